@@ -1,0 +1,149 @@
+/*
+ * libsvm-mi355x device-solver C-ABI.
+ *
+ * Replaces the reference's process-global C++ object
+ *     CudaSolver *cudaSolver        (libsvm/cuda_solver.h:224-246)
+ *     class CudaSolverNU            (libsvm/cuda_solverNU.h:24-57)
+ * with an opaque, handle-scoped solver: every piece of device state lives in
+ * the handle (no __constant__ / __device__ globals), so several solves can
+ * coexist in one process and none of them resets the device.
+ *
+ * Errors: every entry point returns an svmhip_status; nothing throws across
+ * the ABI, nothing calls exit() or hipDeviceReset() (the reference does both,
+ * libsvm/svm_defs.h:108-117, libsvm/cuda_solver.cpp:439-454).  The message of
+ * the last failure on a handle is available from svmhip_last_error().
+ *
+ * Threading: a handle may be used by one host thread at a time; distinct
+ * handles are independent.  Each handle owns one HIP stream.
+ *
+ * Shrinking: the device solver always works on the full variable set (the
+ * reference path requires shrinking=0, svm-train/svm-train.c:260-262);
+ * svm_parameter.shrinking is accepted and ignored.
+ */
+#ifndef LIBSVM_MI355X_SVM_HIP_H
+#define LIBSVM_MI355X_SVM_HIP_H
+
+#include <stddef.h>
+
+#include "svm.h"
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+typedef enum {
+	SVMHIP_OK = 0,
+	SVMHIP_ERR_ARG = 1,       /* invalid argument / parameter */
+	SVMHIP_ERR_DEVICE = 2,    /* HIP runtime error or no usable device */
+	SVMHIP_ERR_NOMEM = 3,     /* device allocation failed */
+	SVMHIP_ERR_STATE = 4,     /* call out of order (e.g. solve before setup) */
+	SVMHIP_ERR_TIMEOUT = 5,   /* a device-side wait exceeded its spin limit */
+	SVMHIP_ERR_MAXITER = 6    /* informational: iteration cap reached */
+} svmhip_status;
+
+/* Which working-set rule the solver uses: C_SVC / ONE_CLASS / EPSILON_SVR use
+ * the standard WSS3 (libsvm/cuda_solver.cpp:493-518); NU_SVC / NU_SVR use the
+ * nu rule (libsvm/cuda_solverNU.cpp:152-178). */
+typedef enum { SVMHIP_KIND_C = 0, SVMHIP_KIND_NU = 1 } svmhip_kind;
+
+/* Device options kept out of svm_parameter so the kept struct stays
+ * byte-identical.  Zero-initialise and set struct_size; zero means "default". */
+typedef struct {
+	size_t struct_size;       /* sizeof(svmhip_options) */
+	int device;               /* HIP device ordinal (default 0) */
+	int grid_blocks;          /* persistent-grid workgroups (0 = autotune) */
+	int block_threads;        /* threads per workgroup (0 = 256) */
+	int max_iter;             /* 0 = LIBSVM rule max(10^7, 100*l) */
+	double cache_bytes;       /* kernel-row cache bytes (0 = param.cache_size MB) */
+	int iters_per_launch;     /* persistent-kernel iterations per launch (0 = all) */
+	int verbose;              /* 1 = print setup and cache statistics to stderr */
+} svmhip_options;
+
+typedef struct svmhip_solver svmhip_solver;
+
+/* Per-solve statistics (filled by svmhip_solve / svmhip_get_stats). */
+typedef struct {
+	int iters;                /* SMO iterations (pairs updated) */
+	int optimal;              /* 1 = stopping rule met, 0 = iteration cap */
+	double obj;               /* 0.5 * sum alpha_i (G_i + p_i), FP64 on device */
+	double rho;               /* Solver::calculate_rho / Solver_NU::calculate_rho */
+	double r;                 /* Solver_NU only */
+	long long row_hits;       /* kernel-row cache hits */
+	long long row_misses;     /* kernel rows computed */
+	double solve_ms;          /* device time of the iteration loop (HIP events) */
+	double setup_ms;          /* device time of G initialisation */
+	int cache_slots;          /* rows the cache holds */
+	int grid_blocks;          /* workgroups used by the persistent kernel */
+} svmhip_stats;
+
+/* CudaSolver(prob, param, quiet) / CudaSolverNU(...)
+ * (libsvm/cuda_solver.cpp:434-448, load_problem_parameters :298-432):
+ * uploads the rows of prob (features truncated to float32, like the
+ * reference) in a SELL-64 layout and the kernel parameters. */
+svmhip_status svmhip_create(const struct svm_problem *prob, const struct svm_parameter *param,
+			    svmhip_kind kind, const svmhip_options *opt, svmhip_solver **out);
+
+/* CudaSolver::setup_rbf_variables (libsvm/cuda_solver.cpp:241-259): squared
+ * row norms for RBF.  Also computes them for every other kernel type as a
+ * no-op-safe call; l must equal prob->l. */
+svmhip_status svmhip_setup_rbf(svmhip_solver *h, int l);
+
+/* CudaSolver::setup_solver (libsvm/cuda_solver.cpp:159-239): copies y, the
+ * initial gradient G (= p, as Solver::Solve passes it, libsvm/svm.cpp:546-557),
+ * alpha and alpha_status, computes QD, initialises G = p + Q alpha on the
+ * device in FP64 and sizes the kernel-row cache.  n_var is l for
+ * classification / one-class and 2l for regression (SVR variable i+l shares
+ * kernel row i). */
+svmhip_status svmhip_setup(svmhip_solver *h, const signed char *y, const double *G, const double *alpha,
+			   const char *alpha_status, double Cp, double Cn, int n_var);
+
+/* CudaSolver::select_working_set / CudaSolverNU::select_working_set
+ * (libsvm/cuda_solver.cpp:493-518, libsvm/cuda_solverNU.cpp:152-178).
+ * *optimal = 1 when the stopping rule holds.  The pair stays on the device,
+ * as in the reference; *out_i / *out_j (may be null) receive it for
+ * inspection, which costs one extra small copy. */
+svmhip_status svmhip_select_working_set(svmhip_solver *h, int n_var, int *optimal, int *out_i, int *out_j);
+
+/* CudaSolver::compute_alpha (libsvm/cuda_solver.cpp:458-463): the two-variable
+ * clipped update in FP64. */
+svmhip_status svmhip_compute_alpha(svmhip_solver *h);
+
+/* CudaSolver::update_gradient (libsvm/cuda_solver.cpp:520-531). */
+svmhip_status svmhip_update_gradient(svmhip_solver *h, int n_var);
+
+/* CudaSolver::update_alpha_status (libsvm/cuda_solver.cpp:465-470). */
+svmhip_status svmhip_update_alpha_status(svmhip_solver *h);
+
+/* CudaSolver::fetch_vectors (libsvm/cuda_solver.cpp:533-555): copies G,
+ * alpha (FP64) and alpha_status back; any pointer may be null. */
+svmhip_status svmhip_fetch(svmhip_solver *h, double *G, double *alpha, char *alpha_status, int n_var);
+
+/* Fused solve: runs the whole SMO loop on the device (one persistent
+ * cooperative launch per chunk of iterations) and computes rho / obj / r in
+ * FP64 on the device.  No counterpart method in the reference: it replaces
+ * the host-driven loop of Solver::Solve (libsvm/svm.cpp:580-776) when the
+ * device solver is active.  max_iter = 0 uses the handle's option. */
+svmhip_status svmhip_solve(svmhip_solver *h, int max_iter, svmhip_stats *stats);
+
+/* rho / obj / r of the current state (after a step-wise loop). */
+svmhip_status svmhip_finalize(svmhip_solver *h, svmhip_stats *stats);
+
+svmhip_status svmhip_get_stats(svmhip_solver *h, svmhip_stats *stats);
+
+/* Device kernel row K(i, 0..l-1) as the solver caches it (float, no label
+ * signs), for testing the row evaluator.  out has l floats (host memory). */
+svmhip_status svmhip_kernel_row(svmhip_solver *h, int i, float *out);
+
+/* Destructor (libsvm/cuda_solver.cpp:450-454) without the device reset. */
+void svmhip_destroy(svmhip_solver *h);
+
+const char *svmhip_last_error(const svmhip_solver *h);
+
+/* Number of usable HIP devices (0 if none); never aborts. */
+int svmhip_device_count(void);
+
+#ifdef __cplusplus
+}
+#endif
+
+#endif /* LIBSVM_MI355X_SVM_HIP_H */

@@ -1,0 +1,727 @@
+// Device-side layout and phase functions of the MI355X SMO solver.
+//
+// Data layout in HBM (one solver handle):
+//   rows of X     SELL-64: rows grouped in slices of 64; slice s stores its
+//                 rows column-major, element p of row 64s+lane at
+//                 vals[slice_ptr[s] + 64p + lane] (and its feature column at
+//                 cols[...] unless the matrix is stored dense).  One lane owns
+//                 one row, so a wave walking p reads 64 consecutive floats:
+//                 coalesced, and every row's dot product is summed in its own
+//                 feature order, which is LIBSVM's merge order
+//                 (libsvm/svm.cpp:305-325) -> bit-identical FP64 dots.
+//   G, alpha, p, QD   FP64 [n_var];  y, status int8 [n_var]
+//   cache rows    float [S][l] kernel rows K(r, .) without label signs;
+//                 Q_ij = y_i y_j K(real(i), real(j)) is formed on use, which is
+//                 exact (sign flips), so SVC / one-class / SVR share one cache.
+//   Ctl           control record: the selected pair, cache decisions, barrier
+//                 counters, iteration count, result scalars.
+//
+// Work split: the persistent grid's workgroup b owns a contiguous range of
+// slices and therefore of real rows; with SVR it handles both variables r and
+// r + l of each row it owns.  Every per-iteration pass touches only the
+// owner's range, so G and the cached rows never cross workgroups inside a
+// pass; the only cross-workgroup data are the per-workgroup partial
+// reductions and the scalars the leader reads (G_i, G_j, Q_ij, alpha_i/j).
+#pragma once
+
+#include <hip/hip_runtime.h>
+#include <stdint.h>
+
+namespace smo {
+
+constexpr int SLICE = 64;
+constexpr double TAU = 1e-12;           // libsvm/svm_defs.h:106, libsvm/svm.cpp:42
+constexpr double INF = __builtin_huge_val();
+
+enum : int8_t { AT_LOWER = 0, AT_UPPER = 1, IS_FREE = 2 };   // libsvm/svm_device.cu:13
+enum { KIND_C = 0, KIND_NU = 1 };
+enum { K_LINEAR = 0, K_POLY = 1, K_RBF = 2, K_SIGMOID = 3, K_PRECOMPUTED = 4 };
+enum { STOP_NONE = 0, STOP_OPTIMAL = 1, STOP_MAXITER = 2 };
+enum { ERR_NONE = 0, ERR_TIMEOUT = 5 };
+
+// Control record.  Written by exactly one thread (the leader's thread 0) at
+// a time; read by all workgroups after a barrier.
+struct Ctl {
+  unsigned arrive;       // grid barrier arrivals
+  unsigned gen;          // grid barrier generation
+  unsigned arrive_lb;    // last-block counter of the step kernels
+  int err;
+  int i, j, ip, in;      // selected variables
+  int slot_i, slot_j, slot_ip, slot_in;
+  int miss_i, miss_j, miss_ip, miss_in;
+  int stop, iter, max_iter, pad0;
+  double dai, daj;       // alpha changes of the last step
+  double Gmax, Gmax2;    // C rule
+  double Gp, Gp2, Gn, Gn2;  // nu rule
+  int lru_head, lru_tail;
+  unsigned long long hits, misses;
+  double rho, obj, r;
+};
+
+// One workgroup's partial reduction (48 bytes).
+struct Partial {
+  double v0, v1, v2, v3;
+  int i0, i1, pad0, pad1;
+};
+
+// Kernel argument block, passed by value.
+struct View {
+  // problem
+  int l, d, nslices, dense;
+  int ktype, degree;
+  double gamma, coef0;
+  const long long *slice_ptr;
+  const float *vals;
+  const int *cols;
+  const int *row_len;
+  const double *xsq;
+  const int *serial;      // PRECOMPUTED: (int)x_r[0].value
+  // solver
+  int nvar, svr, kind, max_iter;
+  double Cp, Cn, eps;
+  double *G, *alpha, *p, *QD;
+  const int8_t *y;
+  int8_t *status;
+  // kernel-row cache
+  float *rows;
+  long long row_stride;   // floats per cached row (>= l)
+  int S, full;
+  int *slot_of_row, *row_of_slot, *lru_prev, *lru_next;
+  int8_t *computed;       // full cache: row already materialised
+  // control
+  Ctl *ctl;
+  Partial *part;
+  float *xs_global;       // per-workgroup query-row scratch when d is too big for LDS
+  int use_lds, nblocks;
+};
+
+// ---------------------------------------------------------------- helpers
+
+__device__ __forceinline__ int real_of(const View &V, int v) { return v < V.l ? v : v - V.l; }
+
+__device__ __forceinline__ double C_of(const View &V, int v) { return V.y[v] > 0 ? V.Cp : V.Cn; }
+
+__device__ __forceinline__ int8_t status_for(const View &V, int v, double a) {
+  // Solver::update_alpha_status, libsvm/svm.cpp:425-432
+  if (a >= C_of(V, v)) return AT_UPPER;
+  if (a <= 0) return AT_LOWER;
+  return IS_FREE;
+}
+
+// argmax with ties to the larger index (the >= scans of libsvm/svm.cpp:851,860)
+__device__ __forceinline__ void max_idx(double &v, int &i, double v2, int i2) {
+  if (v2 > v || (v2 == v && i2 > i)) { v = v2; i = i2; }
+}
+// argmin with ties to the larger index (the <= scans of libsvm/svm.cpp:890,914)
+__device__ __forceinline__ void min_idx(double &v, int &i, double v2, int i2) {
+  if (v2 < v || (v2 == v && i2 > i)) { v = v2; i = i2; }
+}
+
+__device__ __forceinline__ double wave_max(double v) {
+  for (int o = 32; o > 0; o >>= 1) v = fmax(v, __shfl_xor(v, o, 64));
+  return v;
+}
+__device__ __forceinline__ void wave_max_idx(double &v, int &i) {
+  for (int o = 32; o > 0; o >>= 1) {
+    double v2 = __shfl_xor(v, o, 64);
+    int i2 = __shfl_xor(i, o, 64);
+    max_idx(v, i, v2, i2);
+  }
+}
+__device__ __forceinline__ void wave_min_idx(double &v, int &i) {
+  for (int o = 32; o > 0; o >>= 1) {
+    double v2 = __shfl_xor(v, o, 64);
+    int i2 = __shfl_xor(i, o, 64);
+    min_idx(v, i, v2, i2);
+  }
+}
+__device__ __forceinline__ double wave_sum(double v) {
+  for (int o = 32; o > 0; o >>= 1) v += __shfl_xor(v, o, 64);
+  return v;
+}
+__device__ __forceinline__ int wave_sum_i(int v) {
+  for (int o = 32; o > 0; o >>= 1) v += __shfl_xor(v, o, 64);
+  return v;
+}
+
+template <class T>
+__device__ __forceinline__ T ld_acq(const T *p) {
+  return __hip_atomic_load(p, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+}
+
+// powi by repeated squaring, libsvm/svm.cpp:30-40
+__device__ __forceinline__ double powi(double base, int times) {
+  double acc = 1.0, sq = base;
+  for (int t = times; t > 0; t /= 2) {
+    if (t % 2 == 1) acc = acc * sq;
+    sq = sq * sq;
+  }
+  return acc;
+}
+
+// Slice range [sb, se) owned by workgroup b of nb.
+__device__ __forceinline__ void owned_slices(const View &V, int b, int nb, int &sb, int &se) {
+  sb = (int)(((long long)V.nslices * b) / nb);
+  se = (int)(((long long)V.nslices * (b + 1)) / nb);
+}
+
+// ------------------------------------------------------------ row kernel
+
+// Stage row q of X densely (by feature column, or by position for
+// PRECOMPUTED) into xs[0..d).  Whole workgroup; ends with a barrier.
+__device__ void stage_row(const View &V, int q, float *xs) {
+  const int tid = threadIdx.x, nt = blockDim.x;
+  const int len = V.row_len[q];
+  const long long base = V.slice_ptr[q / SLICE] + (q % SLICE);
+  if (!V.dense) {
+    for (int k = tid; k < V.d; k += nt) xs[k] = 0.0f;
+    __syncthreads();
+    const bool by_pos = V.ktype == K_PRECOMPUTED;
+    for (int p = tid; p < len; p += nt) {
+      int c = by_pos ? p : V.cols[base + (long long)p * SLICE];
+      xs[c] = V.vals[base + (long long)p * SLICE];
+    }
+  } else {
+    for (int p = tid; p < len; p += nt) xs[p] = V.vals[base + (long long)p * SLICE];
+  }
+  __syncthreads();
+}
+
+// dot(x_q, x_r) with x_q staged in xs; row r is this lane's row.  Products of
+// two floats are exact in FP64, so fma == multiply-then-add here and the sum
+// is LIBSVM's sequential merge-order sum.
+__device__ __forceinline__ double row_dot(const View &V, const float *xs, int r) {
+  const int len = V.row_len[r];
+  const long long base = V.slice_ptr[r / SLICE] + (r % SLICE);
+  const float *vp = V.vals + base;
+  double acc = 0.0;
+  if (V.dense) {
+    int p = 0;
+    for (; p + 4 <= len; p += 4) {
+      float a0 = vp[(long long)(p + 0) * SLICE], a1 = vp[(long long)(p + 1) * SLICE];
+      float a2 = vp[(long long)(p + 2) * SLICE], a3 = vp[(long long)(p + 3) * SLICE];
+      acc = fma((double)a0, (double)xs[p + 0], acc);
+      acc = fma((double)a1, (double)xs[p + 1], acc);
+      acc = fma((double)a2, (double)xs[p + 2], acc);
+      acc = fma((double)a3, (double)xs[p + 3], acc);
+    }
+    for (; p < len; ++p) acc = fma((double)vp[(long long)p * SLICE], (double)xs[p], acc);
+  } else {
+    const int *cp = V.cols + base;
+    for (int p = 0; p < len; ++p)
+      acc = fma((double)vp[(long long)p * SLICE], (double)xs[cp[(long long)p * SLICE]], acc);
+  }
+  return acc;
+}
+
+// K(q, r) in FP64 given dot(x_q, x_r) (Kernel::kernel_*, libsvm/svm.cpp:243-255).
+__device__ __forceinline__ double kernel_f64(const View &V, int q, int r, double dot) {
+  switch (V.ktype) {
+    case K_LINEAR: return dot;
+    case K_POLY: return powi(V.gamma * dot + V.coef0, V.degree);
+    case K_RBF: return exp(-V.gamma * ((V.xsq[q] + V.xsq[r]) - 2 * dot));
+    case K_SIGMOID: return tanh(V.gamma * dot + V.coef0);
+    default: return 0.0;
+  }
+}
+
+// Cached (float-rounded) kernel value K(q, r); q staged in xs.
+__device__ __forceinline__ float kernel_value(const View &V, const float *xs, int q, int r) {
+  if (V.ktype == K_PRECOMPUTED) {
+    int s = V.serial[r];
+    return (s >= 0 && s < V.d) ? xs[s] : 0.0f;
+  }
+  return (float)kernel_f64(V, q, r, row_dot(V, xs, r));
+}
+
+// Materialise the owned part of kernel row q into cache slot `slot`.
+__device__ void compute_row_part(const View &V, int q, int slot, float *xs, int sb, int se) {
+  stage_row(V, q, xs);
+  float *dst = V.rows + (long long)slot * V.row_stride;
+  const int wave = threadIdx.x / 64, lane = threadIdx.x % 64, nw = blockDim.x / 64;
+  for (int s = sb + wave; s < se; s += nw) {
+    int r = s * SLICE + lane;
+    if (r < V.l) dst[r] = kernel_value(V, xs, q, r);
+  }
+  __syncthreads();
+}
+
+// ------------------------------------------------------- block reductions
+
+struct RedSmem {
+  double v[16][4];
+  int i[16][2];
+  int last;
+  unsigned gen;
+};
+
+// Block-wide reduction of up to 4 (value, index) lanes; op per slot.
+// kinds: 0 = max_idx, 1 = min_idx, 2 = plain max.  Result in thread 0.
+template <int N>
+__device__ void block_reduce(RedSmem &sm, double (&v)[4], int (&ix)[2], const int (&kinds)[4]) {
+  const int wave = threadIdx.x / 64, lane = threadIdx.x % 64, nw = blockDim.x / 64;
+  int slot_i = 0;
+#pragma unroll
+  for (int k = 0; k < N; ++k) {
+    if (kinds[k] == 0) { wave_max_idx(v[k], ix[slot_i]); ++slot_i; }
+    else if (kinds[k] == 1) { wave_min_idx(v[k], ix[slot_i]); ++slot_i; }
+    else v[k] = wave_max(v[k]);
+  }
+  if (lane == 0) {
+#pragma unroll
+    for (int k = 0; k < N; ++k) sm.v[wave][k] = v[k];
+    sm.i[wave][0] = ix[0];
+    sm.i[wave][1] = ix[1];
+  }
+  __syncthreads();
+  if (threadIdx.x == 0) {
+    for (int w = 1; w < nw; ++w) {
+      int si = 0;
+#pragma unroll
+      for (int k = 0; k < N; ++k) {
+        if (kinds[k] == 0) { max_idx(v[k], ix[si], sm.v[w][k], sm.i[w][si]); ++si; }
+        else if (kinds[k] == 1) { min_idx(v[k], ix[si], sm.v[w][k], sm.i[w][si]); ++si; }
+        else v[k] = fmax(v[k], sm.v[w][k]);
+      }
+    }
+  }
+  __syncthreads();
+}
+
+// --------------------------------------------------------------- the cache
+
+__device__ __forceinline__ void lru_touch(const View &V, int s) {
+  Ctl *c = V.ctl;
+  int head = c->lru_head;
+  if (head == s) return;
+  int pv = V.lru_prev[s], nx = V.lru_next[s];
+  if (pv >= 0) V.lru_next[pv] = nx;
+  if (nx >= 0) V.lru_prev[nx] = pv;
+  else c->lru_tail = pv;
+  V.lru_prev[s] = -1;
+  V.lru_next[s] = head;
+  V.lru_prev[head] = s;
+  c->lru_head = s;
+}
+
+// Slot holding kernel row `row`; on a miss the least recently used slot other
+// than `exclude` is reassigned (the row is then computed by every workgroup
+// for its owned range in the next pass).  Single thread.
+__device__ int cache_lookup(const View &V, int row, int exclude, int *miss) {
+  Ctl *c = V.ctl;
+  if (V.full) {
+    *miss = !V.computed[row];
+    V.computed[row] = 1;
+    if (*miss) c->misses++; else c->hits++;
+    return row;
+  }
+  int s = V.slot_of_row[row];
+  if (s >= 0) {
+    *miss = 0;
+    c->hits++;
+    lru_touch(V, s);
+    return s;
+  }
+  int v = c->lru_tail;
+  if (v == exclude) v = V.lru_prev[v];
+  int old = V.row_of_slot[v];
+  if (old >= 0) V.slot_of_row[old] = -1;
+  V.row_of_slot[v] = row;
+  V.slot_of_row[row] = v;
+  lru_touch(V, v);
+  *miss = 1;
+  c->misses++;
+  return v;
+}
+
+// ------------------------------------------------------------ the step math
+
+// Two-variable analytic update with clipping, FP64 (libsvm/svm.cpp:636-721,
+// libsvm/svm_device.cu:828-927).  Single thread.  Qij = Q_i[j] as cached.
+__device__ void alpha_step(const View &V, int i, int j, float Qij) {
+  double *a = V.alpha;
+  const double Ci = C_of(V, i), Cj = C_of(V, j);
+  const double ai0 = a[i], aj0 = a[j];
+  double ai = ai0, aj = aj0;
+  if (V.y[i] != V.y[j]) {
+    double quad = V.QD[i] + V.QD[j] + (double)(2.0f * Qij);
+    if (quad <= 0) quad = TAU;
+    double delta = (-V.G[i] - V.G[j]) / quad;
+    double diff = ai - aj;
+    ai += delta;
+    aj += delta;
+    if (diff > 0) {
+      if (aj < 0) { aj = 0; ai = diff; }
+    } else {
+      if (ai < 0) { ai = 0; aj = -diff; }
+    }
+    if (diff > Ci - Cj) {
+      if (ai > Ci) { ai = Ci; aj = Ci - diff; }
+    } else {
+      if (aj > Cj) { aj = Cj; ai = Cj + diff; }
+    }
+  } else {
+    double quad = V.QD[i] + V.QD[j] - (double)(2.0f * Qij);
+    if (quad <= 0) quad = TAU;
+    double delta = (V.G[i] - V.G[j]) / quad;
+    double sum = ai + aj;
+    ai -= delta;
+    aj += delta;
+    if (sum > Ci) {
+      if (ai > Ci) { ai = Ci; aj = sum - Ci; }
+    } else {
+      if (aj < 0) { aj = 0; ai = sum; }
+    }
+    if (sum > Cj) {
+      if (aj > Cj) { aj = Cj; ai = sum - Cj; }
+    } else {
+      if (ai < 0) { ai = 0; aj = sum; }
+    }
+  }
+  a[i] = ai;
+  a[j] = aj;
+  V.ctl->dai = ai - ai0;
+  V.ctl->daj = aj - aj0;
+}
+
+__device__ __forceinline__ float q_elem(const View &V, int slot, int i, int v) {
+  // Q_iv = y_i y_v K(real i, real v); sign flips of a float are exact
+  float k = V.rows[(long long)slot * V.row_stride + real_of(V, v)];
+  return (V.y[i] == V.y[v]) ? k : -k;
+}
+
+// --------------------------------------------------------------- the passes
+
+// Snapshot of the control scalars a pass needs.
+struct Snap {
+  int i, j, ip, in, slot_i, slot_j, slot_ip, slot_in, miss_i, miss_j, miss_ip, miss_in;
+  int stop, err, iter;
+  double dai, daj, Gmax, Gp, Gn;
+};
+
+__device__ __forceinline__ Snap snap(const Ctl *c) {
+  Snap s;
+  s.i = ld_acq(&c->i); s.j = ld_acq(&c->j); s.ip = ld_acq(&c->ip); s.in = ld_acq(&c->in);
+  s.slot_i = ld_acq(&c->slot_i); s.slot_j = ld_acq(&c->slot_j);
+  s.slot_ip = ld_acq(&c->slot_ip); s.slot_in = ld_acq(&c->slot_in);
+  s.miss_i = ld_acq(&c->miss_i); s.miss_j = ld_acq(&c->miss_j);
+  s.miss_ip = ld_acq(&c->miss_ip); s.miss_in = ld_acq(&c->miss_in);
+  s.stop = ld_acq(&c->stop); s.err = ld_acq(&c->err); s.iter = ld_acq(&c->iter);
+  s.dai = ld_acq(&c->dai); s.daj = ld_acq(&c->daj);
+  s.Gmax = ld_acq(&c->Gmax); s.Gp = ld_acq(&c->Gp); s.Gn = ld_acq(&c->Gn);
+  return s;
+}
+
+// Pass "select i": per-workgroup partial of the WSS i-step over the owned
+// variables (libsvm/svm.cpp:840-863 and the Gmax2 terms of :874-915).
+// If update is set, first applies G_v += Q_iv dai + Q_jv daj
+// (libsvm/svm.cpp:724-736), computing the owned part of row j on a miss.
+template <int KIND>
+__device__ void pass_gradient_select(const View &V, const Snap &c, float *xs, RedSmem &sm, bool update,
+                                     bool select) {
+  int sb, se;
+  owned_slices(V, blockIdx.x, V.nblocks, sb, se);
+  if (update && c.miss_j) compute_row_part(V, real_of(V, c.j), c.slot_j, xs, sb, se);
+  double v[4] = {-INF, -INF, -INF, -INF};
+  int ix[2] = {-1, -1};
+  const int wave = threadIdx.x / 64, lane = threadIdx.x % 64, nw = blockDim.x / 64;
+  const int nv = V.svr ? 2 : 1;
+  for (int s = sb + wave; s < se; s += nw) {
+    int r = s * SLICE + lane;
+    if (r >= V.l) continue;
+    float Kir = 0.f, Kjr = 0.f;
+    if (update) {
+      Kir = V.rows[(long long)c.slot_i * V.row_stride + r];
+      Kjr = V.rows[(long long)c.slot_j * V.row_stride + r];
+    }
+    for (int h = 0; h < nv; ++h) {
+      int k = r + h * V.l;
+      int8_t yk = V.y[k];
+      double g = V.G[k];
+      if (update) {
+        float Qik = (V.y[c.i] == yk) ? Kir : -Kir;
+        float Qjk = (V.y[c.j] == yk) ? Kjr : -Kjr;
+        double t1 = (double)Qik * c.dai;
+        double t2 = (double)Qjk * c.daj;
+        g = g + (t1 + t2);
+        V.G[k] = g;
+      }
+      if (!select) continue;
+      int8_t st = V.status[k];
+      if (KIND == KIND_C) {
+        if (yk == +1) {
+          if (st != AT_UPPER) max_idx(v[0], ix[0], -g, k);
+          if (st != AT_LOWER) v[1] = fmax(v[1], g);
+        } else {
+          if (st != AT_LOWER) max_idx(v[0], ix[0], g, k);
+          if (st != AT_UPPER) v[1] = fmax(v[1], -g);
+        }
+      } else {
+        if (yk == +1) {
+          if (st != AT_UPPER) max_idx(v[0], ix[0], -g, k);
+          if (st != AT_LOWER) v[2] = fmax(v[2], g);
+        } else {
+          if (st != AT_LOWER) max_idx(v[1], ix[1], g, k);
+          if (st != AT_UPPER) v[3] = fmax(v[3], -g);
+        }
+      }
+    }
+  }
+  if (!select) return;
+  if (KIND == KIND_C) {
+    const int kinds[4] = {0, 2, 2, 2};
+    block_reduce<2>(sm, v, ix, kinds);
+  } else {
+    const int kinds[4] = {0, 0, 2, 2};
+    block_reduce<4>(sm, v, ix, kinds);
+  }
+  if (threadIdx.x == 0) {
+    Partial pt;
+    pt.v0 = v[0]; pt.v1 = v[1]; pt.v2 = v[2]; pt.v3 = v[3];
+    pt.i0 = ix[0]; pt.i1 = ix[1]; pt.pad0 = pt.pad1 = 0;
+    V.part[blockIdx.x] = pt;
+  }
+}
+
+// Pass "select j": second-order gain over the owned variables
+// (libsvm/svm.cpp:866-917; nu rule :1123-1172), row i (rows ip / in) computed
+// for the owned range first on a miss.
+template <int KIND>
+__device__ void pass_select_j(const View &V, const Snap &c, float *xs, RedSmem &sm) {
+  int sb, se;
+  owned_slices(V, blockIdx.x, V.nblocks, sb, se);
+  if (KIND == KIND_C) {
+    if (c.miss_i) compute_row_part(V, real_of(V, c.i), c.slot_i, xs, sb, se);
+  } else {
+    if (c.ip >= 0 && c.miss_ip) compute_row_part(V, real_of(V, c.ip), c.slot_ip, xs, sb, se);
+    if (c.in >= 0 && c.miss_in) compute_row_part(V, real_of(V, c.in), c.slot_in, xs, sb, se);
+  }
+  double best = INF;
+  int bj = -1;
+  const int wave = threadIdx.x / 64, lane = threadIdx.x % 64, nw = blockDim.x / 64;
+  const int nv = V.svr ? 2 : 1;
+  for (int s = sb + wave; s < se; s += nw) {
+    int r = s * SLICE + lane;
+    if (r >= V.l) continue;
+    for (int h = 0; h < nv; ++h) {
+      int k = r + h * V.l;
+      int8_t yk = V.y[k];
+      int8_t st = V.status[k];
+      double g = V.G[k];
+      if (KIND == KIND_C) {
+        double gd;
+        if (yk == +1) {
+          if (st == AT_LOWER) continue;
+          gd = c.Gmax + g;
+        } else {
+          if (st == AT_UPPER) continue;
+          gd = c.Gmax - g;
+        }
+        if (gd > 0) {
+          float Kir = V.rows[(long long)c.slot_i * V.row_stride + r];
+          int8_t yi = V.y[c.i];
+          float Qik = (yi == yk) ? Kir : -Kir;
+          double t = (2.0 * yi) * (double)Qik;
+          double quad = (yk == +1) ? (V.QD[c.i] + V.QD[k]) - t : (V.QD[c.i] + V.QD[k]) + t;
+          double od = quad > 0 ? -(gd * gd) / quad : -(gd * gd) / TAU;
+          min_idx(best, bj, od, k);
+        }
+      } else {
+        if (yk == +1) {
+          if (st == AT_LOWER) continue;
+          double gd = c.Gp + g;
+          if (gd > 0) {
+            float Kr = V.rows[(long long)c.slot_ip * V.row_stride + r];
+            float Q = (V.y[c.ip] == yk) ? Kr : -Kr;
+            double quad = (V.QD[c.ip] + V.QD[k]) - (double)(2.0f * Q);
+            double od = quad > 0 ? -(gd * gd) / quad : -(gd * gd) / TAU;
+            min_idx(best, bj, od, k);
+          }
+        } else {
+          if (st == AT_UPPER) continue;
+          double gd = c.Gn - g;
+          if (gd > 0) {
+            float Kr = V.rows[(long long)c.slot_in * V.row_stride + r];
+            float Q = (V.y[c.in] == yk) ? Kr : -Kr;
+            double quad = (V.QD[c.in] + V.QD[k]) - (double)(2.0f * Q);
+            double od = quad > 0 ? -(gd * gd) / quad : -(gd * gd) / TAU;
+            min_idx(best, bj, od, k);
+          }
+        }
+      }
+    }
+  }
+  double v[4] = {best, 0, 0, 0};
+  int ix[2] = {bj, -1};
+  const int kinds[4] = {1, 2, 2, 2};
+  block_reduce<1>(sm, v, ix, kinds);
+  if (threadIdx.x == 0) {
+    Partial pt;
+    pt.v0 = v[0]; pt.v1 = pt.v2 = pt.v3 = 0;
+    pt.i0 = ix[0]; pt.i1 = -1; pt.pad0 = pt.pad1 = 0;
+    V.part[blockIdx.x] = pt;
+  }
+}
+
+// Partials are read with agent-scope loads: they were written by other
+// workgroups, possibly on other XCDs, earlier in the same launch.
+__device__ __forceinline__ Partial load_partial(const Partial *p) {
+  Partial q;
+  q.v0 = ld_acq(&p->v0); q.v1 = ld_acq(&p->v1); q.v2 = ld_acq(&p->v2); q.v3 = ld_acq(&p->v3);
+  q.i0 = ld_acq(&p->i0); q.i1 = ld_acq(&p->i1); q.pad0 = q.pad1 = 0;
+  return q;
+}
+
+// ----------------------------------------------------------- leader work
+// Executed by the one workgroup that arrives last; thread 0 writes Ctl.
+
+template <int KIND>
+__device__ void leader_after_select_i(const View &V, RedSmem &sm, bool count_maxiter) {
+  double v[4] = {-INF, -INF, -INF, -INF};
+  int ix[2] = {-1, -1};
+  for (int b = threadIdx.x; b < V.nblocks; b += blockDim.x) {
+    Partial pt = load_partial(&V.part[b]);
+    if (KIND == KIND_C) {
+      max_idx(v[0], ix[0], pt.v0, pt.i0);
+      v[1] = fmax(v[1], pt.v1);
+    } else {
+      max_idx(v[0], ix[0], pt.v0, pt.i0);
+      max_idx(v[1], ix[1], pt.v1, pt.i1);
+      v[2] = fmax(v[2], pt.v2);
+      v[3] = fmax(v[3], pt.v3);
+    }
+  }
+  if (KIND == KIND_C) {
+    const int kinds[4] = {0, 2, 2, 2};
+    block_reduce<2>(sm, v, ix, kinds);
+  } else {
+    const int kinds[4] = {0, 0, 2, 2};
+    block_reduce<4>(sm, v, ix, kinds);
+  }
+  if (threadIdx.x != 0) return;
+  Ctl *c = V.ctl;
+  int stop = STOP_NONE;
+  if (count_maxiter && c->iter >= V.max_iter) stop = STOP_MAXITER;
+  if (KIND == KIND_C) {
+    c->Gmax = v[0];
+    c->Gmax2 = v[1];
+    c->i = ix[0];
+    if (!stop && v[0] + v[1] < V.eps) stop = STOP_OPTIMAL;   // libsvm/svm.cpp:919
+    if (!stop) {
+      int m;
+      c->slot_i = cache_lookup(V, real_of(V, ix[0]), -1, &m);
+      c->miss_i = m;
+    }
+  } else {
+    c->Gp = v[0]; c->Gn = v[1]; c->Gp2 = v[2]; c->Gn2 = v[3];
+    c->ip = ix[0]; c->in = ix[1];
+    if (!stop && fmax(v[0] + v[2], v[1] + v[3]) < V.eps) stop = STOP_OPTIMAL;  // libsvm/svm.cpp:1174
+    if (!stop) {
+      int m, sp = -1;
+      c->miss_ip = 0; c->miss_in = 0;
+      if (ix[0] >= 0) { sp = cache_lookup(V, real_of(V, ix[0]), -1, &m); c->slot_ip = sp; c->miss_ip = m; }
+      if (ix[1] >= 0) { c->slot_in = cache_lookup(V, real_of(V, ix[1]), sp, &m); c->miss_in = m; }
+    }
+  }
+  c->stop = stop;
+}
+
+// After "select j": pick j, look up its row, and (persistent mode) do the
+// alpha step and the status update of the pair.
+template <int KIND>
+__device__ void leader_after_select_j(const View &V, RedSmem &sm, bool do_step) {
+  double v[4] = {INF, 0, 0, 0};
+  int ix[2] = {-1, -1};
+  for (int b = threadIdx.x; b < V.nblocks; b += blockDim.x) {
+    Partial pt = load_partial(&V.part[b]);
+    min_idx(v[0], ix[0], pt.v0, pt.i0);
+  }
+  const int kinds[4] = {1, 2, 2, 2};
+  block_reduce<1>(sm, v, ix, kinds);
+  if (threadIdx.x != 0) return;
+  Ctl *c = V.ctl;
+  int j = ix[0];
+  if (j < 0) {  // no eligible j: only reachable with eps <= 0 or NaN gradients
+    c->stop = STOP_OPTIMAL;
+    return;
+  }
+  int i, slot_i;
+  if (KIND == KIND_C) {
+    i = c->i;
+    slot_i = c->slot_i;
+  } else {
+    // out_i = y[Gmin_idx] == +1 ? Gmaxp_idx : Gmaxn_idx, libsvm/svm.cpp:1177-1180
+    if (V.y[j] == +1) { i = c->ip; slot_i = c->slot_ip; }
+    else { i = c->in; slot_i = c->slot_in; }
+    c->i = i;
+    c->slot_i = slot_i;
+  }
+  c->j = j;
+  int m;
+  c->slot_j = cache_lookup(V, real_of(V, j), slot_i, &m);
+  c->miss_j = m;
+  c->iter = c->iter + 1;
+  if (do_step) {
+    float Qij = q_elem(V, slot_i, i, j);
+    alpha_step(V, i, j, Qij);
+    V.status[i] = status_for(V, i, V.alpha[i]);
+    V.status[j] = status_for(V, j, V.alpha[j]);
+  }
+}
+
+// ---------------------------------------------------------- synchronisation
+
+// Last-arriving workgroup detection for the non-persistent step kernels.
+__device__ __forceinline__ bool arrive_last(const View &V, RedSmem &sm) {
+  __syncthreads();
+  if (threadIdx.x == 0) {
+    __threadfence();
+    unsigned prev = atomicAdd(&V.ctl->arrive_lb, 1u);
+    sm.last = (prev == (unsigned)V.nblocks - 1u);
+    if (sm.last) atomicExch(&V.ctl->arrive_lb, 0u);
+  }
+  __syncthreads();
+  if (sm.last) __threadfence();
+  return sm.last;
+}
+
+constexpr long long SPIN_LIMIT = 1LL << 23;
+
+// Grid barrier whose last arriver runs `leader` (whole workgroup) before
+// releasing the others.  Waits are bounded: a workgroup that spins past
+// SPIN_LIMIT records ERR_TIMEOUT and leaves, so the kernel always drains.
+template <class F>
+__device__ void grid_leader_barrier(const View &V, RedSmem &sm, F leader) {
+  Ctl *c = V.ctl;
+  __syncthreads();
+  if (threadIdx.x == 0) {
+    sm.gen = ld_acq(&c->gen);
+    __threadfence();
+    unsigned prev = atomicAdd(&c->arrive, 1u);
+    sm.last = (prev == (unsigned)V.nblocks - 1u);
+  }
+  __syncthreads();
+  if (sm.last) {
+    __threadfence();
+    leader();
+    __syncthreads();
+    if (threadIdx.x == 0) {
+      atomicExch(&c->arrive, 0u);
+      __threadfence();
+      atomicAdd(&c->gen, 1u);
+    }
+  } else if (threadIdx.x == 0) {
+    long long spins = 0;
+    while (ld_acq(&c->gen) == sm.gen) {
+      __builtin_amdgcn_s_sleep(2);
+      if (++spins > SPIN_LIMIT) {
+        atomicExch(&c->err, (int)ERR_TIMEOUT);
+        break;
+      }
+    }
+    __threadfence();
+  }
+  __syncthreads();
+}
+
+}  // namespace smo
