@@ -1,0 +1,325 @@
+// gfx950 kernels of the SMO solver and their launch wrappers.
+//
+// Kernel set (each replaces one or more reference kernels, see the comments):
+//   k_self_dot      x^2 per row                (cuda_setup_x_square, svm_device.cu:771-777)
+//   k_qd            QD_v = K(v, v) in FP64     (cuda_setup_QD, svm_device.cu:779-789)
+//   k_ginit_row     G += alpha_v Q_v, one v    (cuda_init_gradient_block, svm_device.cu:717-758)
+//   k_select_i      WSS i-step + Gmax2          (cuda_prep_gmax + cuda_find_gmax, :760-821)
+//   k_select_j      WSS j-step                  (cuda_compute_obj_diff + cuda_find_min_idx, :476-561)
+//   k_alpha         pair update, FP64           (cuda_compute_alpha, :828-927)
+//   k_gupdate       G update                    (cuda_update_gradient, :591-626)
+//   k_status        status of the pair          (cuda_update_alpha_status, :939-948)
+//   k_smo           persistent cooperative kernel running select j -> step ->
+//                   G update + select i for many iterations (no reference
+//                   counterpart: the reference drives ~9 launches and 2 blocking
+//                   copies per iteration from the host, cuda_solver.cpp:493-518)
+//   k_finalize      rho / obj / r in FP64       (Solver::calculate_rho, svm.cpp:1013-1049,
+//                                                Solver_NU::calculate_rho, :1260-1308)
+//   k_kernel_row    one kernel row to a buffer  (test hook)
+#include "smo_device.hpp"
+
+namespace smo {
+
+constexpr int BLOCK = 256;
+
+__global__ void __launch_bounds__(BLOCK) k_self_dot(View V, double *xsq) {
+  int r = blockIdx.x * blockDim.x + threadIdx.x;
+  if (r >= V.l) return;
+  const int len = V.row_len[r];
+  const long long base = V.slice_ptr[r / SLICE] + (r % SLICE);
+  double acc = 0.0;
+  for (int p = 0; p < len; ++p) {
+    double v = (double)V.vals[base + (long long)p * SLICE];
+    acc = fma(v, v, acc);  // v*v is exact: same as multiply-then-add
+  }
+  xsq[r] = acc;
+}
+
+__global__ void __launch_bounds__(BLOCK) k_qd(View V) {
+  int v = blockIdx.x * blockDim.x + threadIdx.x;
+  if (v >= V.nvar) return;
+  int r = real_of(V, v);
+  double q;
+  if (V.ktype == K_PRECOMPUTED) {
+    // x_r[serial_r].value, by node position (libsvm/svm.cpp:255)
+    int s = V.serial[r];
+    const long long base = V.slice_ptr[r / SLICE] + (r % SLICE);
+    q = (s >= 0 && s < V.row_len[r]) ? (double)V.vals[base + (long long)s * SLICE] : 0.0;
+  } else {
+    q = kernel_f64(V, r, r, V.xsq[r]);
+  }
+  V.QD[v] = q;
+}
+
+// G_k += alpha_v * Q_vk for all k: one row of the initial gradient sum
+// (libsvm/svm.cpp:555-569).  Launched once per v with alpha_v != 0, in
+// increasing v, so every G_k accumulates in LIBSVM's order.
+__global__ void __launch_bounds__(BLOCK) k_ginit_row(View V, int v) {
+  extern __shared__ float xs_lds[];
+  float *xs = V.use_lds ? xs_lds : V.xs_global + (long long)blockIdx.x * V.d;
+  int sb, se;
+  owned_slices(V, blockIdx.x, gridDim.x, sb, se);
+  const int q = real_of(V, v);
+  stage_row(V, q, xs);
+  const double av = V.alpha[v];
+  const int8_t yv = V.y[v];
+  const int wave = threadIdx.x / 64, lane = threadIdx.x % 64, nw = blockDim.x / 64;
+  const int nv = V.svr ? 2 : 1;
+  for (int s = sb + wave; s < se; s += nw) {
+    int r = s * SLICE + lane;
+    if (r >= V.l) continue;
+    float K = kernel_value(V, xs, q, r);
+    for (int h = 0; h < nv; ++h) {
+      int k = r + h * V.l;
+      float Q = (V.y[k] == yv) ? K : -K;
+      V.G[k] = V.G[k] + av * (double)Q;
+    }
+  }
+}
+
+template <int KIND>
+__global__ void __launch_bounds__(BLOCK) k_select_i(View V) {
+  __shared__ RedSmem sm;
+  extern __shared__ float xs_lds[];
+  float *xs = V.use_lds ? xs_lds : V.xs_global + (long long)blockIdx.x * V.d;
+  Snap c = snap(V.ctl);
+  pass_gradient_select<KIND>(V, c, xs, sm, false, true);
+  if (arrive_last(V, sm)) leader_after_select_i<KIND>(V, sm, false);
+}
+
+template <int KIND>
+__global__ void __launch_bounds__(BLOCK) k_select_j(View V) {
+  __shared__ RedSmem sm;
+  extern __shared__ float xs_lds[];
+  float *xs = V.use_lds ? xs_lds : V.xs_global + (long long)blockIdx.x * V.d;
+  Snap c = snap(V.ctl);
+  pass_select_j<KIND>(V, c, xs, sm);
+  if (arrive_last(V, sm)) leader_after_select_j<KIND>(V, sm, false);
+}
+
+__global__ void k_alpha(View V) {
+  if (threadIdx.x != 0 || blockIdx.x != 0) return;
+  Ctl *c = V.ctl;
+  int i = c->i, j = c->j;
+  alpha_step(V, i, j, q_elem(V, c->slot_i, i, j));
+}
+
+template <int KIND>
+__global__ void __launch_bounds__(BLOCK) k_gupdate(View V) {
+  __shared__ RedSmem sm;
+  extern __shared__ float xs_lds[];
+  float *xs = V.use_lds ? xs_lds : V.xs_global + (long long)blockIdx.x * V.d;
+  Snap c = snap(V.ctl);
+  pass_gradient_select<KIND>(V, c, xs, sm, true, false);
+}
+
+__global__ void k_status(View V) {
+  if (threadIdx.x != 0 || blockIdx.x != 0) return;
+  Ctl *c = V.ctl;
+  V.status[c->i] = status_for(V, c->i, V.alpha[c->i]);
+  V.status[c->j] = status_for(V, c->j, V.alpha[c->j]);
+}
+
+// The whole SMO loop on the device.  Per iteration: [select j] barrier
+// [leader: j, row-j slot, alpha step, status] [G update + select i] barrier
+// [leader: i, Gmax/Gmax2, stop test, row-i slot].  Requires co-resident
+// workgroups (cooperative launch).
+template <int KIND>
+__global__ void __launch_bounds__(BLOCK) k_smo(View V, int iter_limit) {
+  __shared__ RedSmem sm;
+  extern __shared__ float xs_lds[];
+  float *xs = V.use_lds ? xs_lds : V.xs_global + (long long)blockIdx.x * V.d;
+  Snap c = snap(V.ctl);
+  for (int n = 0; n < iter_limit; ++n) {
+    if (c.stop || c.err) break;
+    pass_select_j<KIND>(V, c, xs, sm);
+    grid_leader_barrier(V, sm, [&] { leader_after_select_j<KIND>(V, sm, true); });
+    c = snap(V.ctl);
+    if (c.stop || c.err) break;
+    pass_gradient_select<KIND>(V, c, xs, sm, true, true);
+    grid_leader_barrier(V, sm, [&] { leader_after_select_i<KIND>(V, sm, true); });
+    c = snap(V.ctl);
+  }
+}
+
+// rho, obj (and r for the nu rule) over all variables.
+template <int KIND>
+__global__ void __launch_bounds__(BLOCK) k_finalize(View V) {
+  __shared__ RedSmem sm;
+  __shared__ double red[BLOCK / 64][8];
+  __shared__ int redi[BLOCK / 64][2];
+  // accumulators: C: s, ub, lb | NU: s1, ub1, lb1, s2, ub2, lb2 ; obj
+  double s1 = 0, ub1 = INF, lb1 = -INF, s2 = 0, ub2 = INF, lb2 = -INF, ob = 0;
+  int n1 = 0, n2 = 0;
+  for (int v = blockIdx.x * blockDim.x + threadIdx.x; v < V.nvar; v += gridDim.x * blockDim.x) {
+    double g = V.G[v];
+    int8_t yv = V.y[v], st = V.status[v];
+    ob += V.alpha[v] * (g + V.p[v]);
+    if (KIND == KIND_C) {
+      double yG = yv * g;
+      if (st == AT_UPPER) {
+        if (yv == -1) ub1 = fmin(ub1, yG); else lb1 = fmax(lb1, yG);
+      } else if (st == AT_LOWER) {
+        if (yv == +1) ub1 = fmin(ub1, yG); else lb1 = fmax(lb1, yG);
+      } else {
+        ++n1;
+        s1 += yG;
+      }
+    } else {
+      if (yv == +1) {
+        if (st == AT_UPPER) lb1 = fmax(lb1, g);
+        else if (st == AT_LOWER) ub1 = fmin(ub1, g);
+        else { ++n1; s1 += g; }
+      } else {
+        if (st == AT_UPPER) lb2 = fmax(lb2, g);
+        else if (st == AT_LOWER) ub2 = fmin(ub2, g);
+        else { ++n2; s2 += g; }
+      }
+    }
+  }
+  // block reduce
+  const int wave = threadIdx.x / 64, lane = threadIdx.x % 64, nw = blockDim.x / 64;
+  s1 = wave_sum(s1); s2 = wave_sum(s2); ob = wave_sum(ob);
+  lb1 = wave_max(lb1); lb2 = wave_max(lb2);
+  ub1 = -wave_max(-ub1); ub2 = -wave_max(-ub2);
+  n1 = wave_sum_i(n1); n2 = wave_sum_i(n2);
+  if (lane == 0) {
+    red[wave][0] = s1; red[wave][1] = ub1; red[wave][2] = lb1; red[wave][3] = s2;
+    red[wave][4] = ub2; red[wave][5] = lb2; red[wave][6] = ob;
+    redi[wave][0] = n1; redi[wave][1] = n2;
+  }
+  __syncthreads();
+  if (threadIdx.x == 0) {
+    for (int w = 1; w < nw; ++w) {
+      s1 += red[w][0]; ub1 = fmin(ub1, red[w][1]); lb1 = fmax(lb1, red[w][2]);
+      s2 += red[w][3]; ub2 = fmin(ub2, red[w][4]); lb2 = fmax(lb2, red[w][5]);
+      ob += red[w][6]; n1 += redi[w][0]; n2 += redi[w][1];
+    }
+    Partial a, b;
+    a.v0 = s1; a.v1 = ub1; a.v2 = lb1; a.v3 = ob; a.i0 = n1; a.i1 = n2; a.pad0 = a.pad1 = 0;
+    b.v0 = s2; b.v1 = ub2; b.v2 = lb2; b.v3 = 0; b.i0 = b.i1 = b.pad0 = b.pad1 = 0;
+    V.part[2 * blockIdx.x] = a;
+    V.part[2 * blockIdx.x + 1] = b;
+  }
+  if (!arrive_last(V, sm)) return;
+  if (threadIdx.x != 0) return;
+  s1 = 0; ub1 = INF; lb1 = -INF; s2 = 0; ub2 = INF; lb2 = -INF; ob = 0; n1 = n2 = 0;
+  for (int b = 0; b < (int)gridDim.x; ++b) {
+    Partial a = load_partial(&V.part[2 * b]), q = load_partial(&V.part[2 * b + 1]);
+    s1 += a.v0; ub1 = fmin(ub1, a.v1); lb1 = fmax(lb1, a.v2); ob += a.v3; n1 += a.i0; n2 += a.i1;
+    s2 += q.v0; ub2 = fmin(ub2, q.v1); lb2 = fmax(lb2, q.v2);
+  }
+  Ctl *c = V.ctl;
+  c->obj = ob / 2;
+  if (KIND == KIND_C) {
+    c->rho = n1 > 0 ? s1 / n1 : (ub1 + lb1) / 2;
+    c->r = 0;
+  } else {
+    double r1 = n1 > 0 ? s1 / n1 : (ub1 + lb1) / 2;
+    double r2 = n2 > 0 ? s2 / n2 : (ub2 + lb2) / 2;
+    c->r = (r1 + r2) / 2;
+    c->rho = (r1 - r2) / 2;
+  }
+}
+
+__global__ void __launch_bounds__(BLOCK) k_kernel_row(View V, int q, float *out) {
+  extern __shared__ float xs_lds[];
+  float *xs = V.use_lds ? xs_lds : V.xs_global + (long long)blockIdx.x * V.d;
+  int sb, se;
+  owned_slices(V, blockIdx.x, gridDim.x, sb, se);
+  stage_row(V, q, xs);
+  const int wave = threadIdx.x / 64, lane = threadIdx.x % 64, nw = blockDim.x / 64;
+  for (int s = sb + wave; s < se; s += nw) {
+    int r = s * SLICE + lane;
+    if (r < V.l) out[r] = kernel_value(V, xs, q, r);
+  }
+}
+
+}  // namespace smo
+
+// ------------------------------------------------------------ launchers
+// Called from the host runtime (smo_runtime.cpp); every launcher returns
+// the launch status and never synchronises.
+extern "C" {
+
+using smo::View;
+static inline unsigned nb_for(long long n, int bs) { return (unsigned)((n + bs - 1) / bs); }
+
+hipError_t smo_launch_self_dot(const View *V, double *xsq, hipStream_t st) {
+  hipLaunchKernelGGL(smo::k_self_dot, dim3(nb_for(V->l, smo::BLOCK)), dim3(smo::BLOCK), 0, st, *V, xsq);
+  return hipGetLastError();
+}
+
+hipError_t smo_launch_qd(const View *V, hipStream_t st) {
+  hipLaunchKernelGGL(smo::k_qd, dim3(nb_for(V->nvar, smo::BLOCK)), dim3(smo::BLOCK), 0, st, *V);
+  return hipGetLastError();
+}
+
+hipError_t smo_launch_ginit_row(const View *V, int v, size_t shmem, hipStream_t st) {
+  hipLaunchKernelGGL(smo::k_ginit_row, dim3(V->nblocks), dim3(smo::BLOCK), shmem, st, *V, v);
+  return hipGetLastError();
+}
+
+hipError_t smo_launch_select_i(const View *V, size_t shmem, hipStream_t st) {
+  if (V->kind == smo::KIND_C)
+    hipLaunchKernelGGL(smo::k_select_i<smo::KIND_C>, dim3(V->nblocks), dim3(smo::BLOCK), shmem, st, *V);
+  else
+    hipLaunchKernelGGL(smo::k_select_i<smo::KIND_NU>, dim3(V->nblocks), dim3(smo::BLOCK), shmem, st, *V);
+  return hipGetLastError();
+}
+
+hipError_t smo_launch_select_j(const View *V, size_t shmem, hipStream_t st) {
+  if (V->kind == smo::KIND_C)
+    hipLaunchKernelGGL(smo::k_select_j<smo::KIND_C>, dim3(V->nblocks), dim3(smo::BLOCK), shmem, st, *V);
+  else
+    hipLaunchKernelGGL(smo::k_select_j<smo::KIND_NU>, dim3(V->nblocks), dim3(smo::BLOCK), shmem, st, *V);
+  return hipGetLastError();
+}
+
+hipError_t smo_launch_alpha(const View *V, hipStream_t st) {
+  hipLaunchKernelGGL(smo::k_alpha, dim3(1), dim3(64), 0, st, *V);
+  return hipGetLastError();
+}
+
+hipError_t smo_launch_gupdate(const View *V, size_t shmem, hipStream_t st) {
+  if (V->kind == smo::KIND_C)
+    hipLaunchKernelGGL(smo::k_gupdate<smo::KIND_C>, dim3(V->nblocks), dim3(smo::BLOCK), shmem, st, *V);
+  else
+    hipLaunchKernelGGL(smo::k_gupdate<smo::KIND_NU>, dim3(V->nblocks), dim3(smo::BLOCK), shmem, st, *V);
+  return hipGetLastError();
+}
+
+hipError_t smo_launch_status(const View *V, hipStream_t st) {
+  hipLaunchKernelGGL(smo::k_status, dim3(1), dim3(64), 0, st, *V);
+  return hipGetLastError();
+}
+
+hipError_t smo_launch_persistent(const View *V, int iter_limit, size_t shmem, hipStream_t st) {
+  View v = *V;
+  void *args[] = {&v, &iter_limit};
+  const void *fn = V->kind == smo::KIND_C ? (const void *)smo::k_smo<smo::KIND_C>
+                                          : (const void *)smo::k_smo<smo::KIND_NU>;
+  return hipLaunchCooperativeKernel(fn, dim3(V->nblocks), dim3(smo::BLOCK), args, (unsigned)shmem, st);
+}
+
+// Largest co-resident grid for the persistent kernel at this LDS size.
+hipError_t smo_persistent_capacity(int kind, size_t shmem, int *blocks_per_cu) {
+  const void *fn = kind == smo::KIND_C ? (const void *)smo::k_smo<smo::KIND_C>
+                                       : (const void *)smo::k_smo<smo::KIND_NU>;
+  return hipOccupancyMaxActiveBlocksPerMultiprocessor(blocks_per_cu, fn, smo::BLOCK, shmem);
+}
+
+hipError_t smo_launch_finalize(const View *V, hipStream_t st) {
+  if (V->kind == smo::KIND_C)
+    hipLaunchKernelGGL(smo::k_finalize<smo::KIND_C>, dim3(V->nblocks), dim3(smo::BLOCK), 0, st, *V);
+  else
+    hipLaunchKernelGGL(smo::k_finalize<smo::KIND_NU>, dim3(V->nblocks), dim3(smo::BLOCK), 0, st, *V);
+  return hipGetLastError();
+}
+
+hipError_t smo_launch_kernel_row(const View *V, int q, float *out, size_t shmem, hipStream_t st) {
+  hipLaunchKernelGGL(smo::k_kernel_row, dim3(V->nblocks), dim3(smo::BLOCK), shmem, st, *V, q, out);
+  return hipGetLastError();
+}
+
+}  // extern "C"

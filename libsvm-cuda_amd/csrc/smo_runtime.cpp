@@ -1,0 +1,612 @@
+// Host runtime of the device SMO solver: the svmhip_* C-ABI (include/svm_hip.h).
+//
+// Owns every device buffer of one solve in a handle (the reference keeps them
+// in a process-global CudaSolver plus __constant__/__device__ symbols,
+// libsvm/cuda_solver.cpp:5, libsvm/svm_device.cu:32-53), converts the
+// svm_node rows to the SELL-64 layout on upload, and drives either the
+// step-wise kernels (mirroring CudaSolver's method-per-phase interface) or
+// the persistent cooperative kernel (svmhip_solve).
+#include <hip/hip_runtime.h>
+#include <math.h>
+#include <stdarg.h>
+#include <stdio.h>
+#include <string.h>
+
+#include <algorithm>
+#include <climits>
+#include <string>
+#include <vector>
+
+#include "../../include/svm_hip.h"
+#include "smo_device.hpp"
+
+extern "C" {
+hipError_t smo_launch_self_dot(const smo::View *V, double *xsq, hipStream_t st);
+hipError_t smo_launch_qd(const smo::View *V, hipStream_t st);
+hipError_t smo_launch_ginit_row(const smo::View *V, int v, size_t shmem, hipStream_t st);
+hipError_t smo_launch_select_i(const smo::View *V, size_t shmem, hipStream_t st);
+hipError_t smo_launch_select_j(const smo::View *V, size_t shmem, hipStream_t st);
+hipError_t smo_launch_alpha(const smo::View *V, hipStream_t st);
+hipError_t smo_launch_gupdate(const smo::View *V, size_t shmem, hipStream_t st);
+hipError_t smo_launch_status(const smo::View *V, hipStream_t st);
+hipError_t smo_launch_persistent(const smo::View *V, int iter_limit, size_t shmem, hipStream_t st);
+hipError_t smo_persistent_capacity(int kind, size_t shmem, int *blocks_per_cu);
+hipError_t smo_launch_finalize(const smo::View *V, hipStream_t st);
+hipError_t smo_launch_kernel_row(const smo::View *V, int q, float *out, size_t shmem, hipStream_t st);
+}
+
+namespace {
+
+constexpr size_t kLdsQueryLimit = 64 * 1024;  // query row staged in LDS up to this size
+
+struct Failure {
+  svmhip_status code;
+};
+
+}  // namespace
+
+struct svmhip_solver {
+  int dev = 0;
+  hipStream_t st = nullptr;
+  hipEvent_t ev[4] = {nullptr, nullptr, nullptr, nullptr};
+  svmhip_kind kind = SVMHIP_KIND_C;
+  svmhip_options opt{};
+  int svm_type = C_SVC;
+  double param_cache_mb = 100;
+  double eps = 1e-3;
+  // problem
+  int l = 0, d = 0, nslices = 0, dense = 0, ktype = RBF, degree = 3;
+  double gamma = 0, coef0 = 0;
+  long long total = 0;
+  // solver
+  int nvar = 0;
+  double Cp = 0, Cn = 0;
+  int S = 0, full = 0;
+  long long row_stride = 0;
+  int nblocks = 1;
+  size_t shmem = 0;
+  int use_lds = 1;
+  bool rbf_ready = false, setup_done = false, have_j = false;
+  smo::Ctl *host_ctl = nullptr;  // pinned mirror
+  // device memory
+  long long *d_slice_ptr = nullptr;
+  float *d_vals = nullptr;
+  int *d_cols = nullptr, *d_row_len = nullptr, *d_serial = nullptr;
+  double *d_xsq = nullptr, *d_G = nullptr, *d_alpha = nullptr, *d_p = nullptr, *d_QD = nullptr;
+  int8_t *d_y = nullptr, *d_status = nullptr, *d_computed = nullptr;
+  float *d_rows = nullptr, *d_xs = nullptr;
+  int *d_slot_of_row = nullptr, *d_row_of_slot = nullptr, *d_prev = nullptr, *d_next = nullptr;
+  smo::Ctl *d_ctl = nullptr;
+  smo::Partial *d_part = nullptr;
+  std::vector<void *> allocs;
+  svmhip_stats stats{};
+  std::string err;
+
+  void fail(svmhip_status code, const char *fmt, ...) {
+    char buf[512];
+    va_list ap;
+    va_start(ap, fmt);
+    vsnprintf(buf, sizeof buf, fmt, ap);
+    va_end(ap);
+    err = buf;
+    throw Failure{code};
+  }
+  void ck(hipError_t e, const char *what) {
+    if (e != hipSuccess) {
+      (void)hipGetLastError();
+      fail(e == hipErrorOutOfMemory ? SVMHIP_ERR_NOMEM : SVMHIP_ERR_DEVICE, "%s: %s", what,
+           hipGetErrorString(e));
+    }
+  }
+  template <class T>
+  T *dalloc(size_t n, const char *what) {
+    void *p = nullptr;
+    size_t bytes = std::max<size_t>(n, 1) * sizeof(T);
+    hipError_t e = hipMalloc(&p, bytes);
+    if (e != hipSuccess) {
+      (void)hipGetLastError();
+      fail(SVMHIP_ERR_NOMEM, "device allocation of %zu bytes for %s failed: %s", bytes, what,
+           hipGetErrorString(e));
+    }
+    allocs.push_back(p);
+    return (T *)p;
+  }
+  void dfree(void *p) {
+    if (!p) return;
+    auto it = std::find(allocs.begin(), allocs.end(), p);
+    if (it != allocs.end()) allocs.erase(it);
+    (void)hipFree(p);
+  }
+  template <class T>
+  void upload(T *dst, const T *src, size_t n, const char *what) {
+    if (n) ck(hipMemcpyAsync(dst, src, n * sizeof(T), hipMemcpyHostToDevice, st), what);
+  }
+
+  smo::View view() const {
+    smo::View V;
+    memset(&V, 0, sizeof V);
+    V.l = l; V.d = d; V.nslices = nslices; V.dense = dense;
+    V.ktype = ktype; V.degree = degree; V.gamma = gamma; V.coef0 = coef0;
+    V.slice_ptr = d_slice_ptr; V.vals = d_vals; V.cols = d_cols; V.row_len = d_row_len;
+    V.xsq = d_xsq; V.serial = d_serial;
+    V.nvar = nvar; V.svr = (nvar == 2 * l && l > 0) ? 1 : 0; V.kind = (int)kind;
+    V.max_iter = host_ctl ? host_ctl->max_iter : 0;
+    V.Cp = Cp; V.Cn = Cn; V.eps = eps;
+    V.G = d_G; V.alpha = d_alpha; V.p = d_p; V.QD = d_QD; V.y = d_y; V.status = d_status;
+    V.rows = d_rows; V.row_stride = row_stride; V.S = S; V.full = full;
+    V.slot_of_row = d_slot_of_row; V.row_of_slot = d_row_of_slot; V.lru_prev = d_prev; V.lru_next = d_next;
+    V.computed = d_computed;
+    V.ctl = d_ctl; V.part = d_part; V.xs_global = d_xs;
+    V.use_lds = use_lds; V.nblocks = nblocks;
+    return V;
+  }
+
+  void read_ctl() {
+    ck(hipMemcpyAsync(host_ctl, d_ctl, sizeof(smo::Ctl), hipMemcpyDeviceToHost, st), "read control record");
+    ck(hipStreamSynchronize(st), "synchronise");
+  }
+
+  ~svmhip_solver() {
+    if (dev >= 0) (void)hipSetDevice(dev);
+    if (st) (void)hipStreamSynchronize(st);
+    for (void *p : allocs) (void)hipFree(p);
+    if (host_ctl) (void)hipHostFree(host_ctl);
+    for (auto &e : ev)
+      if (e) (void)hipEventDestroy(e);
+    if (st) (void)hipStreamDestroy(st);
+  }
+};
+
+#define GUARD_BEGIN try {
+#define GUARD_END                          \
+  }                                        \
+  catch (const Failure &f) {               \
+    return f.code;                         \
+  }                                        \
+  catch (const std::bad_alloc &) {         \
+    h->err = "host allocation failed";     \
+    return SVMHIP_ERR_NOMEM;               \
+  }                                        \
+  return SVMHIP_OK;
+
+// ------------------------------------------------------------- upload
+
+// Rows of prob -> SELL-64 (see smo_device.hpp).  Features are truncated to
+// float32 as the reference does on upload (libsvm/cuda_solver.cpp:366).
+static void build_sell(svmhip_solver *h, const svm_problem *prob) {
+  const int l = prob->l;
+  const bool pre = h->ktype == PRECOMPUTED;
+  std::vector<int> len(l);
+  long long nnz = 0;
+  int maxidx = -1, maxlen = 0;
+  for (int r = 0; r < l; ++r) {
+    const svm_node *x = prob->x[r];
+    if (!x) h->fail(SVMHIP_ERR_ARG, "row %d is null", r);
+    int n = 0, prev = INT_MIN;
+    for (; x[n].index != -1; ++n) {
+      if (!pre) {
+        if (x[n].index < 0) h->fail(SVMHIP_ERR_ARG, "row %d: negative feature index %d", r, x[n].index);
+        if (x[n].index <= prev) h->fail(SVMHIP_ERR_ARG, "row %d: feature indices not ascending", r);
+        prev = x[n].index;
+        maxidx = std::max(maxidx, x[n].index);
+      }
+    }
+    if (pre && n == 0) h->fail(SVMHIP_ERR_ARG, "row %d: precomputed row needs 0:serial", r);
+    len[r] = n;
+    nnz += n;
+    maxlen = std::max(maxlen, n);
+  }
+  h->d = pre ? std::max(maxlen, 1) : std::max(maxidx + 1, 1);
+  h->dense = (!pre && l > 0 && (double)nnz >= 0.5 * (double)l * (double)h->d) ? 1 : 0;
+  h->nslices = (l + smo::SLICE - 1) / smo::SLICE;
+  std::vector<long long> sp(h->nslices + 1, 0);
+  std::vector<int> rl(l);
+  for (int s = 0; s < h->nslices; ++s) {
+    int w = 0;
+    for (int r = s * smo::SLICE; r < std::min(l, (s + 1) * smo::SLICE); ++r) {
+      rl[r] = h->dense ? h->d : len[r];
+      w = std::max(w, rl[r]);
+    }
+    sp[s + 1] = sp[s] + (long long)w * smo::SLICE;
+  }
+  h->total = sp[h->nslices];
+  std::vector<float> vals(std::max<long long>(h->total, 1), 0.0f);
+  std::vector<int> cols;
+  if (!h->dense) cols.assign(std::max<long long>(h->total, 1), 0);
+  std::vector<int> serial;
+  if (pre) serial.resize(l);
+  for (int r = 0; r < l; ++r) {
+    const svm_node *x = prob->x[r];
+    const long long base = sp[r / smo::SLICE] + (r % smo::SLICE);
+    if (pre) serial[r] = (int)x[0].value;
+    for (int p = 0; p < len[r]; ++p) {
+      if (h->dense) {
+        vals[base + (long long)x[p].index * smo::SLICE] = (float)x[p].value;
+      } else {
+        vals[base + (long long)p * smo::SLICE] = (float)x[p].value;
+        cols[base + (long long)p * smo::SLICE] = pre ? p : x[p].index;
+      }
+    }
+  }
+  h->d_slice_ptr = h->dalloc<long long>(sp.size(), "slice pointers");
+  h->d_vals = h->dalloc<float>(vals.size(), "feature values");
+  h->d_row_len = h->dalloc<int>(l, "row lengths");
+  h->upload(h->d_slice_ptr, sp.data(), sp.size(), "upload slice pointers");
+  h->upload(h->d_vals, vals.data(), vals.size(), "upload feature values");
+  h->upload(h->d_row_len, rl.data(), (size_t)l, "upload row lengths");
+  if (!h->dense) {
+    h->d_cols = h->dalloc<int>(cols.size(), "feature columns");
+    h->upload(h->d_cols, cols.data(), cols.size(), "upload feature columns");
+  }
+  if (pre) {
+    h->d_serial = h->dalloc<int>(l, "serials");
+    h->upload(h->d_serial, serial.data(), (size_t)l, "upload serials");
+  }
+  h->d_xsq = h->dalloc<double>(l, "squared norms");
+  // all host vectors die at return: wait for the copies
+  h->ck(hipStreamSynchronize(h->st), "upload");
+}
+
+static int device_cus(int dev) {
+  int cus = 0;
+  if (hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, dev) != hipSuccess) cus = 0;
+  return cus > 0 ? cus : 1;
+}
+
+extern "C" {
+
+int svmhip_device_count(void) {
+  int n = 0;
+  if (hipGetDeviceCount(&n) != hipSuccess) {
+    (void)hipGetLastError();
+    return 0;
+  }
+  return n;
+}
+
+const char *svmhip_last_error(const svmhip_solver *h) { return h ? h->err.c_str() : "null handle"; }
+
+svmhip_status svmhip_create(const svm_problem *prob, const svm_parameter *param, svmhip_kind kind,
+                            const svmhip_options *opt, svmhip_solver **out) {
+  if (!out) return SVMHIP_ERR_ARG;
+  *out = nullptr;
+  if (!prob || !param || prob->l < 0 || (prob->l > 0 && !prob->x)) return SVMHIP_ERR_ARG;
+  if (kind != SVMHIP_KIND_C && kind != SVMHIP_KIND_NU) return SVMHIP_ERR_ARG;
+  svmhip_solver *h = new (std::nothrow) svmhip_solver;
+  if (!h) return SVMHIP_ERR_NOMEM;
+  h->dev = -1;
+  svmhip_status rc = [&]() -> svmhip_status {
+    GUARD_BEGIN
+    if (opt) {
+      size_t n = std::min(opt->struct_size ? opt->struct_size : sizeof(svmhip_options), sizeof(svmhip_options));
+      memcpy(&h->opt, opt, n);
+    }
+    if (h->opt.block_threads != 0 && h->opt.block_threads != 256)
+      h->fail(SVMHIP_ERR_ARG, "block_threads must be 256 (got %d)", h->opt.block_threads);
+    int ndev = svmhip_device_count();
+    if (ndev <= 0) h->fail(SVMHIP_ERR_DEVICE, "no HIP device available");
+    if (h->opt.device < 0 || h->opt.device >= ndev)
+      h->fail(SVMHIP_ERR_ARG, "device %d out of range (%d devices)", h->opt.device, ndev);
+    h->dev = h->opt.device;
+    h->ck(hipSetDevice(h->dev), "hipSetDevice");
+    h->ck(hipStreamCreateWithFlags(&h->st, hipStreamNonBlocking), "create stream");
+    for (auto &e : h->ev) h->ck(hipEventCreate(&e), "create event");
+    h->ck(hipHostMalloc((void **)&h->host_ctl, sizeof(smo::Ctl), hipHostMallocDefault), "pinned control record");
+    memset(h->host_ctl, 0, sizeof(smo::Ctl));
+    h->kind = kind;
+    h->svm_type = param->svm_type;
+    h->ktype = param->kernel_type;
+    h->degree = param->degree;
+    h->gamma = param->gamma;
+    h->coef0 = param->coef0;
+    h->eps = param->eps;
+    h->param_cache_mb = param->cache_size;
+    if (h->ktype < LINEAR || h->ktype > PRECOMPUTED) h->fail(SVMHIP_ERR_ARG, "unknown kernel type %d", h->ktype);
+    if (!(h->eps > 0)) h->fail(SVMHIP_ERR_ARG, "eps <= 0");
+    h->l = prob->l;
+    build_sell(h, prob);
+    // launch geometry: at most one workgroup per slice, one per CU by default
+    int cus = device_cus(h->dev);
+    int want = h->opt.grid_blocks > 0 ? h->opt.grid_blocks : cus;
+    h->nblocks = std::max(1, std::min(want, std::max(h->nslices, 1)));
+    size_t qbytes = (size_t)h->d * sizeof(float);
+    h->use_lds = qbytes <= kLdsQueryLimit;
+    h->shmem = h->use_lds ? qbytes : 0;
+    int per_cu = 0;
+    h->ck(smo_persistent_capacity((int)kind, h->shmem, &per_cu), "occupancy query");
+    int cap = std::max(1, per_cu) * cus;
+    h->nblocks = std::min(h->nblocks, cap);
+    if (!h->use_lds) h->d_xs = h->dalloc<float>((size_t)h->nblocks * h->d, "query-row scratch");
+    h->d_ctl = h->dalloc<smo::Ctl>(1, "control record");
+    h->d_part = h->dalloc<smo::Partial>(2 * (size_t)h->nblocks, "partials");
+    h->ck(hipMemsetAsync(h->d_ctl, 0, sizeof(smo::Ctl), h->st), "clear control record");
+    h->ck(hipStreamSynchronize(h->st), "create");
+    GUARD_END
+  }();
+  if (rc != SVMHIP_OK) {
+    if (h->err.empty()) h->err = "create failed";
+    fprintf(stderr, "svmhip_create: %s\n", h->err.c_str());
+    delete h;
+    return rc;
+  }
+  *out = h;
+  return SVMHIP_OK;
+}
+
+svmhip_status svmhip_setup_rbf(svmhip_solver *h, int l) {
+  if (!h) return SVMHIP_ERR_ARG;
+  GUARD_BEGIN
+  if (l != h->l) h->fail(SVMHIP_ERR_ARG, "setup_rbf: l=%d but the problem has %d rows", l, h->l);
+  h->ck(hipSetDevice(h->dev), "hipSetDevice");
+  smo::View V = h->view();
+  if (h->l > 0) h->ck(smo_launch_self_dot(&V, h->d_xsq, h->st), "x^2 kernel");
+  h->rbf_ready = true;
+  GUARD_END
+}
+
+svmhip_status svmhip_setup(svmhip_solver *h, const signed char *y, const double *G, const double *alpha,
+                           const char *alpha_status, double Cp, double Cn, int n_var) {
+  if (!h) return SVMHIP_ERR_ARG;
+  GUARD_BEGIN
+  if (!y || !G || !alpha || !alpha_status) h->fail(SVMHIP_ERR_ARG, "setup: null array");
+  if (n_var != h->l && n_var != 2 * h->l) h->fail(SVMHIP_ERR_ARG, "setup: n_var=%d must be l or 2l (l=%d)", n_var, h->l);
+  if (n_var == 2 * h->l && h->l > 0 && h->svm_type != EPSILON_SVR && h->svm_type != NU_SVR)
+    h->fail(SVMHIP_ERR_ARG, "setup: n_var=2l is only valid for regression");
+  if (h->setup_done) h->fail(SVMHIP_ERR_STATE, "setup called twice on one handle");
+  if (!(Cp > 0) || !(Cn > 0)) h->fail(SVMHIP_ERR_ARG, "setup: Cp, Cn must be > 0");
+  h->ck(hipSetDevice(h->dev), "hipSetDevice");
+  if (!h->rbf_ready) {
+    svmhip_status s = svmhip_setup_rbf(h, h->l);
+    if (s != SVMHIP_OK) throw Failure{s};
+  }
+  h->nvar = n_var;
+  h->Cp = Cp;
+  h->Cn = Cn;
+  for (int v = 0; v < n_var; ++v)
+    if (y[v] != 1 && y[v] != -1) h->fail(SVMHIP_ERR_ARG, "setup: y[%d]=%d is not +1/-1", v, (int)y[v]);
+  const int nv = n_var;
+  h->d_G = h->dalloc<double>(nv, "G");
+  h->d_alpha = h->dalloc<double>(nv, "alpha");
+  h->d_p = h->dalloc<double>(nv, "p");
+  h->d_QD = h->dalloc<double>(nv, "QD");
+  h->d_y = h->dalloc<int8_t>(nv, "y");
+  h->d_status = h->dalloc<int8_t>(nv, "status");
+  h->upload(h->d_G, G, nv, "upload G");
+  h->upload(h->d_p, G, nv, "upload p");  // Solver::Solve passes G = p here
+  h->upload(h->d_alpha, alpha, nv, "upload alpha");
+  h->upload(h->d_y, (const int8_t *)y, nv, "upload y");
+  h->upload(h->d_status, (const int8_t *)alpha_status, nv, "upload status");
+
+  // kernel-row cache: S rows of l floats (64-bit sized; the reference's
+  // int byte count overflows at 2 GB, libsvm/cuda_solver.cpp:278)
+  double bytes = h->opt.cache_bytes > 0 ? h->opt.cache_bytes : h->param_cache_mb * 1048576.0;
+  h->row_stride = ((long long)h->l + 63) / 64 * 64;
+  long long rowb = std::max<long long>(h->row_stride, 1) * (long long)sizeof(float);
+  long long want = (long long)std::floor(bytes / (double)rowb);
+  size_t freeb = 0, totb = 0;
+  if (hipMemGetInfo(&freeb, &totb) == hipSuccess) {
+    long long cap = (long long)((double)freeb * 0.85) / rowb;
+    want = std::min(want, cap);
+  }
+  want = std::max<long long>(want, 2);
+  if (want >= h->l) {
+    h->full = 1;
+    h->S = std::max(h->l, 1);
+  } else {
+    h->full = 0;
+    h->S = (int)want;
+  }
+  h->d_rows = h->dalloc<float>((size_t)h->S * (size_t)h->row_stride, "kernel-row cache");
+  std::vector<int> init;
+  if (h->full) {
+    h->d_computed = h->dalloc<int8_t>(std::max(h->l, 1), "cache flags");
+    h->ck(hipMemsetAsync(h->d_computed, 0, std::max(h->l, 1), h->st), "clear cache flags");
+  } else {
+    h->d_slot_of_row = h->dalloc<int>(h->l, "cache map");
+    h->d_row_of_slot = h->dalloc<int>(h->S, "cache map");
+    h->d_prev = h->dalloc<int>(h->S, "LRU links");
+    h->d_next = h->dalloc<int>(h->S, "LRU links");
+    init.assign(std::max(h->l, h->S), -1);
+    h->upload(h->d_slot_of_row, init.data(), h->l, "init cache map");
+    h->upload(h->d_row_of_slot, init.data(), h->S, "init cache map");
+    std::vector<int> prv(h->S), nxt(h->S);
+    for (int s = 0; s < h->S; ++s) {
+      prv[s] = s - 1;
+      nxt[s] = s + 1 < h->S ? s + 1 : -1;
+    }
+    h->upload(h->d_prev, prv.data(), h->S, "init LRU");
+    h->upload(h->d_next, nxt.data(), h->S, "init LRU");
+    h->ck(hipStreamSynchronize(h->st), "init LRU");
+  }
+  memset(h->host_ctl, 0, sizeof(smo::Ctl));
+  h->host_ctl->lru_head = 0;
+  h->host_ctl->lru_tail = h->S - 1;
+  int mi = h->opt.max_iter;
+  if (mi <= 0) mi = std::max(10000000, h->l > INT_MAX / 100 ? INT_MAX : 100 * h->l);  // svm.cpp:577
+  h->host_ctl->max_iter = mi;
+  h->ck(hipMemcpyAsync(h->d_ctl, h->host_ctl, sizeof(smo::Ctl), hipMemcpyHostToDevice, h->st), "init control");
+
+  smo::View V = h->view();
+  h->ck(hipEventRecord(h->ev[2], h->st), "event");
+  if (nv > 0) h->ck(smo_launch_qd(&V, h->st), "QD kernel");
+  // G = p + sum_{alpha_v > 0} alpha_v Q_v, in increasing v (svm.cpp:555-569)
+  for (int v = 0; v < nv; ++v)
+    if (alpha_status[v] != smo::AT_LOWER) h->ck(smo_launch_ginit_row(&V, v, h->shmem, h->st), "G init kernel");
+  h->ck(hipEventRecord(h->ev[3], h->st), "event");
+  h->ck(hipStreamSynchronize(h->st), "setup");
+  float ms = 0;
+  (void)hipEventElapsedTime(&ms, h->ev[2], h->ev[3]);
+  h->stats.setup_ms = ms;
+  h->stats.cache_slots = h->S;
+  h->stats.grid_blocks = h->nblocks;
+  h->setup_done = true;
+  h->have_j = false;
+  if (h->opt.verbose)
+    fprintf(stderr, "svmhip: l=%d d=%d %s, n_var=%d, cache %d rows%s, grid %d x 256\n", h->l, h->d,
+            h->dense ? "dense" : "sparse", nv, h->S, h->full ? " (all)" : "", h->nblocks);
+  GUARD_END
+}
+
+svmhip_status svmhip_select_working_set(svmhip_solver *h, int n_var, int *optimal, int *out_i, int *out_j) {
+  if (!h) return SVMHIP_ERR_ARG;
+  GUARD_BEGIN
+  if (!h->setup_done) h->fail(SVMHIP_ERR_STATE, "select_working_set before setup");
+  if (n_var != h->nvar) h->fail(SVMHIP_ERR_ARG, "n_var mismatch");
+  h->ck(hipSetDevice(h->dev), "hipSetDevice");
+  smo::View V = h->view();
+  h->ck(smo_launch_select_i(&V, h->shmem, h->st), "select i");
+  h->read_ctl();  // the Gmax / Gmax2 copy the reference makes (cuda_solver.cpp:83-86)
+  if (h->host_ctl->err) h->fail(SVMHIP_ERR_TIMEOUT, "device wait timed out");
+  if (h->host_ctl->stop) {
+    if (optimal) *optimal = 1;
+    h->have_j = false;
+    return SVMHIP_OK;
+  }
+  h->ck(smo_launch_select_j(&V, h->shmem, h->st), "select j");
+  if (optimal) *optimal = 0;
+  if (out_i || out_j) {
+    h->read_ctl();
+    if (h->host_ctl->stop) {
+      if (optimal) *optimal = 1;
+      return SVMHIP_OK;
+    }
+    if (out_i) *out_i = h->host_ctl->i;
+    if (out_j) *out_j = h->host_ctl->j;
+  }
+  h->have_j = true;
+  GUARD_END
+}
+
+svmhip_status svmhip_compute_alpha(svmhip_solver *h) {
+  if (!h) return SVMHIP_ERR_ARG;
+  GUARD_BEGIN
+  if (!h->have_j) h->fail(SVMHIP_ERR_STATE, "compute_alpha without a selected pair");
+  smo::View V = h->view();
+  h->ck(smo_launch_alpha(&V, h->st), "alpha kernel");
+  GUARD_END
+}
+
+svmhip_status svmhip_update_gradient(svmhip_solver *h, int n_var) {
+  if (!h) return SVMHIP_ERR_ARG;
+  GUARD_BEGIN
+  if (!h->have_j) h->fail(SVMHIP_ERR_STATE, "update_gradient without a selected pair");
+  if (n_var != h->nvar) h->fail(SVMHIP_ERR_ARG, "n_var mismatch");
+  smo::View V = h->view();
+  h->ck(smo_launch_gupdate(&V, h->shmem, h->st), "gradient kernel");
+  GUARD_END
+}
+
+svmhip_status svmhip_update_alpha_status(svmhip_solver *h) {
+  if (!h) return SVMHIP_ERR_ARG;
+  GUARD_BEGIN
+  if (!h->have_j) h->fail(SVMHIP_ERR_STATE, "update_alpha_status without a selected pair");
+  smo::View V = h->view();
+  h->ck(smo_launch_status(&V, h->st), "status kernel");
+  h->have_j = false;
+  GUARD_END
+}
+
+svmhip_status svmhip_fetch(svmhip_solver *h, double *G, double *alpha, char *alpha_status, int n_var) {
+  if (!h) return SVMHIP_ERR_ARG;
+  GUARD_BEGIN
+  if (!h->setup_done) h->fail(SVMHIP_ERR_STATE, "fetch before setup");
+  if (n_var != h->nvar) h->fail(SVMHIP_ERR_ARG, "n_var mismatch");
+  h->ck(hipSetDevice(h->dev), "hipSetDevice");
+  if (G && n_var) h->ck(hipMemcpyAsync(G, h->d_G, sizeof(double) * n_var, hipMemcpyDeviceToHost, h->st), "fetch G");
+  if (alpha && n_var)
+    h->ck(hipMemcpyAsync(alpha, h->d_alpha, sizeof(double) * n_var, hipMemcpyDeviceToHost, h->st), "fetch alpha");
+  if (alpha_status && n_var)
+    h->ck(hipMemcpyAsync(alpha_status, h->d_status, n_var, hipMemcpyDeviceToHost, h->st), "fetch status");
+  h->ck(hipStreamSynchronize(h->st), "fetch");
+  if (h->opt.verbose) {
+    h->read_ctl();
+    fprintf(stderr, "svmhip: cache hits %llu, rows computed %llu\n", h->host_ctl->hits, h->host_ctl->misses);
+  }
+  GUARD_END
+}
+
+svmhip_status svmhip_finalize(svmhip_solver *h, svmhip_stats *stats) {
+  if (!h) return SVMHIP_ERR_ARG;
+  GUARD_BEGIN
+  if (!h->setup_done) h->fail(SVMHIP_ERR_STATE, "finalize before setup");
+  h->ck(hipSetDevice(h->dev), "hipSetDevice");
+  smo::View V = h->view();
+  if (h->nvar > 0) h->ck(smo_launch_finalize(&V, h->st), "finalize kernel");
+  h->read_ctl();
+  h->stats.obj = h->host_ctl->obj;
+  h->stats.rho = h->host_ctl->rho;
+  h->stats.r = h->host_ctl->r;
+  h->stats.iters = h->host_ctl->iter;
+  h->stats.row_hits = (long long)h->host_ctl->hits;
+  h->stats.row_misses = (long long)h->host_ctl->misses;
+  if (stats) *stats = h->stats;
+  GUARD_END
+}
+
+svmhip_status svmhip_solve(svmhip_solver *h, int max_iter, svmhip_stats *stats) {
+  if (!h) return SVMHIP_ERR_ARG;
+  svmhip_status rc = [&]() -> svmhip_status {
+    GUARD_BEGIN
+    if (!h->setup_done) h->fail(SVMHIP_ERR_STATE, "solve before setup");
+    h->ck(hipSetDevice(h->dev), "hipSetDevice");
+    if (max_iter > 0) {
+      h->host_ctl->max_iter = max_iter;
+      h->ck(hipMemcpyAsync(&h->d_ctl->max_iter, &h->host_ctl->max_iter, sizeof(int), hipMemcpyHostToDevice, h->st),
+            "set max_iter");
+    }
+    h->read_ctl();
+    smo::View V = h->view();
+    V.max_iter = h->host_ctl->max_iter;
+    int chunk = h->opt.iters_per_launch > 0 ? h->opt.iters_per_launch : INT_MAX;
+    h->ck(hipEventRecord(h->ev[0], h->st), "event");
+    if (h->nvar > 0) {
+      h->ck(smo_launch_select_i(&V, h->shmem, h->st), "select i");
+      for (;;) {
+        h->ck(smo_launch_persistent(&V, chunk, h->shmem, h->st), "persistent SMO kernel");
+        h->read_ctl();
+        if (h->host_ctl->err) h->fail(SVMHIP_ERR_TIMEOUT, "device wait timed out at iteration %d", h->host_ctl->iter);
+        if (h->host_ctl->stop) break;
+      }
+    }
+    h->ck(hipEventRecord(h->ev[1], h->st), "event");
+    h->ck(hipStreamSynchronize(h->st), "solve");
+    float ms = 0;
+    (void)hipEventElapsedTime(&ms, h->ev[0], h->ev[1]);
+    h->stats.solve_ms = ms;
+    h->have_j = false;
+    svmhip_status s = svmhip_finalize(h, nullptr);
+    if (s != SVMHIP_OK) throw Failure{s};
+    h->stats.optimal = h->host_ctl->stop == smo::STOP_OPTIMAL || h->nvar == 0;
+    GUARD_END
+  }();
+  if (stats) *stats = h->stats;
+  return rc;
+}
+
+svmhip_status svmhip_get_stats(svmhip_solver *h, svmhip_stats *stats) {
+  if (!h || !stats) return SVMHIP_ERR_ARG;
+  *stats = h->stats;
+  return SVMHIP_OK;
+}
+
+svmhip_status svmhip_kernel_row(svmhip_solver *h, int i, float *out) {
+  if (!h) return SVMHIP_ERR_ARG;
+  GUARD_BEGIN
+  if (i < 0 || i >= h->l || !out) h->fail(SVMHIP_ERR_ARG, "kernel_row: bad row %d", i);
+  h->ck(hipSetDevice(h->dev), "hipSetDevice");
+  if (!h->rbf_ready) {
+    svmhip_status s = svmhip_setup_rbf(h, h->l);
+    if (s != SVMHIP_OK) throw Failure{s};
+  }
+  float *buf = h->dalloc<float>(h->l, "row buffer");
+  smo::View V = h->view();
+  h->ck(smo_launch_kernel_row(&V, i, buf, h->shmem, h->st), "kernel row");
+  h->ck(hipMemcpyAsync(out, buf, sizeof(float) * h->l, hipMemcpyDeviceToHost, h->st), "copy row");
+  h->ck(hipStreamSynchronize(h->st), "kernel row");
+  h->dfree(buf);
+  GUARD_END
+}
+
+void svmhip_destroy(svmhip_solver *h) { delete h; }
+
+}  // extern "C"

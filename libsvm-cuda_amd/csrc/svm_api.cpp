@@ -1,0 +1,793 @@
+// The kept LIBSVM C API (include/svm.h) on top of the device solver.
+//
+// Training (svm_train, svm_cross_validation) follows the reference's
+// orchestration (libsvm/svm.cpp:1497-1769 formulations and svm_train_one,
+// :2079-2404 one-vs-one, :2407-2525 cross-validation) and hands every binary,
+// one-class or regression subproblem to the device solver through the
+// svmhip_* C-ABI, which runs the SMO loop on the GPU (the role CudaSolver /
+// CudaSolverNU play in the reference, svm.cpp:1712-1718).  There is no CPU
+// solver in this library: cuda_flag must be 1, and a missing or failing
+// device is reported, never replaced.
+//
+// Prediction, model I/O, parameter checks and accessors are host code, as in
+// the reference (svm.cpp:2527-3232).
+#include <ctype.h>
+#include <float.h>
+#include <limits.h>
+#include <locale.h>
+#include <math.h>
+#include <stdarg.h>
+#include <stdio.h>
+#include <stdlib.h>
+#include <string.h>
+
+#include <algorithm>
+#include <string>
+#include <vector>
+
+#include "../../include/svm.h"
+#include "../../include/svm_hip.h"
+
+int libsvm_version = LIBSVM_VERSION;
+
+namespace {
+
+typedef signed char schar;
+
+void print_stdout(const char *s) {
+  fputs(s, stdout);
+  fflush(stdout);
+}
+void (*g_print)(const char *) = &print_stdout;
+
+void info(const char *fmt, ...) {
+  char buf[4096];
+  va_list ap;
+  va_start(ap, fmt);
+  vsnprintf(buf, sizeof buf, fmt, ap);
+  va_end(ap);
+  g_print(buf);
+}
+
+template <class T>
+T *xmalloc(size_t n) {
+  return (T *)malloc(std::max<size_t>(n, 1) * sizeof(T));
+}
+
+// A device-solver failure; svm_train / svm_cross_validation report it and
+// return without a model (the reference calls exit(1), cuda_solver.cpp:439-446).
+struct DeviceFailure {
+  svmhip_status code;
+};
+
+// Solution of one subproblem (Solver::SolutionInfo + alpha).
+struct Solution {
+  std::vector<double> alpha;  // signed coefficients, length l
+  double rho = 0, obj = 0, r = 0, ub_p = 0, ub_n = 0;
+  int iters = 0;
+};
+
+// Runs the device solver on one formulation: the part of Solver::Solve
+// (libsvm/svm.cpp:515-830) that the reference hands to CudaSolver.
+void device_solve(const svm_problem *prob, const svm_parameter *param, svmhip_kind kind, int n_var,
+                  const std::vector<schar> &y, const std::vector<double> &p, std::vector<double> &alpha, double Cp,
+                  double Cn, Solution &out) {
+  svmhip_options opt;
+  memset(&opt, 0, sizeof opt);
+  opt.struct_size = sizeof opt;
+  const char *e = getenv("SVMHIP_VERBOSE");
+  opt.verbose = e && *e == '1';
+  svmhip_solver *h = nullptr;
+  svmhip_status s = svmhip_create(prob, param, kind, &opt, &h);
+  if (s != SVMHIP_OK) {
+    fprintf(stderr, "ERROR: device solver unavailable (status %d)\n", (int)s);
+    throw DeviceFailure{s};
+  }
+  std::vector<char> status(n_var);
+  for (int v = 0; v < n_var; ++v) {
+    double C = y[v] > 0 ? Cp : Cn;
+    status[v] = alpha[v] >= C ? 1 : (alpha[v] <= 0 ? 0 : 2);  // Solver::update_alpha_status
+  }
+  svmhip_stats st;
+  memset(&st, 0, sizeof st);
+  if ((s = svmhip_setup_rbf(h, prob->l)) != SVMHIP_OK ||
+      (s = svmhip_setup(h, y.data(), p.data(), alpha.data(), status.data(), Cp, Cn, n_var)) != SVMHIP_OK ||
+      (s = svmhip_solve(h, 0, &st)) != SVMHIP_OK ||
+      (s = svmhip_fetch(h, nullptr, alpha.data(), nullptr, n_var)) != SVMHIP_OK) {
+    fprintf(stderr, "ERROR: device solver failed (status %d): %s\n", (int)s, svmhip_last_error(h));
+    svmhip_destroy(h);
+    throw DeviceFailure{s};
+  }
+  svmhip_destroy(h);
+  if (!st.optimal) fprintf(stderr, "\nWARNING: reaching max number of iterations\n");
+  out.rho = st.rho;
+  out.obj = st.obj;
+  out.r = st.r;
+  out.iters = st.iters;
+  out.ub_p = Cp;
+  out.ub_n = Cn;
+  info("\noptimization finished, #iter = %d\n", st.iters);
+}
+
+// Formulations, libsvm/svm.cpp:1500-1696.
+void train_one(const svm_problem *prob, const svm_parameter *param, double Cp, double Cn, Solution &out) {
+  const int l = prob->l;
+  out.alpha.assign(l, 0.0);
+  switch (param->svm_type) {
+    case C_SVC: {
+      std::vector<schar> y(l);
+      std::vector<double> a(l, 0.0), p(l, -1.0);
+      for (int i = 0; i < l; ++i) y[i] = prob->y[i] > 0 ? +1 : -1;
+      device_solve(prob, param, SVMHIP_KIND_C, l, y, p, a, Cp, Cn, out);
+      double sum = 0;
+      for (int i = 0; i < l; ++i) sum += a[i];
+      if (Cp == Cn) info("nu = %f\n", sum / (Cp * l));
+      for (int i = 0; i < l; ++i) out.alpha[i] = a[i] * y[i];
+      break;
+    }
+    case NU_SVC: {
+      std::vector<schar> y(l);
+      for (int i = 0; i < l; ++i) y[i] = prob->y[i] > 0 ? +1 : -1;
+      double pos = param->nu * l / 2, neg = param->nu * l / 2;
+      std::vector<double> a(l), zeros(l, 0.0);
+      for (int i = 0; i < l; ++i) {
+        if (y[i] == +1) { a[i] = std::min(1.0, pos); pos -= a[i]; }
+        else { a[i] = std::min(1.0, neg); neg -= a[i]; }
+      }
+      device_solve(prob, param, SVMHIP_KIND_NU, l, y, zeros, a, 1.0, 1.0, out);
+      double r = out.r;
+      info("C = %f\n", 1 / r);
+      for (int i = 0; i < l; ++i) out.alpha[i] = a[i] * y[i] / r;
+      out.rho /= r;
+      out.obj /= (r * r);
+      out.ub_p = out.ub_n = 1 / r;
+      break;
+    }
+    case ONE_CLASS: {
+      int n = (int)(param->nu * l);
+      std::vector<double> a(l, 0.0), zeros(l, 0.0);
+      std::vector<schar> ones(l, 1);
+      for (int i = 0; i < n; ++i) a[i] = 1;
+      if (n < l) a[n] = param->nu * l - n;
+      device_solve(prob, param, SVMHIP_KIND_C, l, ones, zeros, a, 1.0, 1.0, out);
+      out.alpha = a;
+      break;
+    }
+    case EPSILON_SVR: {
+      std::vector<double> a2(2 * l, 0.0), lin(2 * l);
+      std::vector<schar> y(2 * l);
+      for (int i = 0; i < l; ++i) {
+        lin[i] = param->p - prob->y[i];
+        y[i] = 1;
+        lin[i + l] = param->p + prob->y[i];
+        y[i + l] = -1;
+      }
+      device_solve(prob, param, SVMHIP_KIND_C, 2 * l, y, lin, a2, param->C, param->C, out);
+      double sum = 0;
+      for (int i = 0; i < l; ++i) {
+        out.alpha[i] = a2[i] - a2[i + l];
+        sum += fabs(out.alpha[i]);
+      }
+      info("nu = %f\n", sum / (param->C * l));
+      break;
+    }
+    case NU_SVR: {
+      double C = param->C;
+      std::vector<double> a2(2 * l), lin(2 * l);
+      std::vector<schar> y(2 * l);
+      double sum = C * param->nu * l / 2;
+      for (int i = 0; i < l; ++i) {
+        a2[i] = a2[i + l] = std::min(sum, C);
+        sum -= a2[i];
+        lin[i] = -prob->y[i];
+        y[i] = 1;
+        lin[i + l] = prob->y[i];
+        y[i + l] = -1;
+      }
+      device_solve(prob, param, SVMHIP_KIND_NU, 2 * l, y, lin, a2, C, C, out);
+      info("epsilon = %f\n", -out.r);
+      for (int i = 0; i < l; ++i) out.alpha[i] = a2[i] - a2[i + l];
+      break;
+    }
+  }
+  info("obj = %f, rho = %f\n", out.obj, out.rho);
+  int nSV = 0, nBSV = 0;
+  for (int i = 0; i < l; ++i) {
+    if (fabs(out.alpha[i]) > 0) {
+      ++nSV;
+      double ub = prob->y[i] > 0 ? out.ub_p : out.ub_n;
+      if (fabs(out.alpha[i]) >= ub) ++nBSV;
+    }
+  }
+  info("nSV = %d, nBSV = %d\n", nSV, nBSV);
+}
+
+// svm_group_classes, libsvm/svm.cpp:2081-2157: labels in order of first
+// appearance, except that a -1/+1 problem puts +1 first.
+void group_classes(const svm_problem *prob, std::vector<int> &label, std::vector<int> &start,
+                   std::vector<int> &count, std::vector<int> &perm) {
+  const int l = prob->l;
+  label.clear();
+  count.clear();
+  std::vector<int> cls(l);
+  for (int i = 0; i < l; ++i) {
+    int lab = (int)prob->y[i];
+    size_t c = 0;
+    while (c < label.size() && label[c] != lab) ++c;
+    if (c == label.size()) {
+      label.push_back(lab);
+      count.push_back(0);
+    }
+    ++count[c];
+    cls[i] = (int)c;
+  }
+  const int k = (int)label.size();
+  if (k == 2 && label[0] == -1 && label[1] == 1) {
+    std::swap(label[0], label[1]);
+    std::swap(count[0], count[1]);
+    for (int i = 0; i < l; ++i) cls[i] = 1 - cls[i];
+  }
+  start.assign(k, 0);
+  for (int c = 1; c < k; ++c) start[c] = start[c - 1] + count[c - 1];
+  perm.assign(l, 0);
+  std::vector<int> fill(start);
+  for (int i = 0; i < l; ++i) perm[fill[cls[i]]++] = i;
+}
+
+// Kernel::k_function, libsvm/svm.cpp:327-384 (prediction form).
+double dot(const svm_node *a, const svm_node *b) {
+  double s = 0;
+  while (a->index != -1 && b->index != -1) {
+    if (a->index == b->index) {
+      s += a->value * b->value;
+      ++a;
+      ++b;
+    } else if (a->index > b->index) {
+      ++b;
+    } else {
+      ++a;
+    }
+  }
+  return s;
+}
+
+double powi(double base, int times) {
+  double acc = 1.0, sq = base;
+  for (int t = times; t > 0; t /= 2) {
+    if (t % 2 == 1) acc *= sq;
+    sq = sq * sq;
+  }
+  return acc;
+}
+
+double k_function(const svm_node *x, const svm_node *y, const svm_parameter &p) {
+  switch (p.kernel_type) {
+    case LINEAR: return dot(x, y);
+    case POLY: return powi(p.gamma * dot(x, y) + p.coef0, p.degree);
+    case RBF: {
+      double s = 0;
+      while (x->index != -1 && y->index != -1) {
+        if (x->index == y->index) {
+          double d = x->value - y->value;
+          s += d * d;
+          ++x;
+          ++y;
+        } else if (x->index > y->index) {
+          s += y->value * y->value;
+          ++y;
+        } else {
+          s += x->value * x->value;
+          ++x;
+        }
+      }
+      for (; x->index != -1; ++x) s += x->value * x->value;
+      for (; y->index != -1; ++y) s += y->value * y->value;
+      return exp(-p.gamma * s);
+    }
+    case SIGMOID: return tanh(p.gamma * dot(x, y) + p.coef0);
+    case PRECOMPUTED: return x[(int)(y->value)].value;
+    default: return 0;
+  }
+}
+
+bool is_regression_like(int t) { return t == ONE_CLASS || t == EPSILON_SVR || t == NU_SVR; }
+
+const char *svm_type_table[] = {"c_svc", "nu_svc", "one_class", "epsilon_svr", "nu_svr", NULL};
+const char *kernel_type_table[] = {"linear", "polynomial", "rbf", "sigmoid", "precomputed", NULL};
+
+}  // namespace
+
+extern "C" {
+
+static svm_model *train_model(const svm_problem *prob, const svm_parameter *param);
+
+// Returns NULL (after printing the reason) if the device solver fails.
+svm_model *svm_train(const svm_problem *prob, const svm_parameter *param) {
+  try {
+    return train_model(prob, param);
+  } catch (const DeviceFailure &) {
+    return NULL;
+  }
+}
+
+static svm_model *train_model(const svm_problem *prob, const svm_parameter *param) {
+  svm_model *model = (svm_model *)calloc(1, sizeof(svm_model));
+  model->param = *param;
+  model->free_sv = 0;
+  if (is_regression_like(param->svm_type)) {
+    // libsvm/svm.cpp:2166-2206 (probability output for SVR is not built)
+    model->nr_class = 2;
+    model->sv_coef = xmalloc<double *>(1);
+    Solution f;
+    train_one(prob, param, 0, 0, f);
+    model->rho = xmalloc<double>(1);
+    model->rho[0] = f.rho;
+    int nSV = 0;
+    for (int i = 0; i < prob->l; ++i)
+      if (fabs(f.alpha[i]) > 0) ++nSV;
+    model->l = nSV;
+    model->SV = xmalloc<svm_node *>(nSV);
+    model->sv_coef[0] = xmalloc<double>(nSV);
+    model->sv_indices = xmalloc<int>(nSV);
+    int j = 0;
+    for (int i = 0; i < prob->l; ++i) {
+      if (fabs(f.alpha[i]) > 0) {
+        model->SV[j] = prob->x[i];
+        model->sv_coef[0][j] = f.alpha[i];
+        model->sv_indices[j] = i + 1;
+        ++j;
+      }
+    }
+    return model;
+  }
+
+  // classification, one-vs-one (libsvm/svm.cpp:2207-2404)
+  const int l = prob->l;
+  std::vector<int> label, start, count, perm;
+  group_classes(prob, label, start, count, perm);
+  const int k = (int)label.size();
+  if (k == 1) info("WARNING: training data in only one class. See README for details.\n");
+  std::vector<svm_node *> x(l);
+  for (int i = 0; i < l; ++i) x[i] = prob->x[perm[i]];
+  std::vector<double> wC(k, param->C);
+  for (int w = 0; w < param->nr_weight; ++w) {
+    int c = 0;
+    while (c < k && param->weight_label[w] != label[c]) ++c;
+    if (c == k) fprintf(stderr, "WARNING: class label %d specified in weight is not found\n", param->weight_label[w]);
+    else wC[c] *= param->weight[w];
+  }
+  std::vector<char> nonzero(l, 0);
+  const int npair = k * (k - 1) / 2;
+  std::vector<Solution> f(npair);
+  int p = 0;
+  for (int a = 0; a < k; ++a)
+    for (int b = a + 1; b < k; ++b) {
+      const int sa = start[a], sb = start[b], ca = count[a], cb = count[b];
+      std::vector<svm_node *> sx(ca + cb);
+      std::vector<double> sy(ca + cb);
+      for (int t = 0; t < ca; ++t) { sx[t] = x[sa + t]; sy[t] = +1; }
+      for (int t = 0; t < cb; ++t) { sx[ca + t] = x[sb + t]; sy[ca + t] = -1; }
+      svm_problem sub;
+      sub.l = ca + cb;
+      sub.x = sx.data();
+      sub.y = sy.data();
+      train_one(&sub, param, wC[a], wC[b], f[p]);
+      for (int t = 0; t < ca; ++t)
+        if (fabs(f[p].alpha[t]) > 0) nonzero[sa + t] = 1;
+      for (int t = 0; t < cb; ++t)
+        if (fabs(f[p].alpha[ca + t]) > 0) nonzero[sb + t] = 1;
+      ++p;
+    }
+  model->nr_class = k;
+  model->label = xmalloc<int>(k);
+  for (int c = 0; c < k; ++c) model->label[c] = label[c];
+  model->rho = xmalloc<double>(npair);
+  for (int q = 0; q < npair; ++q) model->rho[q] = f[q].rho;
+  model->probA = model->probB = NULL;
+  int total = 0;
+  std::vector<int> nz_count(k);
+  model->nSV = xmalloc<int>(k);
+  for (int c = 0; c < k; ++c) {
+    int n = 0;
+    for (int t = 0; t < count[c]; ++t)
+      if (nonzero[start[c] + t]) { ++n; ++total; }
+    model->nSV[c] = n;
+    nz_count[c] = n;
+  }
+  info("Total nSV = %d\n", total);
+  model->l = total;
+  model->SV = xmalloc<svm_node *>(total);
+  model->sv_indices = xmalloc<int>(total);
+  p = 0;
+  for (int i = 0; i < l; ++i)
+    if (nonzero[i]) {
+      model->SV[p] = x[i];
+      model->sv_indices[p++] = perm[i] + 1;
+    }
+  std::vector<int> nz_start(k, 0);
+  for (int c = 1; c < k; ++c) nz_start[c] = nz_start[c - 1] + nz_count[c - 1];
+  model->sv_coef = xmalloc<double *>(std::max(k - 1, 1));
+  for (int c = 0; c < k - 1; ++c) model->sv_coef[c] = xmalloc<double>(total);
+  p = 0;
+  for (int a = 0; a < k; ++a)
+    for (int b = a + 1; b < k; ++b) {
+      // classifier (a,b): class-a coefficients in sv_coef[b-1], class-b in sv_coef[a]
+      int q = nz_start[a];
+      for (int t = 0; t < count[a]; ++t)
+        if (nonzero[start[a] + t]) model->sv_coef[b - 1][q++] = f[p].alpha[t];
+      q = nz_start[b];
+      for (int t = 0; t < count[b]; ++t)
+        if (nonzero[start[b] + t]) model->sv_coef[a][q++] = f[p].alpha[count[a] + t];
+      ++p;
+    }
+  return model;
+}
+
+// Stratified folds drawn with rand() as the reference does
+// (libsvm/svm.cpp:2407-2525).
+void svm_cross_validation(const svm_problem *prob, const svm_parameter *param, int nr_fold, double *target) {
+  const int l = prob->l;
+  if (nr_fold > l) {
+    nr_fold = l;
+    fprintf(stderr, "WARNING: # folds > # data. Will use # folds = # data instead (i.e., leave-one-out cross validation)\n");
+  }
+  std::vector<int> perm(l), fold_start(nr_fold + 1);
+  if ((param->svm_type == C_SVC || param->svm_type == NU_SVC) && nr_fold < l) {
+    std::vector<int> label, start, count;
+    group_classes(prob, label, start, count, perm);
+    const int k = (int)label.size();
+    std::vector<int> index(perm), fold_count(nr_fold, 0);
+    for (int c = 0; c < k; ++c)
+      for (int i = 0; i < count[c]; ++i) {
+        int j = i + rand() % (count[c] - i);
+        std::swap(index[start[c] + j], index[start[c] + i]);
+      }
+    for (int i = 0; i < nr_fold; ++i)
+      for (int c = 0; c < k; ++c) fold_count[i] += (i + 1) * count[c] / nr_fold - i * count[c] / nr_fold;
+    fold_start[0] = 0;
+    for (int i = 1; i <= nr_fold; ++i) fold_start[i] = fold_start[i - 1] + fold_count[i - 1];
+    for (int c = 0; c < k; ++c)
+      for (int i = 0; i < nr_fold; ++i) {
+        int b = start[c] + i * count[c] / nr_fold, e = start[c] + (i + 1) * count[c] / nr_fold;
+        for (int j = b; j < e; ++j) perm[fold_start[i]++] = index[j];
+      }
+    fold_start[0] = 0;
+    for (int i = 1; i <= nr_fold; ++i) fold_start[i] = fold_start[i - 1] + fold_count[i - 1];
+  } else {
+    for (int i = 0; i < l; ++i) perm[i] = i;
+    for (int i = 0; i < l; ++i) std::swap(perm[i], perm[i + rand() % (l - i)]);
+    for (int i = 0; i <= nr_fold; ++i) fold_start[i] = i * l / nr_fold;
+  }
+  for (int i = 0; i < nr_fold; ++i) {
+    const int b = fold_start[i], e = fold_start[i + 1];
+    svm_problem sub;
+    sub.l = l - (e - b);
+    std::vector<svm_node *> sx(std::max(sub.l, 1));
+    std::vector<double> sy(std::max(sub.l, 1));
+    int k = 0;
+    for (int j = 0; j < b; ++j) { sx[k] = prob->x[perm[j]]; sy[k++] = prob->y[perm[j]]; }
+    for (int j = e; j < l; ++j) { sx[k] = prob->x[perm[j]]; sy[k++] = prob->y[perm[j]]; }
+    sub.x = sx.data();
+    sub.y = sy.data();
+    svm_model *m = svm_train(&sub, param);
+    if (!m) {  // device failure: mark the fold's targets as NaN
+      for (int j = b; j < e; ++j) target[perm[j]] = NAN;
+      continue;
+    }
+    for (int j = b; j < e; ++j) target[perm[j]] = svm_predict(m, prob->x[perm[j]]);
+    svm_free_and_destroy_model(&m);
+  }
+}
+
+int svm_get_svm_type(const svm_model *model) { return model->param.svm_type; }
+int svm_get_nr_class(const svm_model *model) { return model->nr_class; }
+void svm_get_labels(const svm_model *model, int *label) {
+  if (model->label)
+    for (int i = 0; i < model->nr_class; ++i) label[i] = model->label[i];
+}
+void svm_get_sv_indices(const svm_model *model, int *idx) {
+  if (model->sv_indices)
+    for (int i = 0; i < model->l; ++i) idx[i] = model->sv_indices[i];
+}
+int svm_get_nr_sv(const svm_model *model) { return model->l; }
+double svm_get_svr_probability(const svm_model *model) {
+  if ((model->param.svm_type == EPSILON_SVR || model->param.svm_type == NU_SVR) && model->probA)
+    return model->probA[0];
+  fprintf(stderr, "Model doesn't contain information for SVR probability inference\n");
+  return 0;
+}
+
+double svm_predict_values(const svm_model *model, const svm_node *x, double *dec) {
+  if (is_regression_like(model->param.svm_type)) {
+    const double *c = model->sv_coef[0];
+    double s = 0;
+    for (int i = 0; i < model->l; ++i) s += c[i] * k_function(x, model->SV[i], model->param);
+    s -= model->rho[0];
+    *dec = s;
+    if (model->param.svm_type == ONE_CLASS) return s > 0 ? 1 : -1;
+    return s;
+  }
+  const int k = model->nr_class, l = model->l;
+  std::vector<double> kv(std::max(l, 1));
+  for (int i = 0; i < l; ++i) kv[i] = k_function(x, model->SV[i], model->param);
+  std::vector<int> start(k, 0), vote(k, 0);
+  for (int c = 1; c < k; ++c) start[c] = start[c - 1] + model->nSV[c - 1];
+  int p = 0;
+  for (int a = 0; a < k; ++a)
+    for (int b = a + 1; b < k; ++b) {
+      double s = 0;
+      const double *ca = model->sv_coef[b - 1], *cb = model->sv_coef[a];
+      for (int t = 0; t < model->nSV[a]; ++t) s += ca[start[a] + t] * kv[start[a] + t];
+      for (int t = 0; t < model->nSV[b]; ++t) s += cb[start[b] + t] * kv[start[b] + t];
+      s -= model->rho[p];
+      dec[p] = s;
+      if (dec[p] > 0) ++vote[a];
+      else ++vote[b];
+      ++p;
+    }
+  int best = 0;
+  for (int c = 1; c < k; ++c)
+    if (vote[c] > vote[best]) best = c;
+  return model->label[best];
+}
+
+double svm_predict(const svm_model *model, const svm_node *x) {
+  const int k = model->nr_class;
+  std::vector<double> dec(is_regression_like(model->param.svm_type) ? 1 : std::max(1, k * (k - 1) / 2));
+  return svm_predict_values(model, x, dec.data());
+}
+
+double svm_predict_probability(const svm_model *model, const svm_node *x, double *prob_estimates) {
+  (void)prob_estimates;  // probability models are not trained by this library
+  return svm_predict(model, x);
+}
+
+// Model file writer, libsvm/svm.cpp:2709-2801 (same keys and formats).
+int svm_save_model(const char *name, const svm_model *model) {
+  FILE *fp = fopen(name, "w");
+  if (!fp) return -1;
+  char *old = strdup(setlocale(LC_ALL, NULL));
+  setlocale(LC_ALL, "C");
+  const svm_parameter &pr = model->param;
+  fprintf(fp, "svm_type %s\n", svm_type_table[pr.svm_type]);
+  fprintf(fp, "kernel_type %s\n", kernel_type_table[pr.kernel_type]);
+  if (pr.kernel_type == POLY) fprintf(fp, "degree %d\n", pr.degree);
+  if (pr.kernel_type == POLY || pr.kernel_type == RBF || pr.kernel_type == SIGMOID) fprintf(fp, "gamma %g\n", pr.gamma);
+  if (pr.kernel_type == POLY || pr.kernel_type == SIGMOID) fprintf(fp, "coef0 %g\n", pr.coef0);
+  const int k = model->nr_class, l = model->l;
+  fprintf(fp, "nr_class %d\n", k);
+  fprintf(fp, "total_sv %d\n", l);
+  fprintf(fp, "rho");
+  for (int i = 0; i < k * (k - 1) / 2; ++i) fprintf(fp, " %g", model->rho[i]);
+  fprintf(fp, "\n");
+  if (model->label) {
+    fprintf(fp, "label");
+    for (int i = 0; i < k; ++i) fprintf(fp, " %d", model->label[i]);
+    fprintf(fp, "\n");
+  }
+  if (model->probA) {
+    fprintf(fp, "probA");
+    for (int i = 0; i < k * (k - 1) / 2; ++i) fprintf(fp, " %g", model->probA[i]);
+    fprintf(fp, "\n");
+  }
+  if (model->probB) {
+    fprintf(fp, "probB");
+    for (int i = 0; i < k * (k - 1) / 2; ++i) fprintf(fp, " %g", model->probB[i]);
+    fprintf(fp, "\n");
+  }
+  if (model->nSV) {
+    fprintf(fp, "nr_sv");
+    for (int i = 0; i < k; ++i) fprintf(fp, " %d", model->nSV[i]);
+    fprintf(fp, "\n");
+  }
+  fprintf(fp, "SV\n");
+  for (int i = 0; i < l; ++i) {
+    for (int j = 0; j < k - 1; ++j) fprintf(fp, "%.16g ", model->sv_coef[j][i]);
+    const svm_node *p = model->SV[i];
+    if (pr.kernel_type == PRECOMPUTED) fprintf(fp, "0:%d ", (int)(p->value));
+    else
+      for (; p->index != -1; ++p) fprintf(fp, "%d:%.8g ", p->index, p->value);
+    fprintf(fp, "\n");
+  }
+  setlocale(LC_ALL, old);
+  free(old);
+  if (ferror(fp) != 0 || fclose(fp) != 0) return -1;
+  return 0;
+}
+
+// Model file reader, libsvm/svm.cpp:2803-3044.
+static bool read_header(FILE *fp, svm_model *m) {
+  svm_parameter &pr = m->param;
+  char cmd[81];
+  for (;;) {
+    if (fscanf(fp, "%80s", cmd) != 1) return false;
+    if (!strcmp(cmd, "svm_type")) {
+      if (fscanf(fp, "%80s", cmd) != 1) return false;
+      int i = 0;
+      for (; svm_type_table[i]; ++i)
+        if (!strcmp(svm_type_table[i], cmd)) { pr.svm_type = i; break; }
+      if (!svm_type_table[i]) { fprintf(stderr, "unknown svm type.\n"); return false; }
+    } else if (!strcmp(cmd, "kernel_type")) {
+      if (fscanf(fp, "%80s", cmd) != 1) return false;
+      int i = 0;
+      for (; kernel_type_table[i]; ++i)
+        if (!strcmp(kernel_type_table[i], cmd)) { pr.kernel_type = i; break; }
+      if (!kernel_type_table[i]) { fprintf(stderr, "unknown kernel function.\n"); return false; }
+    } else if (!strcmp(cmd, "degree")) {
+      if (fscanf(fp, "%d", &pr.degree) != 1) return false;
+    } else if (!strcmp(cmd, "gamma")) {
+      if (fscanf(fp, "%lf", &pr.gamma) != 1) return false;
+    } else if (!strcmp(cmd, "coef0")) {
+      if (fscanf(fp, "%lf", &pr.coef0) != 1) return false;
+    } else if (!strcmp(cmd, "nr_class")) {
+      if (fscanf(fp, "%d", &m->nr_class) != 1) return false;
+    } else if (!strcmp(cmd, "total_sv")) {
+      if (fscanf(fp, "%d", &m->l) != 1) return false;
+    } else if (!strcmp(cmd, "rho") || !strcmp(cmd, "probA") || !strcmp(cmd, "probB")) {
+      int n = m->nr_class * (m->nr_class - 1) / 2;
+      double *v = xmalloc<double>(n);
+      for (int i = 0; i < n; ++i)
+        if (fscanf(fp, "%lf", &v[i]) != 1) { free(v); return false; }
+      if (!strcmp(cmd, "rho")) m->rho = v;
+      else if (!strcmp(cmd, "probA")) m->probA = v;
+      else m->probB = v;
+    } else if (!strcmp(cmd, "label") || !strcmp(cmd, "nr_sv")) {
+      int n = m->nr_class;
+      int *v = xmalloc<int>(n);
+      for (int i = 0; i < n; ++i)
+        if (fscanf(fp, "%d", &v[i]) != 1) { free(v); return false; }
+      if (!strcmp(cmd, "label")) m->label = v;
+      else m->nSV = v;
+    } else if (!strcmp(cmd, "SV")) {
+      for (;;) {
+        int c = getc(fp);
+        if (c == EOF || c == '\n') break;
+      }
+      break;
+    } else {
+      fprintf(stderr, "unknown text in model file: [%s]\n", cmd);
+      return false;
+    }
+  }
+  return true;
+}
+
+svm_model *svm_load_model(const char *name) {
+  FILE *fp = fopen(name, "rb");
+  if (!fp) return NULL;
+  char *old = strdup(setlocale(LC_ALL, NULL));
+  setlocale(LC_ALL, "C");
+  svm_model *m = (svm_model *)calloc(1, sizeof(svm_model));
+  if (!read_header(fp, m)) {
+    fprintf(stderr, "ERROR: fscanf failed to read model\n");
+    setlocale(LC_ALL, old);
+    free(old);
+    free(m->rho);
+    free(m->label);
+    free(m->nSV);
+    free(m->probA);
+    free(m->probB);
+    free(m);
+    fclose(fp);
+    return NULL;
+  }
+  // read the SV lines whole
+  std::vector<std::string> lines;
+  {
+    std::string cur;
+    int c;
+    while ((c = getc(fp)) != EOF) {
+      if (c == '\n') { lines.push_back(cur); cur.clear(); }
+      else cur.push_back((char)c);
+    }
+    if (!cur.empty()) lines.push_back(cur);
+  }
+  const int mcls = m->nr_class - 1, l = m->l;
+  long elements = 0;
+  for (int i = 0; i < l && i < (int)lines.size(); ++i)
+    for (char ch : lines[i])
+      if (ch == ':') ++elements;
+  elements += l;
+  m->sv_coef = xmalloc<double *>(std::max(mcls, 1));
+  for (int i = 0; i < mcls; ++i) m->sv_coef[i] = xmalloc<double>(l);
+  m->SV = xmalloc<svm_node *>(l);
+  svm_node *xs = l > 0 ? xmalloc<svm_node>(elements) : NULL;
+  long j = 0;
+  for (int i = 0; i < l; ++i) {
+    std::string line = i < (int)lines.size() ? lines[i] : std::string();
+    std::vector<char> buf(line.begin(), line.end());
+    buf.push_back('\0');
+    m->SV[i] = &xs[j];
+    char *save = nullptr, *tok = strtok_r(buf.data(), " \t", &save);
+    for (int c = 0; c < mcls; ++c) {
+      m->sv_coef[c][i] = tok ? strtod(tok, NULL) : 0.0;
+      tok = c + 1 < mcls ? strtok_r(NULL, " \t", &save) : tok;
+    }
+    for (;;) {
+      char *idx = strtok_r(NULL, ":", &save);
+      char *val = strtok_r(NULL, " \t", &save);
+      if (!val) break;
+      xs[j].index = (int)strtol(idx, NULL, 10);
+      xs[j].value = strtod(val, NULL);
+      ++j;
+    }
+    xs[j++].index = -1;
+  }
+  setlocale(LC_ALL, old);
+  free(old);
+  if (ferror(fp) != 0 || fclose(fp) != 0) return NULL;
+  m->free_sv = 1;
+  return m;
+}
+
+void svm_free_model_content(svm_model *m) {
+  if (m->free_sv && m->l > 0 && m->SV != NULL) free((void *)(m->SV[0]));
+  if (m->sv_coef)
+    for (int i = 0; i < m->nr_class - 1; ++i) free(m->sv_coef[i]);
+  free(m->SV); m->SV = NULL;
+  free(m->sv_coef); m->sv_coef = NULL;
+  free(m->rho); m->rho = NULL;
+  free(m->label); m->label = NULL;
+  free(m->probA); m->probA = NULL;
+  free(m->probB); m->probB = NULL;
+  free(m->sv_indices); m->sv_indices = NULL;
+  free(m->nSV); m->nSV = NULL;
+}
+
+void svm_free_and_destroy_model(svm_model **pp) {
+  if (pp && *pp) {
+    svm_free_model_content(*pp);
+    free(*pp);
+    *pp = NULL;
+  }
+}
+
+void svm_destroy_param(svm_parameter *param) {
+  free(param->weight_label);
+  free(param->weight);
+}
+
+// libsvm/svm.cpp:3097-3216, plus the two restrictions of this build.
+const char *svm_check_parameter(const svm_problem *prob, const svm_parameter *param) {
+  int t = param->svm_type;
+  if (t != C_SVC && t != NU_SVC && t != ONE_CLASS && t != EPSILON_SVR && t != NU_SVR) return "unknown svm type";
+  int kt = param->kernel_type;
+  if (kt != LINEAR && kt != POLY && kt != RBF && kt != SIGMOID && kt != PRECOMPUTED) return "unknown kernel type";
+  if (param->gamma < 0) return "gamma < 0";
+  if (param->degree < 0) return "degree of polynomial kernel < 0";
+  if (param->cache_size <= 0) return "cache_size <= 0";
+  if (param->eps <= 0) return "eps <= 0";
+  if ((t == C_SVC || t == EPSILON_SVR || t == NU_SVR) && param->C <= 0) return "C <= 0";
+  if ((t == NU_SVC || t == ONE_CLASS || t == NU_SVR) && (param->nu <= 0 || param->nu > 1)) return "nu <= 0 or nu > 1";
+  if (t == EPSILON_SVR && param->p < 0) return "p < 0";
+  if (param->shrinking != 0 && param->shrinking != 1) return "shrinking != 0 and shrinking != 1";
+  if (param->probability != 0 && param->probability != 1) return "probability != 0 and probability != 1";
+  if (param->probability == 1 && t == ONE_CLASS) return "one-class SVM probability output not supported yet";
+  if (param->cuda_flag != 1) return "this library has only the GPU solver: set cuda_flag = 1";
+  if (param->probability == 1) return "probability estimates are not supported by the GPU solver";
+  if (t == NU_SVC) {
+    std::vector<int> label, count;
+    for (int i = 0; i < prob->l; ++i) {
+      int lab = (int)prob->y[i];
+      size_t c = 0;
+      while (c < label.size() && label[c] != lab) ++c;
+      if (c == label.size()) { label.push_back(lab); count.push_back(0); }
+      ++count[c];
+    }
+    for (size_t a = 0; a < label.size(); ++a)
+      for (size_t b = a + 1; b < label.size(); ++b)
+        if (param->nu * (count[a] + count[b]) / 2 > std::min(count[a], count[b])) return "specified nu is infeasible";
+  }
+  return NULL;
+}
+
+int svm_check_probability_model(const svm_model *model) {
+  return ((model->param.svm_type == C_SVC || model->param.svm_type == NU_SVC) && model->probA && model->probB) ||
+         ((model->param.svm_type == EPSILON_SVR || model->param.svm_type == NU_SVR) && model->probA);
+}
+
+void svm_set_print_string_function(void (*print_func)(const char *)) {
+  g_print = print_func ? print_func : &print_stdout;
+}
+
+}  // extern "C"

@@ -1,0 +1,369 @@
+"""Python binding of libsvm_hip.so, the MI355X SMO solver.
+
+Two layers, both straight ctypes over the C-ABI declared in include/:
+  * the kept LIBSVM API (include/svm.h): svm_train, svm_predict_values,
+    svm_save_model / svm_load_model, svm_cross_validation, ...
+  * the device-solver handle (include/svm_hip.h): DeviceSolver mirrors the
+    reference's CudaSolver methods (libsvm/cuda_solver.h:224-244) and adds
+    the fused on-device solve.
+
+The library must be built (`make -C libsvm-cuda_amd`); importing this module
+without it raises, there is no fallback.
+"""
+from __future__ import annotations
+
+import ctypes as C
+import os
+
+import numpy as np
+
+from . import ffi
+from .ffi import (C_SVC, EPSILON_SVR, LINEAR, NU_SVC, NU_SVR, ONE_CLASS, POLY,  # noqa: F401
+                  PRECOMPUTED, RBF, SIGMOID, Problem, make_param, model_arrays)
+
+_HERE = os.path.dirname(os.path.abspath(__file__))
+LIB_PATH = os.path.join(os.path.dirname(_HERE), "lib", "libsvm_hip.so")
+
+KIND_C, KIND_NU = 0, 1
+STATUS_NAMES = {0: "OK", 1: "ERR_ARG", 2: "ERR_DEVICE", 3: "ERR_NOMEM", 4: "ERR_STATE",
+                5: "ERR_TIMEOUT", 6: "ERR_MAXITER"}
+
+
+class svmhip_options(C.Structure):
+    _fields_ = [("struct_size", C.c_size_t), ("device", C.c_int), ("grid_blocks", C.c_int),
+                ("block_threads", C.c_int), ("max_iter", C.c_int), ("cache_bytes", C.c_double),
+                ("iters_per_launch", C.c_int), ("verbose", C.c_int)]
+
+
+class svmhip_stats(C.Structure):
+    _fields_ = [("iters", C.c_int), ("optimal", C.c_int), ("obj", C.c_double), ("rho", C.c_double),
+                ("r", C.c_double), ("row_hits", C.c_longlong), ("row_misses", C.c_longlong),
+                ("solve_ms", C.c_double), ("setup_ms", C.c_double), ("cache_slots", C.c_int),
+                ("grid_blocks", C.c_int)]
+
+    def as_dict(self):
+        return {k: getattr(self, k) for k, _ in self._fields_}
+
+
+# every extern "C" symbol declared in include/svm.h and include/svm_hip.h
+SVM_H_SYMBOLS = [
+    "svm_train", "svm_cross_validation", "svm_save_model", "svm_load_model", "svm_get_svm_type",
+    "svm_get_nr_class", "svm_get_labels", "svm_get_sv_indices", "svm_get_nr_sv",
+    "svm_get_svr_probability", "svm_predict_values", "svm_predict", "svm_predict_probability",
+    "svm_free_model_content", "svm_free_and_destroy_model", "svm_destroy_param",
+    "svm_check_parameter", "svm_check_probability_model", "svm_set_print_string_function",
+    "libsvm_version"]
+SVM_HIP_SYMBOLS = [
+    "svmhip_create", "svmhip_setup_rbf", "svmhip_setup", "svmhip_select_working_set",
+    "svmhip_compute_alpha", "svmhip_update_gradient", "svmhip_update_alpha_status", "svmhip_fetch",
+    "svmhip_solve", "svmhip_finalize", "svmhip_get_stats", "svmhip_kernel_row", "svmhip_destroy",
+    "svmhip_last_error", "svmhip_device_count"]
+
+_lib = None
+PRINT_FUNC = C.CFUNCTYPE(None, C.c_char_p)
+_quiet_cb = PRINT_FUNC(lambda s: None)
+
+
+def lib():
+    """Load libsvm_hip.so (raises if it has not been built)."""
+    global _lib
+    if _lib is not None:
+        return _lib
+    if not os.path.exists(LIB_PATH):
+        raise RuntimeError(f"libsvm_hip.so not built: {LIB_PATH} (run `make -C libsvm-cuda_amd`)")
+    L = C.CDLL(LIB_PATH)
+    P, D, I = C.POINTER, C.c_double, C.c_int
+    h = C.c_void_p
+    L.svm_train.argtypes = [P(ffi.svm_problem), P(ffi.svm_parameter)]
+    L.svm_train.restype = P(ffi.svm_model)
+    L.svm_cross_validation.argtypes = [P(ffi.svm_problem), P(ffi.svm_parameter), I, P(D)]
+    L.svm_save_model.argtypes = [C.c_char_p, P(ffi.svm_model)]
+    L.svm_load_model.argtypes = [C.c_char_p]
+    L.svm_load_model.restype = P(ffi.svm_model)
+    L.svm_predict_values.argtypes = [P(ffi.svm_model), P(ffi.svm_node), P(D)]
+    L.svm_predict_values.restype = D
+    L.svm_predict.argtypes = [P(ffi.svm_model), P(ffi.svm_node)]
+    L.svm_predict.restype = D
+    L.svm_free_and_destroy_model.argtypes = [P(P(ffi.svm_model))]
+    L.svm_check_parameter.argtypes = [P(ffi.svm_problem), P(ffi.svm_parameter)]
+    L.svm_check_parameter.restype = C.c_char_p
+    L.svm_set_print_string_function.argtypes = [PRINT_FUNC]
+    L.svm_get_nr_class.argtypes = [P(ffi.svm_model)]
+    L.svm_get_nr_sv.argtypes = [P(ffi.svm_model)]
+    L.svmhip_create.argtypes = [P(ffi.svm_problem), P(ffi.svm_parameter), I, P(svmhip_options),
+                                P(h)]
+    L.svmhip_setup_rbf.argtypes = [h, I]
+    L.svmhip_setup.argtypes = [h, P(C.c_byte), P(D), P(D), P(C.c_char), D, D, I]
+    L.svmhip_select_working_set.argtypes = [h, I, P(I), P(I), P(I)]
+    L.svmhip_compute_alpha.argtypes = [h]
+    L.svmhip_update_gradient.argtypes = [h, I]
+    L.svmhip_update_alpha_status.argtypes = [h]
+    L.svmhip_fetch.argtypes = [h, P(D), P(D), P(C.c_char), I]
+    L.svmhip_solve.argtypes = [h, I, P(svmhip_stats)]
+    L.svmhip_finalize.argtypes = [h, P(svmhip_stats)]
+    L.svmhip_get_stats.argtypes = [h, P(svmhip_stats)]
+    L.svmhip_kernel_row.argtypes = [h, I, P(C.c_float)]
+    L.svmhip_destroy.argtypes = [h]
+    L.svmhip_destroy.restype = None
+    L.svmhip_last_error.argtypes = [h]
+    L.svmhip_last_error.restype = C.c_char_p
+    L.svmhip_device_count.restype = I
+    for name in ("svmhip_create", "svmhip_setup_rbf", "svmhip_setup", "svmhip_select_working_set",
+                 "svmhip_compute_alpha", "svmhip_update_gradient", "svmhip_update_alpha_status",
+                 "svmhip_fetch", "svmhip_solve", "svmhip_finalize", "svmhip_get_stats",
+                 "svmhip_kernel_row"):
+        getattr(L, name).restype = I
+    _lib = L
+    return L
+
+
+def set_quiet(quiet=True):
+    """Route LIBSVM's info() output to nowhere (quiet) or stdout."""
+    lib().svm_set_print_string_function(_quiet_cb if quiet else PRINT_FUNC())
+
+
+def device_count():
+    return lib().svmhip_device_count()
+
+
+class SolverError(RuntimeError):
+    pass
+
+
+def _ptr(a, ct):
+    return a.ctypes.data_as(C.POINTER(ct))
+
+
+class DeviceSolver:
+    """One device solve: a Python view of the svmhip_* handle.
+
+    Method names follow the reference's CudaSolver (libsvm/cuda_solver.h:224-244)."""
+
+    def __init__(self, problem: Problem, param, kind=KIND_C, grid_blocks=0, max_iter=0,
+                 cache_bytes=0.0, iters_per_launch=0, device=0, verbose=0):
+        L = lib()
+        self.problem = problem  # keeps the node buffers alive
+        self.param = param
+        opt = svmhip_options(C.sizeof(svmhip_options), device, grid_blocks, 0, max_iter,
+                             float(cache_bytes), iters_per_launch, verbose)
+        self._h = C.c_void_p()
+        rc = L.svmhip_create(C.byref(problem.struct), C.byref(param), kind, C.byref(opt),
+                             C.byref(self._h))
+        if rc != 0:
+            raise SolverError(f"svmhip_create failed: {STATUS_NAMES.get(rc, rc)}")
+        self.n_var = 0
+
+    def _check(self, rc, what):
+        if rc != 0:
+            msg = lib().svmhip_last_error(self._h).decode()
+            raise SolverError(f"{what}: {STATUS_NAMES.get(rc, rc)}: {msg}")
+
+    def setup_rbf_variables(self):
+        self._check(lib().svmhip_setup_rbf(self._h, self.problem.l), "setup_rbf")
+
+    def setup_solver(self, y, G, alpha, status, Cp, Cn):
+        y = np.ascontiguousarray(y, dtype=np.int8)
+        G = np.ascontiguousarray(G, dtype=np.float64)
+        alpha = np.ascontiguousarray(alpha, dtype=np.float64)
+        status = np.ascontiguousarray(status, dtype=np.int8)
+        self.n_var = len(y)
+        self._check(lib().svmhip_setup(self._h, _ptr(y, C.c_byte), _ptr(G, C.c_double),
+                                       _ptr(alpha, C.c_double), _ptr(status, C.c_char),
+                                       float(Cp), float(Cn), self.n_var), "setup")
+
+    def select_working_set(self, want_pair=False):
+        opt, i, j = C.c_int(), C.c_int(-1), C.c_int(-1)
+        if want_pair:
+            rc = lib().svmhip_select_working_set(self._h, self.n_var, C.byref(opt), C.byref(i),
+                                                 C.byref(j))
+        else:
+            rc = lib().svmhip_select_working_set(self._h, self.n_var, C.byref(opt), None, None)
+        self._check(rc, "select_working_set")
+        return opt.value, i.value, j.value
+
+    def compute_alpha(self):
+        self._check(lib().svmhip_compute_alpha(self._h), "compute_alpha")
+
+    def update_gradient(self):
+        self._check(lib().svmhip_update_gradient(self._h, self.n_var), "update_gradient")
+
+    def update_alpha_status(self):
+        self._check(lib().svmhip_update_alpha_status(self._h), "update_alpha_status")
+
+    def fetch_vectors(self):
+        G = np.zeros(self.n_var)
+        a = np.zeros(self.n_var)
+        st = np.zeros(self.n_var, dtype=np.int8)
+        self._check(lib().svmhip_fetch(self._h, _ptr(G, C.c_double), _ptr(a, C.c_double),
+                                       _ptr(st, C.c_char), self.n_var), "fetch")
+        return G, a, st
+
+    def solve(self, max_iter=0):
+        s = svmhip_stats()
+        self._check(lib().svmhip_solve(self._h, max_iter, C.byref(s)), "solve")
+        return s.as_dict()
+
+    def finalize(self):
+        s = svmhip_stats()
+        self._check(lib().svmhip_finalize(self._h, C.byref(s)), "finalize")
+        return s.as_dict()
+
+    def kernel_row(self, i):
+        out = np.zeros(self.problem.l, dtype=np.float32)
+        self._check(lib().svmhip_kernel_row(self._h, int(i), _ptr(out, C.c_float)), "kernel_row")
+        return out
+
+    def close(self):
+        if self._h:
+            lib().svmhip_destroy(self._h)
+            self._h = C.c_void_p()
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+
+def formulate(problem: Problem, param, Cp=None, Cn=None):
+    """Initial (y, p, alpha, Cp, Cn, kind) of one subproblem, as the
+    formulations of libsvm/svm.cpp:1500-1696 build them."""
+    l = problem.l
+    yl = problem.y
+    st = param.svm_type
+    if st == C_SVC:
+        y = np.where(yl > 0, 1, -1).astype(np.int8)
+        return y, -np.ones(l), np.zeros(l), (param.C if Cp is None else Cp), \
+            (param.C if Cn is None else Cn), KIND_C
+    if st == NU_SVC:
+        y = np.where(yl > 0, 1, -1).astype(np.int8)
+        a = np.zeros(l)
+        pos = neg = param.nu * l / 2
+        for i in range(l):
+            if y[i] == 1:
+                a[i] = min(1.0, pos)
+                pos -= a[i]
+            else:
+                a[i] = min(1.0, neg)
+                neg -= a[i]
+        return y, np.zeros(l), a, 1.0, 1.0, KIND_NU
+    if st == ONE_CLASS:
+        n = int(param.nu * l)
+        a = np.zeros(l)
+        a[:n] = 1
+        if n < l:
+            a[n] = param.nu * l - n
+        return np.ones(l, dtype=np.int8), np.zeros(l), a, 1.0, 1.0, KIND_C
+    if st == EPSILON_SVR:
+        y = np.concatenate([np.ones(l), -np.ones(l)]).astype(np.int8)
+        p = np.concatenate([param.p - yl, param.p + yl])
+        return y, p, np.zeros(2 * l), param.C, param.C, KIND_C
+    if st == NU_SVR:
+        C_ = param.C
+        a = np.zeros(2 * l)
+        s = C_ * param.nu * l / 2
+        for i in range(l):
+            a[i] = a[i + l] = min(s, C_)
+            s -= a[i]
+        y = np.concatenate([np.ones(l), -np.ones(l)]).astype(np.int8)
+        p = np.concatenate([-yl, yl])
+        return y, p, a, C_, C_, KIND_NU
+    raise ValueError(st)
+
+
+def initial_status(y, alpha, Cp, Cn):
+    C_ = np.where(y > 0, Cp, Cn)
+    return np.where(alpha >= C_, 1, np.where(alpha <= 0, 0, 2)).astype(np.int8)
+
+
+def solve_one(problem: Problem, param, Cp=None, Cn=None, step_wise=False, max_iter=0, **opts):
+    """Train one subproblem on the device; returns the raw solver result
+    (alpha_raw, G, status, stats) plus the signed coefficients `alpha` in the
+    convention of svm_train_one (libsvm/svm.cpp:1707-1769)."""
+    y, p, a0, Cp_, Cn_, kind = formulate(problem, param, Cp, Cn)
+    s = DeviceSolver(problem, param, kind=kind, max_iter=max_iter, **opts)
+    s.setup_rbf_variables()
+    s.setup_solver(y, p, a0, initial_status(y, a0, Cp_, Cn_), Cp_, Cn_)
+    if step_wise:
+        limit = max_iter if max_iter > 0 else max(10_000_000, 100 * problem.l)
+        it = 0
+        while it < limit:
+            opt, _, _ = s.select_working_set()
+            if opt:
+                break
+            it += 1
+            s.compute_alpha()
+            s.update_gradient()
+            s.update_alpha_status()
+        stats = s.finalize()
+    else:
+        stats = s.solve(max_iter)
+    G, a, st = s.fetch_vectors()
+    s.close()
+    l = problem.l
+    rho, obj, r = stats["rho"], stats["obj"], stats["r"]
+    if param.svm_type == C_SVC:
+        coef = a * y
+    elif param.svm_type == NU_SVC:
+        coef = a * y / r
+        rho, obj = rho / r, obj / (r * r)
+    elif param.svm_type == ONE_CLASS:
+        coef = a.copy()
+    else:
+        coef = a[:l] - a[l:]
+    return dict(alpha=coef, alpha_raw=a, G=G, status=st, rho=rho, obj=obj, r=r,
+                iters=stats["iters"], stats=stats)
+
+
+class Model:
+    """Owns a struct svm_model* returned by svm_train / svm_load_model."""
+
+    def __init__(self, ptr, problem=None):
+        if not ptr:
+            raise SolverError("svm_train returned no model (device failure)")
+        self.ptr = ptr
+        self.problem = problem
+
+    def arrays(self):
+        return model_arrays(self.ptr)
+
+    def predict_values(self, problem: Problem, rows=None):
+        L = lib()
+        m = self.ptr.contents
+        k = m.nr_class
+        ndec = 1 if m.param.svm_type in (ONE_CLASS, EPSILON_SVR, NU_SVR) else k * (k - 1) // 2
+        rows = range(problem.l) if rows is None else rows
+        dec = np.zeros((len(rows), max(ndec, 1)))
+        lab = np.zeros(len(rows))
+        buf = np.zeros(max(ndec, 1))
+        for n, r in enumerate(rows):
+            lab[n] = L.svm_predict_values(self.ptr, problem.row_nodes(r), _ptr(buf, C.c_double))
+            dec[n] = buf
+        return lab, dec
+
+    def save(self, path):
+        if lib().svm_save_model(path.encode(), self.ptr) != 0:
+            raise SolverError(f"svm_save_model({path}) failed")
+
+    def __del__(self):
+        try:
+            if self.ptr:
+                lib().svm_free_and_destroy_model(C.byref(self.ptr))
+        except Exception:
+            pass
+
+
+def svm_train(problem: Problem, param):
+    return Model(lib().svm_train(C.byref(problem.struct), C.byref(param)), problem)
+
+
+def svm_load_model(path):
+    ptr = lib().svm_load_model(path.encode())
+    if not ptr:
+        raise SolverError(f"svm_load_model({path}) failed")
+    return Model(ptr)
+
+
+def check_parameter(problem: Problem, param):
+    r = lib().svm_check_parameter(C.byref(problem.struct), C.byref(param))
+    return None if r is None else r.decode()

@@ -1,0 +1,235 @@
+"""Parity of the HIP SMO solver with the FP64 oracle (same seeded inputs).
+
+Tolerances (stated per the accuracy contract): dual objective within 1e-6
+relative, rho within 1e-5, identical support-vector sets on these
+well-conditioned problems, decision values within 1e-4.  Where the device
+evaluates exactly the oracle's arithmetic (FP64 G/alpha, float kernel rows
+summed in LIBSVM's order) the iteration counts and the working-set sequence
+are expected to be identical too, and are asserted.
+"""
+import numpy as np
+import pytest
+
+from libsvm_hip import datasets, ffi
+
+pytestmark = pytest.mark.gpu
+
+KERNELS = {"linear": ffi.LINEAR, "poly": ffi.POLY, "rbf": ffi.RBF, "sigmoid": ffi.SIGMOID}
+
+
+def zipf_problem(n=2000):
+    ip, ix, v, y, d = datasets.zipf_binary(n=n)
+    return ffi.Problem(ip, ix, v, y, d), d
+
+
+def small_dense(n=300, d=10, seed=0, regression=False):
+    rng = np.random.default_rng(seed)
+    X = rng.standard_normal((n, d)).astype(np.float32).astype(np.float64)
+    if regression:
+        t = (2 * X[:, 0] + np.sin(X[:, 1])).astype(np.float32).astype(np.float64)
+        return ffi.Problem.from_dense(X, t)
+    y = np.where(X[:, 0] + 0.5 * X[:, 1] + 0.3 * rng.standard_normal(n) > 0, 1.0, -1.0)
+    return ffi.Problem.from_dense(X, y)
+
+
+def assert_solution_close(dev, ref, C=1.0, iters_equal=True):
+    print(f"  device iters={dev['iters']} obj={dev['obj']:.12g} rho={dev['rho']:.12g} | "
+          f"oracle iters={ref['iters']} obj={ref['obj']:.12g} rho={ref['rho']:.12g}", flush=True)
+    if iters_equal:
+        assert dev["iters"] == ref["iters"]
+    assert abs(dev["obj"] - ref["obj"]) <= 1e-6 * max(1.0, abs(ref["obj"]))
+    assert abs(dev["rho"] - ref["rho"]) <= 1e-5
+    sv_dev = np.nonzero(np.abs(dev["alpha"]) > 0)[0]
+    sv_ref = np.nonzero(np.abs(ref["alpha"]) > 0)[0]
+    assert np.array_equal(sv_dev, sv_ref)
+    assert np.max(np.abs(dev["alpha"] - ref["alpha"]), initial=0.0) <= 1e-6 * max(C, 1.0)
+
+
+@pytest.mark.parametrize("kname", list(KERNELS))
+def test_kernel_rows_sparse(hip, ora, kname):
+    prob, d = zipf_problem(600)
+    prm = ffi.make_param(kernel_type=KERNELS[kname], gamma=1.0 / d, coef0=0.25, degree=3)
+    s = hip.DeviceSolver(prob, prm)
+    s.setup_rbf_variables()
+    for i in (0, 1, 77, 599):
+        dev = s.kernel_row(i)
+        ref = ora.kernel_row(prob, prm, i)
+        assert np.mean(dev == ref) > 0.999, (kname, i)
+        np.testing.assert_allclose(dev, ref, rtol=2e-7, atol=1e-30)
+    s.close()
+
+
+@pytest.mark.parametrize("kname", list(KERNELS))
+def test_kernel_rows_dense(hip, ora, kname):
+    X, y = datasets.gmm_dense_matrix(n=700, d=64, seed=7)
+    prob = ffi.Problem.from_dense(X.astype(np.float64), y)
+    prm = ffi.make_param(kernel_type=KERNELS[kname], gamma=1.0 / 64, coef0=0.1, degree=2)
+    s = hip.DeviceSolver(prob, prm)
+    s.setup_rbf_variables()
+    for i in (0, 5, 699):
+        dev = s.kernel_row(i)
+        ref = ora.kernel_row(prob, prm, i)
+        assert np.mean(dev == ref) > 0.999, (kname, i)
+        np.testing.assert_allclose(dev, ref, rtol=2e-7, atol=1e-30)
+    s.close()
+
+
+def test_kernel_rows_precomputed(hip, ora):
+    rng = np.random.default_rng(3)
+    n = 200
+    A = rng.standard_normal((n, 5)).astype(np.float32)
+    K = (A @ A.T).astype(np.float32).astype(np.float64)
+    rows = np.hstack([np.arange(1, n + 1)[:, None].astype(np.float64), K])
+    l, w = rows.shape
+    indptr = np.arange(0, (l + 1) * w, w)
+    indices = np.tile(np.arange(w, dtype=np.int32), l) - 1  # LIBSVM index 0 = serial
+    prob = ffi.Problem(indptr, indices, rows.ravel(), np.where(K[:, 0] > 0, 1.0, -1.0), w)
+    prm = ffi.make_param(kernel_type=ffi.PRECOMPUTED)
+    s = hip.DeviceSolver(prob, prm)
+    for i in (0, 10, 199):
+        np.testing.assert_array_equal(s.kernel_row(i), ora.kernel_row(prob, prm, i))
+    s.close()
+
+
+def test_csvc_sparse_fused_matches_oracle(hip, ora):
+    prob, d = zipf_problem()
+    prm = ffi.make_param(gamma=1.0 / d, C_=1.0, eps=1e-3, shrinking=0)
+    ref = ora.train_one(prob, prm)
+    dev = hip.solve_one(prob, prm)
+    assert_solution_close(dev, ref)
+    assert dev["stats"]["optimal"] == 1
+
+
+def test_csvc_step_api_trace_matches_oracle(hip, ora):
+    """The host-driven CudaSolver-style interface picks the same (i, j)."""
+    prob, d = zipf_problem(800)
+    prm = ffi.make_param(gamma=1.0 / d, C_=1.0, eps=1e-3, shrinking=0)
+    ref = ora.train_one(prob, prm, trace=150)
+    y, p, a0, Cp, Cn, kind = hip.formulate(prob, prm)
+    s = hip.DeviceSolver(prob, prm, kind=kind)
+    s.setup_rbf_variables()
+    s.setup_solver(y, p, a0, hip.initial_status(y, a0, Cp, Cn), Cp, Cn)
+    pairs = []
+    while True:
+        opt, i, j = s.select_working_set(want_pair=len(pairs) < 150)
+        if opt:
+            break
+        if len(pairs) < 150:
+            pairs.append((i, j))
+        s.compute_alpha()
+        s.update_gradient()
+        s.update_alpha_status()
+    st = s.finalize()
+    s.close()
+    np.testing.assert_array_equal(np.array(pairs), ref["trace"][: len(pairs)])
+    assert st["iters"] == ref["iters"]
+    assert abs(st["obj"] - ref["obj"]) <= 1e-9 * abs(ref["obj"])
+
+
+@pytest.mark.parametrize("svm_type", [ffi.C_SVC, ffi.NU_SVC, ffi.ONE_CLASS, ffi.EPSILON_SVR, ffi.NU_SVR])
+@pytest.mark.parametrize("kname", ["rbf", "poly", "linear"])
+def test_all_formulations_match_oracle(hip, ora, svm_type, kname):
+    reg = svm_type in (ffi.EPSILON_SVR, ffi.NU_SVR)
+    prob = small_dense(regression=reg)
+    prm = ffi.make_param(svm_type=svm_type, kernel_type=KERNELS[kname], gamma=0.1, coef0=0.5,
+                         degree=3, C_=2.0, nu=0.4, p=0.1, shrinking=0)
+    ref = ora.train_one(prob, prm)
+    dev = hip.solve_one(prob, prm)
+    assert_solution_close(dev, ref, C=2.0)
+
+
+def test_small_cache_lru_same_result(hip, ora):
+    """A cache of a handful of rows (LRU evictions every iteration) gives the
+    same iterates as a cache holding every row."""
+    prob, d = zipf_problem(1000)
+    prm = ffi.make_param(gamma=1.0 / d, C_=1.0, eps=1e-3, shrinking=0)
+    full = hip.solve_one(prob, prm)
+    row_bytes = 4 * ((prob.l + 63) // 64 * 64)
+    for slots in (2, 3, 17):
+        small = hip.solve_one(prob, prm, cache_bytes=slots * row_bytes)
+        assert small["stats"]["cache_slots"] == slots
+        assert small["iters"] == full["iters"]
+        np.testing.assert_array_equal(small["alpha_raw"], full["alpha_raw"])
+        assert small["stats"]["row_misses"] > full["stats"]["row_misses"]
+
+
+@pytest.mark.parametrize("grid", [1, 3, 7, 64])
+def test_grid_sizes_agree(hip, grid):
+    """Any persistent-grid size gives bit-identical iterates (the reductions
+    break ties by index, so the result does not depend on the split)."""
+    prob, d = zipf_problem(1500)
+    prm = ffi.make_param(gamma=1.0 / d, C_=1.0, eps=1e-3, shrinking=0)
+    ref = hip.solve_one(prob, prm, grid_blocks=0)
+    dev = hip.solve_one(prob, prm, grid_blocks=grid)
+    assert dev["iters"] == ref["iters"]
+    np.testing.assert_array_equal(dev["alpha_raw"], ref["alpha_raw"])
+
+
+def test_max_iter_cap(hip, ora):
+    prob, d = zipf_problem(800)
+    prm = ffi.make_param(gamma=1.0 / d, C_=1.0, eps=1e-3, shrinking=0)
+    ref = ora.train_one(prob, prm, max_iter=37)
+    dev = hip.solve_one(prob, prm, max_iter=37)
+    assert dev["iters"] == 37 == ref["iters"]
+    assert dev["stats"]["optimal"] == 0
+    np.testing.assert_allclose(dev["alpha_raw"], ref["alpha_raw"], rtol=0, atol=1e-12)
+
+
+def test_iters_per_launch_chunks(hip):
+    prob, d = zipf_problem(800)
+    prm = ffi.make_param(gamma=1.0 / d, C_=1.0, eps=1e-3, shrinking=0)
+    one = hip.solve_one(prob, prm)
+    chunked = hip.solve_one(prob, prm, iters_per_launch=13)
+    assert chunked["iters"] == one["iters"]
+    np.testing.assert_array_equal(chunked["alpha_raw"], one["alpha_raw"])
+
+
+def test_svm_train_multiclass_model_matches_oracle(hip, ora):
+    X, y, _ = datasets.clusters_10class(scale=0.03)
+    prob = ffi.Problem.from_dense(X.astype(np.float64), y.astype(np.float64))
+    prm = ffi.make_param(gamma=2.0 ** -7, C_=4.0, eps=1e-3, shrinking=0)
+    assert hip.check_parameter(prob, prm) is None
+    m_dev = hip.svm_train(prob, prm).arrays()
+    m_ref = ora.svm_train(prob, prm).arrays()
+    assert m_dev["nr_class"] == m_ref["nr_class"] == 10
+    np.testing.assert_array_equal(m_dev["label"], m_ref["label"])
+    np.testing.assert_array_equal(m_dev["nSV"], m_ref["nSV"])
+    np.testing.assert_array_equal(m_dev["sv_indices"], m_ref["sv_indices"])
+    np.testing.assert_allclose(m_dev["rho"], m_ref["rho"], atol=1e-5)
+    np.testing.assert_allclose(m_dev["sv_coef"], m_ref["sv_coef"], atol=1e-6 * 4)
+
+
+def test_predict_and_model_file_roundtrip(hip, ora, tmp_path):
+    prob, d = zipf_problem(1200)
+    prm = ffi.make_param(gamma=1.0 / d, C_=1.0, eps=1e-3, shrinking=0)
+    m = hip.svm_train(prob, prm)
+    lab, dec = m.predict_values(prob, rows=range(0, 1200, 7))
+    ref_lab, ref_dec = ora.svm_train(prob, prm).predict_values(prob, rows=range(0, 1200, 7))
+    np.testing.assert_array_equal(lab, ref_lab)
+    np.testing.assert_allclose(dec, ref_dec, atol=1e-4)
+    path = str(tmp_path / "m.model")
+    m.save(path)
+    text = open(path).read()
+    assert text.startswith("svm_type c_svc\nkernel_type rbf\n")
+    m2 = hip.svm_load_model(path)
+    lab2, dec2 = m2.predict_values(prob, rows=range(0, 1200, 7))
+    np.testing.assert_array_equal(lab2, lab)
+    np.testing.assert_allclose(dec2, dec, atol=1e-6)
+
+
+def test_kkt_certificate_blob_csr_20k(hip):
+    """Size-independent check at a larger size: the returned point is
+    feasible and satisfies the eps stopping rule m(alpha) - M(alpha) < eps."""
+    ip, ix, v, y, d = datasets.blob_mask_csr(n=20000)
+    prob = ffi.Problem(ip, ix, v, y, d)
+    prm = ffi.make_param(gamma=0.02, C_=10.0, eps=1e-3, shrinking=0)
+    r = hip.solve_one(prob, prm)
+    a, G, yy = r["alpha_raw"], r["G"], np.where(y > 0, 1, -1)
+    assert np.all(a >= 0) and np.all(a <= 10.0)
+    assert abs(np.dot(a, yy)) < 1e-8 * max(1.0, a.sum())
+    up = ((yy == 1) & (a < 10.0)) | ((yy == -1) & (a > 0))
+    low = ((yy == 1) & (a > 0)) | ((yy == -1) & (a < 10.0))
+    m_up = np.max(-yy[up] * G[up])
+    M_low = np.min(-yy[low] * G[low])
+    print(f"  iters={r['iters']} gap={m_up - M_low:.3e} solve_ms={r['stats']['solve_ms']:.1f}", flush=True)
+    assert m_up - M_low < 1e-3 + 1e-9
