@@ -71,6 +71,8 @@ typedef struct {
 	long long row_hits;       /* kernel-row cache hits */
 	long long row_misses;     /* kernel rows computed */
 	double solve_ms;          /* device time of the iteration loop (HIP events) */
+	double kernel_ms;         /* device time of the persistent-kernel launches alone */
+	int launches;             /* persistent-kernel launches of the last solve */
 	double setup_ms;          /* device time of G initialisation */
 	int cache_slots;          /* rows the cache holds */
 	int grid_blocks;          /* workgroups used by the persistent kernel */
@@ -91,7 +93,9 @@ svmhip_status svmhip_setup_rbf(svmhip_solver *h, int l);
 /* CudaSolver::setup_solver (libsvm/cuda_solver.cpp:159-239): copies y, the
  * initial gradient G (= p, as Solver::Solve passes it, libsvm/svm.cpp:546-557),
  * alpha and alpha_status, computes QD, initialises G = p + Q alpha on the
- * device in FP64 and sizes the kernel-row cache.  n_var is l for
+ * device in FP64 and sizes the kernel-row cache.  Calling it again on the
+ * same handle discards the previous solve's state and starts a new one over
+ * the already uploaded rows (labels, C and alpha may change).  n_var is l for
  * classification / one-class and 2l for regression (SVR variable i+l shares
  * kernel row i). */
 svmhip_status svmhip_setup(svmhip_solver *h, const signed char *y, const double *G, const double *alpha,

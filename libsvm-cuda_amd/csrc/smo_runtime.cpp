@@ -141,6 +141,21 @@ struct svmhip_solver {
     return V;
   }
 
+  // Frees what svmhip_setup allocated, keeping the uploaded problem, so one
+  // handle can run several solves over the same rows.
+  void release_solver_state() {
+    (void)hipStreamSynchronize(st);
+    for (void **p : {(void **)&d_G, (void **)&d_alpha, (void **)&d_p, (void **)&d_QD, (void **)&d_y,
+                     (void **)&d_status, (void **)&d_rows, (void **)&d_computed, (void **)&d_slot_of_row,
+                     (void **)&d_row_of_slot, (void **)&d_prev, (void **)&d_next}) {
+      dfree(*p);
+      *p = nullptr;
+    }
+    setup_done = false;
+    have_j = false;
+    stats = svmhip_stats{};
+  }
+
   void read_ctl() {
     ck(hipMemcpyAsync(host_ctl, d_ctl, sizeof(smo::Ctl), hipMemcpyDeviceToHost, st), "read control record");
     ck(hipStreamSynchronize(st), "synchronise");
@@ -352,7 +367,7 @@ svmhip_status svmhip_setup(svmhip_solver *h, const signed char *y, const double 
   if (n_var != h->l && n_var != 2 * h->l) h->fail(SVMHIP_ERR_ARG, "setup: n_var=%d must be l or 2l (l=%d)", n_var, h->l);
   if (n_var == 2 * h->l && h->l > 0 && h->svm_type != EPSILON_SVR && h->svm_type != NU_SVR)
     h->fail(SVMHIP_ERR_ARG, "setup: n_var=2l is only valid for regression");
-  if (h->setup_done) h->fail(SVMHIP_ERR_STATE, "setup called twice on one handle");
+  if (h->setup_done) h->release_solver_state();  // re-setup reuses the uploaded rows
   if (!(Cp > 0) || !(Cn > 0)) h->fail(SVMHIP_ERR_ARG, "setup: Cp, Cn must be > 0");
   h->ck(hipSetDevice(h->dev), "hipSetDevice");
   if (!h->rbf_ready) {
@@ -561,9 +576,17 @@ svmhip_status svmhip_solve(svmhip_solver *h, int max_iter, svmhip_stats *stats) 
     h->ck(hipEventRecord(h->ev[0], h->st), "event");
     if (h->nvar > 0) {
       h->ck(smo_launch_select_i(&V, h->shmem, h->st), "select i");
+      h->stats.kernel_ms = 0;
+      h->stats.launches = 0;
       for (;;) {
+        h->ck(hipEventRecord(h->ev[2], h->st), "event");
         h->ck(smo_launch_persistent(&V, chunk, h->shmem, h->st), "persistent SMO kernel");
+        h->ck(hipEventRecord(h->ev[3], h->st), "event");
         h->read_ctl();
+        float kms = 0;
+        (void)hipEventElapsedTime(&kms, h->ev[2], h->ev[3]);
+        h->stats.kernel_ms += kms;
+        h->stats.launches += 1;
         if (h->host_ctl->err) h->fail(SVMHIP_ERR_TIMEOUT, "device wait timed out at iteration %d", h->host_ctl->iter);
         if (h->host_ctl->stop) break;
       }

@@ -38,7 +38,8 @@ class svmhip_options(C.Structure):
 class svmhip_stats(C.Structure):
     _fields_ = [("iters", C.c_int), ("optimal", C.c_int), ("obj", C.c_double), ("rho", C.c_double),
                 ("r", C.c_double), ("row_hits", C.c_longlong), ("row_misses", C.c_longlong),
-                ("solve_ms", C.c_double), ("setup_ms", C.c_double), ("cache_slots", C.c_int),
+                ("solve_ms", C.c_double), ("kernel_ms", C.c_double), ("launches", C.c_int),
+                ("setup_ms", C.c_double), ("cache_slots", C.c_int),
                 ("grid_blocks", C.c_int)]
 
     def as_dict(self):
