@@ -60,6 +60,18 @@ def cpu_baseline(prob, prm, seconds):
                       f"-m 100 MB row cache, shrinking on), {dt:.1f} s"}
 
 
+def copy_probe(gib):
+    """A plain device-to-device copy of known size through torch's copy
+    kernel: its FETCH_SIZE / WRITE_SIZE readings calibrate the counters."""
+    import torch
+    n = int(gib * 2 ** 30) // 4
+    a = torch.ones(n, dtype=torch.float32, device="cuda:0")
+    b = torch.empty_like(a)
+    b.copy_(a)
+    torch.cuda.synchronize()
+    log(f"[bench] copy probe: {4 * n} bytes read, {4 * n} bytes written")
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -71,7 +83,13 @@ def main():
     ap.add_argument("--cache-gb", type=float, default=64.0)
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--cpu-seconds", type=float, default=20.0)
+    ap.add_argument("--copy-probe-gib", type=float, default=0.0,
+                    help="instead of the solve, run a plain 1-pass device copy of this many GiB "
+                         "(reference for the FETCH_SIZE / WRITE_SIZE counters)")
     args = ap.parse_args()
+    if args.copy_probe_gib > 0:
+        copy_probe(args.copy_probe_gib)
+        return
     if args.gpus != 1 or int(os.environ.get("WORLD_SIZE", "1")) > 1:
         print("bench.py measures one GPU only: this solver build has no multi-GPU path "
               "(run with --gpus 1)", file=sys.stderr)

@@ -52,7 +52,7 @@ typedef struct {
 	size_t struct_size;       /* sizeof(svmhip_options) */
 	int device;               /* HIP device ordinal (default 0) */
 	int grid_blocks;          /* persistent-grid workgroups (0 = autotune) */
-	int block_threads;        /* threads per workgroup (0 = 256) */
+	int block_threads;        /* threads per workgroup (0 = 512, the only size built) */
 	int max_iter;             /* 0 = LIBSVM rule max(10^7, 100*l) */
 	double cache_bytes;       /* kernel-row cache bytes (0 = param.cache_size MB) */
 	int iters_per_launch;     /* persistent-kernel iterations per launch (0 = all) */

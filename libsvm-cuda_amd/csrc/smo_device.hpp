@@ -91,6 +91,9 @@ struct View {
   // control
   Ctl *ctl;
   Partial *part;
+  Partial *gpart;         // per-group partials of the two-level barrier
+  unsigned *grp;          // per-group arrival counters (GROUP_STRIDE apart)
+  int ngroups, pad_g;
   float *xs_global;       // per-workgroup query-row scratch when d is too big for LDS
   int use_lds, nblocks;
 };
@@ -189,27 +192,51 @@ __device__ void stage_row(const View &V, int q, float *xs) {
 
 // dot(x_q, x_r) with x_q staged in xs; row r is this lane's row.  Products of
 // two floats are exact in FP64, so fma == multiply-then-add here and the sum
-// is LIBSVM's sequential merge-order sum.
+// is LIBSVM's sequential merge-order sum.  Loads are issued in batches of
+// ROW_UNROLL per wave (16 KB in flight per wave) so a row pass is bound by
+// HBM bandwidth rather than by one memory round trip per few elements.
+constexpr int ROW_UNROLL = 32;
+
 __device__ __forceinline__ double row_dot(const View &V, const float *xs, int r) {
-  const int len = V.row_len[r];
-  const long long base = V.slice_ptr[r / SLICE] + (r % SLICE);
+  const int s = r / SLICE;
+  const long long base = V.slice_ptr[s] + (r % SLICE);
   const float *vp = V.vals + base;
   double acc = 0.0;
+  constexpr int U = ROW_UNROLL;
   if (V.dense) {
+    const int len = V.d;
     int p = 0;
-    for (; p + 4 <= len; p += 4) {
-      float a0 = vp[(long long)(p + 0) * SLICE], a1 = vp[(long long)(p + 1) * SLICE];
-      float a2 = vp[(long long)(p + 2) * SLICE], a3 = vp[(long long)(p + 3) * SLICE];
-      acc = fma((double)a0, (double)xs[p + 0], acc);
-      acc = fma((double)a1, (double)xs[p + 1], acc);
-      acc = fma((double)a2, (double)xs[p + 2], acc);
-      acc = fma((double)a3, (double)xs[p + 3], acc);
+    for (; p + U <= len; p += U) {
+      float a[U];
+#pragma unroll
+      for (int u = 0; u < U; ++u) a[u] = vp[(long long)(p + u) * SLICE];
+#pragma unroll
+      for (int u = 0; u < U; ++u) acc = fma((double)a[u], (double)xs[p + u], acc);
     }
     for (; p < len; ++p) acc = fma((double)vp[(long long)p * SLICE], (double)xs[p], acc);
   } else {
+    // walk the slice's common width (rows are zero-padded to it) so the trip
+    // count is uniform across the wave; entries past this row's length are
+    // masked out of the sum
+    const int len = V.row_len[r];
+    const int width = (int)((V.slice_ptr[s + 1] - V.slice_ptr[s]) / SLICE);
     const int *cp = V.cols + base;
-    for (int p = 0; p < len; ++p)
-      acc = fma((double)vp[(long long)p * SLICE], (double)xs[cp[(long long)p * SLICE]], acc);
+    int p = 0;
+    for (; p + U <= width; p += U) {
+      float a[U];
+      int c[U];
+#pragma unroll
+      for (int u = 0; u < U; ++u) {
+        a[u] = vp[(long long)(p + u) * SLICE];
+        c[u] = cp[(long long)(p + u) * SLICE];
+      }
+#pragma unroll
+      for (int u = 0; u < U; ++u) {
+        double t = (p + u < len) ? (double)a[u] * (double)xs[c[u]] : 0.0;
+        acc = acc + t;
+      }
+    }
+    for (; p < len; ++p) acc = fma((double)vp[(long long)p * SLICE], (double)xs[cp[(long long)p * SLICE]], acc);
   }
   return acc;
 }
@@ -573,74 +600,95 @@ __device__ __forceinline__ Partial load_partial(const Partial *p) {
 }
 
 // ----------------------------------------------------------- leader work
-// Executed by the one workgroup that arrives last; thread 0 writes Ctl.
+// Decisions taken once per pass by a single thread (thread 0 of the leader
+// workgroup) from the grid-wide reduced partial.
 
-template <int KIND>
-__device__ void leader_after_select_i(const View &V, RedSmem &sm, bool count_maxiter) {
-  double v[4] = {-INF, -INF, -INF, -INF};
-  int ix[2] = {-1, -1};
-  for (int b = threadIdx.x; b < V.nblocks; b += blockDim.x) {
-    Partial pt = load_partial(&V.part[b]);
-    if (KIND == KIND_C) {
-      max_idx(v[0], ix[0], pt.v0, pt.i0);
-      v[1] = fmax(v[1], pt.v1);
-    } else {
-      max_idx(v[0], ix[0], pt.v0, pt.i0);
-      max_idx(v[1], ix[1], pt.v1, pt.i1);
-      v[2] = fmax(v[2], pt.v2);
-      v[3] = fmax(v[3], pt.v3);
-    }
+enum { PH_I = 0, PH_J = 1 };
+
+template <int KIND, int PH>
+__device__ __forceinline__ Partial partial_identity() {
+  Partial p;
+  if (PH == PH_J) { p.v0 = INF; p.v1 = p.v2 = p.v3 = 0; }
+  else { p.v0 = p.v1 = p.v2 = p.v3 = -INF; }
+  p.i0 = p.i1 = -1;
+  p.pad0 = p.pad1 = 0;
+  return p;
+}
+
+template <int KIND, int PH>
+__device__ __forceinline__ void partial_combine(Partial &a, const Partial &b) {
+  if (PH == PH_J) {
+    min_idx(a.v0, a.i0, b.v0, b.i0);
+  } else if (KIND == KIND_C) {
+    max_idx(a.v0, a.i0, b.v0, b.i0);
+    a.v1 = fmax(a.v1, b.v1);
+  } else {
+    max_idx(a.v0, a.i0, b.v0, b.i0);
+    max_idx(a.v1, a.i1, b.v1, b.i1);
+    a.v2 = fmax(a.v2, b.v2);
+    a.v3 = fmax(a.v3, b.v3);
   }
-  if (KIND == KIND_C) {
+}
+
+// Reduce src[first + t*stride], t < count, over the whole workgroup; the
+// result is valid in thread 0.
+template <int KIND, int PH>
+__device__ Partial reduce_partials(const Partial *src, int first, int count, int stride, RedSmem &sm) {
+  Partial a = partial_identity<KIND, PH>();
+  for (int t = threadIdx.x; t < count; t += blockDim.x) partial_combine<KIND, PH>(a, load_partial(&src[first + t * stride]));
+  double v[4] = {a.v0, a.v1, a.v2, a.v3};
+  int ix[2] = {a.i0, a.i1};
+  if (PH == PH_J) {
+    const int kinds[4] = {1, 2, 2, 2};
+    block_reduce<1>(sm, v, ix, kinds);
+  } else if (KIND == KIND_C) {
     const int kinds[4] = {0, 2, 2, 2};
     block_reduce<2>(sm, v, ix, kinds);
   } else {
     const int kinds[4] = {0, 0, 2, 2};
     block_reduce<4>(sm, v, ix, kinds);
   }
-  if (threadIdx.x != 0) return;
+  a.v0 = v[0]; a.v1 = v[1]; a.v2 = v[2]; a.v3 = v[3];
+  a.i0 = ix[0]; a.i1 = ix[1];
+  return a;
+}
+
+// After "select i": i (ip / in), Gmax / Gmax2, the stopping test, row slot(s).
+template <int KIND>
+__device__ void finish_select_i(const View &V, const Partial &t, bool count_maxiter) {
   Ctl *c = V.ctl;
   int stop = STOP_NONE;
   if (count_maxiter && c->iter >= V.max_iter) stop = STOP_MAXITER;
   if (KIND == KIND_C) {
-    c->Gmax = v[0];
-    c->Gmax2 = v[1];
-    c->i = ix[0];
-    if (!stop && v[0] + v[1] < V.eps) stop = STOP_OPTIMAL;   // libsvm/svm.cpp:919
+    c->Gmax = t.v0;
+    c->Gmax2 = t.v1;
+    c->i = t.i0;
+    if (!stop && t.v0 + t.v1 < V.eps) stop = STOP_OPTIMAL;   // libsvm/svm.cpp:919
     if (!stop) {
       int m;
-      c->slot_i = cache_lookup(V, real_of(V, ix[0]), -1, &m);
+      c->slot_i = cache_lookup(V, real_of(V, t.i0), -1, &m);
       c->miss_i = m;
     }
   } else {
-    c->Gp = v[0]; c->Gn = v[1]; c->Gp2 = v[2]; c->Gn2 = v[3];
-    c->ip = ix[0]; c->in = ix[1];
-    if (!stop && fmax(v[0] + v[2], v[1] + v[3]) < V.eps) stop = STOP_OPTIMAL;  // libsvm/svm.cpp:1174
+    c->Gp = t.v0; c->Gn = t.v1; c->Gp2 = t.v2; c->Gn2 = t.v3;
+    c->ip = t.i0; c->in = t.i1;
+    if (!stop && fmax(t.v0 + t.v2, t.v1 + t.v3) < V.eps) stop = STOP_OPTIMAL;  // libsvm/svm.cpp:1174
     if (!stop) {
       int m, sp = -1;
       c->miss_ip = 0; c->miss_in = 0;
-      if (ix[0] >= 0) { sp = cache_lookup(V, real_of(V, ix[0]), -1, &m); c->slot_ip = sp; c->miss_ip = m; }
-      if (ix[1] >= 0) { c->slot_in = cache_lookup(V, real_of(V, ix[1]), sp, &m); c->miss_in = m; }
+      if (t.i0 >= 0) { sp = cache_lookup(V, real_of(V, t.i0), -1, &m); c->slot_ip = sp; c->miss_ip = m; }
+      if (t.i1 >= 0) { c->slot_in = cache_lookup(V, real_of(V, t.i1), sp, &m); c->miss_in = m; }
     }
   }
   c->stop = stop;
 }
 
-// After "select j": pick j, look up its row, and (persistent mode) do the
-// alpha step and the status update of the pair.
+// After "select j": j, its row slot, and (persistent mode) the alpha step
+// and the status update of the pair.
 template <int KIND>
-__device__ void leader_after_select_j(const View &V, RedSmem &sm, bool do_step) {
-  double v[4] = {INF, 0, 0, 0};
-  int ix[2] = {-1, -1};
-  for (int b = threadIdx.x; b < V.nblocks; b += blockDim.x) {
-    Partial pt = load_partial(&V.part[b]);
-    min_idx(v[0], ix[0], pt.v0, pt.i0);
-  }
-  const int kinds[4] = {1, 2, 2, 2};
-  block_reduce<1>(sm, v, ix, kinds);
-  if (threadIdx.x != 0) return;
+__device__ void finish_select_j(const View &V, const Partial &t, bool do_step) {
   Ctl *c = V.ctl;
-  int j = ix[0];
+  int j = t.i0;
   if (j < 0) {  // no eligible j: only reachable with eps <= 0 or NaN gradients
     c->stop = STOP_OPTIMAL;
     return;
@@ -669,6 +717,18 @@ __device__ void leader_after_select_j(const View &V, RedSmem &sm, bool do_step) 
   }
 }
 
+template <int KIND>
+__device__ void leader_after_select_i(const View &V, RedSmem &sm, bool count_maxiter) {
+  Partial t = reduce_partials<KIND, PH_I>(V.part, 0, V.nblocks, 1, sm);
+  if (threadIdx.x == 0) finish_select_i<KIND>(V, t, count_maxiter);
+}
+
+template <int KIND>
+__device__ void leader_after_select_j(const View &V, RedSmem &sm, bool do_step) {
+  Partial t = reduce_partials<KIND, PH_J>(V.part, 0, V.nblocks, 1, sm);
+  if (threadIdx.x == 0) finish_select_j<KIND>(V, t, do_step);
+}
+
 // ---------------------------------------------------------- synchronisation
 
 // Last-arriving workgroup detection for the non-persistent step kernels.
@@ -686,34 +746,56 @@ __device__ __forceinline__ bool arrive_last(const View &V, RedSmem &sm) {
 }
 
 constexpr long long SPIN_LIMIT = 1LL << 23;
+constexpr int GROUP_STRIDE = 32;  // group counters 128 bytes apart
 
-// Grid barrier whose last arriver runs `leader` (whole workgroup) before
-// releasing the others.  Waits are bounded: a workgroup that spins past
+// Two-level grid barrier with reduction.  Workgroup b belongs to group
+// b % ngroups; the last arriver of a group reduces the group's partials into
+// gpart[group] and arrives at the grid counter; the last group leader reduces
+// the group partials, runs `finish` (thread 0) and releases everyone by
+// bumping the generation.  Counters see ngroups-way instead of nblocks-way
+// atomic contention.  Waits are bounded: a workgroup that spins past
 // SPIN_LIMIT records ERR_TIMEOUT and leaves, so the kernel always drains.
-template <class F>
-__device__ void grid_leader_barrier(const View &V, RedSmem &sm, F leader) {
+template <int KIND, int PH, class F>
+__device__ void grid_reduce_barrier(const View &V, RedSmem &sm, F finish) {
   Ctl *c = V.ctl;
+  const int ng = V.ngroups, g = blockIdx.x % ng;
+  const int members = (V.nblocks - 1 - g) / ng + 1;
+  unsigned *gcount = V.grp + g * GROUP_STRIDE;
   __syncthreads();
   if (threadIdx.x == 0) {
     sm.gen = ld_acq(&c->gen);
     __threadfence();
-    unsigned prev = atomicAdd(&c->arrive, 1u);
-    sm.last = (prev == (unsigned)V.nblocks - 1u);
+    unsigned prev = atomicAdd(gcount, 1u);
+    sm.last = (prev == (unsigned)members - 1u);
+    if (sm.last) atomicExch(gcount, 0u);
   }
   __syncthreads();
+  bool top = false;
   if (sm.last) {
     __threadfence();
-    leader();
-    __syncthreads();
+    Partial gp = reduce_partials<KIND, PH>(V.part, g, members, ng, sm);
     if (threadIdx.x == 0) {
-      atomicExch(&c->arrive, 0u);
+      V.gpart[g] = gp;
+      __threadfence();
+      unsigned prev = atomicAdd(&c->arrive, 1u);
+      sm.last = (prev == (unsigned)ng - 1u);
+      if (sm.last) atomicExch(&c->arrive, 0u);
+    }
+    __syncthreads();
+    top = sm.last;
+  }
+  if (top) {
+    __threadfence();
+    Partial t = reduce_partials<KIND, PH>(V.gpart, 0, ng, 1, sm);
+    if (threadIdx.x == 0) {
+      finish(t);
       __threadfence();
       atomicAdd(&c->gen, 1u);
     }
   } else if (threadIdx.x == 0) {
     long long spins = 0;
     while (ld_acq(&c->gen) == sm.gen) {
-      __builtin_amdgcn_s_sleep(2);
+      __builtin_amdgcn_s_sleep(1);
       if (++spins > SPIN_LIMIT) {
         atomicExch(&c->err, (int)ERR_TIMEOUT);
         break;

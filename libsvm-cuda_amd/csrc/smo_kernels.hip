@@ -20,7 +20,7 @@
 
 namespace smo {
 
-constexpr int BLOCK = 256;
+constexpr int BLOCK = 512;
 
 __global__ void __launch_bounds__(BLOCK) k_self_dot(View V, double *xsq) {
   int r = blockIdx.x * blockDim.x + threadIdx.x;
@@ -133,11 +133,11 @@ __global__ void __launch_bounds__(BLOCK) k_smo(View V, int iter_limit) {
   for (int n = 0; n < iter_limit; ++n) {
     if (c.stop || c.err) break;
     pass_select_j<KIND>(V, c, xs, sm);
-    grid_leader_barrier(V, sm, [&] { leader_after_select_j<KIND>(V, sm, true); });
+    grid_reduce_barrier<KIND, PH_J>(V, sm, [&](const Partial &t) { finish_select_j<KIND>(V, t, true); });
     c = snap(V.ctl);
     if (c.stop || c.err) break;
     pass_gradient_select<KIND>(V, c, xs, sm, true, true);
-    grid_leader_barrier(V, sm, [&] { leader_after_select_i<KIND>(V, sm, true); });
+    grid_reduce_barrier<KIND, PH_I>(V, sm, [&](const Partial &t) { finish_select_i<KIND>(V, t, true); });
     c = snap(V.ctl);
   }
 }

@@ -77,7 +77,9 @@ struct svmhip_solver {
   float *d_rows = nullptr, *d_xs = nullptr;
   int *d_slot_of_row = nullptr, *d_row_of_slot = nullptr, *d_prev = nullptr, *d_next = nullptr;
   smo::Ctl *d_ctl = nullptr;
-  smo::Partial *d_part = nullptr;
+  smo::Partial *d_part = nullptr, *d_gpart = nullptr;
+  unsigned *d_grp = nullptr;
+  int ngroups = 1;
   std::vector<void *> allocs;
   svmhip_stats stats{};
   std::string err;
@@ -138,6 +140,7 @@ struct svmhip_solver {
     V.computed = d_computed;
     V.ctl = d_ctl; V.part = d_part; V.xs_global = d_xs;
     V.use_lds = use_lds; V.nblocks = nblocks;
+    V.gpart = d_gpart; V.grp = d_grp; V.ngroups = ngroups;
     return V;
   }
 
@@ -296,8 +299,8 @@ svmhip_status svmhip_create(const svm_problem *prob, const svm_parameter *param,
       size_t n = std::min(opt->struct_size ? opt->struct_size : sizeof(svmhip_options), sizeof(svmhip_options));
       memcpy(&h->opt, opt, n);
     }
-    if (h->opt.block_threads != 0 && h->opt.block_threads != 256)
-      h->fail(SVMHIP_ERR_ARG, "block_threads must be 256 (got %d)", h->opt.block_threads);
+    if (h->opt.block_threads != 0 && h->opt.block_threads != 512)
+      h->fail(SVMHIP_ERR_ARG, "block_threads must be 512 (got %d)", h->opt.block_threads);
     int ndev = svmhip_device_count();
     if (ndev <= 0) h->fail(SVMHIP_ERR_DEVICE, "no HIP device available");
     if (h->opt.device < 0 || h->opt.device >= ndev)
@@ -334,6 +337,11 @@ svmhip_status svmhip_create(const svm_problem *prob, const svm_parameter *param,
     if (!h->use_lds) h->d_xs = h->dalloc<float>((size_t)h->nblocks * h->d, "query-row scratch");
     h->d_ctl = h->dalloc<smo::Ctl>(1, "control record");
     h->d_part = h->dalloc<smo::Partial>(2 * (size_t)h->nblocks, "partials");
+    h->ngroups = std::max(1, std::min(32, (int)std::ceil(std::sqrt((double)h->nblocks))));
+    h->d_gpart = h->dalloc<smo::Partial>(h->ngroups, "group partials");
+    h->d_grp = h->dalloc<unsigned>((size_t)h->ngroups * smo::GROUP_STRIDE, "group counters");
+    h->ck(hipMemsetAsync(h->d_grp, 0, sizeof(unsigned) * h->ngroups * smo::GROUP_STRIDE, h->st),
+          "clear group counters");
     h->ck(hipMemsetAsync(h->d_ctl, 0, sizeof(smo::Ctl), h->st), "clear control record");
     h->ck(hipStreamSynchronize(h->st), "create");
     GUARD_END
@@ -457,7 +465,7 @@ svmhip_status svmhip_setup(svmhip_solver *h, const signed char *y, const double 
   h->setup_done = true;
   h->have_j = false;
   if (h->opt.verbose)
-    fprintf(stderr, "svmhip: l=%d d=%d %s, n_var=%d, cache %d rows%s, grid %d x 256\n", h->l, h->d,
+    fprintf(stderr, "svmhip: l=%d d=%d %s, n_var=%d, cache %d rows%s, grid %d x 512\n", h->l, h->d,
             h->dense ? "dense" : "sparse", nv, h->S, h->full ? " (all)" : "", h->nblocks);
   GUARD_END
 }
