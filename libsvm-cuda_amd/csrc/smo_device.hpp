@@ -56,6 +56,7 @@ struct Ctl {
   int lru_head, lru_tail;
   unsigned long long hits, misses;
   double rho, obj, r;
+  alignas(8) unsigned char dec_bytes[384];   // persistent kernel's Dec between launches
 };
 
 // One workgroup's partial reduction (48 bytes).
@@ -91,9 +92,10 @@ struct View {
   // control
   Ctl *ctl;
   Partial *part;
-  Partial *gpart;         // per-group partials of the two-level barrier
-  unsigned *grp;          // per-group arrival counters (GROUP_STRIDE apart)
-  int ngroups, pad_g;
+  struct XPart *xpart;    // persistent kernel: 2 x nblocks exchange records
+  int8_t *pm_flags;       // persistent kernel, full cache: nblocks x l row flags
+  int *pm_lru;            // persistent kernel, LRU cache: nblocks x pm_stride ints
+  long long pm_stride;
   float *xs_global;       // per-workgroup query-row scratch when d is too big for LDS
   int use_lds, nblocks;
 };
@@ -317,63 +319,83 @@ __device__ void block_reduce(RedSmem &sm, double (&v)[4], int (&ix)[2], const in
 
 // --------------------------------------------------------------- the cache
 
-__device__ __forceinline__ void lru_touch(const View &V, int s) {
-  Ctl *c = V.ctl;
-  int head = c->lru_head;
+// Where one copy of the cache's bookkeeping lives: the shared copy used by
+// the step kernels (head/tail/counters in Ctl) or a workgroup's private copy
+// used by the persistent kernel.
+struct CacheMeta {
+  int8_t *computed;                 // full cache: row already materialised
+  int *slot_of_row, *row_of_slot, *prev, *next;
+  int *head, *tail;
+  unsigned long long *hits, *misses;
+};
+
+__device__ __forceinline__ void lru_touch(CacheMeta &M, int s) {
+  int head = *M.head;
   if (head == s) return;
-  int pv = V.lru_prev[s], nx = V.lru_next[s];
-  if (pv >= 0) V.lru_next[pv] = nx;
-  if (nx >= 0) V.lru_prev[nx] = pv;
-  else c->lru_tail = pv;
-  V.lru_prev[s] = -1;
-  V.lru_next[s] = head;
-  V.lru_prev[head] = s;
-  c->lru_head = s;
+  int pv = M.prev[s], nx = M.next[s];
+  if (pv >= 0) M.next[pv] = nx;
+  if (nx >= 0) M.prev[nx] = pv;
+  else *M.tail = pv;
+  M.prev[s] = -1;
+  M.next[s] = head;
+  M.prev[head] = s;
+  *M.head = s;
 }
 
 // Slot holding kernel row `row`; on a miss the least recently used slot other
 // than `exclude` is reassigned (the row is then computed by every workgroup
 // for its owned range in the next pass).  Single thread.
-__device__ int cache_lookup(const View &V, int row, int exclude, int *miss) {
-  Ctl *c = V.ctl;
+__device__ int cache_lookup(const View &V, CacheMeta &M, int row, int exclude, int *miss) {
   if (V.full) {
-    *miss = !V.computed[row];
-    V.computed[row] = 1;
-    if (*miss) c->misses++; else c->hits++;
+    *miss = !M.computed[row];
+    M.computed[row] = 1;
+    if (*miss) ++*M.misses; else ++*M.hits;
     return row;
   }
-  int s = V.slot_of_row[row];
+  int s = M.slot_of_row[row];
   if (s >= 0) {
     *miss = 0;
-    c->hits++;
-    lru_touch(V, s);
+    ++*M.hits;
+    lru_touch(M, s);
     return s;
   }
-  int v = c->lru_tail;
-  if (v == exclude) v = V.lru_prev[v];
-  int old = V.row_of_slot[v];
-  if (old >= 0) V.slot_of_row[old] = -1;
-  V.row_of_slot[v] = row;
-  V.slot_of_row[row] = v;
-  lru_touch(V, v);
+  int v = *M.tail;
+  if (v == exclude) v = M.prev[v];
+  int old = M.row_of_slot[v];
+  if (old >= 0) M.slot_of_row[old] = -1;
+  M.row_of_slot[v] = row;
+  M.slot_of_row[row] = v;
+  lru_touch(M, v);
+  *M.misses += 1;
   *miss = 1;
-  c->misses++;
   return v;
+}
+
+__device__ __forceinline__ CacheMeta shared_meta(const View &V) {
+  CacheMeta M;
+  M.computed = V.computed;
+  M.slot_of_row = V.slot_of_row; M.row_of_slot = V.row_of_slot; M.prev = V.lru_prev; M.next = V.lru_next;
+  M.head = &V.ctl->lru_head; M.tail = &V.ctl->lru_tail;
+  M.hits = &V.ctl->hits; M.misses = &V.ctl->misses;
+  return M;
+}
+
+__device__ __forceinline__ int cache_lookup(const View &V, int row, int exclude, int *miss) {
+  CacheMeta M = shared_meta(V);
+  return cache_lookup(V, M, row, exclude, miss);
 }
 
 // ------------------------------------------------------------ the step math
 
 // Two-variable analytic update with clipping, FP64 (libsvm/svm.cpp:636-721,
-// libsvm/svm_device.cu:828-927).  Single thread.  Qij = Q_i[j] as cached.
-__device__ void alpha_step(const View &V, int i, int j, float Qij) {
-  double *a = V.alpha;
-  const double Ci = C_of(V, i), Cj = C_of(V, j);
-  const double ai0 = a[i], aj0 = a[j];
-  double ai = ai0, aj = aj0;
-  if (V.y[i] != V.y[j]) {
-    double quad = V.QD[i] + V.QD[j] + (double)(2.0f * Qij);
+// libsvm/svm_device.cu:828-927), as a pure function of the pair's scalars.
+// Qij = Q_i[j] as cached (float).  Returns the new (alpha_i, alpha_j).
+__device__ __forceinline__ void alpha_pair(int yi, int yj, double Ci, double Cj, double QDi, double QDj, double Gi,
+                                           double Gj, float Qij, double &ai, double &aj) {
+  if (yi != yj) {
+    double quad = QDi + QDj + (double)(2.0f * Qij);
     if (quad <= 0) quad = TAU;
-    double delta = (-V.G[i] - V.G[j]) / quad;
+    double delta = (-Gi - Gj) / quad;
     double diff = ai - aj;
     ai += delta;
     aj += delta;
@@ -388,9 +410,9 @@ __device__ void alpha_step(const View &V, int i, int j, float Qij) {
       if (aj > Cj) { aj = Cj; ai = Cj + diff; }
     }
   } else {
-    double quad = V.QD[i] + V.QD[j] - (double)(2.0f * Qij);
+    double quad = QDi + QDj - (double)(2.0f * Qij);
     if (quad <= 0) quad = TAU;
-    double delta = (V.G[i] - V.G[j]) / quad;
+    double delta = (Gi - Gj) / quad;
     double sum = ai + aj;
     ai -= delta;
     aj += delta;
@@ -405,8 +427,16 @@ __device__ void alpha_step(const View &V, int i, int j, float Qij) {
       if (ai < 0) { ai = 0; aj = sum; }
     }
   }
-  a[i] = ai;
-  a[j] = aj;
+}
+
+// Step-mode wrapper: reads the pair from the arrays, writes alpha and the
+// changes into Ctl.  Single thread.
+__device__ void alpha_step(const View &V, int i, int j, float Qij) {
+  double ai = V.alpha[i], aj = V.alpha[j];
+  const double ai0 = ai, aj0 = aj;
+  alpha_pair(V.y[i], V.y[j], C_of(V, i), C_of(V, j), V.QD[i], V.QD[j], V.G[i], V.G[j], Qij, ai, aj);
+  V.alpha[i] = ai;
+  V.alpha[j] = aj;
   V.ctl->dai = ai - ai0;
   V.ctl->daj = aj - aj0;
 }
@@ -443,11 +473,31 @@ __device__ __forceinline__ Snap snap(const Ctl *c) {
 // variables (libsvm/svm.cpp:840-863 and the Gmax2 terms of :874-915).
 // If update is set, first applies G_v += Q_iv dai + Q_jv daj
 // (libsvm/svm.cpp:724-736), computing the owned part of row j on a miss.
+// Alpha / status of the last pair, written by the workgroup that owns each
+// variable (persistent kernel: every workgroup computes the pair update, only
+// the owners store it).
+struct Pend {
+  int i, j;
+  double ai, aj;
+};
+
+__device__ __forceinline__ bool owns(const View &V, int sb, int se, int v) {
+  int r = real_of(V, v);
+  return r >= sb * SLICE && r < se * SLICE;
+}
+
 template <int KIND>
-__device__ void pass_gradient_select(const View &V, const Snap &c, float *xs, RedSmem &sm, bool update,
-                                     bool select) {
+__device__ Partial pass_gradient_select(const View &V, const Snap &c, float *xs, RedSmem &sm, bool update,
+                                        bool select, bool to_part = true, const Pend *pend = nullptr) {
   int sb, se;
   owned_slices(V, blockIdx.x, V.nblocks, sb, se);
+  if (pend) {
+    if (threadIdx.x == 0) {
+      if (owns(V, sb, se, pend->i)) { V.alpha[pend->i] = pend->ai; V.status[pend->i] = status_for(V, pend->i, pend->ai); }
+      if (owns(V, sb, se, pend->j)) { V.alpha[pend->j] = pend->aj; V.status[pend->j] = status_for(V, pend->j, pend->aj); }
+    }
+    __syncthreads();
+  }
   if (update && c.miss_j) compute_row_part(V, real_of(V, c.j), c.slot_j, xs, sb, se);
   double v[4] = {-INF, -INF, -INF, -INF};
   int ix[2] = {-1, -1};
@@ -494,7 +544,8 @@ __device__ void pass_gradient_select(const View &V, const Snap &c, float *xs, Re
       }
     }
   }
-  if (!select) return;
+  Partial pt;
+  if (!select) return pt;
   if (KIND == KIND_C) {
     const int kinds[4] = {0, 2, 2, 2};
     block_reduce<2>(sm, v, ix, kinds);
@@ -502,19 +553,17 @@ __device__ void pass_gradient_select(const View &V, const Snap &c, float *xs, Re
     const int kinds[4] = {0, 0, 2, 2};
     block_reduce<4>(sm, v, ix, kinds);
   }
-  if (threadIdx.x == 0) {
-    Partial pt;
-    pt.v0 = v[0]; pt.v1 = v[1]; pt.v2 = v[2]; pt.v3 = v[3];
-    pt.i0 = ix[0]; pt.i1 = ix[1]; pt.pad0 = pt.pad1 = 0;
-    V.part[blockIdx.x] = pt;
-  }
+  pt.v0 = v[0]; pt.v1 = v[1]; pt.v2 = v[2]; pt.v3 = v[3];
+  pt.i0 = ix[0]; pt.i1 = ix[1]; pt.pad0 = pt.pad1 = 0;
+  if (to_part && threadIdx.x == 0) V.part[blockIdx.x] = pt;
+  return pt;
 }
 
 // Pass "select j": second-order gain over the owned variables
 // (libsvm/svm.cpp:866-917; nu rule :1123-1172), row i (rows ip / in) computed
 // for the owned range first on a miss.
 template <int KIND>
-__device__ void pass_select_j(const View &V, const Snap &c, float *xs, RedSmem &sm) {
+__device__ Partial pass_select_j(const View &V, const Snap &c, float *xs, RedSmem &sm, bool to_part = true) {
   int sb, se;
   owned_slices(V, blockIdx.x, V.nblocks, sb, se);
   if (KIND == KIND_C) {
@@ -582,12 +631,11 @@ __device__ void pass_select_j(const View &V, const Snap &c, float *xs, RedSmem &
   int ix[2] = {bj, -1};
   const int kinds[4] = {1, 2, 2, 2};
   block_reduce<1>(sm, v, ix, kinds);
-  if (threadIdx.x == 0) {
-    Partial pt;
-    pt.v0 = v[0]; pt.v1 = pt.v2 = pt.v3 = 0;
-    pt.i0 = ix[0]; pt.i1 = -1; pt.pad0 = pt.pad1 = 0;
-    V.part[blockIdx.x] = pt;
-  }
+  Partial pt;
+  pt.v0 = v[0]; pt.v1 = pt.v2 = pt.v3 = 0;
+  pt.i0 = ix[0]; pt.i1 = -1; pt.pad0 = pt.pad1 = 0;
+  if (to_part && threadIdx.x == 0) V.part[blockIdx.x] = pt;
+  return pt;
 }
 
 // Partials are read with agent-scope loads: they were written by other
@@ -746,64 +794,232 @@ __device__ __forceinline__ bool arrive_last(const View &V, RedSmem &sm) {
 }
 
 constexpr long long SPIN_LIMIT = 1LL << 23;
-constexpr int GROUP_STRIDE = 32;  // group counters 128 bytes apart
 
-// Two-level grid barrier with reduction.  Workgroup b belongs to group
-// b % ngroups; the last arriver of a group reduces the group's partials into
-// gpart[group] and arrives at the grid counter; the last group leader reduces
-// the group partials, runs `finish` (thread 0) and releases everyone by
-// bumping the generation.  Counters see ngroups-way instead of nblocks-way
-// atomic contention.  Waits are bounded: a workgroup that spins past
-// SPIN_LIMIT records ERR_TIMEOUT and leaves, so the kernel always drains.
-template <int KIND, int PH, class F>
-__device__ void grid_reduce_barrier(const View &V, RedSmem &sm, F finish) {
-  Ctl *c = V.ctl;
-  const int ng = V.ngroups, g = blockIdx.x % ng;
-  const int members = (V.nblocks - 1 - g) / ng + 1;
-  unsigned *gcount = V.grp + g * GROUP_STRIDE;
-  __syncthreads();
+// ------------------------------------------------ persistent-kernel exchange
+//
+// The persistent kernel keeps every vector (G, alpha, status, cached kernel
+// rows) partitioned by owner workgroup, and lets every workgroup take the
+// same decisions from the same data.  The only data that crosses workgroups
+// are the per-pass partial records below: each carries a workgroup's best
+// candidate together with the scalars the decision needs (G, alpha and, for
+// the j pass, Q_ij of that candidate).  A record is published with
+// agent-scope (write-through) stores, a wait for their completion, and then
+// its sequence number; readers poll the sequence number and then load the
+// record.  No read-modify-write atomics, no L2 write-back or invalidation.
+
+struct XPart {
+  unsigned seq;
+  int i0, i1, pad;
+  double v0, v1, v2, v3;
+  double g0, a0, g1, a1;   // G and alpha of the winner(s)
+  double qd0, qd1;         // QD of the winner(s)
+  int y0, y1;              // label / sign of the winner(s)
+  float q0, fpad;          // j record: Q_ij
+};
+
+// Control state every workgroup holds (identical in all of them); also the
+// record block 0 leaves in Ctl between launches.
+struct Dec {
+  int phase, stop, iter, epoch;
+  int i, j, ip, in;
+  int slot_i, slot_j, slot_ip, slot_in;
+  int miss_i, miss_j, miss_ip, miss_in;
+  int pend, err, pad0, pad1;
+  double ai_new, aj_new, dai, daj;
+  double Gmax, Gmax2, Gp, Gn, Gp2, Gn2;
+  double Gi, ai, Gip, aip, Gin, ain;
+  double QDi, QDip, QDin;
+  int yi, yip, yin, pad2;
+  unsigned long long hits, misses;
+};
+
+static_assert(sizeof(Dec) <= sizeof(((Ctl *)nullptr)->dec_bytes), "Dec must fit in Ctl");
+
+__host__ __device__ __forceinline__ Dec &ctl_dec(Ctl *c) { return *reinterpret_cast<Dec *>(c->dec_bytes); }
+
+template <class T>
+__device__ __forceinline__ void st_agent(T *p, T v) {
+  __hip_atomic_store(p, v, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+}
+
+__device__ __forceinline__ void publish(XPart *slot, const XPart &p, unsigned seq) {
+  st_agent(&slot->i0, p.i0); st_agent(&slot->i1, p.i1);
+  st_agent(&slot->v0, p.v0); st_agent(&slot->v1, p.v1); st_agent(&slot->v2, p.v2); st_agent(&slot->v3, p.v3);
+  st_agent(&slot->g0, p.g0); st_agent(&slot->a0, p.a0); st_agent(&slot->g1, p.g1); st_agent(&slot->a1, p.a1);
+  st_agent(&slot->q0, p.q0);
+  st_agent(&slot->qd0, p.qd0); st_agent(&slot->qd1, p.qd1);
+  st_agent(&slot->y0, p.y0); st_agent(&slot->y1, p.y1);
+  // the record's stores complete (are visible at agent scope) before its
+  // sequence number is written
+  __asm__ volatile("s_waitcnt vmcnt(0)" ::: "memory");
+  st_agent(&slot->seq, seq);
+}
+
+template <int KIND, int PH>
+__device__ __forceinline__ XPart xpart_identity() {
+  XPart p;
+  p.seq = 0; p.i0 = p.i1 = -1; p.pad = 0;
+  if (PH == PH_J) { p.v0 = INF; p.v1 = p.v2 = p.v3 = 0; }
+  else { p.v0 = p.v1 = p.v2 = p.v3 = -INF; }
+  p.g0 = p.a0 = p.g1 = p.a1 = 0;
+  p.qd0 = p.qd1 = 0; p.y0 = p.y1 = 0;
+  p.q0 = 0; p.fpad = 0;
+  return p;
+}
+
+struct XSmem {
+  XPart win[2];
+  int widx[2];
+  int err;
+};
+
+// Publish this workgroup's record and gather everyone's: returns, in every
+// thread, the combined record (reduced values; winners' payloads in g0/a0/q0
+// and, for the nu rule's second argmax, g1/a1).
+template <int KIND, int PH>
+__device__ XPart exchange(const View &V, RedSmem &sm, XSmem &xs, const XPart &mine, unsigned seq) {
+  XPart *buf = V.xpart + (size_t)(seq & 1u) * V.nblocks;
   if (threadIdx.x == 0) {
-    sm.gen = ld_acq(&c->gen);
-    __threadfence();
-    unsigned prev = atomicAdd(gcount, 1u);
-    sm.last = (prev == (unsigned)members - 1u);
-    if (sm.last) atomicExch(gcount, 0u);
+    publish(&buf[blockIdx.x], mine, seq);
+    xs.err = 0;
   }
-  __syncthreads();
-  bool top = false;
-  if (sm.last) {
-    __threadfence();
-    Partial gp = reduce_partials<KIND, PH>(V.part, g, members, ng, sm);
-    if (threadIdx.x == 0) {
-      V.gpart[g] = gp;
-      __threadfence();
-      unsigned prev = atomicAdd(&c->arrive, 1u);
-      sm.last = (prev == (unsigned)ng - 1u);
-      if (sm.last) atomicExch(&c->arrive, 0u);
-    }
-    __syncthreads();
-    top = sm.last;
-  }
-  if (top) {
-    __threadfence();
-    Partial t = reduce_partials<KIND, PH>(V.gpart, 0, ng, 1, sm);
-    if (threadIdx.x == 0) {
-      finish(t);
-      __threadfence();
-      atomicAdd(&c->gen, 1u);
-    }
-  } else if (threadIdx.x == 0) {
+  XPart q = xpart_identity<KIND, PH>();
+  const int t = threadIdx.x;
+  if (t < V.nblocks) {
     long long spins = 0;
-    while (ld_acq(&c->gen) == sm.gen) {
+    while (ld_acq(&buf[t].seq) != seq) {
       __builtin_amdgcn_s_sleep(1);
-      if (++spins > SPIN_LIMIT) {
-        atomicExch(&c->err, (int)ERR_TIMEOUT);
-        break;
-      }
+      if (++spins > SPIN_LIMIT) { xs.err = 1; break; }
     }
-    __threadfence();
+    __asm__ volatile("" ::: "memory");
+    q.i0 = ld_acq(&buf[t].i0); q.i1 = ld_acq(&buf[t].i1);
+    q.v0 = ld_acq(&buf[t].v0); q.v1 = ld_acq(&buf[t].v1); q.v2 = ld_acq(&buf[t].v2); q.v3 = ld_acq(&buf[t].v3);
+    q.g0 = ld_acq(&buf[t].g0); q.a0 = ld_acq(&buf[t].a0); q.g1 = ld_acq(&buf[t].g1); q.a1 = ld_acq(&buf[t].a1);
+    q.q0 = ld_acq(&buf[t].q0);
+    q.qd0 = ld_acq(&buf[t].qd0); q.qd1 = ld_acq(&buf[t].qd1);
+    q.y0 = ld_acq(&buf[t].y0); q.y1 = ld_acq(&buf[t].y1);
+  }
+  double v[4] = {q.v0, q.v1, q.v2, q.v3};
+  int ix[2] = {q.i0, q.i1};
+  if (PH == PH_J) {
+    const int kinds[4] = {1, 2, 2, 2};
+    block_reduce<1>(sm, v, ix, kinds);
+  } else if (KIND == KIND_C) {
+    const int kinds[4] = {0, 2, 2, 2};
+    block_reduce<2>(sm, v, ix, kinds);
+  } else {
+    const int kinds[4] = {0, 0, 2, 2};
+    block_reduce<4>(sm, v, ix, kinds);
+  }
+  if (threadIdx.x == 0) {
+    xs.widx[0] = ix[0];
+    xs.widx[1] = ix[1];
+    xs.win[0] = xpart_identity<KIND, PH>();
+    xs.win[1] = xpart_identity<KIND, PH>();
+    xs.win[0].v0 = v[0]; xs.win[0].v1 = v[1]; xs.win[0].v2 = v[2]; xs.win[0].v3 = v[3];
+    xs.win[0].i0 = ix[0]; xs.win[0].i1 = ix[1];
   }
   __syncthreads();
+  // the record holding each winning index supplies its payload
+  if (t < V.nblocks) {
+    if (ix[0] >= 0 && q.i0 == xs.widx[0]) {
+      xs.win[0].g0 = q.g0; xs.win[0].a0 = q.a0; xs.win[0].q0 = q.q0; xs.win[0].qd0 = q.qd0; xs.win[0].y0 = q.y0;
+    }
+    if (KIND == KIND_NU && PH == PH_I && xs.widx[1] >= 0 && q.i1 == xs.widx[1]) {
+      xs.win[0].g1 = q.g1; xs.win[0].a1 = q.a1; xs.win[0].qd1 = q.qd1; xs.win[0].y1 = q.y1;
+    }
+  }
+  __syncthreads();
+  XPart r = xs.win[0];
+  r.seq = xs.err;
+  __syncthreads();
+  return r;
+}
+
+__device__ __forceinline__ CacheMeta private_meta(const View &V, Dec &d) {
+  CacheMeta M;
+  if (V.full) {
+    M.computed = V.pm_flags + (long long)blockIdx.x * V.l;
+  } else {
+    int *base = V.pm_lru + (long long)blockIdx.x * V.pm_stride;
+    M.slot_of_row = base;
+    M.row_of_slot = base + V.l;
+    M.prev = base + V.l + V.S;
+    M.next = base + V.l + 2 * V.S;
+    M.head = base + V.l + 3 * V.S;
+    M.tail = M.head + 1;
+  }
+  M.hits = &d.hits;
+  M.misses = &d.misses;
+  return M;
+}
+
+__device__ __forceinline__ Snap snap_of(const Dec &d) {
+  Snap c;
+  c.i = d.i; c.j = d.j; c.ip = d.ip; c.in = d.in;
+  c.slot_i = d.slot_i; c.slot_j = d.slot_j; c.slot_ip = d.slot_ip; c.slot_in = d.slot_in;
+  c.miss_i = d.miss_i; c.miss_j = d.miss_j; c.miss_ip = d.miss_ip; c.miss_in = d.miss_in;
+  c.stop = d.stop; c.err = d.err; c.iter = d.iter;
+  c.dai = d.dai; c.daj = d.daj; c.Gmax = d.Gmax; c.Gp = d.Gp; c.Gn = d.Gn;
+  return c;
+}
+
+// Decisions after the i exchange (every workgroup, thread 0).
+template <int KIND>
+__device__ void decide_i(const View &V, Dec &d, const XPart &r, bool count_maxiter) {
+  CacheMeta M = private_meta(V, d);
+  int stop = STOP_NONE;
+  if (count_maxiter && d.iter >= V.max_iter) stop = STOP_MAXITER;
+  if (KIND == KIND_C) {
+    d.Gmax = r.v0; d.Gmax2 = r.v1; d.i = r.i0; d.Gi = r.g0; d.ai = r.a0; d.QDi = r.qd0; d.yi = r.y0;
+    if (!stop && r.v0 + r.v1 < V.eps) stop = STOP_OPTIMAL;   // libsvm/svm.cpp:919
+    if (!stop) d.slot_i = cache_lookup(V, M, real_of(V, r.i0), -1, &d.miss_i);
+  } else {
+    d.Gp = r.v0; d.Gn = r.v1; d.Gp2 = r.v2; d.Gn2 = r.v3;
+    d.ip = r.i0; d.in = r.i1;
+    d.Gip = r.g0; d.aip = r.a0; d.Gin = r.g1; d.ain = r.a1;
+    d.QDip = r.qd0; d.yip = r.y0; d.QDin = r.qd1; d.yin = r.y1;
+    if (!stop && fmax(r.v0 + r.v2, r.v1 + r.v3) < V.eps) stop = STOP_OPTIMAL;  // libsvm/svm.cpp:1174
+    if (!stop) {
+      int sp = -1;
+      d.miss_ip = d.miss_in = 0;
+      if (d.ip >= 0) { sp = cache_lookup(V, M, real_of(V, d.ip), -1, &d.miss_ip); d.slot_ip = sp; }
+      if (d.in >= 0) d.slot_in = cache_lookup(V, M, real_of(V, d.in), sp, &d.miss_in);
+    }
+  }
+  d.stop = stop;
+}
+
+// Decisions after the j exchange: the pair, the row-j slot and the alpha
+// step, computed identically in every workgroup (thread 0).
+template <int KIND>
+__device__ void decide_j(const View &V, Dec &d, const XPart &r) {
+  CacheMeta M = private_meta(V, d);
+  const int j = r.i0;
+  if (j < 0) {  // only reachable with eps <= 0 or NaN gradients
+    d.stop = STOP_OPTIMAL;
+    return;
+  }
+  double Gi, ai, QDi;
+  int yi;
+  if (KIND == KIND_C) {
+    Gi = d.Gi; ai = d.ai; QDi = d.QDi; yi = d.yi;
+  } else if (r.y0 == +1) {  // libsvm/svm.cpp:1177-1180
+    d.i = d.ip; d.slot_i = d.slot_ip; Gi = d.Gip; ai = d.aip; QDi = d.QDip; yi = d.yip;
+  } else {
+    d.i = d.in; d.slot_i = d.slot_in; Gi = d.Gin; ai = d.ain; QDi = d.QDin; yi = d.yin;
+  }
+  const int yj = r.y0;
+  d.j = j;
+  d.slot_j = cache_lookup(V, M, real_of(V, j), d.slot_i, &d.miss_j);
+  d.iter += 1;
+  double ai2 = ai, aj2 = r.a0;
+  alpha_pair(yi, yj, yi > 0 ? V.Cp : V.Cn, yj > 0 ? V.Cp : V.Cn, QDi, r.qd0, Gi, r.g0, r.q0, ai2, aj2);
+  d.dai = ai2 - ai;
+  d.daj = aj2 - r.a0;
+  d.ai_new = ai2;
+  d.aj_new = aj2;
+  d.pend = 1;
 }
 
 }  // namespace smo

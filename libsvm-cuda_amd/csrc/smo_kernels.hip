@@ -120,25 +120,129 @@ __global__ void k_status(View V) {
   V.status[c->j] = status_for(V, c->j, V.alpha[c->j]);
 }
 
-// The whole SMO loop on the device.  Per iteration: [select j] barrier
-// [leader: j, row-j slot, alpha step, status] [G update + select i] barrier
-// [leader: i, Gmax/Gmax2, stop test, row-i slot].  Requires co-resident
-// workgroups (cooperative launch).
+// Record of this workgroup's i-pass winner(s) with their G and alpha
+// (thread 0; the pass's block reduction has synchronised the workgroup).
+template <int KIND>
+__device__ XPart make_i_record(const View &V, const Partial &bp) {
+  XPart m = xpart_identity<KIND, PH_I>();
+  m.v0 = bp.v0; m.v1 = bp.v1; m.v2 = bp.v2; m.v3 = bp.v3;
+  m.i0 = bp.i0; m.i1 = bp.i1;
+  if (threadIdx.x == 0) {
+    if (bp.i0 >= 0) { m.g0 = V.G[bp.i0]; m.a0 = V.alpha[bp.i0]; m.qd0 = V.QD[bp.i0]; m.y0 = V.y[bp.i0]; }
+    if (KIND == KIND_NU && bp.i1 >= 0) {
+      m.g1 = V.G[bp.i1]; m.a1 = V.alpha[bp.i1]; m.qd1 = V.QD[bp.i1]; m.y1 = V.y[bp.i1];
+    }
+  }
+  return m;
+}
+
+// Record of this workgroup's j-pass winner with Q_ij, G_j and alpha_j.
+template <int KIND>
+__device__ XPart make_j_record(const View &V, const Snap &c, const Partial &bp) {
+  XPart m = xpart_identity<KIND, PH_J>();
+  m.v0 = bp.v0;
+  m.i0 = bp.i0;
+  if (threadIdx.x == 0 && bp.i0 >= 0) {
+    const int j = bp.i0;
+    int i = c.i, slot = c.slot_i;
+    if (KIND == KIND_NU) {
+      if (V.y[j] == +1) { i = c.ip; slot = c.slot_ip; }
+      else { i = c.in; slot = c.slot_in; }
+    }
+    m.q0 = q_elem(V, slot, i, j);
+    m.g0 = V.G[j];
+    m.a0 = V.alpha[j];
+    m.qd0 = V.QD[j];
+    m.y0 = V.y[j];
+  }
+  return m;
+}
+
+// The whole SMO loop on the device, one iteration per trip:
+//   [select j over owned variables, computing row i on a miss] exchange
+//   [every workgroup: j, row-j slot, alpha step]
+//   [owners store the pair's alpha/status; G update, computing row j on a
+//    miss; select i] exchange
+//   [every workgroup: i, Gmax/Gmax2, stop test, row-i slot]
+// Requires co-resident workgroups (cooperative launch).  State that must
+// survive to the next launch is left in Ctl by workgroup 0.
 template <int KIND>
 __global__ void __launch_bounds__(BLOCK) k_smo(View V, int iter_limit) {
   __shared__ RedSmem sm;
+  __shared__ XSmem xsm;
+  __shared__ Dec d;
   extern __shared__ float xs_lds[];
   float *xs = V.use_lds ? xs_lds : V.xs_global + (long long)blockIdx.x * V.d;
-  Snap c = snap(V.ctl);
+  if (threadIdx.x == 0) d = ctl_dec(V.ctl);
+  __syncthreads();
+  if (d.phase == 0 && !d.stop && !d.err) {
+    Snap c = snap_of(d);
+    Partial bp = pass_gradient_select<KIND>(V, c, xs, sm, false, true, false, nullptr);
+    XPart mine = make_i_record<KIND>(V, bp);
+    const unsigned seq = (unsigned)d.epoch + 1u;
+    XPart r = exchange<KIND, PH_I>(V, sm, xsm, mine, seq);
+    if (threadIdx.x == 0) {
+      d.epoch = (int)seq;
+      d.phase = 1;
+      if (r.seq) d.err = 1;
+      else decide_i<KIND>(V, d, r, true);
+    }
+    __syncthreads();
+  }
   for (int n = 0; n < iter_limit; ++n) {
-    if (c.stop || c.err) break;
-    pass_select_j<KIND>(V, c, xs, sm);
-    grid_reduce_barrier<KIND, PH_J>(V, sm, [&](const Partial &t) { finish_select_j<KIND>(V, t, true); });
-    c = snap(V.ctl);
-    if (c.stop || c.err) break;
-    pass_gradient_select<KIND>(V, c, xs, sm, true, true);
-    grid_reduce_barrier<KIND, PH_I>(V, sm, [&](const Partial &t) { finish_select_i<KIND>(V, t, true); });
-    c = snap(V.ctl);
+    if (d.stop || d.err) break;
+    Snap c = snap_of(d);
+    Partial bp = pass_select_j<KIND>(V, c, xs, sm, false);
+    XPart mine = make_j_record<KIND>(V, c, bp);
+    unsigned seq = (unsigned)d.epoch + 1u;
+    XPart r = exchange<KIND, PH_J>(V, sm, xsm, mine, seq);
+    if (threadIdx.x == 0) {
+      d.epoch = (int)seq;
+      if (r.seq) d.err = 1;
+      else decide_j<KIND>(V, d, r);
+    }
+    __syncthreads();
+    if (d.stop || d.err) break;
+    c = snap_of(d);
+    Pend pd;
+    pd.i = d.i; pd.j = d.j; pd.ai = d.ai_new; pd.aj = d.aj_new;
+    Partial bp2 = pass_gradient_select<KIND>(V, c, xs, sm, true, true, false, &pd);
+    XPart mine2 = make_i_record<KIND>(V, bp2);
+    seq = (unsigned)d.epoch + 1u;
+    XPart r2 = exchange<KIND, PH_I>(V, sm, xsm, mine2, seq);
+    if (threadIdx.x == 0) {
+      d.epoch = (int)seq;
+      d.pend = 0;
+      if (r2.seq) d.err = 1;
+      else decide_i<KIND>(V, d, r2, true);
+    }
+    __syncthreads();
+  }
+  if (blockIdx.x == 0 && threadIdx.x == 0) {
+    Ctl *c = V.ctl;
+    ctl_dec(c) = d;
+    c->iter = d.iter;
+    c->stop = d.stop;
+    c->err = d.err ? (int)ERR_TIMEOUT : 0;
+    c->hits = d.hits;
+    c->misses = d.misses;
+    c->i = d.i; c->j = d.j;
+    c->Gmax = d.Gmax; c->Gmax2 = d.Gmax2;
+  }
+}
+
+// Resets every workgroup's private LRU bookkeeping (persistent kernel).
+__global__ void __launch_bounds__(BLOCK) k_lru_private_init(View V) {
+  int *base = V.pm_lru + (long long)blockIdx.x * V.pm_stride;
+  for (int r = threadIdx.x; r < V.l; r += blockDim.x) base[r] = -1;
+  for (int s = threadIdx.x; s < V.S; s += blockDim.x) {
+    base[V.l + s] = -1;
+    base[V.l + V.S + s] = s - 1;
+    base[V.l + 2 * V.S + s] = s + 1 < V.S ? s + 1 : -1;
+  }
+  if (threadIdx.x == 0) {
+    base[V.l + 3 * V.S] = 0;
+    base[V.l + 3 * V.S + 1] = V.S - 1;
   }
 }
 
@@ -291,6 +395,11 @@ hipError_t smo_launch_gupdate(const View *V, size_t shmem, hipStream_t st) {
 
 hipError_t smo_launch_status(const View *V, hipStream_t st) {
   hipLaunchKernelGGL(smo::k_status, dim3(1), dim3(64), 0, st, *V);
+  return hipGetLastError();
+}
+
+hipError_t smo_launch_lru_private_init(const View *V, hipStream_t st) {
+  hipLaunchKernelGGL(smo::k_lru_private_init, dim3(V->nblocks), dim3(smo::BLOCK), 0, st, *V);
   return hipGetLastError();
 }
 

@@ -30,6 +30,7 @@ hipError_t smo_launch_alpha(const smo::View *V, hipStream_t st);
 hipError_t smo_launch_gupdate(const smo::View *V, size_t shmem, hipStream_t st);
 hipError_t smo_launch_status(const smo::View *V, hipStream_t st);
 hipError_t smo_launch_persistent(const smo::View *V, int iter_limit, size_t shmem, hipStream_t st);
+hipError_t smo_launch_lru_private_init(const smo::View *V, hipStream_t st);
 hipError_t smo_persistent_capacity(int kind, size_t shmem, int *blocks_per_cu);
 hipError_t smo_launch_finalize(const smo::View *V, hipStream_t st);
 hipError_t smo_launch_kernel_row(const smo::View *V, int q, float *out, size_t shmem, hipStream_t st);
@@ -77,9 +78,12 @@ struct svmhip_solver {
   float *d_rows = nullptr, *d_xs = nullptr;
   int *d_slot_of_row = nullptr, *d_row_of_slot = nullptr, *d_prev = nullptr, *d_next = nullptr;
   smo::Ctl *d_ctl = nullptr;
-  smo::Partial *d_part = nullptr, *d_gpart = nullptr;
-  unsigned *d_grp = nullptr;
-  int ngroups = 1;
+  smo::Partial *d_part = nullptr;
+  smo::XPart *d_xpart = nullptr;
+  int8_t *d_pm_flags = nullptr;
+  int *d_pm_lru = nullptr;
+  long long pm_stride = 0;
+  bool shared_meta_dirty = false;  // the persistent kernel overwrote cache slots
   std::vector<void *> allocs;
   svmhip_stats stats{};
   std::string err;
@@ -140,7 +144,7 @@ struct svmhip_solver {
     V.computed = d_computed;
     V.ctl = d_ctl; V.part = d_part; V.xs_global = d_xs;
     V.use_lds = use_lds; V.nblocks = nblocks;
-    V.gpart = d_gpart; V.grp = d_grp; V.ngroups = ngroups;
+    V.xpart = d_xpart; V.pm_flags = d_pm_flags; V.pm_lru = d_pm_lru; V.pm_stride = pm_stride;
     return V;
   }
 
@@ -150,7 +154,8 @@ struct svmhip_solver {
     (void)hipStreamSynchronize(st);
     for (void **p : {(void **)&d_G, (void **)&d_alpha, (void **)&d_p, (void **)&d_QD, (void **)&d_y,
                      (void **)&d_status, (void **)&d_rows, (void **)&d_computed, (void **)&d_slot_of_row,
-                     (void **)&d_row_of_slot, (void **)&d_prev, (void **)&d_next}) {
+                     (void **)&d_row_of_slot, (void **)&d_prev, (void **)&d_next, (void **)&d_pm_flags,
+                     (void **)&d_pm_lru}) {
       dfree(*p);
       *p = nullptr;
     }
@@ -265,6 +270,31 @@ static void build_sell(svmhip_solver *h, const svm_problem *prob) {
   h->ck(hipStreamSynchronize(h->st), "upload");
 }
 
+// Empties the cache bookkeeping the step kernels share (all rows invalid).
+static void reset_shared_cache(svmhip_solver *h) {
+  if (h->full) {
+    h->ck(hipMemsetAsync(h->d_computed, 0, std::max(h->l, 1), h->st), "clear cache flags");
+  } else {
+    std::vector<int> init(std::max(h->l, h->S), -1);
+    h->upload(h->d_slot_of_row, init.data(), h->l, "init cache map");
+    h->upload(h->d_row_of_slot, init.data(), h->S, "init cache map");
+    std::vector<int> prv(h->S), nxt(h->S);
+    for (int s = 0; s < h->S; ++s) {
+      prv[s] = s - 1;
+      nxt[s] = s + 1 < h->S ? s + 1 : -1;
+    }
+    h->upload(h->d_prev, prv.data(), h->S, "init LRU");
+    h->upload(h->d_next, nxt.data(), h->S, "init LRU");
+    h->ck(hipStreamSynchronize(h->st), "init LRU");  // host vectors die here
+  }
+  h->host_ctl->lru_head = 0;
+  h->host_ctl->lru_tail = h->S - 1;
+  h->ck(hipMemcpyAsync(&h->d_ctl->lru_head, &h->host_ctl->lru_head, 2 * sizeof(int), hipMemcpyHostToDevice, h->st),
+        "reset LRU ends");
+  h->ck(hipStreamSynchronize(h->st), "reset cache");
+  h->shared_meta_dirty = false;
+}
+
 static int device_cus(int dev) {
   int cus = 0;
   if (hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, dev) != hipSuccess) cus = 0;
@@ -325,7 +355,11 @@ svmhip_status svmhip_create(const svm_problem *prob, const svm_parameter *param,
     build_sell(h, prob);
     // launch geometry: at most one workgroup per slice, one per CU by default
     int cus = device_cus(h->dev);
-    int want = h->opt.grid_blocks > 0 ? h->opt.grid_blocks : cus;
+    // default: about six 64-row slices per workgroup, 32..128 workgroups
+    // (fewer workgroups make each exchange cheaper, more make row passes
+    // faster; this balance was measured on 20k-100k row problems)
+    int want = h->opt.grid_blocks > 0 ? h->opt.grid_blocks
+                                      : std::min(std::min(128, cus), std::max(32, h->nslices / 6));
     h->nblocks = std::max(1, std::min(want, std::max(h->nslices, 1)));
     size_t qbytes = (size_t)h->d * sizeof(float);
     h->use_lds = qbytes <= kLdsQueryLimit;
@@ -334,14 +368,11 @@ svmhip_status svmhip_create(const svm_problem *prob, const svm_parameter *param,
     h->ck(smo_persistent_capacity((int)kind, h->shmem, &per_cu), "occupancy query");
     int cap = std::max(1, per_cu) * cus;
     h->nblocks = std::min(h->nblocks, cap);
+    h->nblocks = std::min(h->nblocks, 512);  // the exchange reads one record per thread
     if (!h->use_lds) h->d_xs = h->dalloc<float>((size_t)h->nblocks * h->d, "query-row scratch");
     h->d_ctl = h->dalloc<smo::Ctl>(1, "control record");
     h->d_part = h->dalloc<smo::Partial>(2 * (size_t)h->nblocks, "partials");
-    h->ngroups = std::max(1, std::min(32, (int)std::ceil(std::sqrt((double)h->nblocks))));
-    h->d_gpart = h->dalloc<smo::Partial>(h->ngroups, "group partials");
-    h->d_grp = h->dalloc<unsigned>((size_t)h->ngroups * smo::GROUP_STRIDE, "group counters");
-    h->ck(hipMemsetAsync(h->d_grp, 0, sizeof(unsigned) * h->ngroups * smo::GROUP_STRIDE, h->st),
-          "clear group counters");
+    h->d_xpart = h->dalloc<smo::XPart>(2 * (size_t)h->nblocks, "exchange records");
     h->ck(hipMemsetAsync(h->d_ctl, 0, sizeof(smo::Ctl), h->st), "clear control record");
     h->ck(hipStreamSynchronize(h->st), "create");
     GUARD_END
@@ -420,30 +451,19 @@ svmhip_status svmhip_setup(svmhip_solver *h, const signed char *y, const double 
     h->S = (int)want;
   }
   h->d_rows = h->dalloc<float>((size_t)h->S * (size_t)h->row_stride, "kernel-row cache");
-  std::vector<int> init;
   if (h->full) {
     h->d_computed = h->dalloc<int8_t>(std::max(h->l, 1), "cache flags");
-    h->ck(hipMemsetAsync(h->d_computed, 0, std::max(h->l, 1), h->st), "clear cache flags");
+    h->d_pm_flags = h->dalloc<int8_t>((size_t)h->nblocks * std::max(h->l, 1), "private cache flags");
   } else {
     h->d_slot_of_row = h->dalloc<int>(h->l, "cache map");
     h->d_row_of_slot = h->dalloc<int>(h->S, "cache map");
     h->d_prev = h->dalloc<int>(h->S, "LRU links");
     h->d_next = h->dalloc<int>(h->S, "LRU links");
-    init.assign(std::max(h->l, h->S), -1);
-    h->upload(h->d_slot_of_row, init.data(), h->l, "init cache map");
-    h->upload(h->d_row_of_slot, init.data(), h->S, "init cache map");
-    std::vector<int> prv(h->S), nxt(h->S);
-    for (int s = 0; s < h->S; ++s) {
-      prv[s] = s - 1;
-      nxt[s] = s + 1 < h->S ? s + 1 : -1;
-    }
-    h->upload(h->d_prev, prv.data(), h->S, "init LRU");
-    h->upload(h->d_next, nxt.data(), h->S, "init LRU");
-    h->ck(hipStreamSynchronize(h->st), "init LRU");
+    h->pm_stride = ((long long)h->l + 3LL * h->S + 2 + 31) / 32 * 32;
+    h->d_pm_lru = h->dalloc<int>((size_t)h->nblocks * (size_t)h->pm_stride, "private LRU bookkeeping");
   }
   memset(h->host_ctl, 0, sizeof(smo::Ctl));
-  h->host_ctl->lru_head = 0;
-  h->host_ctl->lru_tail = h->S - 1;
+  reset_shared_cache(h);
   int mi = h->opt.max_iter;
   if (mi <= 0) mi = std::max(10000000, h->l > INT_MAX / 100 ? INT_MAX : 100 * h->l);  // svm.cpp:577
   h->host_ctl->max_iter = mi;
@@ -476,6 +496,7 @@ svmhip_status svmhip_select_working_set(svmhip_solver *h, int n_var, int *optima
   if (!h->setup_done) h->fail(SVMHIP_ERR_STATE, "select_working_set before setup");
   if (n_var != h->nvar) h->fail(SVMHIP_ERR_ARG, "n_var mismatch");
   h->ck(hipSetDevice(h->dev), "hipSetDevice");
+  if (h->shared_meta_dirty) reset_shared_cache(h);
   smo::View V = h->view();
   h->ck(smo_launch_select_i(&V, h->shmem, h->st), "select i");
   h->read_ctl();  // the Gmax / Gmax2 copy the reference makes (cuda_solver.cpp:83-86)
@@ -578,12 +599,28 @@ svmhip_status svmhip_solve(svmhip_solver *h, int max_iter, svmhip_stats *stats) 
             "set max_iter");
     }
     h->read_ctl();
+    // persistent-kernel state: start from the current alpha / G / status with
+    // every workgroup's private cache bookkeeping empty
+    {
+      smo::Dec &dc = smo::ctl_dec(h->host_ctl);
+      memset(&dc, 0, sizeof dc);
+      dc.iter = h->host_ctl->iter;
+      h->host_ctl->stop = 0;
+      h->host_ctl->err = 0;
+      h->ck(hipMemcpyAsync(h->d_ctl, h->host_ctl, sizeof(smo::Ctl), hipMemcpyHostToDevice, h->st), "seed control");
+      h->ck(hipMemsetAsync(h->d_xpart, 0, 2 * sizeof(smo::XPart) * h->nblocks, h->st), "clear exchange records");
+      smo::View V0 = h->view();
+      if (h->full)
+        h->ck(hipMemsetAsync(h->d_pm_flags, 0, (size_t)h->nblocks * std::max(h->l, 1), h->st), "clear cache flags");
+      else
+        h->ck(smo_launch_lru_private_init(&V0, h->st), "init private LRU");
+      h->shared_meta_dirty = true;
+    }
     smo::View V = h->view();
     V.max_iter = h->host_ctl->max_iter;
     int chunk = h->opt.iters_per_launch > 0 ? h->opt.iters_per_launch : INT_MAX;
     h->ck(hipEventRecord(h->ev[0], h->st), "event");
     if (h->nvar > 0) {
-      h->ck(smo_launch_select_i(&V, h->shmem, h->st), "select i");
       h->stats.kernel_ms = 0;
       h->stats.launches = 0;
       for (;;) {
