@@ -143,6 +143,53 @@ void svmhip_destroy(svmhip_solver *h);
 
 const char *svmhip_last_error(const svmhip_solver *h);
 
+/* ---------------------------------------------------------------------------
+ * Batched solves.  No counterpart in the reference, which trains the
+ * k(k-1)/2 one-vs-one pairs and every cross-validation fold one after the
+ * other (libsvm/svm.cpp:2246-2295, :2491-2521).  Here many independent
+ * subproblems over row subsets of one problem run in one launch, one
+ * workgroup per subproblem, each with its own alpha/G/kernel-row cache.
+ * ------------------------------------------------------------------------- */
+
+typedef struct svmhip_batch svmhip_batch;
+
+/* One subproblem: a row set of the batch's problem plus one formulation, given
+ * exactly as Solver::Solve receives it (libsvm/svm.cpp:515): n_var = n or 2n
+ * variables (2n: regression, variable v+n uses row v), y = +1/-1 per variable,
+ * p = linear term, alpha = initial point (overwritten with the solution). */
+typedef struct {
+	int rowset;               /* id from svmhip_batch_add_rowset */
+	int kind;                 /* svmhip_kind */
+	int n_var;
+	const signed char *y;     /* [n_var] */
+	const double *p;          /* [n_var] */
+	double *alpha;            /* [n_var] in: initial point, out: solution */
+	double Cp, Cn;
+	double gamma;             /* kernel gamma of this subproblem */
+	double eps;
+	/* outputs */
+	double rho, obj, r;
+	int iters;
+	int status;               /* svmhip_status of this subproblem */
+} svmhip_subproblem;
+
+/* Uploads every row of prob once (SELL-64, float32 features). */
+svmhip_status svmhip_batch_create(const struct svm_problem *prob, const struct svm_parameter *param,
+				  const svmhip_options *opt, svmhip_batch **out);
+
+/* Registers a row set (indices into prob, any order, repeats not allowed);
+ * its rows are packed on the device.  Returns the id (>= 0) or -status. */
+int svmhip_batch_add_rowset(svmhip_batch *b, const int *rows, int n);
+
+/* Solves all subproblems; per-subproblem kernel-row cache of
+ * opt.cache_bytes (or param.cache_size MB).  stats (may be null) sums
+ * iterations and cache counters; solve_ms is the batch's device time. */
+svmhip_status svmhip_batch_solve(svmhip_batch *b, svmhip_subproblem *subs, int nsub, svmhip_stats *stats);
+
+void svmhip_batch_destroy(svmhip_batch *b);
+
+const char *svmhip_batch_last_error(const svmhip_batch *b);
+
 /* Number of usable HIP devices (0 if none); never aborts. */
 int svmhip_device_count(void);
 

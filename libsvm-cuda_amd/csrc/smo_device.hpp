@@ -98,11 +98,28 @@ struct View {
   long long pm_stride;
   float *xs_global;       // per-workgroup query-row scratch when d is too big for LDS
   int use_lds, nblocks;
+  int solo;               // 1 = one workgroup owns the whole problem (batched solves)
+  int pad_solo;
+};
+
+// Kernel-row cache arenas of the batched solver: K arenas, claimed by a
+// workgroup for the lifetime of one subproblem.
+struct BatchPool {
+  float *rows;             // K arenas of arena_floats floats
+  long long arena_floats;
+  int8_t *flags;           // K x flag_stride row flags (full-cache arenas)
+  int *lru;                // K x lru_stride ints (LRU arenas)
+  long long flag_stride, lru_stride;
+  int *busy;               // K claim flags
+  int K, pad;
 };
 
 // ---------------------------------------------------------------- helpers
 
 __device__ __forceinline__ int real_of(const View &V, int v) { return v < V.l ? v : v - V.l; }
+
+// Owner index of this workgroup (0 when one workgroup owns everything).
+__device__ __forceinline__ int my_owner(const View &V) { return V.solo ? 0 : (int)blockIdx.x; }
 
 __device__ __forceinline__ double C_of(const View &V, int v) { return V.y[v] > 0 ? V.Cp : V.Cn; }
 
@@ -490,7 +507,7 @@ template <int KIND>
 __device__ Partial pass_gradient_select(const View &V, const Snap &c, float *xs, RedSmem &sm, bool update,
                                         bool select, bool to_part = true, const Pend *pend = nullptr) {
   int sb, se;
-  owned_slices(V, blockIdx.x, V.nblocks, sb, se);
+  owned_slices(V, my_owner(V), V.nblocks, sb, se);
   if (pend) {
     if (threadIdx.x == 0) {
       if (owns(V, sb, se, pend->i)) { V.alpha[pend->i] = pend->ai; V.status[pend->i] = status_for(V, pend->i, pend->ai); }
@@ -565,7 +582,7 @@ __device__ Partial pass_gradient_select(const View &V, const Snap &c, float *xs,
 template <int KIND>
 __device__ Partial pass_select_j(const View &V, const Snap &c, float *xs, RedSmem &sm, bool to_part = true) {
   int sb, se;
-  owned_slices(V, blockIdx.x, V.nblocks, sb, se);
+  owned_slices(V, my_owner(V), V.nblocks, sb, se);
   if (KIND == KIND_C) {
     if (c.miss_i) compute_row_part(V, real_of(V, c.i), c.slot_i, xs, sb, se);
   } else {
@@ -939,9 +956,9 @@ __device__ XPart exchange(const View &V, RedSmem &sm, XSmem &xs, const XPart &mi
 __device__ __forceinline__ CacheMeta private_meta(const View &V, Dec &d) {
   CacheMeta M;
   if (V.full) {
-    M.computed = V.pm_flags + (long long)blockIdx.x * V.l;
+    M.computed = V.pm_flags + (long long)my_owner(V) * V.l;
   } else {
-    int *base = V.pm_lru + (long long)blockIdx.x * V.pm_stride;
+    int *base = V.pm_lru + (long long)my_owner(V) * V.pm_stride;
     M.slot_of_row = base;
     M.row_of_slot = base + V.l;
     M.prev = base + V.l + V.S;

@@ -339,6 +339,218 @@ __global__ void __launch_bounds__(BLOCK) k_kernel_row(View V, int q, float *out)
   }
 }
 
+// ------------------------------------------------------------ batched solves
+//
+// Many independent subproblems (one-vs-one pairs, cross-validation folds,
+// gamma grid points) in one launch: workgroup b solves subproblem order[b]
+// from start to finish with the same passes as the persistent kernel, but as
+// the only owner of its subproblem, so every decision is local and the only
+// synchronisation is the workgroup barrier.  Kernel-row cache arenas come
+// from a pool of K >= (resident workgroups) arenas; a workgroup claims one
+// with a compare-and-swap and returns it when done.
+
+
+
+__device__ void solo_init_cache(View &V) {
+  if (V.full) {
+    for (int r = threadIdx.x; r < V.l; r += blockDim.x) V.pm_flags[r] = 0;
+  } else {
+    int *base = V.pm_lru;
+    for (int r = threadIdx.x; r < V.l; r += blockDim.x) base[r] = -1;
+    for (int t = threadIdx.x; t < V.S; t += blockDim.x) {
+      base[V.l + t] = -1;
+      base[V.l + V.S + t] = t - 1;
+      base[V.l + 2 * V.S + t] = t + 1 < V.S ? t + 1 : -1;
+    }
+    if (threadIdx.x == 0) {
+      base[V.l + 3 * V.S] = 0;
+      base[V.l + 3 * V.S + 1] = V.S - 1;
+    }
+  }
+}
+
+// rho / obj / r of a subproblem, whole workgroup; thread 0 writes Ctl.
+template <int KIND>
+__device__ void solo_finalize(const View &V) {
+  __shared__ double red[BLOCK / 64][8];
+  __shared__ int redi[BLOCK / 64][2];
+  double s1 = 0, ub1 = INF, lb1 = -INF, s2 = 0, ub2 = INF, lb2 = -INF, ob = 0;
+  int n1 = 0, n2 = 0;
+  for (int v = threadIdx.x; v < V.nvar; v += blockDim.x) {
+    double g = V.G[v];
+    int8_t yv = V.y[v], st = V.status[v];
+    ob += V.alpha[v] * (g + V.p[v]);
+    if (KIND == KIND_C) {
+      double yG = yv * g;
+      if (st == AT_UPPER) {
+        if (yv == -1) ub1 = fmin(ub1, yG); else lb1 = fmax(lb1, yG);
+      } else if (st == AT_LOWER) {
+        if (yv == +1) ub1 = fmin(ub1, yG); else lb1 = fmax(lb1, yG);
+      } else {
+        ++n1;
+        s1 += yG;
+      }
+    } else {
+      if (yv == +1) {
+        if (st == AT_UPPER) lb1 = fmax(lb1, g);
+        else if (st == AT_LOWER) ub1 = fmin(ub1, g);
+        else { ++n1; s1 += g; }
+      } else {
+        if (st == AT_UPPER) lb2 = fmax(lb2, g);
+        else if (st == AT_LOWER) ub2 = fmin(ub2, g);
+        else { ++n2; s2 += g; }
+      }
+    }
+  }
+  const int wave = threadIdx.x / 64, lane = threadIdx.x % 64, nw = blockDim.x / 64;
+  s1 = wave_sum(s1); s2 = wave_sum(s2); ob = wave_sum(ob);
+  lb1 = wave_max(lb1); lb2 = wave_max(lb2);
+  ub1 = -wave_max(-ub1); ub2 = -wave_max(-ub2);
+  n1 = wave_sum_i(n1); n2 = wave_sum_i(n2);
+  if (lane == 0) {
+    red[wave][0] = s1; red[wave][1] = ub1; red[wave][2] = lb1; red[wave][3] = s2;
+    red[wave][4] = ub2; red[wave][5] = lb2; red[wave][6] = ob;
+    redi[wave][0] = n1; redi[wave][1] = n2;
+  }
+  __syncthreads();
+  if (threadIdx.x == 0) {
+    for (int w = 1; w < nw; ++w) {
+      s1 += red[w][0]; ub1 = fmin(ub1, red[w][1]); lb1 = fmax(lb1, red[w][2]);
+      s2 += red[w][3]; ub2 = fmin(ub2, red[w][4]); lb2 = fmax(lb2, red[w][5]);
+      ob += red[w][6]; n1 += redi[w][0]; n2 += redi[w][1];
+    }
+    Ctl *c = V.ctl;
+    c->obj = ob / 2;
+    if (KIND == KIND_C) {
+      c->rho = n1 > 0 ? s1 / n1 : (ub1 + lb1) / 2;
+      c->r = 0;
+    } else {
+      double r1 = n1 > 0 ? s1 / n1 : (ub1 + lb1) / 2;
+      double r2 = n2 > 0 ? s2 / n2 : (ub2 + lb2) / 2;
+      c->r = (r1 + r2) / 2;
+      c->rho = (r1 - r2) / 2;
+    }
+  }
+  __syncthreads();
+}
+
+template <int KIND>
+__global__ void __launch_bounds__(BLOCK) k_smo_batch(const View *views, const int *order, BatchPool pool) {
+  __shared__ RedSmem sm;
+  __shared__ Dec d;
+  __shared__ int s_arena;
+  extern __shared__ float xs_lds[];
+  View V = views[order[blockIdx.x]];
+  float *xs = xs_lds;
+  // claim a cache arena (bounded probing; K exceeds the resident workgroups)
+  if (threadIdx.x == 0) {
+    int a = (int)(blockIdx.x % (unsigned)pool.K), found = -1;
+    for (long long probe = 0; probe < (1LL << 22); ++probe) {
+      if (atomicCAS(&pool.busy[a], 0, 1) == 0) { found = a; break; }
+      a = a + 1 == pool.K ? 0 : a + 1;
+      if (a == 0) __builtin_amdgcn_s_sleep(4);
+    }
+    s_arena = found;
+    __threadfence();
+  }
+  __syncthreads();
+  const int arena = s_arena;
+  if (arena < 0) {
+    if (threadIdx.x == 0) { V.ctl->err = ERR_TIMEOUT; V.ctl->stop = STOP_NONE; }
+    return;
+  }
+  V.rows = pool.rows + (long long)arena * pool.arena_floats;
+  V.row_stride = ((long long)V.l + 63) / 64 * 64;
+  long long fit = pool.arena_floats / (V.row_stride > 0 ? V.row_stride : 1);
+  if (fit >= V.l) { V.full = 1; V.S = V.l > 0 ? V.l : 1; }
+  else { V.full = 0; V.S = (int)(fit < 2 ? 2 : fit); }
+  V.pm_flags = pool.flags + (long long)arena * pool.flag_stride;
+  V.pm_lru = pool.lru + (long long)arena * pool.lru_stride;
+  V.pm_stride = 0;
+  solo_init_cache(V);
+  // QD_v = K(v, v) in FP64 (gamma is per subproblem)
+  for (int v = threadIdx.x; v < V.nvar; v += blockDim.x) {
+    int r = real_of(V, v);
+    V.QD[v] = kernel_f64(V, r, r, V.xsq[r]);
+  }
+  __syncthreads();
+  // G = p + sum_{alpha_v > 0} alpha_v Q_v in increasing v (libsvm/svm.cpp:555-569)
+  const int nv = V.svr ? 2 : 1;
+  for (int v = 0; v < V.nvar; ++v) {
+    if (V.status[v] == AT_LOWER) continue;
+    const int q = real_of(V, v);
+    stage_row(V, q, xs);
+    const double av = V.alpha[v];
+    const int8_t yv = V.y[v];
+    for (int r = threadIdx.x; r < V.l; r += blockDim.x) {
+      // row r of a slice is lane r % 64 of wave r / 64 in the SELL layout
+      float K = kernel_value(V, xs, q, r);
+      for (int h = 0; h < nv; ++h) {
+        int k = r + h * V.l;
+        float Qv = (V.y[k] == yv) ? K : -K;
+        V.G[k] = V.G[k] + av * (double)Qv;
+      }
+    }
+    __syncthreads();
+  }
+  if (threadIdx.x == 0) {
+    memset(&d, 0, sizeof d);
+    d.iter = 0;
+  }
+  __syncthreads();
+  {
+    Snap c = snap_of(d);
+    Partial bp = pass_gradient_select<KIND>(V, c, xs, sm, false, true, false, nullptr);
+    XPart m = make_i_record<KIND>(V, bp);
+    if (threadIdx.x == 0) { d.phase = 1; decide_i<KIND>(V, d, m, true); }
+    __syncthreads();
+  }
+  while (!d.stop) {
+    Snap c = snap_of(d);
+    Partial bp = pass_select_j<KIND>(V, c, xs, sm, false);
+    XPart m = make_j_record<KIND>(V, c, bp);
+    if (threadIdx.x == 0) decide_j<KIND>(V, d, m);
+    __syncthreads();
+    if (d.stop) break;
+    c = snap_of(d);
+    Pend pd;
+    pd.i = d.i; pd.j = d.j; pd.ai = d.ai_new; pd.aj = d.aj_new;
+    Partial bp2 = pass_gradient_select<KIND>(V, c, xs, sm, true, true, false, &pd);
+    XPart m2 = make_i_record<KIND>(V, bp2);
+    if (threadIdx.x == 0) decide_i<KIND>(V, d, m2, true);
+    __syncthreads();
+  }
+  solo_finalize<KIND>(V);
+  if (threadIdx.x == 0) {
+    Ctl *c = V.ctl;
+    c->iter = d.iter;
+    c->stop = d.stop;
+    c->err = 0;
+    c->hits = d.hits;
+    c->misses = d.misses;
+    __threadfence();
+    atomicExch(&pool.busy[arena], 0);
+  }
+}
+
+// Copies the rows of one row set out of the full problem's SELL arrays into
+// the row set's own SELL arrays (slice pointers precomputed on the host).
+__global__ void __launch_bounds__(BLOCK) k_pack_rowset(View src, const int *rows, int n, const long long *dst_slice_ptr,
+                                                      float *dst_vals, int *dst_cols, int *dst_len, double *dst_xsq) {
+  int r = blockIdx.x * blockDim.x + threadIdx.x;
+  if (r >= n) return;
+  const int g = rows[r];
+  const int len = src.row_len[g];
+  const long long sbase = src.slice_ptr[g / SLICE] + (g % SLICE);
+  const long long dbase = dst_slice_ptr[r / SLICE] + (r % SLICE);
+  for (int p = 0; p < len; ++p) {
+    dst_vals[dbase + (long long)p * SLICE] = src.vals[sbase + (long long)p * SLICE];
+    if (!src.dense) dst_cols[dbase + (long long)p * SLICE] = src.cols[sbase + (long long)p * SLICE];
+  }
+  dst_len[r] = len;
+  dst_xsq[r] = src.xsq[g];
+}
+
 }  // namespace smo
 
 // ------------------------------------------------------------ launchers
@@ -423,6 +635,29 @@ hipError_t smo_launch_finalize(const View *V, hipStream_t st) {
     hipLaunchKernelGGL(smo::k_finalize<smo::KIND_C>, dim3(V->nblocks), dim3(smo::BLOCK), 0, st, *V);
   else
     hipLaunchKernelGGL(smo::k_finalize<smo::KIND_NU>, dim3(V->nblocks), dim3(smo::BLOCK), 0, st, *V);
+  return hipGetLastError();
+}
+
+hipError_t smo_launch_batch(int kind, const View *views, const int *order, int count, const smo::BatchPool *pool,
+                            size_t shmem, hipStream_t st) {
+  if (count <= 0) return hipSuccess;
+  if (kind == smo::KIND_C)
+    hipLaunchKernelGGL(smo::k_smo_batch<smo::KIND_C>, dim3(count), dim3(smo::BLOCK), shmem, st, views, order, *pool);
+  else
+    hipLaunchKernelGGL(smo::k_smo_batch<smo::KIND_NU>, dim3(count), dim3(smo::BLOCK), shmem, st, views, order, *pool);
+  return hipGetLastError();
+}
+
+hipError_t smo_batch_capacity(size_t shmem, int *blocks_per_cu) {
+  return hipOccupancyMaxActiveBlocksPerMultiprocessor(blocks_per_cu, (const void *)smo::k_smo_batch<smo::KIND_NU>,
+                                                      smo::BLOCK, shmem);
+}
+
+hipError_t smo_launch_pack_rowset(const View *src, const int *rows, int n, const long long *dst_slice_ptr,
+                                  float *dst_vals, int *dst_cols, int *dst_len, double *dst_xsq, hipStream_t st) {
+  if (n <= 0) return hipSuccess;
+  hipLaunchKernelGGL(smo::k_pack_rowset, dim3((n + smo::BLOCK - 1) / smo::BLOCK), dim3(smo::BLOCK), 0, st, *src, rows, n,
+                     dst_slice_ptr, dst_vals, dst_cols, dst_len, dst_xsq);
   return hipGetLastError();
 }
 

@@ -34,6 +34,11 @@ hipError_t smo_launch_lru_private_init(const smo::View *V, hipStream_t st);
 hipError_t smo_persistent_capacity(int kind, size_t shmem, int *blocks_per_cu);
 hipError_t smo_launch_finalize(const smo::View *V, hipStream_t st);
 hipError_t smo_launch_kernel_row(const smo::View *V, int q, float *out, size_t shmem, hipStream_t st);
+hipError_t smo_launch_batch(int kind, const smo::View *views, const int *order, int count, const smo::BatchPool *pool,
+                            size_t shmem, hipStream_t st);
+hipError_t smo_batch_capacity(size_t shmem, int *blocks_per_cu);
+hipError_t smo_launch_pack_rowset(const smo::View *src, const int *rows, int n, const long long *dst_slice_ptr,
+                                  float *dst_vals, int *dst_cols, int *dst_len, double *dst_xsq, hipStream_t st);
 }
 
 namespace {
@@ -59,6 +64,7 @@ struct svmhip_solver {
   int l = 0, d = 0, nslices = 0, dense = 0, ktype = RBF, degree = 3;
   double gamma = 0, coef0 = 0;
   long long total = 0;
+  std::vector<int> host_row_len;  // SELL row lengths (kept for packing row sets)
   // solver
   int nvar = 0;
   double Cp = 0, Cn = 0;
@@ -266,6 +272,7 @@ static void build_sell(svmhip_solver *h, const svm_problem *prob) {
     h->upload(h->d_serial, serial.data(), (size_t)l, "upload serials");
   }
   h->d_xsq = h->dalloc<double>(l, "squared norms");
+  h->host_row_len = rl;
   // all host vectors die at return: wait for the copies
   h->ck(hipStreamSynchronize(h->st), "upload");
 }
@@ -676,5 +683,260 @@ svmhip_status svmhip_kernel_row(svmhip_solver *h, int i, float *out) {
 }
 
 void svmhip_destroy(svmhip_solver *h) { delete h; }
+
+}  // extern "C"
+
+// ------------------------------------------------------------ batched solves
+
+struct svmhip_batch {
+  svmhip_solver *g = nullptr;  // owns the uploaded problem and every allocation
+  struct Rowset {
+    int n, nslices;
+    long long *slice_ptr;
+    float *vals;
+    int *cols, *len;
+    double *xsq;
+  };
+  std::vector<Rowset> rowsets;
+  double param_cache_mb = 100;
+  std::string err;
+};
+
+extern "C" {
+
+const char *svmhip_batch_last_error(const svmhip_batch *b) {
+  if (!b) return "null batch";
+  return !b->err.empty() ? b->err.c_str() : (b->g ? b->g->err.c_str() : "");
+}
+
+svmhip_status svmhip_batch_create(const svm_problem *prob, const svm_parameter *param, const svmhip_options *opt,
+                                  svmhip_batch **out) {
+  if (!out) return SVMHIP_ERR_ARG;
+  *out = nullptr;
+  if (param && param->kernel_type == PRECOMPUTED) return SVMHIP_ERR_ARG;  // row sets keep no serials
+  svmhip_solver *g = nullptr;
+  svmhip_status st = svmhip_create(prob, param, SVMHIP_KIND_C, opt, &g);
+  if (st != SVMHIP_OK) return st;
+  st = svmhip_setup_rbf(g, prob->l);
+  if (st != SVMHIP_OK) {
+    svmhip_destroy(g);
+    return st;
+  }
+  if (!g->use_lds) {
+    fprintf(stderr, "svmhip_batch_create: %d features exceed the batched solver's LDS query row\n", g->d);
+    svmhip_destroy(g);
+    return SVMHIP_ERR_ARG;
+  }
+  svmhip_batch *b = new (std::nothrow) svmhip_batch;
+  if (!b) {
+    svmhip_destroy(g);
+    return SVMHIP_ERR_NOMEM;
+  }
+  b->g = g;
+  b->param_cache_mb = param->cache_size;
+  *out = b;
+  return SVMHIP_OK;
+}
+
+int svmhip_batch_add_rowset(svmhip_batch *b, const int *rows, int n) {
+  if (!b || !rows || n <= 0) return -(int)SVMHIP_ERR_ARG;
+  svmhip_solver *h = b->g;
+  svmhip_status rc = [&]() -> svmhip_status {
+    GUARD_BEGIN
+    std::vector<char> seen(h->l, 0);
+    for (int r = 0; r < n; ++r) {
+      if (rows[r] < 0 || rows[r] >= h->l) h->fail(SVMHIP_ERR_ARG, "row set: index %d out of range", rows[r]);
+      if (seen[rows[r]]) h->fail(SVMHIP_ERR_ARG, "row set: row %d repeated", rows[r]);
+      seen[rows[r]] = 1;
+    }
+    svmhip_batch::Rowset rs;
+    rs.n = n;
+    rs.nslices = (n + smo::SLICE - 1) / smo::SLICE;
+    std::vector<long long> sp(rs.nslices + 1, 0);
+    for (int s = 0; s < rs.nslices; ++s) {
+      int w = 0;
+      for (int r = s * smo::SLICE; r < std::min(n, (s + 1) * smo::SLICE); ++r) w = std::max(w, h->host_row_len[rows[r]]);
+      sp[s + 1] = sp[s] + (long long)w * smo::SLICE;
+    }
+    rs.slice_ptr = h->dalloc<long long>(sp.size(), "row-set slice pointers");
+    rs.vals = h->dalloc<float>(std::max<long long>(sp.back(), 1), "row-set values");
+    rs.cols = h->dense ? nullptr : h->dalloc<int>(std::max<long long>(sp.back(), 1), "row-set columns");
+    rs.len = h->dalloc<int>(n, "row-set lengths");
+    rs.xsq = h->dalloc<double>(n, "row-set norms");
+    int *d_rows = h->dalloc<int>(n, "row-set indices");
+    h->upload(rs.slice_ptr, sp.data(), sp.size(), "upload row-set slices");
+    h->upload(d_rows, rows, (size_t)n, "upload row-set indices");
+    if (sp.back() > 0)
+      h->ck(hipMemsetAsync(rs.vals, 0, sizeof(float) * sp.back(), h->st), "clear row-set values");
+    smo::View V = h->view();
+    h->ck(smo_launch_pack_rowset(&V, d_rows, n, rs.slice_ptr, rs.vals, rs.cols, rs.len, rs.xsq, h->st), "pack row set");
+    h->ck(hipStreamSynchronize(h->st), "pack row set");
+    h->dfree(d_rows);
+    b->rowsets.push_back(rs);
+    GUARD_END
+  }();
+  if (rc != SVMHIP_OK) return -(int)rc;
+  return (int)b->rowsets.size() - 1;
+}
+
+svmhip_status svmhip_batch_solve(svmhip_batch *b, svmhip_subproblem *subs, int nsub, svmhip_stats *stats) {
+  if (!b || (nsub > 0 && !subs)) return SVMHIP_ERR_ARG;
+  svmhip_solver *h = b->g;
+  svmhip_stats tot{};
+  std::vector<void *> temps;
+  svmhip_status rc = [&]() -> svmhip_status {
+    GUARD_BEGIN
+    h->ck(hipSetDevice(h->dev), "hipSetDevice");
+    long long nv_total = 0;
+    int maxn = 1;
+    for (int k = 0; k < nsub; ++k) {
+      const svmhip_subproblem &sp = subs[k];
+      if (sp.rowset < 0 || sp.rowset >= (int)b->rowsets.size()) h->fail(SVMHIP_ERR_ARG, "subproblem %d: bad row set", k);
+      const int n = b->rowsets[sp.rowset].n;
+      if (sp.n_var != n && sp.n_var != 2 * n) h->fail(SVMHIP_ERR_ARG, "subproblem %d: n_var must be n or 2n", k);
+      if (sp.kind != SVMHIP_KIND_C && sp.kind != SVMHIP_KIND_NU) h->fail(SVMHIP_ERR_ARG, "subproblem %d: bad kind", k);
+      if (!sp.y || !sp.p || !sp.alpha) h->fail(SVMHIP_ERR_ARG, "subproblem %d: null array", k);
+      if (!(sp.Cp > 0) || !(sp.Cn > 0) || !(sp.eps > 0)) h->fail(SVMHIP_ERR_ARG, "subproblem %d: Cp, Cn, eps must be > 0", k);
+      for (int v = 0; v < sp.n_var; ++v)
+        if (sp.y[v] != 1 && sp.y[v] != -1) h->fail(SVMHIP_ERR_ARG, "subproblem %d: y must be +1/-1", k);
+      nv_total += sp.n_var;
+      maxn = std::max(maxn, n);
+    }
+    if (nsub == 0) return SVMHIP_OK;
+    // pooled per-subproblem state
+    auto talloc = [&](size_t bytes, const char *what) {
+      void *p = h->dalloc<char>(bytes, what);
+      temps.push_back(p);
+      return p;
+    };
+    double *dG = (double *)talloc(sizeof(double) * nv_total, "batch G");
+    double *dA = (double *)talloc(sizeof(double) * nv_total, "batch alpha");
+    double *dP = (double *)talloc(sizeof(double) * nv_total, "batch p");
+    double *dQ = (double *)talloc(sizeof(double) * nv_total, "batch QD");
+    int8_t *dY = (int8_t *)talloc(nv_total, "batch y");
+    int8_t *dS = (int8_t *)talloc(nv_total, "batch status");
+    smo::Ctl *dC = (smo::Ctl *)talloc(sizeof(smo::Ctl) * nsub, "batch control");
+    smo::View *dV = (smo::View *)talloc(sizeof(smo::View) * nsub, "batch views");
+    std::vector<double> hA(nv_total), hP(nv_total);
+    std::vector<int8_t> hY(nv_total), hS(nv_total);
+    std::vector<smo::View> views(nsub);
+    long long off = 0;
+    const smo::View base = h->view();
+    for (int k = 0; k < nsub; ++k) {
+      const svmhip_subproblem &sp = subs[k];
+      const svmhip_batch::Rowset &rs = b->rowsets[sp.rowset];
+      for (int v = 0; v < sp.n_var; ++v) {
+        hA[off + v] = sp.alpha[v];
+        hP[off + v] = sp.p[v];
+        hY[off + v] = sp.y[v];
+        double C = sp.y[v] > 0 ? sp.Cp : sp.Cn;
+        hS[off + v] = sp.alpha[v] >= C ? smo::AT_UPPER : (sp.alpha[v] <= 0 ? smo::AT_LOWER : smo::IS_FREE);
+      }
+      smo::View V = base;
+      V.l = rs.n; V.nslices = rs.nslices;
+      V.slice_ptr = rs.slice_ptr; V.vals = rs.vals; V.cols = rs.cols; V.row_len = rs.len; V.xsq = rs.xsq;
+      V.serial = nullptr;
+      V.gamma = sp.gamma;
+      V.nvar = sp.n_var; V.svr = sp.n_var == 2 * rs.n ? 1 : 0; V.kind = sp.kind;
+      V.max_iter = h->opt.max_iter > 0 ? h->opt.max_iter : std::max(10000000, rs.n > INT_MAX / 100 ? INT_MAX : 100 * rs.n);
+      V.Cp = sp.Cp; V.Cn = sp.Cn; V.eps = sp.eps;
+      V.G = dG + off; V.alpha = dA + off; V.p = dP + off; V.QD = dQ + off; V.y = dY + off; V.status = dS + off;
+      V.rows = nullptr; V.S = 0; V.full = 0;
+      V.slot_of_row = V.row_of_slot = V.lru_prev = V.lru_next = nullptr;
+      V.computed = nullptr;
+      V.ctl = dC + k; V.part = nullptr; V.xpart = nullptr; V.xs_global = nullptr;
+      V.use_lds = 1; V.nblocks = 1; V.solo = 1;
+      views[k] = V;
+      off += sp.n_var;
+    }
+    h->upload(dA, hA.data(), nv_total, "upload batch alpha");
+    h->upload(dP, hP.data(), nv_total, "upload batch p");
+    h->upload(dG, hP.data(), nv_total, "upload batch G");
+    h->upload(dY, hY.data(), nv_total, "upload batch y");
+    h->upload(dS, hS.data(), nv_total, "upload batch status");
+    h->upload(dV, views.data(), nsub, "upload batch views");
+    h->ck(hipMemsetAsync(dC, 0, sizeof(smo::Ctl) * nsub, h->st), "clear batch control");
+    // cache arenas: one per resident workgroup
+    int per_cu = 0;
+    h->ck(smo_batch_capacity(h->shmem, &per_cu), "occupancy query");
+    int resident = std::max(1, per_cu) * device_cus(h->dev);
+    int K = std::min(nsub, resident + 8);
+    double want = h->opt.cache_bytes > 0 ? h->opt.cache_bytes : b->param_cache_mb * 1048576.0;
+    long long maxstride = ((long long)maxn + 63) / 64 * 64;
+    long long arena = std::max<long long>((long long)(want / sizeof(float)), 2 * maxstride);
+    size_t freeb = 0, totb = 0;
+    if (hipMemGetInfo(&freeb, &totb) == hipSuccess) {
+      long long cap = (long long)(0.7 * (double)freeb / sizeof(float)) / K;
+      arena = std::max<long long>(std::min(arena, cap), 2 * maxstride);
+    }
+    smo::BatchPool pool;
+    pool.K = K;
+    pool.pad = 0;
+    pool.arena_floats = arena;
+    pool.rows = (float *)talloc(sizeof(float) * (size_t)arena * K, "batch row-cache arenas");
+    pool.flag_stride = maxn;
+    pool.flags = (int8_t *)talloc((size_t)maxn * K, "batch cache flags");
+    pool.lru_stride = 4LL * maxn + 2;
+    pool.lru = (int *)talloc(sizeof(int) * (size_t)pool.lru_stride * K, "batch LRU bookkeeping");
+    pool.busy = (int *)talloc(sizeof(int) * K, "batch arena claims");
+    h->ck(hipMemsetAsync(pool.busy, 0, sizeof(int) * K, h->st), "clear arena claims");
+    // largest subproblems first (longest-processing-time order), one launch per rule
+    int *dOrd = (int *)talloc(sizeof(int) * nsub, "batch order");
+    std::vector<int> ord;
+    std::vector<int> count_kind(2, 0);
+    for (int kind = 0; kind < 2; ++kind) {
+      std::vector<int> o;
+      for (int k = 0; k < nsub; ++k)
+        if (subs[k].kind == kind) o.push_back(k);
+      std::stable_sort(o.begin(), o.end(), [&](int a, int c) { return subs[a].n_var > subs[c].n_var; });
+      count_kind[kind] = (int)o.size();
+      ord.insert(ord.end(), o.begin(), o.end());
+    }
+    h->upload(dOrd, ord.data(), ord.size(), "upload batch order");
+    h->ck(hipEventRecord(h->ev[0], h->st), "event");
+    h->ck(smo_launch_batch(smo::KIND_C, dV, dOrd, count_kind[0], &pool, h->shmem, h->st), "batched SMO kernel");
+    h->ck(smo_launch_batch(smo::KIND_NU, dV, dOrd + count_kind[0], count_kind[1], &pool, h->shmem, h->st),
+          "batched SMO kernel");
+    h->ck(hipEventRecord(h->ev[1], h->st), "event");
+    std::vector<smo::Ctl> hC(nsub);
+    h->ck(hipMemcpyAsync(hA.data(), dA, sizeof(double) * nv_total, hipMemcpyDeviceToHost, h->st), "fetch batch alpha");
+    h->ck(hipMemcpyAsync(hC.data(), dC, sizeof(smo::Ctl) * nsub, hipMemcpyDeviceToHost, h->st), "fetch batch control");
+    h->ck(hipStreamSynchronize(h->st), "batched solve");
+    float ms = 0;
+    (void)hipEventElapsedTime(&ms, h->ev[0], h->ev[1]);
+    tot.solve_ms = ms;
+    tot.kernel_ms = ms;
+    tot.launches = (count_kind[0] > 0) + (count_kind[1] > 0);
+    tot.optimal = 1;
+    off = 0;
+    for (int k = 0; k < nsub; ++k) {
+      svmhip_subproblem &sp = subs[k];
+      for (int v = 0; v < sp.n_var; ++v) sp.alpha[v] = hA[off + v];
+      off += sp.n_var;
+      sp.rho = hC[k].rho;
+      sp.obj = hC[k].obj;
+      sp.r = hC[k].r;
+      sp.iters = hC[k].iter;
+      sp.status = hC[k].err ? SVMHIP_ERR_TIMEOUT : (hC[k].stop == smo::STOP_OPTIMAL ? SVMHIP_OK : SVMHIP_ERR_MAXITER);
+      tot.iters += hC[k].iter;
+      tot.row_hits += (long long)hC[k].hits;
+      tot.row_misses += (long long)hC[k].misses;
+      if (hC[k].stop != smo::STOP_OPTIMAL) tot.optimal = 0;
+      if (hC[k].err) h->fail(SVMHIP_ERR_TIMEOUT, "subproblem %d: cache arena unavailable", k);
+    }
+    tot.cache_slots = (int)std::min<long long>(INT_MAX, arena / maxstride);
+    tot.grid_blocks = resident;
+    GUARD_END
+  }();
+  for (void *p : temps) h->dfree(p);
+  if (stats) *stats = tot;
+  return rc;
+}
+
+void svmhip_batch_destroy(svmhip_batch *b) {
+  if (!b) return;
+  svmhip_destroy(b->g);
+  delete b;
+}
 
 }  // extern "C"

@@ -46,6 +46,14 @@ class svmhip_stats(C.Structure):
         return {k: getattr(self, k) for k, _ in self._fields_}
 
 
+class svmhip_subproblem(C.Structure):
+    _fields_ = [("rowset", C.c_int), ("kind", C.c_int), ("n_var", C.c_int),
+                ("y", C.POINTER(C.c_byte)), ("p", C.POINTER(C.c_double)),
+                ("alpha", C.POINTER(C.c_double)), ("Cp", C.c_double), ("Cn", C.c_double),
+                ("gamma", C.c_double), ("eps", C.c_double), ("rho", C.c_double),
+                ("obj", C.c_double), ("r", C.c_double), ("iters", C.c_int), ("status", C.c_int)]
+
+
 # every extern "C" symbol declared in include/svm.h and include/svm_hip.h
 SVM_H_SYMBOLS = [
     "svm_train", "svm_cross_validation", "svm_save_model", "svm_load_model", "svm_get_svm_type",
@@ -58,7 +66,8 @@ SVM_HIP_SYMBOLS = [
     "svmhip_create", "svmhip_setup_rbf", "svmhip_setup", "svmhip_select_working_set",
     "svmhip_compute_alpha", "svmhip_update_gradient", "svmhip_update_alpha_status", "svmhip_fetch",
     "svmhip_solve", "svmhip_finalize", "svmhip_get_stats", "svmhip_kernel_row", "svmhip_destroy",
-    "svmhip_last_error", "svmhip_device_count"]
+    "svmhip_last_error", "svmhip_device_count", "svmhip_batch_create", "svmhip_batch_add_rowset",
+    "svmhip_batch_solve", "svmhip_batch_destroy", "svmhip_batch_last_error"]
 
 _lib = None
 PRINT_FUNC = C.CFUNCTYPE(None, C.c_char_p)
@@ -109,6 +118,16 @@ def lib():
     L.svmhip_last_error.argtypes = [h]
     L.svmhip_last_error.restype = C.c_char_p
     L.svmhip_device_count.restype = I
+    L.svmhip_batch_create.argtypes = [P(ffi.svm_problem), P(ffi.svm_parameter), P(svmhip_options), P(h)]
+    L.svmhip_batch_create.restype = I
+    L.svmhip_batch_add_rowset.argtypes = [h, P(I), I]
+    L.svmhip_batch_add_rowset.restype = I
+    L.svmhip_batch_solve.argtypes = [h, P(svmhip_subproblem), I, P(svmhip_stats)]
+    L.svmhip_batch_solve.restype = I
+    L.svmhip_batch_destroy.argtypes = [h]
+    L.svmhip_batch_destroy.restype = None
+    L.svmhip_batch_last_error.argtypes = [h]
+    L.svmhip_batch_last_error.restype = C.c_char_p
     for name in ("svmhip_create", "svmhip_setup_rbf", "svmhip_setup", "svmhip_select_working_set",
                  "svmhip_compute_alpha", "svmhip_update_gradient", "svmhip_update_alpha_status",
                  "svmhip_fetch", "svmhip_solve", "svmhip_finalize", "svmhip_get_stats",
@@ -368,3 +387,81 @@ def svm_load_model(path):
 def check_parameter(problem: Problem, param):
     r = lib().svm_check_parameter(C.byref(problem.struct), C.byref(param))
     return None if r is None else r.decode()
+
+
+class BatchSolver:
+    """Many subproblems over row subsets of one uploaded problem, solved in
+    one launch (one workgroup each): the svmhip_batch_* C-ABI."""
+
+    def __init__(self, problem: Problem, param, cache_bytes=0.0, max_iter=0, device=0):
+        L = lib()
+        self.problem = problem
+        self.param = param
+        opt = svmhip_options(C.sizeof(svmhip_options), device, 0, 0, max_iter, float(cache_bytes), 0, 0)
+        self._h = C.c_void_p()
+        rc = L.svmhip_batch_create(C.byref(problem.struct), C.byref(param), C.byref(opt), C.byref(self._h))
+        if rc != 0:
+            raise SolverError(f"svmhip_batch_create failed: {STATUS_NAMES.get(rc, rc)}")
+
+    def add_rowset(self, rows):
+        rows = np.ascontiguousarray(rows, dtype=np.int32)
+        rid = lib().svmhip_batch_add_rowset(self._h, _ptr(rows, C.c_int), len(rows))
+        if rid < 0:
+            raise SolverError(f"add_rowset: {STATUS_NAMES.get(-rid, -rid)}: "
+                              f"{lib().svmhip_batch_last_error(self._h).decode()}")
+        return rid
+
+    def solve(self, subs):
+        """subs: list of dicts with rowset, kind, y, p, alpha0, Cp, Cn, gamma, eps.
+        Returns (list of result dicts with alpha/rho/obj/r/iters/status, stats)."""
+        n = len(subs)
+        arr = (svmhip_subproblem * max(n, 1))()
+        keep = []
+        for k, sp in enumerate(subs):
+            y = np.ascontiguousarray(sp["y"], dtype=np.int8)
+            p = np.ascontiguousarray(sp["p"], dtype=np.float64)
+            a = np.array(sp["alpha0"], dtype=np.float64)
+            keep += [y, p, a]
+            arr[k] = svmhip_subproblem(sp["rowset"], sp["kind"], len(y), _ptr(y, C.c_byte),
+                                       _ptr(p, C.c_double), _ptr(a, C.c_double), sp["Cp"], sp["Cn"],
+                                       sp["gamma"], sp["eps"], 0.0, 0.0, 0.0, 0, 0)
+        st = svmhip_stats()
+        rc = lib().svmhip_batch_solve(self._h, arr, n, C.byref(st))
+        if rc != 0:
+            raise SolverError(f"batch_solve: {STATUS_NAMES.get(rc, rc)}: "
+                              f"{lib().svmhip_batch_last_error(self._h).decode()}")
+        out = []
+        for k in range(n):
+            out.append(dict(alpha_raw=keep[3 * k + 2], rho=arr[k].rho, obj=arr[k].obj, r=arr[k].r,
+                            iters=arr[k].iters, status=arr[k].status))
+        return out, st.as_dict()
+
+    def close(self):
+        if self._h:
+            lib().svmhip_batch_destroy(self._h)
+            self._h = C.c_void_p()
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+
+def finish_solution(problem_y, param, res, y, r=None):
+    """Signed coefficients, rho and obj from a raw solver result, as the
+    formulations of libsvm/svm.cpp:1500-1696 post-process them."""
+    a = res["alpha_raw"]
+    rho, obj, rr = res["rho"], res["obj"], res["r"]
+    st = param.svm_type
+    l = len(problem_y)
+    if st == C_SVC:
+        coef = a * y
+    elif st == NU_SVC:
+        coef = a * y / rr
+        rho, obj = rho / rr, obj / (rr * rr)
+    elif st == ONE_CLASS:
+        coef = a.copy()
+    else:
+        coef = a[:l] - a[l:]
+    return coef, rho, obj
