@@ -190,6 +190,16 @@ void svmhip_batch_destroy(svmhip_batch *b);
 
 const char *svmhip_batch_last_error(const svmhip_batch *b);
 
+/* Decision values of n rows against a trained model on the device: the
+ * batched form of svm_predict_values (libsvm/svm.cpp:2569-2643), which the
+ * reference only has on the CPU.  dec_values receives n x npair values
+ * (npair = nr_class*(nr_class-1)/2, or 1 for one-class / regression),
+ * labels (may be null) the predicted label per row as svm_predict returns
+ * it.  Kernel values use FP64 arithmetic on float32 features; PRECOMPUTED
+ * kernels and rows wider than 16384 features return SVMHIP_ERR_ARG. */
+svmhip_status svmhip_predict_batch(const struct svm_model *model, const struct svm_node *const *x, int n,
+				   const svmhip_options *opt, double *dec_values, double *labels);
+
 /* Number of usable HIP devices (0 if none); never aborts. */
 int svmhip_device_count(void);
 

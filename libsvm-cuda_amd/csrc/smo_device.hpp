@@ -1039,4 +1039,17 @@ __device__ void decide_j(const View &V, Dec &d, const XPart &r) {
   d.pend = 1;
 }
 
+// Arguments of the batched prediction kernel.
+struct PredictArgs {
+  View sv;                 // support vectors (l = nSV), kernel parameters
+  View tv;                 // test rows
+  const double *coef;      // [(k-1) x nsv]
+  const double *rho;       // [npair]
+  const int *cls_start;    // [k] first SV of each class (classification)
+  const int *cls_count;    // [k]
+  int k, npair, tb, regression_like;
+  double *kv;              // scratch: gridDim x tb x nsv
+  double *dec;             // [ntest x npair]
+};
+
 }  // namespace smo

@@ -551,6 +551,93 @@ __global__ void __launch_bounds__(BLOCK) k_pack_rowset(View src, const int *rows
   dst_xsq[r] = src.xsq[g];
 }
 
+// ------------------------------------------------------------ prediction
+//
+// Decision values of many rows against one model (svm_predict_values,
+// libsvm/svm.cpp:2569-2643).  Workgroup b takes test rows [b*TB, b*TB+TB):
+// it stages them densely in LDS, every lane walks one support vector's row
+// (SELL, coalesced) against all TB staged rows with FP64 accumulation, the
+// kernel values go to the workgroup's scratch, and each one-vs-one decision
+// value is then summed by one wave in a fixed order (deterministic).
+
+
+
+__global__ void __launch_bounds__(BLOCK) k_predict(PredictArgs A, int first_row) {
+  extern __shared__ float xt[];  // tb x d
+  const View &S = A.sv;
+  const View &T = A.tv;
+  const int d = S.d > T.d ? S.d : T.d;
+  const int t0 = first_row + blockIdx.x * A.tb;
+  const int nt = min(A.tb, T.l - t0);
+  for (int e = threadIdx.x; e < A.tb * d; e += blockDim.x) xt[e] = 0.0f;
+  __syncthreads();
+  for (int t = 0; t < nt; ++t) {
+    const int q = t0 + t;
+    const int len = T.row_len[q];
+    const long long base = T.slice_ptr[q / SLICE] + (q % SLICE);
+    for (int p = threadIdx.x; p < len; p += blockDim.x) {
+      int c = T.dense ? p : T.cols[base + (long long)p * SLICE];
+      if (c < d) xt[t * d + c] = T.vals[base + (long long)p * SLICE];
+    }
+  }
+  __syncthreads();
+  double *kv = A.kv + (long long)blockIdx.x * A.tb * S.l;
+  const int wave = threadIdx.x / 64, lane = threadIdx.x % 64, nw = blockDim.x / 64;
+  constexpr int TBMAX = 16;
+  for (int s0 = wave * SLICE; s0 < S.nslices * SLICE; s0 += nw * SLICE) {
+    const int r = s0 + lane;
+    if (r >= S.l) continue;
+    double acc[TBMAX];
+#pragma unroll
+    for (int t = 0; t < TBMAX; ++t) acc[t] = 0.0;
+    const int len = S.row_len[r];
+    const long long base = S.slice_ptr[r / SLICE] + (r % SLICE);
+    for (int p = 0; p < len; ++p) {
+      const double v = (double)S.vals[base + (long long)p * SLICE];
+      const int c = S.dense ? p : S.cols[base + (long long)p * SLICE];
+      if (c >= d) continue;
+#pragma unroll
+      for (int t = 0; t < TBMAX; ++t)
+        if (t < nt) acc[t] = fma(v, (double)xt[t * d + c], acc[t]);
+    }
+#pragma unroll
+    for (int t = 0; t < TBMAX; ++t) {
+      if (t >= nt) break;
+      const double dot = acc[t];
+      double K;
+      switch (S.ktype) {
+        case K_LINEAR: K = dot; break;
+        case K_POLY: K = powi(S.gamma * dot + S.coef0, S.degree); break;
+        case K_RBF: K = exp(-S.gamma * ((T.xsq[t0 + t] + S.xsq[r]) - 2 * dot)); break;
+        case K_SIGMOID: K = tanh(S.gamma * dot + S.coef0); break;
+        default: K = 0.0;
+      }
+      kv[(long long)t * S.l + r] = K;
+    }
+  }
+  __syncthreads();
+  // decision values: one wave per (test row, pair)
+  for (int job = wave; job < nt * A.npair; job += nw) {
+    const int t = job / A.npair, pr = job % A.npair;
+    const double *kt = kv + (long long)t * S.l;
+    double sum = 0.0;
+    if (A.regression_like) {
+      for (int s = lane; s < S.l; s += 64) sum += A.coef[s] * kt[s];
+    } else {
+      // pair index pr -> classes (a, b), a < b, in LIBSVM's order
+      int a = 0, rem = pr;
+      while (rem >= A.k - 1 - a) { rem -= A.k - 1 - a; ++a; }
+      const int b = a + 1 + rem;
+      const double *ca = A.coef + (long long)(b - 1) * S.l;
+      const double *cb = A.coef + (long long)a * S.l;
+      for (int s = A.cls_start[a] + lane; s < A.cls_start[a] + A.cls_count[a]; s += 64) sum += ca[s] * kt[s];
+      for (int s = A.cls_start[b] + lane; s < A.cls_start[b] + A.cls_count[b]; s += 64) sum += cb[s] * kt[s];
+    }
+    sum = wave_sum(sum);
+    if (lane == 0) A.dec[(long long)(t0 + t) * A.npair + pr] = sum - A.rho[pr];
+  }
+}
+
 }  // namespace smo
 
 // ------------------------------------------------------------ launchers
@@ -658,6 +745,13 @@ hipError_t smo_launch_pack_rowset(const View *src, const int *rows, int n, const
   if (n <= 0) return hipSuccess;
   hipLaunchKernelGGL(smo::k_pack_rowset, dim3((n + smo::BLOCK - 1) / smo::BLOCK), dim3(smo::BLOCK), 0, st, *src, rows, n,
                      dst_slice_ptr, dst_vals, dst_cols, dst_len, dst_xsq);
+  return hipGetLastError();
+}
+
+hipError_t smo_launch_predict_offset(const smo::PredictArgs *A, int first_row, int nblocks, size_t shmem,
+                                    hipStream_t st) {
+  if (nblocks <= 0) return hipSuccess;
+  hipLaunchKernelGGL(smo::k_predict, dim3(nblocks), dim3(smo::BLOCK), shmem, st, *A, first_row);
   return hipGetLastError();
 }
 

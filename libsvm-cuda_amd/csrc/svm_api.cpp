@@ -22,6 +22,7 @@
 #include <string.h>
 
 #include <algorithm>
+#include <map>
 #include <string>
 #include <vector>
 
@@ -67,139 +68,266 @@ struct Solution {
   int iters = 0;
 };
 
-// Runs the device solver on one formulation: the part of Solver::Solve
-// (libsvm/svm.cpp:515-830) that the reference hands to CudaSolver.
-void device_solve(const svm_problem *prob, const svm_parameter *param, svmhip_kind kind, int n_var,
-                  const std::vector<schar> &y, const std::vector<double> &p, std::vector<double> &alpha, double Cp,
-                  double Cn, Solution &out) {
-  svmhip_options opt;
-  memset(&opt, 0, sizeof opt);
-  opt.struct_size = sizeof opt;
-  const char *e = getenv("SVMHIP_VERBOSE");
-  opt.verbose = e && *e == '1';
-  svmhip_solver *h = nullptr;
-  svmhip_status s = svmhip_create(prob, param, kind, &opt, &h);
-  if (s != SVMHIP_OK) {
-    fprintf(stderr, "ERROR: device solver unavailable (status %d)\n", (int)s);
-    throw DeviceFailure{s};
-  }
-  std::vector<char> status(n_var);
-  for (int v = 0; v < n_var; ++v) {
-    double C = y[v] > 0 ? Cp : Cn;
-    status[v] = alpha[v] >= C ? 1 : (alpha[v] <= 0 ? 0 : 2);  // Solver::update_alpha_status
-  }
-  svmhip_stats st;
-  memset(&st, 0, sizeof st);
-  if ((s = svmhip_setup_rbf(h, prob->l)) != SVMHIP_OK ||
-      (s = svmhip_setup(h, y.data(), p.data(), alpha.data(), status.data(), Cp, Cn, n_var)) != SVMHIP_OK ||
-      (s = svmhip_solve(h, 0, &st)) != SVMHIP_OK ||
-      (s = svmhip_fetch(h, nullptr, alpha.data(), nullptr, n_var)) != SVMHIP_OK) {
-    fprintf(stderr, "ERROR: device solver failed (status %d): %s\n", (int)s, svmhip_last_error(h));
-    svmhip_destroy(h);
-    throw DeviceFailure{s};
-  }
-  svmhip_destroy(h);
-  if (!st.optimal) fprintf(stderr, "\nWARNING: reaching max number of iterations\n");
-  out.rho = st.rho;
-  out.obj = st.obj;
-  out.r = st.r;
-  out.iters = st.iters;
-  out.ub_p = Cp;
-  out.ub_n = Cn;
-  info("\noptimization finished, #iter = %d\n", st.iters);
-}
+// One subproblem: rows of the caller's problem (in solver order), their
+// labels / targets, and the formulation Solver::Solve receives.
+struct Job {
+  std::vector<int> rows;
+  std::vector<double> ylab;
+  double Cp = 0, Cn = 0;
+  svmhip_kind kind = SVMHIP_KIND_C;
+  int n_var = 0;
+  std::vector<schar> y;
+  std::vector<double> p, alpha;
+  double fCp = 0, fCn = 0;
+  Solution sol;
+};
 
-// Formulations, libsvm/svm.cpp:1500-1696.
-void train_one(const svm_problem *prob, const svm_parameter *param, double Cp, double Cn, Solution &out) {
-  const int l = prob->l;
-  out.alpha.assign(l, 0.0);
+// Formulations, libsvm/svm.cpp:1500-1696: y, p, the initial alpha and the
+// bounds of each svm_type.
+void formulate(const svm_parameter *param, Job &j) {
+  const int l = (int)j.rows.size();
   switch (param->svm_type) {
-    case C_SVC: {
-      std::vector<schar> y(l);
-      std::vector<double> a(l, 0.0), p(l, -1.0);
-      for (int i = 0; i < l; ++i) y[i] = prob->y[i] > 0 ? +1 : -1;
-      device_solve(prob, param, SVMHIP_KIND_C, l, y, p, a, Cp, Cn, out);
-      double sum = 0;
-      for (int i = 0; i < l; ++i) sum += a[i];
-      if (Cp == Cn) info("nu = %f\n", sum / (Cp * l));
-      for (int i = 0; i < l; ++i) out.alpha[i] = a[i] * y[i];
+    case C_SVC:
+      j.kind = SVMHIP_KIND_C;
+      j.n_var = l;
+      j.y.resize(l);
+      for (int i = 0; i < l; ++i) j.y[i] = j.ylab[i] > 0 ? +1 : -1;
+      j.p.assign(l, -1.0);
+      j.alpha.assign(l, 0.0);
+      j.fCp = j.Cp;
+      j.fCn = j.Cn;
+      break;
+    case NU_SVC: {
+      j.kind = SVMHIP_KIND_NU;
+      j.n_var = l;
+      j.y.resize(l);
+      for (int i = 0; i < l; ++i) j.y[i] = j.ylab[i] > 0 ? +1 : -1;
+      double pos = param->nu * l / 2, neg = param->nu * l / 2;
+      j.alpha.resize(l);
+      for (int i = 0; i < l; ++i) {
+        if (j.y[i] == +1) { j.alpha[i] = std::min(1.0, pos); pos -= j.alpha[i]; }
+        else { j.alpha[i] = std::min(1.0, neg); neg -= j.alpha[i]; }
+      }
+      j.p.assign(l, 0.0);
+      j.fCp = j.fCn = 1.0;
       break;
     }
-    case NU_SVC: {
-      std::vector<schar> y(l);
-      for (int i = 0; i < l; ++i) y[i] = prob->y[i] > 0 ? +1 : -1;
-      double pos = param->nu * l / 2, neg = param->nu * l / 2;
-      std::vector<double> a(l), zeros(l, 0.0);
+    case ONE_CLASS: {
+      j.kind = SVMHIP_KIND_C;
+      j.n_var = l;
+      int n = (int)(param->nu * l);
+      j.alpha.assign(l, 0.0);
+      for (int i = 0; i < n; ++i) j.alpha[i] = 1;
+      if (n < l) j.alpha[n] = param->nu * l - n;
+      j.y.assign(l, 1);
+      j.p.assign(l, 0.0);
+      j.fCp = j.fCn = 1.0;
+      break;
+    }
+    case EPSILON_SVR:
+      j.kind = SVMHIP_KIND_C;
+      j.n_var = 2 * l;
+      j.alpha.assign(2 * l, 0.0);
+      j.p.resize(2 * l);
+      j.y.resize(2 * l);
       for (int i = 0; i < l; ++i) {
-        if (y[i] == +1) { a[i] = std::min(1.0, pos); pos -= a[i]; }
-        else { a[i] = std::min(1.0, neg); neg -= a[i]; }
+        j.p[i] = param->p - j.ylab[i];
+        j.y[i] = 1;
+        j.p[i + l] = param->p + j.ylab[i];
+        j.y[i + l] = -1;
       }
-      device_solve(prob, param, SVMHIP_KIND_NU, l, y, zeros, a, 1.0, 1.0, out);
-      double r = out.r;
+      j.fCp = j.fCn = param->C;
+      break;
+    case NU_SVR: {
+      j.kind = SVMHIP_KIND_NU;
+      j.n_var = 2 * l;
+      const double C = param->C;
+      double sum = C * param->nu * l / 2;
+      j.alpha.resize(2 * l);
+      j.p.resize(2 * l);
+      j.y.resize(2 * l);
+      for (int i = 0; i < l; ++i) {
+        j.alpha[i] = j.alpha[i + l] = std::min(sum, C);
+        sum -= j.alpha[i];
+        j.p[i] = -j.ylab[i];
+        j.y[i] = 1;
+        j.p[i + l] = j.ylab[i];
+        j.y[i + l] = -1;
+      }
+      j.fCp = j.fCn = C;
+      break;
+    }
+  }
+}
+
+// Post-processing of the solver output per svm_type (libsvm/svm.cpp:1520-1696)
+// and the summary lines of svm_train_one (:1739-1766).
+void finish(const svm_parameter *param, Job &j, double rho, double obj, double r, int iters, bool optimal) {
+  const int l = (int)j.rows.size();
+  Solution &out = j.sol;
+  out.alpha.assign(l, 0.0);
+  out.rho = rho;
+  out.obj = obj;
+  out.r = r;
+  out.iters = iters;
+  out.ub_p = j.fCp;
+  out.ub_n = j.fCn;
+  if (!optimal) fprintf(stderr, "\nWARNING: reaching max number of iterations\n");
+  info("\noptimization finished, #iter = %d\n", iters);
+  switch (param->svm_type) {
+    case C_SVC: {
+      double sum = 0;
+      for (int i = 0; i < l; ++i) sum += j.alpha[i];
+      if (j.Cp == j.Cn) info("nu = %f\n", sum / (j.Cp * l));
+      for (int i = 0; i < l; ++i) out.alpha[i] = j.alpha[i] * j.y[i];
+      break;
+    }
+    case NU_SVC:
       info("C = %f\n", 1 / r);
-      for (int i = 0; i < l; ++i) out.alpha[i] = a[i] * y[i] / r;
+      for (int i = 0; i < l; ++i) out.alpha[i] = j.alpha[i] * j.y[i] / r;
       out.rho /= r;
       out.obj /= (r * r);
       out.ub_p = out.ub_n = 1 / r;
       break;
-    }
-    case ONE_CLASS: {
-      int n = (int)(param->nu * l);
-      std::vector<double> a(l, 0.0), zeros(l, 0.0);
-      std::vector<schar> ones(l, 1);
-      for (int i = 0; i < n; ++i) a[i] = 1;
-      if (n < l) a[n] = param->nu * l - n;
-      device_solve(prob, param, SVMHIP_KIND_C, l, ones, zeros, a, 1.0, 1.0, out);
-      out.alpha = a;
+    case ONE_CLASS:
+      for (int i = 0; i < l; ++i) out.alpha[i] = j.alpha[i];
       break;
-    }
     case EPSILON_SVR: {
-      std::vector<double> a2(2 * l, 0.0), lin(2 * l);
-      std::vector<schar> y(2 * l);
-      for (int i = 0; i < l; ++i) {
-        lin[i] = param->p - prob->y[i];
-        y[i] = 1;
-        lin[i + l] = param->p + prob->y[i];
-        y[i + l] = -1;
-      }
-      device_solve(prob, param, SVMHIP_KIND_C, 2 * l, y, lin, a2, param->C, param->C, out);
       double sum = 0;
       for (int i = 0; i < l; ++i) {
-        out.alpha[i] = a2[i] - a2[i + l];
+        out.alpha[i] = j.alpha[i] - j.alpha[i + l];
         sum += fabs(out.alpha[i]);
       }
       info("nu = %f\n", sum / (param->C * l));
       break;
     }
-    case NU_SVR: {
-      double C = param->C;
-      std::vector<double> a2(2 * l), lin(2 * l);
-      std::vector<schar> y(2 * l);
-      double sum = C * param->nu * l / 2;
-      for (int i = 0; i < l; ++i) {
-        a2[i] = a2[i + l] = std::min(sum, C);
-        sum -= a2[i];
-        lin[i] = -prob->y[i];
-        y[i] = 1;
-        lin[i + l] = prob->y[i];
-        y[i + l] = -1;
-      }
-      device_solve(prob, param, SVMHIP_KIND_NU, 2 * l, y, lin, a2, C, C, out);
-      info("epsilon = %f\n", -out.r);
-      for (int i = 0; i < l; ++i) out.alpha[i] = a2[i] - a2[i + l];
+    case NU_SVR:
+      info("epsilon = %f\n", -r);
+      for (int i = 0; i < l; ++i) out.alpha[i] = j.alpha[i] - j.alpha[i + l];
       break;
-    }
   }
   info("obj = %f, rho = %f\n", out.obj, out.rho);
   int nSV = 0, nBSV = 0;
   for (int i = 0; i < l; ++i) {
     if (fabs(out.alpha[i]) > 0) {
       ++nSV;
-      double ub = prob->y[i] > 0 ? out.ub_p : out.ub_n;
+      double ub = j.ylab[i] > 0 ? out.ub_p : out.ub_n;
       if (fabs(out.alpha[i]) >= ub) ++nBSV;
     }
   }
   info("nSV = %d, nBSV = %d\n", nSV, nBSV);
+}
+
+svmhip_options device_options() {
+  svmhip_options opt;
+  memset(&opt, 0, sizeof opt);
+  opt.struct_size = sizeof opt;
+  const char *e = getenv("SVMHIP_VERBOSE");
+  opt.verbose = e && *e == '1';
+  return opt;
+}
+
+// Problems up to this many rows go to the batched (one workgroup) solver
+// even when alone; larger single problems use the persistent grid.
+constexpr int kSoloMaxRows = 16384;
+
+// One large subproblem on the persistent multi-workgroup solver.
+void run_single(const svm_problem *prob, const svm_parameter *param, Job &j) {
+  const int l = (int)j.rows.size();
+  std::vector<svm_node *> sx(std::max(l, 1));
+  for (int i = 0; i < l; ++i) sx[i] = prob->x[j.rows[i]];
+  svm_problem sub;
+  sub.l = l;
+  sub.x = sx.data();
+  sub.y = j.ylab.data();
+  svmhip_options opt = device_options();
+  svmhip_solver *h = nullptr;
+  svmhip_status s = svmhip_create(&sub, param, j.kind, &opt, &h);
+  if (s != SVMHIP_OK) {
+    fprintf(stderr, "ERROR: device solver unavailable (status %d)\n", (int)s);
+    throw DeviceFailure{s};
+  }
+  std::vector<char> status(j.n_var);
+  for (int v = 0; v < j.n_var; ++v) {
+    double C = j.y[v] > 0 ? j.fCp : j.fCn;
+    status[v] = j.alpha[v] >= C ? 1 : (j.alpha[v] <= 0 ? 0 : 2);
+  }
+  svmhip_stats st;
+  memset(&st, 0, sizeof st);
+  if ((s = svmhip_setup_rbf(h, l)) != SVMHIP_OK ||
+      (s = svmhip_setup(h, j.y.data(), j.p.data(), j.alpha.data(), status.data(), j.fCp, j.fCn, j.n_var)) != SVMHIP_OK ||
+      (s = svmhip_solve(h, 0, &st)) != SVMHIP_OK ||
+      (s = svmhip_fetch(h, nullptr, j.alpha.data(), nullptr, j.n_var)) != SVMHIP_OK) {
+    fprintf(stderr, "ERROR: device solver failed (status %d): %s\n", (int)s, svmhip_last_error(h));
+    svmhip_destroy(h);
+    throw DeviceFailure{s};
+  }
+  svmhip_destroy(h);
+  finish(param, j, st.rho, st.obj, st.r, st.iters, st.optimal != 0);
+}
+
+// Every subproblem on the device: one large one on the persistent grid,
+// otherwise all of them in one batched launch (one workgroup each).  This is
+// where the reference would create and run one CudaSolver per subproblem
+// (libsvm/svm.cpp:1712-1738).
+void run_jobs(const svm_problem *prob, const svm_parameter *param, std::vector<Job> &jobs) {
+  for (Job &j : jobs) formulate(param, j);
+  if (jobs.empty()) return;
+  if (jobs.size() == 1 && (int)jobs[0].rows.size() > kSoloMaxRows) {
+    run_single(prob, param, jobs[0]);
+    return;
+  }
+  svmhip_options opt = device_options();
+  svmhip_batch *b = nullptr;
+  svmhip_status s = svmhip_batch_create(prob, param, &opt, &b);
+  if (s != SVMHIP_OK) {
+    if (s == SVMHIP_ERR_ARG) {  // e.g. precomputed kernels or very wide rows: one solve at a time
+      for (Job &j : jobs) run_single(prob, param, j);
+      return;
+    }
+    fprintf(stderr, "ERROR: device solver unavailable (status %d)\n", (int)s);
+    throw DeviceFailure{s};
+  }
+  std::map<std::vector<int>, int> ids;
+  std::vector<svmhip_subproblem> subs(jobs.size());
+  for (size_t k = 0; k < jobs.size(); ++k) {
+    Job &j = jobs[k];
+    auto it = ids.find(j.rows);
+    int id;
+    if (it != ids.end()) {
+      id = it->second;
+    } else {
+      id = svmhip_batch_add_rowset(b, j.rows.data(), (int)j.rows.size());
+      if (id < 0) {
+        fprintf(stderr, "ERROR: device row set failed: %s\n", svmhip_batch_last_error(b));
+        svmhip_batch_destroy(b);
+        throw DeviceFailure{(svmhip_status)(-id)};
+      }
+      ids.emplace(j.rows, id);
+    }
+    svmhip_subproblem &sp = subs[k];
+    memset(&sp, 0, sizeof sp);
+    sp.rowset = id;
+    sp.kind = j.kind;
+    sp.n_var = j.n_var;
+    sp.y = j.y.data();
+    sp.p = j.p.data();
+    sp.alpha = j.alpha.data();
+    sp.Cp = j.fCp;
+    sp.Cn = j.fCn;
+    sp.gamma = param->gamma;
+    sp.eps = param->eps;
+  }
+  svmhip_stats st;
+  s = svmhip_batch_solve(b, subs.data(), (int)subs.size(), &st);
+  if (s != SVMHIP_OK) {
+    fprintf(stderr, "ERROR: batched device solve failed (status %d): %s\n", (int)s, svmhip_batch_last_error(b));
+    svmhip_batch_destroy(b);
+    throw DeviceFailure{s};
+  }
+  svmhip_batch_destroy(b);
+  for (size_t k = 0; k < jobs.size(); ++k)
+    finish(param, jobs[k], subs[k].rho, subs[k].obj, subs[k].r, subs[k].iters, subs[k].status == SVMHIP_OK);
 }
 
 // svm_group_classes, libsvm/svm.cpp:2081-2157: labels in order of first
@@ -299,132 +427,170 @@ const char *kernel_type_table[] = {"linear", "polynomial", "rbf", "sigmoid", "pr
 
 extern "C" {
 
-static svm_model *train_model(const svm_problem *prob, const svm_parameter *param);
+// Trains one model per training subset (rows of prob, in order) with every
+// subproblem of every model in one device batch.  Model layout as in
+// svm_train (libsvm/svm.cpp:2159-2404): one-vs-one pairs, weighted C,
+// support vectors grouped by class, sv_coef[class][...] packing.
+static std::vector<svm_model *> train_models(const svm_problem *prob, const svm_parameter *param,
+                                             const std::vector<std::vector<int>> &subsets) {
+  struct Plan {
+    std::vector<int> rows;            // subset, grouped by class for classification
+    std::vector<int> label, start, count;
+    std::vector<double> wC;
+    size_t first_job = 0;
+  };
+  const bool regression_like = is_regression_like(param->svm_type);
+  std::vector<Plan> plans(subsets.size());
+  std::vector<Job> jobs;
+  for (size_t m = 0; m < subsets.size(); ++m) {
+    Plan &pl = plans[m];
+    const std::vector<int> &sub = subsets[m];
+    pl.first_job = jobs.size();
+    if (regression_like) {
+      Job j;
+      j.rows = sub;
+      for (int r : sub) j.ylab.push_back(prob->y[r]);
+      j.Cp = j.Cn = 0;
+      jobs.push_back(std::move(j));
+      continue;
+    }
+    // svm_group_classes over the subset
+    std::vector<svm_node *> sx(sub.size());
+    std::vector<double> sy(sub.size());
+    for (size_t i = 0; i < sub.size(); ++i) { sx[i] = prob->x[sub[i]]; sy[i] = prob->y[sub[i]]; }
+    svm_problem sp;
+    sp.l = (int)sub.size();
+    sp.x = sx.data();
+    sp.y = sy.data();
+    std::vector<int> perm;
+    group_classes(&sp, pl.label, pl.start, pl.count, perm);
+    const int k = (int)pl.label.size();
+    if (k == 1) info("WARNING: training data in only one class. See README for details.\n");
+    pl.rows.resize(sub.size());
+    for (size_t i = 0; i < sub.size(); ++i) pl.rows[i] = sub[perm[i]];
+    pl.wC.assign(k, param->C);
+    for (int w = 0; w < param->nr_weight; ++w) {
+      int c = 0;
+      while (c < k && param->weight_label[w] != pl.label[c]) ++c;
+      if (c == k) fprintf(stderr, "WARNING: class label %d specified in weight is not found\n", param->weight_label[w]);
+      else pl.wC[c] *= param->weight[w];
+    }
+    for (int a = 0; a < k; ++a)
+      for (int b = a + 1; b < k; ++b) {
+        Job j;
+        for (int t = 0; t < pl.count[a]; ++t) { j.rows.push_back(pl.rows[pl.start[a] + t]); j.ylab.push_back(+1); }
+        for (int t = 0; t < pl.count[b]; ++t) { j.rows.push_back(pl.rows[pl.start[b] + t]); j.ylab.push_back(-1); }
+        j.Cp = pl.wC[a];
+        j.Cn = pl.wC[b];
+        jobs.push_back(std::move(j));
+      }
+  }
+  run_jobs(prob, param, jobs);
+
+  std::vector<svm_model *> models(subsets.size());
+  for (size_t m = 0; m < subsets.size(); ++m) {
+    const Plan &pl = plans[m];
+    const std::vector<int> &sub = subsets[m];
+    svm_model *model = (svm_model *)calloc(1, sizeof(svm_model));
+    model->param = *param;
+    model->free_sv = 0;
+    models[m] = model;
+    if (regression_like) {
+      const Solution &f = jobs[pl.first_job].sol;
+      model->nr_class = 2;
+      model->sv_coef = xmalloc<double *>(1);
+      model->rho = xmalloc<double>(1);
+      model->rho[0] = f.rho;
+      int nSV = 0;
+      for (size_t i = 0; i < sub.size(); ++i)
+        if (fabs(f.alpha[i]) > 0) ++nSV;
+      model->l = nSV;
+      model->SV = xmalloc<svm_node *>(nSV);
+      model->sv_coef[0] = xmalloc<double>(nSV);
+      model->sv_indices = xmalloc<int>(nSV);
+      int j = 0;
+      for (size_t i = 0; i < sub.size(); ++i)
+        if (fabs(f.alpha[i]) > 0) {
+          model->SV[j] = prob->x[sub[i]];
+          model->sv_coef[0][j] = f.alpha[i];
+          model->sv_indices[j] = (int)i + 1;
+          ++j;
+        }
+      continue;
+    }
+    const int k = (int)pl.label.size();
+    const int l = (int)sub.size();
+    const int npair = k * (k - 1) / 2;
+    // position in the subset of each grouped row (for sv_indices)
+    std::vector<int> pos_of(prob->l, -1);
+    for (int i = 0; i < l; ++i) pos_of[sub[i]] = i;
+    std::vector<char> nonzero(l, 0);
+    for (int a = 0, q = 0; a < k; ++a)
+      for (int b = a + 1; b < k; ++b, ++q) {
+        const Solution &f = jobs[pl.first_job + q].sol;
+        for (int t = 0; t < pl.count[a]; ++t)
+          if (fabs(f.alpha[t]) > 0) nonzero[pl.start[a] + t] = 1;
+        for (int t = 0; t < pl.count[b]; ++t)
+          if (fabs(f.alpha[pl.count[a] + t]) > 0) nonzero[pl.start[b] + t] = 1;
+      }
+    model->nr_class = k;
+    model->label = xmalloc<int>(k);
+    for (int c = 0; c < k; ++c) model->label[c] = pl.label[c];
+    model->rho = xmalloc<double>(npair);
+    for (int q = 0; q < npair; ++q) model->rho[q] = jobs[pl.first_job + q].sol.rho;
+    model->probA = model->probB = NULL;
+    int total = 0;
+    std::vector<int> nz_count(k);
+    model->nSV = xmalloc<int>(k);
+    for (int c = 0; c < k; ++c) {
+      int n = 0;
+      for (int t = 0; t < pl.count[c]; ++t)
+        if (nonzero[pl.start[c] + t]) { ++n; ++total; }
+      model->nSV[c] = n;
+      nz_count[c] = n;
+    }
+    info("Total nSV = %d\n", total);
+    model->l = total;
+    model->SV = xmalloc<svm_node *>(total);
+    model->sv_indices = xmalloc<int>(total);
+    int p = 0;
+    for (int i = 0; i < l; ++i)
+      if (nonzero[i]) {
+        model->SV[p] = prob->x[pl.rows[i]];
+        model->sv_indices[p++] = pos_of[pl.rows[i]] + 1;
+      }
+    std::vector<int> nz_start(k, 0);
+    for (int c = 1; c < k; ++c) nz_start[c] = nz_start[c - 1] + nz_count[c - 1];
+    model->sv_coef = xmalloc<double *>(std::max(k - 1, 1));
+    for (int c = 0; c < k - 1; ++c) model->sv_coef[c] = xmalloc<double>(total);
+    for (int a = 0, q = 0; a < k; ++a)
+      for (int b = a + 1; b < k; ++b, ++q) {
+        // classifier (a,b): class-a coefficients in sv_coef[b-1], class-b in sv_coef[a]
+        const Solution &f = jobs[pl.first_job + q].sol;
+        int t2 = nz_start[a];
+        for (int t = 0; t < pl.count[a]; ++t)
+          if (nonzero[pl.start[a] + t]) model->sv_coef[b - 1][t2++] = f.alpha[t];
+        t2 = nz_start[b];
+        for (int t = 0; t < pl.count[b]; ++t)
+          if (nonzero[pl.start[b] + t]) model->sv_coef[a][t2++] = f.alpha[pl.count[a] + t];
+      }
+  }
+  return models;
+}
 
 // Returns NULL (after printing the reason) if the device solver fails.
 svm_model *svm_train(const svm_problem *prob, const svm_parameter *param) {
   try {
-    return train_model(prob, param);
+    std::vector<int> all(prob->l);
+    for (int i = 0; i < prob->l; ++i) all[i] = i;
+    return train_models(prob, param, {all})[0];
   } catch (const DeviceFailure &) {
     return NULL;
   }
 }
 
-static svm_model *train_model(const svm_problem *prob, const svm_parameter *param) {
-  svm_model *model = (svm_model *)calloc(1, sizeof(svm_model));
-  model->param = *param;
-  model->free_sv = 0;
-  if (is_regression_like(param->svm_type)) {
-    // libsvm/svm.cpp:2166-2206 (probability output for SVR is not built)
-    model->nr_class = 2;
-    model->sv_coef = xmalloc<double *>(1);
-    Solution f;
-    train_one(prob, param, 0, 0, f);
-    model->rho = xmalloc<double>(1);
-    model->rho[0] = f.rho;
-    int nSV = 0;
-    for (int i = 0; i < prob->l; ++i)
-      if (fabs(f.alpha[i]) > 0) ++nSV;
-    model->l = nSV;
-    model->SV = xmalloc<svm_node *>(nSV);
-    model->sv_coef[0] = xmalloc<double>(nSV);
-    model->sv_indices = xmalloc<int>(nSV);
-    int j = 0;
-    for (int i = 0; i < prob->l; ++i) {
-      if (fabs(f.alpha[i]) > 0) {
-        model->SV[j] = prob->x[i];
-        model->sv_coef[0][j] = f.alpha[i];
-        model->sv_indices[j] = i + 1;
-        ++j;
-      }
-    }
-    return model;
-  }
-
-  // classification, one-vs-one (libsvm/svm.cpp:2207-2404)
-  const int l = prob->l;
-  std::vector<int> label, start, count, perm;
-  group_classes(prob, label, start, count, perm);
-  const int k = (int)label.size();
-  if (k == 1) info("WARNING: training data in only one class. See README for details.\n");
-  std::vector<svm_node *> x(l);
-  for (int i = 0; i < l; ++i) x[i] = prob->x[perm[i]];
-  std::vector<double> wC(k, param->C);
-  for (int w = 0; w < param->nr_weight; ++w) {
-    int c = 0;
-    while (c < k && param->weight_label[w] != label[c]) ++c;
-    if (c == k) fprintf(stderr, "WARNING: class label %d specified in weight is not found\n", param->weight_label[w]);
-    else wC[c] *= param->weight[w];
-  }
-  std::vector<char> nonzero(l, 0);
-  const int npair = k * (k - 1) / 2;
-  std::vector<Solution> f(npair);
-  int p = 0;
-  for (int a = 0; a < k; ++a)
-    for (int b = a + 1; b < k; ++b) {
-      const int sa = start[a], sb = start[b], ca = count[a], cb = count[b];
-      std::vector<svm_node *> sx(ca + cb);
-      std::vector<double> sy(ca + cb);
-      for (int t = 0; t < ca; ++t) { sx[t] = x[sa + t]; sy[t] = +1; }
-      for (int t = 0; t < cb; ++t) { sx[ca + t] = x[sb + t]; sy[ca + t] = -1; }
-      svm_problem sub;
-      sub.l = ca + cb;
-      sub.x = sx.data();
-      sub.y = sy.data();
-      train_one(&sub, param, wC[a], wC[b], f[p]);
-      for (int t = 0; t < ca; ++t)
-        if (fabs(f[p].alpha[t]) > 0) nonzero[sa + t] = 1;
-      for (int t = 0; t < cb; ++t)
-        if (fabs(f[p].alpha[ca + t]) > 0) nonzero[sb + t] = 1;
-      ++p;
-    }
-  model->nr_class = k;
-  model->label = xmalloc<int>(k);
-  for (int c = 0; c < k; ++c) model->label[c] = label[c];
-  model->rho = xmalloc<double>(npair);
-  for (int q = 0; q < npair; ++q) model->rho[q] = f[q].rho;
-  model->probA = model->probB = NULL;
-  int total = 0;
-  std::vector<int> nz_count(k);
-  model->nSV = xmalloc<int>(k);
-  for (int c = 0; c < k; ++c) {
-    int n = 0;
-    for (int t = 0; t < count[c]; ++t)
-      if (nonzero[start[c] + t]) { ++n; ++total; }
-    model->nSV[c] = n;
-    nz_count[c] = n;
-  }
-  info("Total nSV = %d\n", total);
-  model->l = total;
-  model->SV = xmalloc<svm_node *>(total);
-  model->sv_indices = xmalloc<int>(total);
-  p = 0;
-  for (int i = 0; i < l; ++i)
-    if (nonzero[i]) {
-      model->SV[p] = x[i];
-      model->sv_indices[p++] = perm[i] + 1;
-    }
-  std::vector<int> nz_start(k, 0);
-  for (int c = 1; c < k; ++c) nz_start[c] = nz_start[c - 1] + nz_count[c - 1];
-  model->sv_coef = xmalloc<double *>(std::max(k - 1, 1));
-  for (int c = 0; c < k - 1; ++c) model->sv_coef[c] = xmalloc<double>(total);
-  p = 0;
-  for (int a = 0; a < k; ++a)
-    for (int b = a + 1; b < k; ++b) {
-      // classifier (a,b): class-a coefficients in sv_coef[b-1], class-b in sv_coef[a]
-      int q = nz_start[a];
-      for (int t = 0; t < count[a]; ++t)
-        if (nonzero[start[a] + t]) model->sv_coef[b - 1][q++] = f[p].alpha[t];
-      q = nz_start[b];
-      for (int t = 0; t < count[b]; ++t)
-        if (nonzero[start[b] + t]) model->sv_coef[a][q++] = f[p].alpha[count[a] + t];
-      ++p;
-    }
-  return model;
-}
-
 // Stratified folds drawn with rand() as the reference does
-// (libsvm/svm.cpp:2407-2525).
+// (libsvm/svm.cpp:2407-2525); all folds' subproblems train in one batch.
 void svm_cross_validation(const svm_problem *prob, const svm_parameter *param, int nr_fold, double *target) {
   const int l = prob->l;
   if (nr_fold > l) {
@@ -458,24 +624,38 @@ void svm_cross_validation(const svm_problem *prob, const svm_parameter *param, i
     for (int i = 0; i < l; ++i) std::swap(perm[i], perm[i + rand() % (l - i)]);
     for (int i = 0; i <= nr_fold; ++i) fold_start[i] = i * l / nr_fold;
   }
-  for (int i = 0; i < nr_fold; ++i) {
-    const int b = fold_start[i], e = fold_start[i + 1];
-    svm_problem sub;
-    sub.l = l - (e - b);
-    std::vector<svm_node *> sx(std::max(sub.l, 1));
-    std::vector<double> sy(std::max(sub.l, 1));
-    int k = 0;
-    for (int j = 0; j < b; ++j) { sx[k] = prob->x[perm[j]]; sy[k++] = prob->y[perm[j]]; }
-    for (int j = e; j < l; ++j) { sx[k] = prob->x[perm[j]]; sy[k++] = prob->y[perm[j]]; }
-    sub.x = sx.data();
-    sub.y = sy.data();
-    svm_model *m = svm_train(&sub, param);
-    if (!m) {  // device failure: mark the fold's targets as NaN
+  std::vector<std::vector<int>> subsets(nr_fold);
+  for (int f = 0; f < nr_fold; ++f) {
+    for (int j = 0; j < fold_start[f]; ++j) subsets[f].push_back(perm[j]);
+    for (int j = fold_start[f + 1]; j < l; ++j) subsets[f].push_back(perm[j]);
+  }
+  std::vector<svm_model *> models;
+  try {
+    models = train_models(prob, param, subsets);
+  } catch (const DeviceFailure &) {  // device failure: no predictions
+    for (int i = 0; i < l; ++i) target[i] = NAN;
+    return;
+  }
+  for (int f = 0; f < nr_fold; ++f) {
+    const int b = fold_start[f], e = fold_start[f + 1];
+    std::vector<const svm_node *> rows;
+    for (int j = b; j < e; ++j) rows.push_back(prob->x[perm[j]]);
+    std::vector<double> lab(rows.size()), dec;
+    const int k = models[f]->nr_class;
+    dec.resize(rows.size() * (size_t)(is_regression_like(param->svm_type) ? 1 : std::max(1, k * (k - 1) / 2)));
+    svmhip_options opt = device_options();
+    svmhip_status s = rows.empty() ? SVMHIP_OK
+                                   : svmhip_predict_batch(models[f], rows.data(), (int)rows.size(), &opt, dec.data(),
+                                                          lab.data());
+    if (s == SVMHIP_OK) {
+      for (int j = b; j < e; ++j) target[perm[j]] = lab[j - b];
+    } else if (s == SVMHIP_ERR_ARG) {  // precomputed kernels / very wide rows: per-row host prediction
+      for (int j = b; j < e; ++j) target[perm[j]] = svm_predict(models[f], prob->x[perm[j]]);
+    } else {
+      fprintf(stderr, "ERROR: device prediction failed (status %d)\n", (int)s);
       for (int j = b; j < e; ++j) target[perm[j]] = NAN;
-      continue;
     }
-    for (int j = b; j < e; ++j) target[perm[j]] = svm_predict(m, prob->x[perm[j]]);
-    svm_free_and_destroy_model(&m);
+    svm_free_and_destroy_model(&models[f]);
   }
 }
 

@@ -67,7 +67,7 @@ SVM_HIP_SYMBOLS = [
     "svmhip_compute_alpha", "svmhip_update_gradient", "svmhip_update_alpha_status", "svmhip_fetch",
     "svmhip_solve", "svmhip_finalize", "svmhip_get_stats", "svmhip_kernel_row", "svmhip_destroy",
     "svmhip_last_error", "svmhip_device_count", "svmhip_batch_create", "svmhip_batch_add_rowset",
-    "svmhip_batch_solve", "svmhip_batch_destroy", "svmhip_batch_last_error"]
+    "svmhip_batch_solve", "svmhip_batch_destroy", "svmhip_batch_last_error", "svmhip_predict_batch"]
 
 _lib = None
 PRINT_FUNC = C.CFUNCTYPE(None, C.c_char_p)
@@ -128,6 +128,8 @@ def lib():
     L.svmhip_batch_destroy.restype = None
     L.svmhip_batch_last_error.argtypes = [h]
     L.svmhip_batch_last_error.restype = C.c_char_p
+    L.svmhip_predict_batch.argtypes = [P(ffi.svm_model), P(P(ffi.svm_node)), I, P(svmhip_options), P(D), P(D)]
+    L.svmhip_predict_batch.restype = I
     for name in ("svmhip_create", "svmhip_setup_rbf", "svmhip_setup", "svmhip_select_working_set",
                  "svmhip_compute_alpha", "svmhip_update_gradient", "svmhip_update_alpha_status",
                  "svmhip_fetch", "svmhip_solve", "svmhip_finalize", "svmhip_get_stats",
@@ -361,6 +363,25 @@ class Model:
             dec[n] = buf
         return lab, dec
 
+    def predict_batch(self, problem: Problem, rows=None):
+        """Decision values and labels of many rows on the device
+        (svmhip_predict_batch)."""
+        m = self.ptr.contents
+        k = m.nr_class
+        ndec = 1 if m.param.svm_type in (ONE_CLASS, EPSILON_SVR, NU_SVR) else k * (k - 1) // 2
+        rows = np.arange(problem.l) if rows is None else np.asarray(rows)
+        ptrs = (C.POINTER(ffi.svm_node) * max(len(rows), 1))()
+        for n, r in enumerate(rows):
+            ptrs[n] = problem.row_nodes(int(r))
+        dec = np.zeros((len(rows), ndec))
+        lab = np.zeros(len(rows))
+        opt = svmhip_options(C.sizeof(svmhip_options), 0, 0, 0, 0, 0.0, 0, 0)
+        rc = lib().svmhip_predict_batch(self.ptr, ptrs, len(rows), C.byref(opt), _ptr(dec, C.c_double),
+                                        _ptr(lab, C.c_double))
+        if rc != 0:
+            raise SolverError(f"svmhip_predict_batch: {STATUS_NAMES.get(rc, rc)}")
+        return lab, dec
+
     def save(self, path):
         if lib().svm_save_model(path.encode(), self.ptr) != 0:
             raise SolverError(f"svm_save_model({path}) failed")
@@ -465,3 +486,10 @@ def finish_solution(problem_y, param, res, y, r=None):
     else:
         coef = a[:l] - a[l:]
     return coef, rho, obj
+
+
+def svm_cross_validation(problem: Problem, param, nr_fold):
+    """The kept svm_cross_validation (rand()-drawn stratified folds)."""
+    target = np.zeros(problem.l)
+    lib().svm_cross_validation(C.byref(problem.struct), C.byref(param), nr_fold, _ptr(target, C.c_double))
+    return target

@@ -66,6 +66,8 @@ def lib():
         L.ora_cross_validation_folds.argtypes = [C.POINTER(ffi.svm_problem), C.POINTER(ffi.svm_parameter),
                                                  C.c_int, C.POINTER(C.c_int), C.POINTER(C.c_double)]
         L.ora_free_model.argtypes = [C.POINTER(ffi.svm_model)]
+        L.ora_svm_cross_validation.argtypes = [C.POINTER(ffi.svm_problem), C.POINTER(ffi.svm_parameter),
+                                               C.c_int, C.POINTER(C.c_double)]
         _lib = L
     return _lib
 
@@ -163,4 +165,13 @@ def cross_validation(problem: ffi.Problem, param, folds):
     L.ora_cross_validation_folds(C.byref(problem.struct), C.byref(param), int(folds.max()) + 1,
                                  folds.ctypes.data_as(C.POINTER(C.c_int)),
                                  target.ctypes.data_as(C.POINTER(C.c_double)))
+    return target
+
+
+def svm_cross_validation(problem: ffi.Problem, param, nr_fold):
+    """rand()-drawn folds exactly as the reference (call srand first)."""
+    L = lib()
+    target = np.zeros(problem.l)
+    L.ora_svm_cross_validation(C.byref(problem.struct), C.byref(param), nr_fold,
+                               target.ctypes.data_as(C.POINTER(C.c_double)))
     return target

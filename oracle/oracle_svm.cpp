@@ -1177,6 +1177,8 @@ static double predict_values(const svm_model *m, const svm_node *x, double *dec)
 /* ------------------------------------------------------------------------ */
 extern "C" {
 
+void ora_free_model(svm_model *m);
+
 void ora_set_options(const ora_options *opt)
 {
 	if (opt) g_opt = *opt;
@@ -1279,6 +1281,54 @@ void ora_cross_validation_folds(const svm_problem *prob, const svm_parameter *pa
 		free(m->nSV);
 		free(m->sv_indices);
 		free(m);
+	}
+}
+
+/* svm_cross_validation with the reference's rand()-drawn stratified folds
+ * (libsvm/svm.cpp:2407-2525). */
+void ora_svm_cross_validation(const svm_problem *prob, const svm_parameter *param, int nr_fold, double *target)
+{
+	int l = prob->l;
+	if (nr_fold > l) nr_fold = l;
+	std::vector<int> perm(l), fold_start(nr_fold + 1);
+	if ((param->svm_type == C_SVC || param->svm_type == NU_SVC) && nr_fold < l) {
+		std::vector<int> label, start, count;
+		ora::group_classes(prob, label, start, count, perm);
+		int k = (int)label.size();
+		std::vector<int> index(perm), fold_count(nr_fold, 0);
+		for (int c = 0; c < k; c++)
+			for (int i = 0; i < count[c]; i++) {
+				int j = i + rand() % (count[c] - i);
+				std::swap(index[start[c] + j], index[start[c] + i]);
+			}
+		for (int i = 0; i < nr_fold; i++)
+			for (int c = 0; c < k; c++) fold_count[i] += (i + 1) * count[c] / nr_fold - i * count[c] / nr_fold;
+		fold_start[0] = 0;
+		for (int i = 1; i <= nr_fold; i++) fold_start[i] = fold_start[i - 1] + fold_count[i - 1];
+		for (int c = 0; c < k; c++)
+			for (int i = 0; i < nr_fold; i++) {
+				int b = start[c] + i * count[c] / nr_fold, e = start[c] + (i + 1) * count[c] / nr_fold;
+				for (int j = b; j < e; j++) perm[fold_start[i]++] = index[j];
+			}
+		fold_start[0] = 0;
+		for (int i = 1; i <= nr_fold; i++) fold_start[i] = fold_start[i - 1] + fold_count[i - 1];
+	} else {
+		for (int i = 0; i < l; i++) perm[i] = i;
+		for (int i = 0; i < l; i++) std::swap(perm[i], perm[i + rand() % (l - i)]);
+		for (int i = 0; i <= nr_fold; i++) fold_start[i] = i * l / nr_fold;
+	}
+	for (int f = 0; f < nr_fold; f++) {
+		std::vector<svm_node *> sx;
+		std::vector<double> sy;
+		for (int j = 0; j < fold_start[f]; j++) { sx.push_back(prob->x[perm[j]]); sy.push_back(prob->y[perm[j]]); }
+		for (int j = fold_start[f + 1]; j < l; j++) { sx.push_back(prob->x[perm[j]]); sy.push_back(prob->y[perm[j]]); }
+		svm_problem sub;
+		sub.l = (int)sx.size();
+		sub.x = sx.data();
+		sub.y = sy.data();
+		svm_model *m = ora::train(&sub, param);
+		for (int j = fold_start[f]; j < fold_start[f + 1]; j++) target[perm[j]] = ora_svm_predict(m, prob->x[perm[j]]);
+		ora_free_model(m);
 	}
 }
 

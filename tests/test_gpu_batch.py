@@ -59,3 +59,55 @@ def test_batch_small_arena_lru(hip, ora):
         assert r["iters"] == ref["iters"]
         np.testing.assert_allclose(r["alpha_raw"], ref["alpha_raw"], rtol=0, atol=1e-12)
     assert st["row_misses"] > 3 * 100
+
+
+def test_svm_cross_validation_matches_oracle(hip, ora):
+    """svm_cross_validation trains every fold's one-vs-one pairs in one batch;
+    with the same rand() stream it must predict exactly what the oracle's
+    fold-by-fold LIBSVM loop predicts."""
+    import ctypes
+    libc = ctypes.CDLL("libc.so.6")
+    X, y, _ = datasets.clusters_10class(scale=0.05, seed=21)
+    prob = ffi.Problem.from_dense(X.astype(np.float64), y.astype(np.float64))
+    prm = ffi.make_param(gamma=2.0 ** -7, C_=4.0, eps=1e-3, shrinking=0)
+    libc.srand(7)
+    dev = hip.svm_cross_validation(prob, prm, 5)
+    libc.srand(7)
+    ref = ora.svm_cross_validation(prob, prm, 5)
+    agree = np.mean(dev == ref)
+    print(f"  CV l={prob.l} accuracy dev={np.mean(dev == y):.4f} ref={np.mean(ref == y):.4f} agree={agree}", flush=True)
+    assert agree == 1.0
+
+
+def test_svm_train_binary_small_uses_batch_and_matches(hip, ora):
+    ip, ix, v, y, d = datasets.zipf_binary(n=2000)
+    prob = ffi.Problem(ip, ix, v, y, d)
+    prm = ffi.make_param(gamma=1.0 / d, C_=1.0, eps=1e-3, shrinking=0)
+    m_dev = hip.svm_train(prob, prm).arrays()
+    m_ref = ora.svm_train(prob, prm).arrays()
+    np.testing.assert_array_equal(m_dev["sv_indices"], m_ref["sv_indices"])
+    np.testing.assert_allclose(m_dev["sv_coef"], m_ref["sv_coef"], atol=1e-9)
+    np.testing.assert_allclose(m_dev["rho"], m_ref["rho"], atol=1e-9)
+
+
+@pytest.mark.parametrize("kname,svm_type", [("rbf", ffi.C_SVC), ("poly", ffi.C_SVC), ("rbf", ffi.NU_SVC),
+                                            ("linear", ffi.C_SVC), ("rbf", ffi.EPSILON_SVR),
+                                            ("rbf", ffi.ONE_CLASS), ("sigmoid", ffi.C_SVC)])
+def test_predict_batch_matches_oracle(hip, ora, kname, svm_type):
+    kt = {"rbf": ffi.RBF, "poly": ffi.POLY, "linear": ffi.LINEAR, "sigmoid": ffi.SIGMOID}[kname]
+    X, y, _ = datasets.clusters_10class(scale=0.04, seed=5)
+    if svm_type == ffi.EPSILON_SVR:
+        y = X[:, 0].astype(np.float64) + 0.1 * X[:, 1]
+    prob = ffi.Problem.from_dense(X.astype(np.float64), y.astype(np.float64))
+    prm = ffi.make_param(svm_type=svm_type, kernel_type=kt, gamma=2.0 ** -7, coef0=0.1, degree=2,
+                         C_=4.0, nu=0.2, eps=1e-3, shrinking=0)
+    m_ref = ora.svm_train(prob, prm)
+    rows = np.arange(0, prob.l, 3)
+    lab_r, dec_r = m_ref.predict_values(prob, rows=rows)
+    m_dev = hip.svm_train(prob, prm)
+    lab_d, dec_d = m_dev.predict_batch(prob, rows=rows)
+    lab_h, dec_h = m_dev.predict_values(prob, rows=rows)  # host path of the same model
+    np.testing.assert_allclose(dec_d, dec_h, rtol=0, atol=1e-9)
+    np.testing.assert_allclose(dec_d, dec_r, rtol=0, atol=1e-4)
+    if svm_type != ffi.EPSILON_SVR:
+        assert np.mean(lab_d == lab_r) >= 0.999
