@@ -72,6 +72,115 @@ def copy_probe(gib):
     log(f"[bench] copy probe: {4 * n} bytes read, {4 * n} bytes written")
 
 
+def ovo_cv_grid_jobs(X, y, fold, gammas, C):
+    """Subproblems of 5-fold CV x gamma grid x one-vs-one pairs, formed as
+    svm_cross_validation / svm_train form them (classes in order of first
+    appearance in each training subset)."""
+    jobs = []
+    nf = int(fold.max()) + 1
+    for f in range(nf):
+        train = np.nonzero(fold != f)[0]
+        labels = list(dict.fromkeys(y[train].tolist()))
+        by = {c: train[y[train] == c] for c in labels}
+        for a in range(len(labels)):
+            for b in range(a + 1, len(labels)):
+                rows = np.concatenate([by[labels[a]], by[labels[b]]])
+                yy = np.concatenate([np.ones(len(by[labels[a]])), -np.ones(len(by[labels[b]]))]).astype(np.int8)
+                jobs.append((f, a, b, rows, yy))
+    return jobs
+
+
+def bench_ovo_cv_grid(args):
+    """10 classes, ~40k x 128 dense, 5-fold CV x 4 gammas x 45 pairs = 900
+    solves per step through the batched solver (one workgroup or a team of
+    workgroups per solve, all solves in flight together)."""
+    gammas = [2.0 ** -9, 2.0 ** -7, 2.0 ** -5, 2.0 ** -3]
+    C = 4.0
+    X, y, fold = datasets.clusters_10class()
+    prob = ffi.Problem.from_dense(X.astype(np.float64), y.astype(np.float64), keep_zeros=True)
+    prm = ffi.make_param(gamma=gammas[0], C_=C, eps=1e-3, shrinking=0)
+    jobs = ovo_cv_grid_jobs(X, y, fold, gammas, C)
+    b = libsvm_hip.BatchSolver(prob, prm, cache_bytes=args.batch_cache_mb * 2 ** 20)
+    t0 = time.perf_counter()
+    rids = [b.add_rowset(rows) for (_, _, _, rows, _) in jobs]
+    log(f"[bench] {len(jobs)} row sets packed in {time.perf_counter() - t0:.1f}s (n={prob.l})")
+    subs = []
+    for rid, (_, _, _, rows, yy) in zip(rids, jobs):
+        for g in gammas:
+            n = len(rows)
+            subs.append(dict(rowset=rid, kind=libsvm_hip.KIND_C, y=yy, p=-np.ones(n), alpha0=np.zeros(n),
+                             Cp=C, Cn=C, gamma=g, eps=1e-3))
+    for w in range(args.warmup):
+        _, st = b.solve(subs)
+        log(f"[bench] warmup {w}: {st['iters']} iters, device {st['solve_ms']:.0f} ms")
+    walls, stats = [], []
+    for k in range(args.steps):
+        t = time.perf_counter()
+        res, st = b.solve(subs)
+        walls.append(time.perf_counter() - t)
+        stats.append(st)
+        log(f"[bench] step {k}: {len(subs)} solves, {st['iters']} iters, {walls[-1]:.2f}s wall, "
+            f"device {st['solve_ms']:.0f} ms in {st['launches']} launches, rows computed {st['row_misses']}")
+    iters = sum(st["iters"] for st in stats)
+    wall = sum(walls)
+    # plain path (one persistent solve at a time, the whole grid per solve)
+    # on fold 0: its 45 pairs at all 4 gammas, a fifth of the workload
+    sample = [(k, g) for k, jb in enumerate(jobs) if jb[0] == 0 for g in gammas]
+    p_iters, p_time = 0, 0.0
+    if not args.no_plain:
+        for n_done, (k, g) in enumerate(sample):
+            _, _, _, rows, yy = jobs[k]
+            sub = prob.subset(rows, y=yy.astype(np.float64))
+            sp = ffi.make_param(gamma=g, C_=C, eps=1e-3, shrinking=0, cache_size=4096.0)
+            t = time.perf_counter()
+            r = libsvm_hip.solve_one(sub, sp)
+            p_time += time.perf_counter() - t
+            p_iters += r["iters"]
+            if n_done % 45 == 44:
+                log(f"[bench] plain path: {n_done + 1} solves, {p_iters} iters, {p_time:.1f}s")
+        log(f"[bench] plain persistent path, {len(sample)} solves one at a time: {p_iters} iters in {p_time:.2f}s")
+    last = stats[-1]
+    out = {
+        "metric": METRIC,
+        "value": iters / wall,
+        "unit": "SMO iters/s",
+        "n_gpus": 1, "steps": args.steps, "warmup": args.warmup,
+        "ms_per_step": 1e3 * wall / args.steps,
+        "higher_is_better": True, "scaling": "weak", "vs_baseline": None,
+        "dtype": "f64", "data": "synthetic",
+        "config": {"workload": f"clusters_10class n={prob.l} d=128: 5-fold CV x 4 gammas x 45 OvO pairs, "
+                               f"C-SVC C=4 eps=1e-3, {len(subs)} solves per step in one batched launch",
+                   "solves_per_s": len(subs) * args.steps / wall, "train_time_s": wall / args.steps,
+                   "iters_per_step": last["iters"], "rows_computed_per_step": last["row_misses"],
+                   "stages_per_step": last["launches"],
+                   "plain_path_sample": (f"fold 0, {len(sample)} solves one at a time: {p_iters} iters "
+                                         f"in {p_time:.2f}s") if p_time else None,
+                   "plain_path_iters_per_s": (p_iters / p_time) if p_time else None,
+                   "plain_path_train_time_s_extrapolated": (p_time * last["iters"] / p_iters) if p_time else None,
+                   "speedup_vs_plain_path": ((iters / wall) / (p_iters / p_time)) if p_time else None},
+    }
+    if not args.no_cpu:
+        sys.path.insert(0, os.path.join(ROOT, "oracle"))
+        import oracle  # CPU baseline leg only
+        t = time.perf_counter()
+        c_iters = 0
+        done = 0
+        for k, g in sorted(sample, key=lambda kg: (len(jobs[kg[0]][3]), kg[1])):
+            _, _, _, rows, yy = jobs[k]
+            sub = prob.subset(rows, y=yy.astype(np.float64))
+            sp = ffi.make_param(gamma=g, C_=C, eps=1e-3, cache_size=100.0, shrinking=1, cuda_flag=0)
+            c_iters += oracle.train_one(sub, sp)["iters"]
+            done += 1
+            if time.perf_counter() - t > args.cpu_seconds:
+                break
+        dt = time.perf_counter() - t
+        out["cpu_baseline"] = {"value": c_iters / dt, "unit": "SMO iters/s", "cores": 1, "kind": "port",
+                               "sample": f"{done} fold-0 solves (smallest pairs first, all 4 gammas), "
+                                         f"oracle with shrinking and a 100 MB cache, {dt:.1f}s"}
+    b.close()
+    print(json.dumps(out), flush=True)
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -86,6 +195,9 @@ def main():
     ap.add_argument("--copy-probe-gib", type=float, default=0.0,
                     help="instead of the solve, run a plain 1-pass device copy of this many GiB "
                          "(reference for the FETCH_SIZE / WRITE_SIZE counters)")
+    ap.add_argument("--workload", default="gmm_dense", choices=["gmm_dense", "ovo_cv_grid"])
+    ap.add_argument("--batch-cache-mb", type=float, default=400.0)
+    ap.add_argument("--no-plain", action="store_true")
     args = ap.parse_args()
     if args.copy_probe_gib > 0:
         copy_probe(args.copy_probe_gib)
@@ -96,6 +208,9 @@ def main():
         sys.exit(2)
 
     libsvm_hip.set_quiet(True)
+    if args.workload == "ovo_cv_grid":
+        bench_ovo_cv_grid(args)
+        return
     n, d = args.n, args.d
     t0 = time.perf_counter()
     X, y = datasets.gmm_dense_matrix(n=n, d=d, seed=1003)

@@ -98,28 +98,38 @@ struct View {
   long long pm_stride;
   float *xs_global;       // per-workgroup query-row scratch when d is too big for LDS
   int use_lds, nblocks;
-  int solo;               // 1 = one workgroup owns the whole problem (batched solves)
-  int pad_solo;
+  int rank;               // batched solves: this workgroup's owner index (-1: blockIdx.x)
+  int resume;             // batched solves: G, alpha, status and QD carry over from an earlier stage
+  int iter0;              // batched solves: iterations already done
+  int pad_rank;
 };
 
-// Kernel-row cache arenas of the batched solver: K arenas, claimed by a
-// workgroup for the lifetime of one subproblem.
+// Work queue and kernel-row cache arenas of the batched solver.  Item q of
+// a launch is (subproblem, rank).  With by_item = 0 every item is a whole
+// subproblem and workgroup b uses arena b; with by_item = 1 item q uses
+// arena q, and the T consecutive items of a team use arenas [q0, q0 + T)
+// as one row cache (each member keeps its own cache metadata).
 struct BatchPool {
   float *rows;             // K arenas of arena_floats floats
   long long arena_floats;
   int8_t *flags;           // K x flag_stride row flags (full-cache arenas)
   int *lru;                // K x lru_stride ints (LRU arenas)
   long long flag_stride, lru_stride;
-  int *busy;               // K claim flags
-  int K, pad;
+  int *queue;              // next item (zeroed before each launch)
+  int K, by_item;
 };
 
 // ---------------------------------------------------------------- helpers
 
 __device__ __forceinline__ int real_of(const View &V, int v) { return v < V.l ? v : v - V.l; }
 
-// Owner index of this workgroup (0 when one workgroup owns everything).
-__device__ __forceinline__ int my_owner(const View &V) { return V.solo ? 0 : (int)blockIdx.x; }
+// Owner index of this workgroup: its rank in a batched solve, else its
+// block index in the persistent grid.
+__device__ __forceinline__ int my_owner(const View &V) { return V.rank >= 0 ? V.rank : (int)blockIdx.x; }
+
+// Index of this workgroup's private cache metadata within V.pm_flags /
+// V.pm_lru (batched solves point those at the workgroup's own arena).
+__device__ __forceinline__ long long pm_index(const View &V) { return V.rank >= 0 ? 0 : (long long)blockIdx.x; }
 
 __device__ __forceinline__ double C_of(const View &V, int v) { return V.y[v] > 0 ? V.Cp : V.Cn; }
 
@@ -281,7 +291,7 @@ __device__ __forceinline__ float kernel_value(const View &V, const float *xs, in
 }
 
 // Materialise the owned part of kernel row q into cache slot `slot`.
-__device__ void compute_row_part(const View &V, int q, int slot, float *xs, int sb, int se) {
+__device__ __forceinline__ void compute_row_part(const View &V, int q, int slot, float *xs, int sb, int se) {
   stage_row(V, q, xs);
   float *dst = V.rows + (long long)slot * V.row_stride;
   const int wave = threadIdx.x / 64, lane = threadIdx.x % 64, nw = blockDim.x / 64;
@@ -897,7 +907,7 @@ template <int KIND, int PH>
 __device__ XPart exchange(const View &V, RedSmem &sm, XSmem &xs, const XPart &mine, unsigned seq) {
   XPart *buf = V.xpart + (size_t)(seq & 1u) * V.nblocks;
   if (threadIdx.x == 0) {
-    publish(&buf[blockIdx.x], mine, seq);
+    publish(&buf[my_owner(V)], mine, seq);
     xs.err = 0;
   }
   XPart q = xpart_identity<KIND, PH>();
@@ -956,9 +966,9 @@ __device__ XPart exchange(const View &V, RedSmem &sm, XSmem &xs, const XPart &mi
 __device__ __forceinline__ CacheMeta private_meta(const View &V, Dec &d) {
   CacheMeta M;
   if (V.full) {
-    M.computed = V.pm_flags + (long long)my_owner(V) * V.l;
+    M.computed = V.pm_flags + pm_index(V) * V.l;
   } else {
-    int *base = V.pm_lru + (long long)my_owner(V) * V.pm_stride;
+    int *base = V.pm_lru + pm_index(V) * V.pm_stride;
     M.slot_of_row = base;
     M.row_of_slot = base + V.l;
     M.prev = base + V.l + V.S;

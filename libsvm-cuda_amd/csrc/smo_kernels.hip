@@ -16,6 +16,11 @@
 //   k_finalize      rho / obj / r in FP64       (Solver::calculate_rho, svm.cpp:1013-1049,
 //                                                Solver_NU::calculate_rho, :1260-1308)
 //   k_kernel_row    one kernel row to a buffer  (test hook)
+//   k_smo_batch     many subproblems per launch, each by one workgroup or a
+//                   team of them (no reference counterpart: the reference
+//                   solves one-vs-one pairs and folds one after another)
+//   k_pack_rowset   copies a row subset into its own SELL arrays
+//   k_predict       decision values of many rows     (svm_predict_values, svm.cpp:2569-2643)
 #include "smo_device.hpp"
 
 namespace smo {
@@ -158,37 +163,32 @@ __device__ XPart make_j_record(const View &V, const Snap &c, const Partial &bp) 
   return m;
 }
 
-// The whole SMO loop on the device, one iteration per trip:
+// First selection of a solve (or of a resumed solve): i from the current G,
+// exchanged, then the stop test and the row-i slot.
+template <int KIND>
+__device__ __forceinline__ void smo_select_first(const View &V, Dec &d, RedSmem &sm, XSmem &xsm, float *xs) {
+  Snap c = snap_of(d);
+  Partial bp = pass_gradient_select<KIND>(V, c, xs, sm, false, true, false, nullptr);
+  XPart mine = make_i_record<KIND>(V, bp);
+  const unsigned seq = (unsigned)d.epoch + 1u;
+  XPart r = exchange<KIND, PH_I>(V, sm, xsm, mine, seq);
+  if (threadIdx.x == 0) {
+    d.epoch = (int)seq;
+    d.phase = 1;
+    if (r.seq) d.err = 1;
+    else decide_i<KIND>(V, d, r, true);
+  }
+  __syncthreads();
+}
+
+// The SMO loop across co-resident owners, one iteration per trip:
 //   [select j over owned variables, computing row i on a miss] exchange
 //   [every workgroup: j, row-j slot, alpha step]
 //   [owners store the pair's alpha/status; G update, computing row j on a
 //    miss; select i] exchange
 //   [every workgroup: i, Gmax/Gmax2, stop test, row-i slot]
-// Requires co-resident workgroups (cooperative launch).  State that must
-// survive to the next launch is left in Ctl by workgroup 0.
 template <int KIND>
-__global__ void __launch_bounds__(BLOCK) k_smo(View V, int iter_limit) {
-  __shared__ RedSmem sm;
-  __shared__ XSmem xsm;
-  __shared__ Dec d;
-  extern __shared__ float xs_lds[];
-  float *xs = V.use_lds ? xs_lds : V.xs_global + (long long)blockIdx.x * V.d;
-  if (threadIdx.x == 0) d = ctl_dec(V.ctl);
-  __syncthreads();
-  if (d.phase == 0 && !d.stop && !d.err) {
-    Snap c = snap_of(d);
-    Partial bp = pass_gradient_select<KIND>(V, c, xs, sm, false, true, false, nullptr);
-    XPart mine = make_i_record<KIND>(V, bp);
-    const unsigned seq = (unsigned)d.epoch + 1u;
-    XPart r = exchange<KIND, PH_I>(V, sm, xsm, mine, seq);
-    if (threadIdx.x == 0) {
-      d.epoch = (int)seq;
-      d.phase = 1;
-      if (r.seq) d.err = 1;
-      else decide_i<KIND>(V, d, r, true);
-    }
-    __syncthreads();
-  }
+__device__ __forceinline__ void smo_iterations(const View &V, Dec &d, RedSmem &sm, XSmem &xsm, float *xs, int iter_limit) {
   for (int n = 0; n < iter_limit; ++n) {
     if (d.stop || d.err) break;
     Snap c = snap_of(d);
@@ -218,6 +218,22 @@ __global__ void __launch_bounds__(BLOCK) k_smo(View V, int iter_limit) {
     }
     __syncthreads();
   }
+}
+
+// The whole SMO loop on the device, the workgroups of the grid being the
+// owners.  Requires co-resident workgroups (cooperative launch).  State that
+// must survive to the next launch is left in Ctl by workgroup 0.
+template <int KIND>
+__global__ void __launch_bounds__(BLOCK) k_smo(View V, int iter_limit) {
+  __shared__ RedSmem sm;
+  __shared__ XSmem xsm;
+  __shared__ Dec d;
+  extern __shared__ float xs_lds[];
+  float *xs = V.use_lds ? xs_lds : V.xs_global + (long long)blockIdx.x * V.d;
+  if (threadIdx.x == 0) d = ctl_dec(V.ctl);
+  __syncthreads();
+  if (d.phase == 0 && !d.stop && !d.err) smo_select_first<KIND>(V, d, sm, xsm, xs);
+  smo_iterations<KIND>(V, d, sm, xsm, xs, iter_limit);
   if (blockIdx.x == 0 && threadIdx.x == 0) {
     Ctl *c = V.ctl;
     ctl_dec(c) = d;
@@ -342,16 +358,18 @@ __global__ void __launch_bounds__(BLOCK) k_kernel_row(View V, int q, float *out)
 // ------------------------------------------------------------ batched solves
 //
 // Many independent subproblems (one-vs-one pairs, cross-validation folds,
-// gamma grid points) in one launch: workgroup b solves subproblem order[b]
-// from start to finish with the same passes as the persistent kernel, but as
-// the only owner of its subproblem, so every decision is local and the only
-// synchronisation is the workgroup barrier.  Kernel-row cache arenas come
-// from a pool of K >= (resident workgroups) arenas; a workgroup claims one
-// with a compare-and-swap and returns it when done.
+// gamma grid points) in one cooperative launch.  The workgroups of the grid
+// take items from a queue.  An item is a whole subproblem (rank 0 of a team
+// of one: every decision local, no exchange) or one member of a team of T
+// co-resident workgroups that owns a subproblem the way the persistent
+// kernel's grid does.  A team's T items are consecutive and the host queues
+// teams only when every item of the launch fits in the grid at once, so
+// every member is resident from the start.  Each item runs up to its
+// subproblem's iteration cap (V.max_iter); the host resumes unfinished
+// subproblems in a later launch from the G, alpha and status left in memory,
+// which continues the same sequence of working sets.
 
-
-
-__device__ void solo_init_cache(View &V) {
+__device__ __forceinline__ void solo_init_cache(View &V) {
   if (V.full) {
     for (int r = threadIdx.x; r < V.l; r += blockDim.x) V.pm_flags[r] = 0;
   } else {
@@ -369,36 +387,75 @@ __device__ void solo_init_cache(View &V) {
   }
 }
 
-// rho / obj / r of a subproblem, whole workgroup; thread 0 writes Ctl.
+// QD_v = K(v, v) in FP64 for every variable (gamma is per subproblem).
+__device__ __forceinline__ void batch_qd(const View &V) {
+  for (int v = threadIdx.x; v < V.nvar; v += blockDim.x) {
+    int r = real_of(V, v);
+    V.QD[v] = kernel_f64(V, r, r, V.xsq[r]);
+  }
+}
+
+// G = p + sum_{alpha_v > 0} alpha_v Q_v in increasing v (libsvm/svm.cpp:555-569),
+// for rows [r0, r1) (r0 a multiple of the slice height).  Ends with a barrier.
+__device__ __forceinline__ void batch_ginit(const View &V, float *xs, int r0, int r1) {
+  const int nv = V.svr ? 2 : 1;
+  for (int v = 0; v < V.nvar; ++v) {
+    if (V.status[v] == AT_LOWER) continue;
+    const int q = real_of(V, v);
+    stage_row(V, q, xs);
+    const double av = V.alpha[v];
+    const int8_t yv = V.y[v];
+    for (int r = r0 + threadIdx.x; r < r1; r += blockDim.x) {
+      // row r of a slice is lane r % 64 of wave (r % 512) / 64 in the SELL layout
+      float K = kernel_value(V, xs, q, r);
+      for (int h = 0; h < nv; ++h) {
+        int k = r + h * V.l;
+        float Qv = (V.y[k] == yv) ? K : -K;
+        V.G[k] = V.G[k] + av * (double)Qv;
+      }
+    }
+    __syncthreads();
+  }
+  __syncthreads();
+}
+
+// Terms of rho / obj / r over the variables of rows [r0, r1), reduced over
+// the workgroup into thread 0's f[0..6] (s1, ub1, lb1, s2, ub2, lb2, obj) and
+// n[0..1] (Solver::calculate_rho, libsvm/svm.cpp:1013-1049; Solver_NU::calculate_rho, :1260-1308;
+// the objective as in Solver::Solve, :794-803).
 template <int KIND>
-__device__ void solo_finalize(const View &V) {
-  __shared__ double red[BLOCK / 64][8];
+__device__ __forceinline__ void final_terms(const View &V, int r0, int r1, double f[7], int n[2]) {
+  __shared__ double red[BLOCK / 64][7];
   __shared__ int redi[BLOCK / 64][2];
   double s1 = 0, ub1 = INF, lb1 = -INF, s2 = 0, ub2 = INF, lb2 = -INF, ob = 0;
   int n1 = 0, n2 = 0;
-  for (int v = threadIdx.x; v < V.nvar; v += blockDim.x) {
-    double g = V.G[v];
-    int8_t yv = V.y[v], st = V.status[v];
-    ob += V.alpha[v] * (g + V.p[v]);
-    if (KIND == KIND_C) {
-      double yG = yv * g;
-      if (st == AT_UPPER) {
-        if (yv == -1) ub1 = fmin(ub1, yG); else lb1 = fmax(lb1, yG);
-      } else if (st == AT_LOWER) {
-        if (yv == +1) ub1 = fmin(ub1, yG); else lb1 = fmax(lb1, yG);
+  const int nv = V.svr ? 2 : 1;
+  for (int rr = r0 + threadIdx.x; rr < r1; rr += blockDim.x) {
+    for (int h = 0; h < nv; ++h) {
+      const int v = rr + h * V.l;
+      double g = V.G[v];
+      int8_t yv = V.y[v], st = V.status[v];
+      ob += V.alpha[v] * (g + V.p[v]);
+      if (KIND == KIND_C) {
+        double yG = yv * g;
+        if (st == AT_UPPER) {
+          if (yv == -1) ub1 = fmin(ub1, yG); else lb1 = fmax(lb1, yG);
+        } else if (st == AT_LOWER) {
+          if (yv == +1) ub1 = fmin(ub1, yG); else lb1 = fmax(lb1, yG);
+        } else {
+          ++n1;
+          s1 += yG;
+        }
       } else {
-        ++n1;
-        s1 += yG;
-      }
-    } else {
-      if (yv == +1) {
-        if (st == AT_UPPER) lb1 = fmax(lb1, g);
-        else if (st == AT_LOWER) ub1 = fmin(ub1, g);
-        else { ++n1; s1 += g; }
-      } else {
-        if (st == AT_UPPER) lb2 = fmax(lb2, g);
-        else if (st == AT_LOWER) ub2 = fmin(ub2, g);
-        else { ++n2; s2 += g; }
+        if (yv == +1) {
+          if (st == AT_UPPER) lb1 = fmax(lb1, g);
+          else if (st == AT_LOWER) ub1 = fmin(ub1, g);
+          else { ++n1; s1 += g; }
+        } else {
+          if (st == AT_UPPER) lb2 = fmax(lb2, g);
+          else if (st == AT_LOWER) ub2 = fmin(ub2, g);
+          else { ++n2; s2 += g; }
+        }
       }
     }
   }
@@ -419,83 +476,46 @@ __device__ void solo_finalize(const View &V) {
       s2 += red[w][3]; ub2 = fmin(ub2, red[w][4]); lb2 = fmax(lb2, red[w][5]);
       ob += red[w][6]; n1 += redi[w][0]; n2 += redi[w][1];
     }
-    Ctl *c = V.ctl;
-    c->obj = ob / 2;
-    if (KIND == KIND_C) {
-      c->rho = n1 > 0 ? s1 / n1 : (ub1 + lb1) / 2;
-      c->r = 0;
-    } else {
-      double r1 = n1 > 0 ? s1 / n1 : (ub1 + lb1) / 2;
-      double r2 = n2 > 0 ? s2 / n2 : (ub2 + lb2) / 2;
-      c->r = (r1 + r2) / 2;
-      c->rho = (r1 - r2) / 2;
-    }
+    f[0] = s1; f[1] = ub1; f[2] = lb1; f[3] = s2; f[4] = ub2; f[5] = lb2; f[6] = ob;
+    n[0] = n1; n[1] = n2;
   }
   __syncthreads();
 }
 
+// rho, obj and r from the combined terms into Ctl (thread 0).
 template <int KIND>
-__global__ void __launch_bounds__(BLOCK) k_smo_batch(const View *views, const int *order, BatchPool pool) {
-  __shared__ RedSmem sm;
-  __shared__ Dec d;
-  __shared__ int s_arena;
-  extern __shared__ float xs_lds[];
-  View V = views[order[blockIdx.x]];
-  float *xs = xs_lds;
-  // claim a cache arena (bounded probing; K exceeds the resident workgroups)
-  if (threadIdx.x == 0) {
-    int a = (int)(blockIdx.x % (unsigned)pool.K), found = -1;
-    for (long long probe = 0; probe < (1LL << 22); ++probe) {
-      if (atomicCAS(&pool.busy[a], 0, 1) == 0) { found = a; break; }
-      a = a + 1 == pool.K ? 0 : a + 1;
-      if (a == 0) __builtin_amdgcn_s_sleep(4);
-    }
-    s_arena = found;
-    __threadfence();
+__device__ void write_final(Ctl *c, const double f[7], const int n[2]) {
+  c->obj = f[6] / 2;
+  if (KIND == KIND_C) {
+    c->rho = n[0] > 0 ? f[0] / n[0] : (f[1] + f[2]) / 2;
+    c->r = 0;
+  } else {
+    double r1 = n[0] > 0 ? f[0] / n[0] : (f[1] + f[2]) / 2;
+    double r2 = n[1] > 0 ? f[3] / n[1] : (f[4] + f[5]) / 2;
+    c->r = (r1 + r2) / 2;
+    c->rho = (r1 - r2) / 2;
   }
-  __syncthreads();
-  const int arena = s_arena;
-  if (arena < 0) {
-    if (threadIdx.x == 0) { V.ctl->err = ERR_TIMEOUT; V.ctl->stop = STOP_NONE; }
-    return;
-  }
-  V.rows = pool.rows + (long long)arena * pool.arena_floats;
-  V.row_stride = ((long long)V.l + 63) / 64 * 64;
-  long long fit = pool.arena_floats / (V.row_stride > 0 ? V.row_stride : 1);
-  if (fit >= V.l) { V.full = 1; V.S = V.l > 0 ? V.l : 1; }
-  else { V.full = 0; V.S = (int)(fit < 2 ? 2 : fit); }
-  V.pm_flags = pool.flags + (long long)arena * pool.flag_stride;
-  V.pm_lru = pool.lru + (long long)arena * pool.lru_stride;
-  V.pm_stride = 0;
+}
+
+__device__ void write_ctl_state(Ctl *c, const Dec &d, int err) {
+  c->iter = d.iter;
+  c->stop = d.stop;
+  c->err = err;
+  c->hits = d.hits;
+  c->misses = d.misses;
+}
+
+// One subproblem owned by this workgroup alone.
+template <int KIND>
+__device__ __forceinline__ void solo_solve(View &V, Dec &d, RedSmem &sm, float *xs) {
   solo_init_cache(V);
-  // QD_v = K(v, v) in FP64 (gamma is per subproblem)
-  for (int v = threadIdx.x; v < V.nvar; v += blockDim.x) {
-    int r = real_of(V, v);
-    V.QD[v] = kernel_f64(V, r, r, V.xsq[r]);
-  }
-  __syncthreads();
-  // G = p + sum_{alpha_v > 0} alpha_v Q_v in increasing v (libsvm/svm.cpp:555-569)
-  const int nv = V.svr ? 2 : 1;
-  for (int v = 0; v < V.nvar; ++v) {
-    if (V.status[v] == AT_LOWER) continue;
-    const int q = real_of(V, v);
-    stage_row(V, q, xs);
-    const double av = V.alpha[v];
-    const int8_t yv = V.y[v];
-    for (int r = threadIdx.x; r < V.l; r += blockDim.x) {
-      // row r of a slice is lane r % 64 of wave r / 64 in the SELL layout
-      float K = kernel_value(V, xs, q, r);
-      for (int h = 0; h < nv; ++h) {
-        int k = r + h * V.l;
-        float Qv = (V.y[k] == yv) ? K : -K;
-        V.G[k] = V.G[k] + av * (double)Qv;
-      }
-    }
-    __syncthreads();
+  if (!V.resume) {
+    batch_qd(V);
+    batch_ginit(V, xs, 0, V.l);
   }
   if (threadIdx.x == 0) {
     memset(&d, 0, sizeof d);
-    d.iter = 0;
+    d.iter = V.iter0;
   }
   __syncthreads();
   {
@@ -520,16 +540,107 @@ __global__ void __launch_bounds__(BLOCK) k_smo_batch(const View *views, const in
     if (threadIdx.x == 0) decide_i<KIND>(V, d, m2, true);
     __syncthreads();
   }
-  solo_finalize<KIND>(V);
+  double f[7];
+  int n[2];
+  final_terms<KIND>(V, 0, V.l, f, n);
   if (threadIdx.x == 0) {
-    Ctl *c = V.ctl;
-    c->iter = d.iter;
-    c->stop = d.stop;
-    c->err = 0;
-    c->hits = d.hits;
-    c->misses = d.misses;
-    __threadfence();
-    atomicExch(&pool.busy[arena], 0);
+    write_final<KIND>(V.ctl, f, n);
+    write_ctl_state(V.ctl, d, 0);
+  }
+}
+
+// One member (rank V.rank of V.nblocks) of a team owning a subproblem.
+template <int KIND>
+__device__ __forceinline__ void team_solve(View &V, Dec &d, RedSmem &sm, XSmem &xsm, float *xs) {
+  int sb, se;
+  owned_slices(V, V.rank, V.nblocks, sb, se);
+  const int r0 = sb * SLICE, r1 = min(se * SLICE, V.l);
+  solo_init_cache(V);
+  if (!V.resume) {
+    batch_qd(V);  // every member writes every QD (identical values): row i's QD is read by all
+    batch_ginit(V, xs, r0, r1);
+  }
+  if (threadIdx.x == 0) {
+    memset(&d, 0, sizeof d);
+    d.iter = V.iter0;
+  }
+  __syncthreads();
+  smo_select_first<KIND>(V, d, sm, xsm, xs);
+  smo_iterations<KIND>(V, d, sm, xsm, xs, INT_MAX);
+  // combine the members' terms: rank 0 gathers the records in rank order
+  double f[7];
+  int n[2];
+  final_terms<KIND>(V, r0, r1, f, n);
+  if (threadIdx.x == 0) {
+    const unsigned seq = (unsigned)d.epoch + 1u;
+    XPart *buf = V.xpart + (size_t)(seq & 1u) * V.nblocks;
+    XPart m = xpart_identity<KIND, PH_I>();
+    m.v0 = f[0]; m.v1 = f[1]; m.v2 = f[2]; m.v3 = f[3]; m.g0 = f[4]; m.a0 = f[5]; m.g1 = f[6];
+    m.i0 = n[0]; m.i1 = n[1]; m.a1 = d.err ? 1.0 : 0.0;
+    publish(&buf[V.rank], m, seq);
+    if (V.rank == 0) {
+      double F[7] = {0, INF, -INF, 0, INF, -INF, 0};
+      int N[2] = {0, 0};
+      int err = d.err;
+      for (int t = 0; t < V.nblocks && !err; ++t) {
+        long long spins = 0;
+        while (ld_acq(&buf[t].seq) != seq) {
+          __builtin_amdgcn_s_sleep(1);
+          if (++spins > SPIN_LIMIT) { err = 1; break; }
+        }
+        if (err) break;
+        __asm__ volatile("" ::: "memory");
+        F[0] += ld_acq(&buf[t].v0); F[1] = fmin(F[1], ld_acq(&buf[t].v1)); F[2] = fmax(F[2], ld_acq(&buf[t].v2));
+        F[3] += ld_acq(&buf[t].v3); F[4] = fmin(F[4], ld_acq(&buf[t].g0)); F[5] = fmax(F[5], ld_acq(&buf[t].a0));
+        F[6] += ld_acq(&buf[t].g1);
+        N[0] += ld_acq(&buf[t].i0); N[1] += ld_acq(&buf[t].i1);
+        if (ld_acq(&buf[t].a1) != 0.0) err = 1;
+      }
+      write_final<KIND>(V.ctl, F, N);
+      write_ctl_state(V.ctl, d, err ? (int)ERR_TIMEOUT : 0);
+    }
+  }
+  __syncthreads();
+}
+
+__global__ void __launch_bounds__(BLOCK) k_smo_batch(const View *views, const int *items, int nitems,
+                                                    BatchPool pool) {
+  __shared__ RedSmem sm;
+  __shared__ XSmem xsm;
+  __shared__ Dec d;
+  __shared__ int s_item;
+  extern __shared__ float xs_lds[];
+  float *xs = xs_lds;
+  for (;;) {
+    if (threadIdx.x == 0) s_item = atomicAdd(pool.queue, 1);
+    __syncthreads();
+    // uniform across the workgroup: keeps the View in scalar registers
+    const int q = __builtin_amdgcn_readfirstlane(s_item);
+    if (q >= nitems) break;
+    const int sub = __builtin_amdgcn_readfirstlane(items[2 * q]);
+    View V = views[sub];
+    V.rank = __builtin_amdgcn_readfirstlane(items[2 * q + 1]);
+    const long long own = pool.by_item ? q : (long long)blockIdx.x;
+    const long long first = pool.by_item ? own - V.rank : own;
+    const int team = pool.by_item ? V.nblocks : 1;
+    V.rows = pool.rows + first * pool.arena_floats;
+    V.row_stride = ((long long)V.l + 63) / 64 * 64;
+    const long long fit = team * pool.arena_floats / (V.row_stride > 0 ? V.row_stride : 1);
+    if (fit >= V.l) { V.full = 1; V.S = V.l > 0 ? V.l : 1; }
+    else { V.full = 0; V.S = (int)(fit < 2 ? 2 : fit); }
+    V.pm_flags = pool.flags + own * pool.flag_stride;
+    V.pm_lru = pool.lru + own * pool.lru_stride;
+    V.pm_stride = 0;
+    if (team > 1) {
+      if (V.kind == KIND_C) team_solve<KIND_C>(V, d, sm, xsm, xs);
+      else team_solve<KIND_NU>(V, d, sm, xsm, xs);
+    } else {
+      V.nblocks = 1;
+      V.rank = 0;
+      if (V.kind == KIND_C) solo_solve<KIND_C>(V, d, sm, xs);
+      else solo_solve<KIND_NU>(V, d, sm, xs);
+    }
+    __syncthreads();
   }
 }
 
@@ -725,18 +836,17 @@ hipError_t smo_launch_finalize(const View *V, hipStream_t st) {
   return hipGetLastError();
 }
 
-hipError_t smo_launch_batch(int kind, const View *views, const int *order, int count, const smo::BatchPool *pool,
+hipError_t smo_launch_batch(const View *views, const int *items, int nitems, int grid, const smo::BatchPool *pool,
                             size_t shmem, hipStream_t st) {
-  if (count <= 0) return hipSuccess;
-  if (kind == smo::KIND_C)
-    hipLaunchKernelGGL(smo::k_smo_batch<smo::KIND_C>, dim3(count), dim3(smo::BLOCK), shmem, st, views, order, *pool);
-  else
-    hipLaunchKernelGGL(smo::k_smo_batch<smo::KIND_NU>, dim3(count), dim3(smo::BLOCK), shmem, st, views, order, *pool);
-  return hipGetLastError();
+  if (nitems <= 0 || grid <= 0) return hipSuccess;
+  smo::BatchPool P = *pool;
+  void *args[] = {(void *)&views, (void *)&items, (void *)&nitems, (void *)&P};
+  return hipLaunchCooperativeKernel((const void *)smo::k_smo_batch, dim3(grid), dim3(smo::BLOCK), args,
+                                    (unsigned)shmem, st);
 }
 
 hipError_t smo_batch_capacity(size_t shmem, int *blocks_per_cu) {
-  return hipOccupancyMaxActiveBlocksPerMultiprocessor(blocks_per_cu, (const void *)smo::k_smo_batch<smo::KIND_NU>,
+  return hipOccupancyMaxActiveBlocksPerMultiprocessor(blocks_per_cu, (const void *)smo::k_smo_batch,
                                                       smo::BLOCK, shmem);
 }
 

@@ -34,7 +34,7 @@ hipError_t smo_launch_lru_private_init(const smo::View *V, hipStream_t st);
 hipError_t smo_persistent_capacity(int kind, size_t shmem, int *blocks_per_cu);
 hipError_t smo_launch_finalize(const smo::View *V, hipStream_t st);
 hipError_t smo_launch_kernel_row(const smo::View *V, int q, float *out, size_t shmem, hipStream_t st);
-hipError_t smo_launch_batch(int kind, const smo::View *views, const int *order, int count, const smo::BatchPool *pool,
+hipError_t smo_launch_batch(const smo::View *views, const int *items, int nitems, int grid, const smo::BatchPool *pool,
                             size_t shmem, hipStream_t st);
 hipError_t smo_batch_capacity(size_t shmem, int *blocks_per_cu);
 hipError_t smo_launch_predict_offset(const smo::PredictArgs *A, int first_row, int nblocks, size_t shmem,
@@ -151,7 +151,7 @@ struct svmhip_solver {
     V.slot_of_row = d_slot_of_row; V.row_of_slot = d_row_of_slot; V.lru_prev = d_prev; V.lru_next = d_next;
     V.computed = d_computed;
     V.ctl = d_ctl; V.part = d_part; V.xs_global = d_xs;
-    V.use_lds = use_lds; V.nblocks = nblocks;
+    V.use_lds = use_lds; V.nblocks = nblocks; V.rank = -1;
     V.xpart = d_xpart; V.pm_flags = d_pm_flags; V.pm_lru = d_pm_lru; V.pm_stride = pm_stride;
     return V;
   }
@@ -694,6 +694,7 @@ struct svmhip_batch {
   svmhip_solver *g = nullptr;  // owns the uploaded problem and every allocation
   struct Rowset {
     int n, nslices;
+    double width;            // mean stored entries per row (SELL slice widths)
     long long *slice_ptr;
     float *vals;
     int *cols, *len;
@@ -760,6 +761,7 @@ int svmhip_batch_add_rowset(svmhip_batch *b, const int *rows, int n) {
       for (int r = s * smo::SLICE; r < std::min(n, (s + 1) * smo::SLICE); ++r) w = std::max(w, h->host_row_len[rows[r]]);
       sp[s + 1] = sp[s] + (long long)w * smo::SLICE;
     }
+    rs.width = (double)sp.back() / ((double)rs.nslices * smo::SLICE);
     rs.slice_ptr = h->dalloc<long long>(sp.size(), "row-set slice pointers");
     rs.vals = h->dalloc<float>(std::max<long long>(sp.back(), 1), "row-set values");
     rs.cols = h->dense ? nullptr : h->dalloc<int>(std::max<long long>(sp.back(), 1), "row-set columns");
@@ -805,7 +807,7 @@ svmhip_status svmhip_batch_solve(svmhip_batch *b, svmhip_subproblem *subs, int n
       maxn = std::max(maxn, n);
     }
     if (nsub == 0) return SVMHIP_OK;
-    // pooled per-subproblem state
+    // pooled per-subproblem state, resident across the stages of the solve
     auto talloc = [&](size_t bytes, const char *what) {
       void *p = h->dalloc<char>(bytes, what);
       temps.push_back(p);
@@ -822,6 +824,7 @@ svmhip_status svmhip_batch_solve(svmhip_batch *b, svmhip_subproblem *subs, int n
     std::vector<double> hA(nv_total), hP(nv_total);
     std::vector<int8_t> hY(nv_total), hS(nv_total);
     std::vector<smo::View> views(nsub);
+    std::vector<int> true_max(nsub);
     long long off = 0;
     const smo::View base = h->view();
     for (int k = 0; k < nsub; ++k) {
@@ -840,14 +843,15 @@ svmhip_status svmhip_batch_solve(svmhip_batch *b, svmhip_subproblem *subs, int n
       V.serial = nullptr;
       V.gamma = sp.gamma;
       V.nvar = sp.n_var; V.svr = sp.n_var == 2 * rs.n ? 1 : 0; V.kind = sp.kind;
-      V.max_iter = h->opt.max_iter > 0 ? h->opt.max_iter : std::max(10000000, rs.n > INT_MAX / 100 ? INT_MAX : 100 * rs.n);
+      true_max[k] = h->opt.max_iter > 0 ? h->opt.max_iter : std::max(10000000, rs.n > INT_MAX / 100 ? INT_MAX : 100 * rs.n);
+      V.max_iter = true_max[k];
       V.Cp = sp.Cp; V.Cn = sp.Cn; V.eps = sp.eps;
       V.G = dG + off; V.alpha = dA + off; V.p = dP + off; V.QD = dQ + off; V.y = dY + off; V.status = dS + off;
       V.rows = nullptr; V.S = 0; V.full = 0;
       V.slot_of_row = V.row_of_slot = V.lru_prev = V.lru_next = nullptr;
       V.computed = nullptr;
       V.ctl = dC + k; V.part = nullptr; V.xpart = nullptr; V.xs_global = nullptr;
-      V.use_lds = 1; V.nblocks = 1; V.solo = 1;
+      V.use_lds = 1; V.nblocks = 1; V.rank = 0; V.resume = 0; V.iter0 = 0;
       views[k] = V;
       off += sp.n_var;
     }
@@ -856,78 +860,171 @@ svmhip_status svmhip_batch_solve(svmhip_batch *b, svmhip_subproblem *subs, int n
     h->upload(dG, hP.data(), nv_total, "upload batch G");
     h->upload(dY, hY.data(), nv_total, "upload batch y");
     h->upload(dS, hS.data(), nv_total, "upload batch status");
-    h->upload(dV, views.data(), nsub, "upload batch views");
-    h->ck(hipMemsetAsync(dC, 0, sizeof(smo::Ctl) * nsub, h->st), "clear batch control");
-    // cache arenas: one per resident workgroup
+    // one cache arena per resident workgroup
     int per_cu = 0;
     h->ck(smo_batch_capacity(h->shmem, &per_cu), "occupancy query");
-    int resident = std::max(1, per_cu) * device_cus(h->dev);
-    int K = std::min(nsub, resident + 8);
-    double want = h->opt.cache_bytes > 0 ? h->opt.cache_bytes : b->param_cache_mb * 1048576.0;
+    const int G = std::max(1, per_cu) * device_cus(h->dev);
+    const int K = G;
+    // arena size: opt.cache_bytes if set, else an equal share of 70% of the
+    // free device memory (kernel-row caching never changes a result, so the
+    // LIBSVM cache_size, sized for host memory, is not applied here)
     long long maxstride = ((long long)maxn + 63) / 64 * 64;
-    long long arena = std::max<long long>((long long)(want / sizeof(float)), 2 * maxstride);
     size_t freeb = 0, totb = 0;
-    if (hipMemGetInfo(&freeb, &totb) == hipSuccess) {
-      long long cap = (long long)(0.7 * (double)freeb / sizeof(float)) / K;
-      arena = std::max<long long>(std::min(arena, cap), 2 * maxstride);
-    }
+    h->ck(hipMemGetInfo(&freeb, &totb), "hipMemGetInfo");
+    const long long share = (long long)(0.7 * (double)freeb / sizeof(float)) / K;
+    long long arena = h->opt.cache_bytes > 0 ? (long long)(h->opt.cache_bytes / sizeof(float)) : share;
+    arena = std::max<long long>(std::min(arena, std::max<long long>(share, 2 * maxstride)), 2 * maxstride);
+    // no subproblem needs more than its whole matrix
+    arena = std::min<long long>(arena, std::max<long long>((long long)maxn * maxstride, 2 * maxstride));
     smo::BatchPool pool;
     pool.K = K;
-    pool.pad = 0;
+    pool.by_item = 0;
     pool.arena_floats = arena;
     pool.rows = (float *)talloc(sizeof(float) * (size_t)arena * K, "batch row-cache arenas");
     pool.flag_stride = maxn;
     pool.flags = (int8_t *)talloc((size_t)maxn * K, "batch cache flags");
     pool.lru_stride = 4LL * maxn + 2;
     pool.lru = (int *)talloc(sizeof(int) * (size_t)pool.lru_stride * K, "batch LRU bookkeeping");
-    pool.busy = (int *)talloc(sizeof(int) * K, "batch arena claims");
-    h->ck(hipMemsetAsync(pool.busy, 0, sizeof(int) * K, h->st), "clear arena claims");
-    // largest subproblems first (longest-processing-time order), one launch per rule
-    int *dOrd = (int *)talloc(sizeof(int) * nsub, "batch order");
-    std::vector<int> ord;
-    std::vector<int> count_kind(2, 0);
-    for (int kind = 0; kind < 2; ++kind) {
-      std::vector<int> o;
-      for (int k = 0; k < nsub; ++k)
-        if (subs[k].kind == kind) o.push_back(k);
-      std::stable_sort(o.begin(), o.end(), [&](int a, int c) { return subs[a].n_var > subs[c].n_var; });
-      count_kind[kind] = (int)o.size();
-      ord.insert(ord.end(), o.begin(), o.end());
-    }
-    h->upload(dOrd, ord.data(), ord.size(), "upload batch order");
-    h->ck(hipEventRecord(h->ev[0], h->st), "event");
-    h->ck(smo_launch_batch(smo::KIND_C, dV, dOrd, count_kind[0], &pool, h->shmem, h->st), "batched SMO kernel");
-    h->ck(smo_launch_batch(smo::KIND_NU, dV, dOrd + count_kind[0], count_kind[1], &pool, h->shmem, h->st),
-          "batched SMO kernel");
-    h->ck(hipEventRecord(h->ev[1], h->st), "event");
+    pool.queue = (int *)talloc(sizeof(int), "batch queue");
+    smo::XPart *dX = (smo::XPart *)talloc(sizeof(smo::XPart) * 2 * (size_t)G, "team exchange records");
+    int *dItems = (int *)talloc(sizeof(int) * 2 * (size_t)std::max(nsub, G), "batch items");
+
+    // Stages.  Every unfinished subproblem runs until it converges or reaches
+    // this stage's iteration cap, sized so that its estimated device time is
+    // about tau; tau grows 4x each stage.  While more subproblems remain than
+    // the grid has workgroups each is solved by one workgroup (largest first);
+    // once fewer remain, the grid is shared out as teams in proportion to
+    // their sizes.  The cap only decides where a solve pauses: a resumed
+    // subproblem continues the same sequence of working sets.
+    const int team_max = h->opt.grid_blocks > 0 ? std::min(h->opt.grid_blocks, 64) : 32;
+    const int fixed_chunk = h->opt.iters_per_launch;
+    std::vector<int> rem(nsub), iters(nsub, 0);
+    bool all_optimal = true;
+    for (int k = 0; k < nsub; ++k) rem[k] = k;
     std::vector<smo::Ctl> hC(nsub);
+    std::vector<int> T(nsub, 1), items;
+    double tau = 0.05;
+    int stages = 0;
+    float total_ms = 0;
+    auto est_iter_s = [&](int k, int t) {
+      const svmhip_batch::Rowset &rs = b->rowsets[subs[k].rowset];
+      double solo = 2.5e-9 * subs[k].n_var * (1.0 + rs.width / 16.0) + 1e-6;
+      return t > 1 ? solo / t + 6e-6 : solo;
+    };
+    while (!rem.empty()) {
+      // team sizes
+      // team shares in proportion to n_var x (iterations so far + 1): the
+      // subproblems that have run longest are the likeliest to run longest
+      double wsum = 0;
+      for (int k : rem) wsum += (double)subs[k].n_var * (iters[k] + 1.0);
+      int tsum = 0;
+      for (int k : rem) {
+        const svmhip_batch::Rowset &rs = b->rowsets[subs[k].rowset];
+        int t = 1;
+        if ((int)rem.size() < G && team_max > 1) {
+          t = (int)((double)G * subs[k].n_var * (iters[k] + 1.0) / wsum);
+          t = std::max(1, std::min({t, team_max, rs.nslices / 2}));
+        }
+        T[k] = t;
+        tsum += t;
+      }
+      // members given to the one-workgroup minimum can push the sum past the
+      // grid: take them back from the largest teams
+      while (tsum > G) {
+        int big = -1;
+        for (int k : rem)
+          if (T[k] > 1 && (big < 0 || T[k] > T[big])) big = k;
+        if (big < 0) break;
+        --T[big];
+        --tsum;
+      }
+      const bool teams = tsum > (int)rem.size();
+      if (teams && tsum > G) h->fail(SVMHIP_ERR_STATE, "team plan exceeds the grid");
+      // queue order: teams (largest first), then single workgroups (largest first)
+      std::vector<int> ord = rem;
+      std::stable_sort(ord.begin(), ord.end(), [&](int a, int c) {
+        if ((T[a] > 1) != (T[c] > 1)) return T[a] > 1;
+        return subs[a].n_var > subs[c].n_var;
+      });
+      items.clear();
+      int xoff = 0;
+      for (int k : ord) {
+        smo::View &V = views[k];
+        V.nblocks = T[k];
+        V.xpart = T[k] > 1 ? dX + xoff : nullptr;
+        if (T[k] > 1) xoff += 2 * T[k];
+        V.resume = stages > 0 ? 1 : 0;
+        V.iter0 = iters[k];
+        long long cap = fixed_chunk > 0 ? fixed_chunk : (long long)(tau / est_iter_s(k, T[k]));
+        cap = std::max<long long>(cap, 64);
+        V.max_iter = (int)std::min<long long>((long long)iters[k] + cap, true_max[k]);
+        for (int r = 0; r < T[k]; ++r) {
+          items.push_back(k);
+          items.push_back(r);
+        }
+      }
+      const int nitems = (int)items.size() / 2;
+      pool.by_item = teams ? 1 : 0;
+      h->upload(dV, views.data(), nsub, "upload batch views");
+      h->upload(dItems, items.data(), items.size(), "upload batch items");
+      h->ck(hipMemsetAsync(pool.queue, 0, sizeof(int), h->st), "clear batch queue");
+      if (xoff > 0) h->ck(hipMemsetAsync(dX, 0, sizeof(smo::XPart) * xoff, h->st), "clear team records");
+      h->ck(hipMemsetAsync(dC, 0, sizeof(smo::Ctl) * nsub, h->st), "clear batch control");
+      h->ck(hipEventRecord(h->ev[0], h->st), "event");
+      h->ck(smo_launch_batch(dV, dItems, nitems, std::min(G, nitems), &pool, h->shmem, h->st), "batched SMO kernel");
+      h->ck(hipEventRecord(h->ev[1], h->st), "event");
+      h->ck(hipMemcpyAsync(hC.data(), dC, sizeof(smo::Ctl) * nsub, hipMemcpyDeviceToHost, h->st), "fetch batch control");
+      h->ck(hipStreamSynchronize(h->st), "batched solve");
+      float ms = 0;
+      (void)hipEventElapsedTime(&ms, h->ev[0], h->ev[1]);
+      total_ms += ms;
+      ++stages;
+      std::vector<int> next;
+      for (int k : rem) {
+        const smo::Ctl &c = hC[k];
+        if (c.err) h->fail(SVMHIP_ERR_TIMEOUT, "subproblem %d: a team member did not respond", k);
+        iters[k] = c.iter;
+        tot.row_hits += (long long)c.hits;
+        tot.row_misses += (long long)c.misses;
+        svmhip_subproblem &sp = subs[k];
+        sp.rho = c.rho;
+        sp.obj = c.obj;
+        sp.r = c.r;
+        sp.iters = c.iter;
+        if (c.stop == smo::STOP_OPTIMAL) {
+          sp.status = SVMHIP_OK;
+        } else if (c.iter >= true_max[k]) {
+          sp.status = SVMHIP_ERR_MAXITER;
+          all_optimal = false;
+        } else {
+          next.push_back(k);
+        }
+      }
+      if (h->opt.verbose) {
+        int nteam = 0, tmax = 1;
+        for (int k : rem)
+          if (T[k] > 1) { ++nteam; tmax = std::max(tmax, T[k]); }
+        fprintf(stderr, "[svmhip] batch stage %d: %zu subproblems (%d teams, largest %d), %d items, %.1f ms, %zu left\n",
+                stages, rem.size(), nteam, tmax, nitems, ms, next.size());
+      }
+      rem.swap(next);
+      tau *= 4;
+    }
     h->ck(hipMemcpyAsync(hA.data(), dA, sizeof(double) * nv_total, hipMemcpyDeviceToHost, h->st), "fetch batch alpha");
-    h->ck(hipMemcpyAsync(hC.data(), dC, sizeof(smo::Ctl) * nsub, hipMemcpyDeviceToHost, h->st), "fetch batch control");
-    h->ck(hipStreamSynchronize(h->st), "batched solve");
-    float ms = 0;
-    (void)hipEventElapsedTime(&ms, h->ev[0], h->ev[1]);
-    tot.solve_ms = ms;
-    tot.kernel_ms = ms;
-    tot.launches = (count_kind[0] > 0) + (count_kind[1] > 0);
-    tot.optimal = 1;
+    h->ck(hipStreamSynchronize(h->st), "fetch batch alpha");
+    tot.solve_ms = total_ms;
+    tot.kernel_ms = total_ms;
+    tot.launches = stages;
+    tot.optimal = all_optimal ? 1 : 0;
     off = 0;
     for (int k = 0; k < nsub; ++k) {
       svmhip_subproblem &sp = subs[k];
       for (int v = 0; v < sp.n_var; ++v) sp.alpha[v] = hA[off + v];
       off += sp.n_var;
-      sp.rho = hC[k].rho;
-      sp.obj = hC[k].obj;
-      sp.r = hC[k].r;
-      sp.iters = hC[k].iter;
-      sp.status = hC[k].err ? SVMHIP_ERR_TIMEOUT : (hC[k].stop == smo::STOP_OPTIMAL ? SVMHIP_OK : SVMHIP_ERR_MAXITER);
-      tot.iters += hC[k].iter;
-      tot.row_hits += (long long)hC[k].hits;
-      tot.row_misses += (long long)hC[k].misses;
-      if (hC[k].stop != smo::STOP_OPTIMAL) tot.optimal = 0;
-      if (hC[k].err) h->fail(SVMHIP_ERR_TIMEOUT, "subproblem %d: cache arena unavailable", k);
+      tot.iters += sp.iters;
     }
     tot.cache_slots = (int)std::min<long long>(INT_MAX, arena / maxstride);
-    tot.grid_blocks = resident;
+    tot.grid_blocks = G;
     GUARD_END
   }();
   for (void *p : temps) h->dfree(p);

@@ -414,11 +414,16 @@ class BatchSolver:
     """Many subproblems over row subsets of one uploaded problem, solved in
     one launch (one workgroup each): the svmhip_batch_* C-ABI."""
 
-    def __init__(self, problem: Problem, param, cache_bytes=0.0, max_iter=0, device=0):
+    def __init__(self, problem: Problem, param, cache_bytes=0.0, max_iter=0, device=0, team_max=0, chunk=0,
+                 verbose=0):
+        """team_max: largest team of workgroups per subproblem (0 = default,
+        1 = one workgroup each); chunk: iterations per subproblem per launch
+        (0 = sized by the scheduler)."""
         L = lib()
         self.problem = problem
         self.param = param
-        opt = svmhip_options(C.sizeof(svmhip_options), device, 0, 0, max_iter, float(cache_bytes), 0, 0)
+        opt = svmhip_options(C.sizeof(svmhip_options), device, team_max, 0, max_iter, float(cache_bytes), chunk,
+                             verbose)
         self._h = C.c_void_p()
         rc = L.svmhip_batch_create(C.byref(problem.struct), C.byref(param), C.byref(opt), C.byref(self._h))
         if rc != 0:

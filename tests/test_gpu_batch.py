@@ -1,6 +1,7 @@
-"""Batched solves (one workgroup per subproblem) against the oracle: each
-subproblem of a batch must give the same iterates as the oracle on the same
-row subset (identical iteration count, alpha to 1e-9, objective to 1e-9)."""
+"""Batched solves against the oracle: each subproblem of a batch, whether
+one workgroup or a team of them solves it and however many launches it is
+split over, must give the same iterates as the oracle on the same row subset
+(identical iteration count, alpha to 1e-9, objective to 1e-9)."""
 import numpy as np
 import pytest
 
@@ -9,12 +10,14 @@ from libsvm_hip import datasets, ffi
 pytestmark = pytest.mark.gpu
 
 
-def test_batch_subsets_match_oracle(hip, ora):
+@pytest.mark.parametrize("team_max,chunk", [(0, 0), (1, 0), (1, 53), (4, 37), (16, 0)])
+def test_batch_subsets_match_oracle(hip, ora, team_max, chunk):
     ip, ix, v, y, d = datasets.zipf_binary(n=3000, seed=9)
     prob = ffi.Problem(ip, ix, v, y, d)
     prm = ffi.make_param(gamma=1.0 / d, C_=1.0, eps=1e-3, shrinking=0)
     rng = np.random.default_rng(0)
-    b = hip.BatchSolver(prob, prm)
+    print(f"  team_max={team_max} chunk={chunk}", flush=True)
+    b = hip.BatchSolver(prob, prm, team_max=team_max, chunk=chunk)
     cases = []
     for k in range(12):
         n = int(rng.integers(200, 2500))
@@ -41,15 +44,19 @@ def test_batch_subsets_match_oracle(hip, ora):
         assert abs(obj - ref["obj"]) <= 1e-9 * abs(ref["obj"])
         assert abs(rho - ref["rho"]) <= 1e-9
     assert st["iters"] == sum(r["iters"] for r in res)
+    if chunk:
+        assert st["launches"] > 1
 
 
-def test_batch_small_arena_lru(hip, ora):
-    """Arenas too small for a full cache: LRU eviction inside each workgroup."""
+@pytest.mark.parametrize("team_max", [1, 0])
+def test_batch_small_arena_lru(hip, ora, team_max):
+    """Arenas too small for a full cache: LRU eviction inside each workgroup
+    (team_max 1) or across a team's shared arena (team_max 0)."""
     ip, ix, v, y, d = datasets.zipf_binary(n=2000, seed=2)
     prob = ffi.Problem(ip, ix, v, y, d)
     prm = ffi.make_param(gamma=1.0 / d, C_=1.0, eps=1e-3, shrinking=0)
     rows = np.arange(0, 2000, 2)
-    b = hip.BatchSolver(prob, prm, cache_bytes=4 * 1024 * 5)  # 5 rows of 1024 floats
+    b = hip.BatchSolver(prob, prm, cache_bytes=4 * 1024 * 5, team_max=team_max)  # 5 rows of 1024 floats
     rid = b.add_rowset(rows)
     sub = prob.subset(rows)
     yy, p, a0, Cp, Cn, kind = hip.formulate(sub, prm)
