@@ -147,8 +147,12 @@ const char *svmhip_last_error(const svmhip_solver *h);
  * Batched solves.  No counterpart in the reference, which trains the
  * k(k-1)/2 one-vs-one pairs and every cross-validation fold one after the
  * other (libsvm/svm.cpp:2246-2295, :2491-2521).  Here many independent
- * subproblems over row subsets of one problem run in one launch, one
- * workgroup per subproblem, each with its own alpha/G/kernel-row cache.
+ * subproblems over row subsets of one problem are in flight together, each
+ * solved by one workgroup or by a team of workgroups, each with its own
+ * alpha/G/kernel-row cache.  Options: grid_blocks = largest team (0 = 32,
+ * 1 = one workgroup per subproblem); iters_per_launch = iterations per
+ * subproblem per launch (0 = sized by the scheduler); cache_bytes = arena
+ * per workgroup (0 = an equal share of 70% of free device memory).
  * ------------------------------------------------------------------------- */
 
 typedef struct svmhip_batch svmhip_batch;
@@ -181,9 +185,12 @@ svmhip_status svmhip_batch_create(const struct svm_problem *prob, const struct s
  * its rows are packed on the device.  Returns the id (>= 0) or -status. */
 int svmhip_batch_add_rowset(svmhip_batch *b, const int *rows, int n);
 
-/* Solves all subproblems; per-subproblem kernel-row cache of
- * opt.cache_bytes (or param.cache_size MB).  stats (may be null) sums
- * iterations and cache counters; solve_ms is the batch's device time. */
+/* Solves all subproblems.  The launches run in stages: a subproblem pauses
+ * at its stage's iteration cap and resumes in the next stage from the same
+ * state (the iterates are those of an uninterrupted solve); unfinished
+ * subproblems are re-planned into teams as the set shrinks.  stats (may be
+ * null) sums iterations and cache counters; solve_ms is the device time of
+ * all stages, launches the number of stages. */
 svmhip_status svmhip_batch_solve(svmhip_batch *b, svmhip_subproblem *subs, int nsub, svmhip_stats *stats);
 
 void svmhip_batch_destroy(svmhip_batch *b);
@@ -199,6 +206,74 @@ const char *svmhip_batch_last_error(const svmhip_batch *b);
  * kernels and rows wider than 16384 features return SVMHIP_ERR_ARG. */
 svmhip_status svmhip_predict_batch(const struct svm_model *model, const struct svm_node *const *x, int n,
 				   const svmhip_options *opt, double *dec_values, double *labels);
+
+/* ---------------------------------------------------------------------------
+ * Serving.  No counterpart in the reference (one solve per process, host
+ * driven).  A server keeps many requests on the device at once: binary
+ * training solves (C_SVC or NU_SVC, as svm_train_one runs them,
+ * libsvm/svm.cpp:1707-1769) and predictions against the models of finished
+ * solves.  Requests are admitted first come first served while their device
+ * footprint (rows, solver state, kernel-row cache) fits the HBM budget;
+ * every svmhip_server_step admits what fits, runs the admitted predictions,
+ * then advances every admitted solve by one stage of the batched kernel
+ * (about stage_ms of device time each), so requests that arrive while others
+ * run join at the next stage.  A solve's state and cache stay on the device
+ * between stages and its iterates do not depend on what else is in flight.
+ * ------------------------------------------------------------------------- */
+
+typedef struct svmhip_server svmhip_server;
+
+typedef struct {
+	int id;                   /* request id */
+	int type;                 /* 0 = train, 1 = predict */
+	int status;               /* svmhip_status */
+	int n;                    /* rows of the request */
+	int iters;                /* train: SMO iterations */
+	int nSV;                  /* train: support vectors (the resident model) */
+	double rho, obj, r;       /* train: as Solve reports them (r: nu only) */
+	double device_ms;         /* device time of the stages/launches it took part in */
+} svmhip_completion;
+
+/* hbm_budget_bytes = 0: 80% of the free device memory.  opt as for the
+ * batched solver; cache_bytes caps each solve's kernel-row cache
+ * (0 = its whole matrix up to 2 GiB). */
+svmhip_status svmhip_server_create(const svmhip_options *opt, double hbm_budget_bytes, svmhip_server **out);
+
+/* Queues a binary training solve of prob with param (C_SVC or NU_SVC,
+ * exactly two labels, kernel other than PRECOMPUTED).  The rows are packed
+ * on the host now and uploaded at admission.  Returns the id (>= 0) or
+ * -status. */
+int svmhip_server_submit_train(svmhip_server *s, const struct svm_problem *prob, const struct svm_parameter *param);
+
+/* Queues the decision values of n rows against the resident model of the
+ * finished training request model_id.  Returns the id or -status. */
+int svmhip_server_submit_predict(svmhip_server *s, int model_id, const struct svm_node *const *x, int n);
+
+/* One round: admit, predict, one stage of every admitted solve.  stage_ms
+ * <= 0 uses 10 ms.  *active / *queued (may be null) receive the counts after
+ * the round. */
+svmhip_status svmhip_server_step(svmhip_server *s, double stage_ms, int *active, int *queued);
+
+/* Completions since the last call, at most max; returns their number. */
+int svmhip_server_poll(svmhip_server *s, svmhip_completion *out, int max);
+
+/* Results of a finished request: a train request's signed coefficients
+ * (alpha_i * y_i, scaled by 1/r for NU_SVC, as svm_train_one leaves them;
+ * n = its rows) and labels[2] (label of y = +1, label of y = -1); a predict
+ * request's decision values (n = its rows) and predicted labels (may be
+ * null). */
+svmhip_status svmhip_server_train_result(svmhip_server *s, int id, double *coef, int n, double *labels2);
+svmhip_status svmhip_server_predict_result(svmhip_server *s, int id, double *dec_values, double *labels, int n);
+
+/* Frees a finished request (for a train request: its resident model). */
+svmhip_status svmhip_server_release(svmhip_server *s, int id);
+
+/* Device bytes held by admitted requests and resident models. */
+double svmhip_server_bytes_in_use(const svmhip_server *s);
+
+void svmhip_server_destroy(svmhip_server *s);
+
+const char *svmhip_server_last_error(const svmhip_server *s);
 
 /* Number of usable HIP devices (0 if none); never aborts. */
 int svmhip_device_count(void);

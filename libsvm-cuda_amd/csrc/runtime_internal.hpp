@@ -1,0 +1,221 @@
+// Internals shared by the runtime files (smo_runtime.cpp, serve.cpp): the
+// solver handle with its error / allocation helpers, the kernel launchers
+// and the SELL-64 packing of svm_node rows.
+#pragma once
+#include <hip/hip_runtime.h>
+#include <math.h>
+#include <stdarg.h>
+#include <stdio.h>
+#include <string.h>
+
+#include <algorithm>
+#include <climits>
+#include <string>
+#include <vector>
+
+#include "../../include/svm_hip.h"
+#include "smo_device.hpp"
+
+extern "C" {
+hipError_t smo_launch_self_dot(const smo::View *V, double *xsq, hipStream_t st);
+hipError_t smo_launch_qd(const smo::View *V, hipStream_t st);
+hipError_t smo_launch_ginit_row(const smo::View *V, int v, size_t shmem, hipStream_t st);
+hipError_t smo_launch_select_i(const smo::View *V, size_t shmem, hipStream_t st);
+hipError_t smo_launch_select_j(const smo::View *V, size_t shmem, hipStream_t st);
+hipError_t smo_launch_alpha(const smo::View *V, hipStream_t st);
+hipError_t smo_launch_gupdate(const smo::View *V, size_t shmem, hipStream_t st);
+hipError_t smo_launch_status(const smo::View *V, hipStream_t st);
+hipError_t smo_launch_persistent(const smo::View *V, int iter_limit, size_t shmem, hipStream_t st);
+hipError_t smo_launch_lru_private_init(const smo::View *V, hipStream_t st);
+hipError_t smo_persistent_capacity(int kind, size_t shmem, int *blocks_per_cu);
+hipError_t smo_launch_finalize(const smo::View *V, hipStream_t st);
+hipError_t smo_launch_kernel_row(const smo::View *V, int q, float *out, size_t shmem, hipStream_t st);
+hipError_t smo_launch_batch(const smo::View *views, const int *items, int nitems, int grid, const smo::BatchPool *pool,
+                            size_t shmem, hipStream_t st);
+hipError_t smo_batch_capacity(size_t shmem, int *blocks_per_cu);
+hipError_t smo_launch_predict_offset(const smo::PredictArgs *A, int first_row, int nblocks, size_t shmem,
+                                    hipStream_t st);
+hipError_t smo_launch_pack_rowset(const smo::View *src, const int *rows, int n, const long long *dst_slice_ptr,
+                                  float *dst_vals, int *dst_cols, int *dst_len, double *dst_xsq, hipStream_t st);
+}
+
+namespace svmhip_detail {
+
+constexpr size_t kLdsQueryLimit = 64 * 1024;  // query row staged in LDS up to this size
+
+struct Failure {
+  svmhip_status code;
+};
+
+}  // namespace svmhip_detail
+using namespace svmhip_detail;
+
+struct svmhip_solver {
+  int dev = 0;
+  hipStream_t st = nullptr;
+  hipEvent_t ev[4] = {nullptr, nullptr, nullptr, nullptr};
+  svmhip_kind kind = SVMHIP_KIND_C;
+  svmhip_options opt{};
+  int svm_type = C_SVC;
+  double param_cache_mb = 100;
+  double eps = 1e-3;
+  // problem
+  int l = 0, d = 0, nslices = 0, dense = 0, ktype = RBF, degree = 3;
+  double gamma = 0, coef0 = 0;
+  long long total = 0;
+  std::vector<int> host_row_len;  // SELL row lengths (kept for packing row sets)
+  // solver
+  int nvar = 0;
+  double Cp = 0, Cn = 0;
+  int S = 0, full = 0;
+  long long row_stride = 0;
+  int nblocks = 1;
+  size_t shmem = 0;
+  int use_lds = 1;
+  bool rbf_ready = false, setup_done = false, have_j = false;
+  smo::Ctl *host_ctl = nullptr;  // pinned mirror
+  // device memory
+  long long *d_slice_ptr = nullptr;
+  float *d_vals = nullptr;
+  int *d_cols = nullptr, *d_row_len = nullptr, *d_serial = nullptr;
+  double *d_xsq = nullptr, *d_G = nullptr, *d_alpha = nullptr, *d_p = nullptr, *d_QD = nullptr;
+  int8_t *d_y = nullptr, *d_status = nullptr, *d_computed = nullptr;
+  float *d_rows = nullptr, *d_xs = nullptr;
+  int *d_slot_of_row = nullptr, *d_row_of_slot = nullptr, *d_prev = nullptr, *d_next = nullptr;
+  smo::Ctl *d_ctl = nullptr;
+  smo::Partial *d_part = nullptr;
+  smo::XPart *d_xpart = nullptr;
+  int8_t *d_pm_flags = nullptr;
+  int *d_pm_lru = nullptr;
+  long long pm_stride = 0;
+  bool shared_meta_dirty = false;  // the persistent kernel overwrote cache slots
+  std::vector<void *> allocs;
+  svmhip_stats stats{};
+  std::string err;
+
+  void fail(svmhip_status code, const char *fmt, ...) {
+    char buf[512];
+    va_list ap;
+    va_start(ap, fmt);
+    vsnprintf(buf, sizeof buf, fmt, ap);
+    va_end(ap);
+    err = buf;
+    throw Failure{code};
+  }
+  void ck(hipError_t e, const char *what) {
+    if (e != hipSuccess) {
+      (void)hipGetLastError();
+      fail(e == hipErrorOutOfMemory ? SVMHIP_ERR_NOMEM : SVMHIP_ERR_DEVICE, "%s: %s", what,
+           hipGetErrorString(e));
+    }
+  }
+  template <class T>
+  T *dalloc(size_t n, const char *what) {
+    void *p = nullptr;
+    size_t bytes = std::max<size_t>(n, 1) * sizeof(T);
+    hipError_t e = hipMalloc(&p, bytes);
+    if (e != hipSuccess) {
+      (void)hipGetLastError();
+      fail(SVMHIP_ERR_NOMEM, "device allocation of %zu bytes for %s failed: %s", bytes, what,
+           hipGetErrorString(e));
+    }
+    allocs.push_back(p);
+    return (T *)p;
+  }
+  void dfree(void *p) {
+    if (!p) return;
+    auto it = std::find(allocs.begin(), allocs.end(), p);
+    if (it != allocs.end()) allocs.erase(it);
+    (void)hipFree(p);
+  }
+  template <class T>
+  void upload(T *dst, const T *src, size_t n, const char *what) {
+    if (n) ck(hipMemcpyAsync(dst, src, n * sizeof(T), hipMemcpyHostToDevice, st), what);
+  }
+
+  smo::View view() const {
+    smo::View V;
+    memset(&V, 0, sizeof V);
+    V.l = l; V.d = d; V.nslices = nslices; V.dense = dense;
+    V.ktype = ktype; V.degree = degree; V.gamma = gamma; V.coef0 = coef0;
+    V.slice_ptr = d_slice_ptr; V.vals = d_vals; V.cols = d_cols; V.row_len = d_row_len;
+    V.xsq = d_xsq; V.serial = d_serial;
+    V.nvar = nvar; V.svr = (nvar == 2 * l && l > 0) ? 1 : 0; V.kind = (int)kind;
+    V.max_iter = host_ctl ? host_ctl->max_iter : 0;
+    V.Cp = Cp; V.Cn = Cn; V.eps = eps;
+    V.G = d_G; V.alpha = d_alpha; V.p = d_p; V.QD = d_QD; V.y = d_y; V.status = d_status;
+    V.rows = d_rows; V.row_stride = row_stride; V.S = S; V.full = full;
+    V.slot_of_row = d_slot_of_row; V.row_of_slot = d_row_of_slot; V.lru_prev = d_prev; V.lru_next = d_next;
+    V.computed = d_computed;
+    V.ctl = d_ctl; V.part = d_part; V.xs_global = d_xs;
+    V.use_lds = use_lds; V.nblocks = nblocks; V.rank = -1;
+    V.xpart = d_xpart; V.pm_flags = d_pm_flags; V.pm_lru = d_pm_lru; V.pm_stride = pm_stride;
+    return V;
+  }
+
+  // Frees what svmhip_setup allocated, keeping the uploaded problem, so one
+  // handle can run several solves over the same rows.
+  void release_solver_state() {
+    (void)hipStreamSynchronize(st);
+    for (void **p : {(void **)&d_G, (void **)&d_alpha, (void **)&d_p, (void **)&d_QD, (void **)&d_y,
+                     (void **)&d_status, (void **)&d_rows, (void **)&d_computed, (void **)&d_slot_of_row,
+                     (void **)&d_row_of_slot, (void **)&d_prev, (void **)&d_next, (void **)&d_pm_flags,
+                     (void **)&d_pm_lru}) {
+      dfree(*p);
+      *p = nullptr;
+    }
+    setup_done = false;
+    have_j = false;
+    stats = svmhip_stats{};
+  }
+
+  void read_ctl() {
+    ck(hipMemcpyAsync(host_ctl, d_ctl, sizeof(smo::Ctl), hipMemcpyDeviceToHost, st), "read control record");
+    ck(hipStreamSynchronize(st), "synchronise");
+  }
+
+  ~svmhip_solver() {
+    if (dev >= 0) (void)hipSetDevice(dev);
+    if (st) (void)hipStreamSynchronize(st);
+    for (void *p : allocs) (void)hipFree(p);
+    if (host_ctl) (void)hipHostFree(host_ctl);
+    for (auto &e : ev)
+      if (e) (void)hipEventDestroy(e);
+    if (st) (void)hipStreamDestroy(st);
+  }
+};
+
+#define GUARD_BEGIN try {
+#define GUARD_END                          \
+  }                                        \
+  catch (const Failure &f) {               \
+    return f.code;                         \
+  }                                        \
+  catch (const std::bad_alloc &) {         \
+    h->err = "host allocation failed";     \
+    return SVMHIP_ERR_NOMEM;               \
+  }                                        \
+  return SVMHIP_OK;
+
+
+// Rows of a problem packed on the host in the SELL-64 layout
+// (see smo_device.hpp), ready for upload.
+struct HostSell {
+  int l = 0, d = 1, nslices = 0, dense = 0;
+  long long total = 0;
+  std::vector<long long> sp;   // nslices + 1 slice offsets
+  std::vector<float> vals;
+  std::vector<int> cols;       // empty when dense
+  std::vector<int> rl;         // stored entries per row
+  std::vector<int> serial;     // PRECOMPUTED only
+};
+
+// Packs prob's rows (features truncated to float32, as the reference does on
+// upload, libsvm/cuda_solver.cpp:366); errors go through h->fail.
+void make_host_sell(svmhip_solver *h, const svm_problem *prob, int ktype, HostSell &out);
+
+inline int device_cus(int dev) {
+  int cus = 0;
+  if (hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, dev) != hipSuccess) cus = 0;
+  return cus > 0 ? cus : 1;
+}

@@ -101,6 +101,9 @@ struct View {
   int rank;               // batched solves: this workgroup's owner index (-1: blockIdx.x)
   int resume;             // batched solves: G, alpha, status and QD carry over from an earlier stage
   int iter0;              // batched solves: iterations already done
+  int own_cache;          // batched solves: rows/S/full/row_stride and the cache metadata
+                          // (nblocks copies, pm_stride apart) are the subproblem's own
+  int keep_cache;         // batched solves: the own cache is valid from an earlier stage
   int pad_rank;
 };
 

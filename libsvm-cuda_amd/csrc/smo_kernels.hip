@@ -370,6 +370,7 @@ __global__ void __launch_bounds__(BLOCK) k_kernel_row(View V, int q, float *out)
 // which continues the same sequence of working sets.
 
 __device__ __forceinline__ void solo_init_cache(View &V) {
+  if (V.keep_cache) return;
   if (V.full) {
     for (int r = threadIdx.x; r < V.l; r += blockDim.x) V.pm_flags[r] = 0;
   } else {
@@ -620,16 +621,25 @@ __global__ void __launch_bounds__(BLOCK) k_smo_batch(const View *views, const in
     const int sub = __builtin_amdgcn_readfirstlane(items[2 * q]);
     View V = views[sub];
     V.rank = __builtin_amdgcn_readfirstlane(items[2 * q + 1]);
-    const long long own = pool.by_item ? q : (long long)blockIdx.x;
-    const long long first = pool.by_item ? own - V.rank : own;
-    const int team = pool.by_item ? V.nblocks : 1;
-    V.rows = pool.rows + first * pool.arena_floats;
-    V.row_stride = ((long long)V.l + 63) / 64 * 64;
-    const long long fit = team * pool.arena_floats / (V.row_stride > 0 ? V.row_stride : 1);
-    if (fit >= V.l) { V.full = 1; V.S = V.l > 0 ? V.l : 1; }
-    else { V.full = 0; V.S = (int)(fit < 2 ? 2 : fit); }
-    V.pm_flags = pool.flags + own * pool.flag_stride;
-    V.pm_lru = pool.lru + own * pool.lru_stride;
+    int team;
+    if (V.own_cache) {
+      // the subproblem's own arena; member r keeps metadata copy r
+      team = V.nblocks;
+      V.pm_flags += (long long)V.rank * V.l;
+      V.pm_lru += (long long)V.rank * V.pm_stride;
+    } else {
+      const long long own = pool.by_item ? q : (long long)blockIdx.x;
+      const long long first = pool.by_item ? own - V.rank : own;
+      team = pool.by_item ? V.nblocks : 1;
+      V.rows = pool.rows + first * pool.arena_floats;
+      V.row_stride = ((long long)V.l + 63) / 64 * 64;
+      const long long fit = team * pool.arena_floats / (V.row_stride > 0 ? V.row_stride : 1);
+      if (fit >= V.l) { V.full = 1; V.S = V.l > 0 ? V.l : 1; }
+      else { V.full = 0; V.S = (int)(fit < 2 ? 2 : fit); }
+      V.pm_flags = pool.flags + own * pool.flag_stride;
+      V.pm_lru = pool.lru + own * pool.lru_stride;
+      V.keep_cache = 0;
+    }
     V.pm_stride = 0;
     if (team > 1) {
       if (V.kind == KIND_C) team_solve<KIND_C>(V, d, sm, xsm, xs);

@@ -54,6 +54,12 @@ class svmhip_subproblem(C.Structure):
                 ("obj", C.c_double), ("r", C.c_double), ("iters", C.c_int), ("status", C.c_int)]
 
 
+class svmhip_completion(C.Structure):
+    _fields_ = [("id", C.c_int), ("type", C.c_int), ("status", C.c_int), ("n", C.c_int),
+                ("iters", C.c_int), ("nSV", C.c_int), ("rho", C.c_double), ("obj", C.c_double),
+                ("r", C.c_double), ("device_ms", C.c_double)]
+
+
 # every extern "C" symbol declared in include/svm.h and include/svm_hip.h
 SVM_H_SYMBOLS = [
     "svm_train", "svm_cross_validation", "svm_save_model", "svm_load_model", "svm_get_svm_type",
@@ -67,7 +73,11 @@ SVM_HIP_SYMBOLS = [
     "svmhip_compute_alpha", "svmhip_update_gradient", "svmhip_update_alpha_status", "svmhip_fetch",
     "svmhip_solve", "svmhip_finalize", "svmhip_get_stats", "svmhip_kernel_row", "svmhip_destroy",
     "svmhip_last_error", "svmhip_device_count", "svmhip_batch_create", "svmhip_batch_add_rowset",
-    "svmhip_batch_solve", "svmhip_batch_destroy", "svmhip_batch_last_error", "svmhip_predict_batch"]
+    "svmhip_batch_solve", "svmhip_batch_destroy", "svmhip_batch_last_error", "svmhip_predict_batch",
+    "svmhip_server_create", "svmhip_server_submit_train", "svmhip_server_submit_predict",
+    "svmhip_server_step", "svmhip_server_poll", "svmhip_server_train_result",
+    "svmhip_server_predict_result", "svmhip_server_release", "svmhip_server_bytes_in_use",
+    "svmhip_server_destroy", "svmhip_server_last_error"]
 
 _lib = None
 PRINT_FUNC = C.CFUNCTYPE(None, C.c_char_p)
@@ -130,6 +140,28 @@ def lib():
     L.svmhip_batch_last_error.restype = C.c_char_p
     L.svmhip_predict_batch.argtypes = [P(ffi.svm_model), P(P(ffi.svm_node)), I, P(svmhip_options), P(D), P(D)]
     L.svmhip_predict_batch.restype = I
+    L.svmhip_server_create.argtypes = [P(svmhip_options), D, P(h)]
+    L.svmhip_server_create.restype = I
+    L.svmhip_server_submit_train.argtypes = [h, P(ffi.svm_problem), P(ffi.svm_parameter)]
+    L.svmhip_server_submit_train.restype = I
+    L.svmhip_server_submit_predict.argtypes = [h, I, P(P(ffi.svm_node)), I]
+    L.svmhip_server_submit_predict.restype = I
+    L.svmhip_server_step.argtypes = [h, D, P(I), P(I)]
+    L.svmhip_server_step.restype = I
+    L.svmhip_server_poll.argtypes = [h, P(svmhip_completion), I]
+    L.svmhip_server_poll.restype = I
+    L.svmhip_server_train_result.argtypes = [h, I, P(D), I, P(D)]
+    L.svmhip_server_train_result.restype = I
+    L.svmhip_server_predict_result.argtypes = [h, I, P(D), P(D), I]
+    L.svmhip_server_predict_result.restype = I
+    L.svmhip_server_release.argtypes = [h, I]
+    L.svmhip_server_release.restype = I
+    L.svmhip_server_bytes_in_use.argtypes = [h]
+    L.svmhip_server_bytes_in_use.restype = D
+    L.svmhip_server_destroy.argtypes = [h]
+    L.svmhip_server_destroy.restype = None
+    L.svmhip_server_last_error.argtypes = [h]
+    L.svmhip_server_last_error.restype = C.c_char_p
     for name in ("svmhip_create", "svmhip_setup_rbf", "svmhip_setup", "svmhip_select_working_set",
                  "svmhip_compute_alpha", "svmhip_update_gradient", "svmhip_update_alpha_status",
                  "svmhip_fetch", "svmhip_solve", "svmhip_finalize", "svmhip_get_stats",
@@ -498,3 +530,91 @@ def svm_cross_validation(problem: Problem, param, nr_fold):
     target = np.zeros(problem.l)
     lib().svm_cross_validation(C.byref(problem.struct), C.byref(param), nr_fold, _ptr(target, C.c_double))
     return target
+
+
+class Server:
+    """Device-resident request server (the svmhip_server_* C-ABI): binary
+    C-SVC / nu-SVC training solves and predictions against their resident
+    models, admitted by HBM budget and advanced together one stage per step."""
+
+    def __init__(self, budget_bytes=0.0, cache_bytes=0.0, team_max=0, max_iter=0, device=0):
+        L = lib()
+        opt = svmhip_options(C.sizeof(svmhip_options), device, team_max, 0, max_iter, float(cache_bytes), 0, 0)
+        self._h = C.c_void_p()
+        rc = L.svmhip_server_create(C.byref(opt), float(budget_bytes), C.byref(self._h))
+        if rc != 0:
+            raise SolverError(f"svmhip_server_create failed: {STATUS_NAMES.get(rc, rc)}")
+        self._keep = {}
+
+    def _err(self):
+        return lib().svmhip_server_last_error(self._h).decode()
+
+    def submit_train(self, problem: Problem, param):
+        rid = lib().svmhip_server_submit_train(self._h, C.byref(problem.struct), C.byref(param))
+        if rid < 0:
+            raise SolverError(f"submit_train: {STATUS_NAMES.get(-rid, -rid)}: {self._err()}")
+        return rid
+
+    def submit_predict(self, model_id, problem: Problem):
+        rid = lib().svmhip_server_submit_predict(self._h, model_id, problem.struct.x, problem.l)
+        if rid < 0:
+            raise SolverError(f"submit_predict: {STATUS_NAMES.get(-rid, -rid)}: {self._err()}")
+        return rid
+
+    def step(self, stage_ms=10.0):
+        a, q = C.c_int(), C.c_int()
+        rc = lib().svmhip_server_step(self._h, float(stage_ms), C.byref(a), C.byref(q))
+        if rc != 0:
+            raise SolverError(f"server step: {STATUS_NAMES.get(rc, rc)}: {self._err()}")
+        return a.value, q.value
+
+    def poll(self, max_n=4096):
+        buf = (svmhip_completion * max_n)()
+        n = lib().svmhip_server_poll(self._h, buf, max_n)
+        return [dict(id=c.id, type=c.type, status=c.status, n=c.n, iters=c.iters, nSV=c.nSV, rho=c.rho,
+                     obj=c.obj, r=c.r, device_ms=c.device_ms) for c in buf[:n]]
+
+    def train_result(self, rid, n):
+        coef = np.zeros(n)
+        lab = np.zeros(2)
+        rc = lib().svmhip_server_train_result(self._h, rid, _ptr(coef, C.c_double), n, _ptr(lab, C.c_double))
+        if rc != 0:
+            raise SolverError(f"train_result: {STATUS_NAMES.get(rc, rc)}: {self._err()}")
+        return coef, lab
+
+    def predict_result(self, rid, n):
+        dec = np.zeros(n)
+        lab = np.zeros(n)
+        rc = lib().svmhip_server_predict_result(self._h, rid, _ptr(dec, C.c_double), _ptr(lab, C.c_double), n)
+        if rc != 0:
+            raise SolverError(f"predict_result: {STATUS_NAMES.get(rc, rc)}: {self._err()}")
+        return dec, lab
+
+    def release(self, rid):
+        rc = lib().svmhip_server_release(self._h, rid)
+        if rc != 0:
+            raise SolverError(f"release: {STATUS_NAMES.get(rc, rc)}: {self._err()}")
+
+    def bytes_in_use(self):
+        return lib().svmhip_server_bytes_in_use(self._h)
+
+    def run_until_idle(self, stage_ms=10.0, max_steps=100000):
+        """Steps until nothing is active or queued; returns all completions."""
+        out = []
+        for _ in range(max_steps):
+            a, q = self.step(stage_ms)
+            out += self.poll()
+            if a == 0 and q == 0:
+                return out
+        raise SolverError("server did not drain")
+
+    def close(self):
+        if self._h:
+            lib().svmhip_server_destroy(self._h)
+            self._h = C.c_void_p()
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass

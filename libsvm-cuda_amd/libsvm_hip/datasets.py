@@ -137,14 +137,16 @@ def clusters_10class(classes=10, d=128, lo=500, hi=12000, total_range=(36000, 44
     return X, y, fold
 
 
-def serving_request(k, base_seed=1005):
+def serving_request(k, base_seed=1005, n_range=(500, 20000)):
     """Request k of the serving stream: C-SVC (80%) or nu-SVC (nu = 0.3),
-    n log-uniform in [500, 20000], d in {32, 128, 512}, CSR density in
+    n log-uniform in n_range, d in {32, 128, 512}, CSR density in
     {0.05, 0.3, 1.0}, U(0,1) features, labels from a sparse random
-    hyperplane with 3% flips (balanced at the median)."""
+    hyperplane (10% support, at least one feature) split exactly at the
+    median (ties broken at random, so both classes hold half the rows), then
+    3% flips."""
     rng = np.random.Generator(np.random.PCG64(base_seed * 10 ** 6 + k))
     kind = "nu_svc" if rng.random() < 0.2 else "c_svc"
-    n = int(round(np.exp(rng.uniform(np.log(500), np.log(20000)))))
+    n = int(round(np.exp(rng.uniform(np.log(n_range[0]), np.log(n_range[1])))))
     d = int(rng.choice([32, 128, 512]))
     dens = float(rng.choice([0.05, 0.3, 1.0]))
     C = float(rng.choice([0.5, 1.0, 8.0]))
@@ -152,10 +154,23 @@ def serving_request(k, base_seed=1005):
     vals = rng.random((n, d)).astype(np.float32).astype(np.float64)
     X = np.where(mask, vals, 0.0)
     w = np.where(rng.random(d) < 0.1, rng.standard_normal(d), 0.0)
+    if not w.any():
+        w[rng.integers(d)] = rng.standard_normal()
     s = X @ w
-    y = np.where(s > np.median(s), 1.0, -1.0)
+    order = np.lexsort((rng.random(n), s))
+    y = np.full(n, -1.0)
+    y[order[n // 2:]] = 1.0
     flip = rng.random(n) < 0.03
     y[flip] = -y[flip]
     indptr, indices, data = _csr_from_mask(mask, vals)
     return dict(kind=kind, n=n, d=d, density=dens, C=C, gamma=1.0 / d, nu=0.3,
                 indptr=indptr, indices=indices, data=data, y=y)
+
+
+def serving_rows(seed, n, d, density):
+    """n test rows shaped like a serving request's: CSR, U(0,1) at positions
+    active with probability `density`.  Returns (indptr, indices, data)."""
+    rng = np.random.Generator(np.random.PCG64(seed))
+    mask = rng.random((n, d)) < density
+    vals = rng.random((n, d)).astype(np.float32).astype(np.float64)
+    return _csr_from_mask(mask, vals)

@@ -52,6 +52,9 @@ int main(void) {
   P(svm_parameter, shrinking) P(svm_parameter, probability)
   P(svm_model, nr_class) P(svm_model, SV) P(svm_model, sv_indices) P(svm_model, free_sv)
   printf("svmhip_options %zu\nsvmhip_stats %zu\n", sizeof(svmhip_options), sizeof(svmhip_stats));
+  printf("svmhip_subproblem %zu\nsvmhip_completion %zu\n", sizeof(svmhip_subproblem), sizeof(svmhip_completion));
+  printf("svmhip_completion.rho %zu\nsvmhip_completion.device_ms %zu\n", offsetof(svmhip_completion, rho),
+         offsetof(svmhip_completion, device_ms));
   return 0;
 }
 """
@@ -75,6 +78,10 @@ def test_struct_layout_matches_c_compiler(tmp_path):
         assert got[f"svm_model.{f}"] == getattr(ffi.svm_model, f).offset
     assert got["svmhip_options"] == C.sizeof(libsvm_hip.svmhip_options)
     assert got["svmhip_stats"] == C.sizeof(libsvm_hip.svmhip_stats)
+    assert got["svmhip_subproblem"] == C.sizeof(libsvm_hip.svmhip_subproblem)
+    assert got["svmhip_completion"] == C.sizeof(libsvm_hip.svmhip_completion)
+    assert got["svmhip_completion.rho"] == libsvm_hip.svmhip_completion.rho.offset
+    assert got["svmhip_completion.device_ms"] == libsvm_hip.svmhip_completion.device_ms.offset
 
 
 def test_check_parameter_messages():
