@@ -181,6 +181,167 @@ def bench_ovo_cv_grid(args):
     print(json.dumps(out), flush=True)
 
 
+def _gen_request(k):
+    r = datasets.serving_request(k)
+    return {key: r[key] for key in ("kind", "n", "d", "density", "C", "gamma", "nu", "indptr", "indices", "data", "y")}
+
+
+def _serving_problem(r):
+    return ffi.Problem(r["indptr"], r["indices"], r["data"], r["y"], r["d"])
+
+
+def _serving_param(r):
+    t = ffi.NU_SVC if r["kind"] == "nu_svc" else ffi.C_SVC
+    return ffi.make_param(svm_type=t, gamma=r["gamma"], C_=r["C"], nu=r["nu"], eps=1e-3, shrinking=0)
+
+
+def bench_serving(args):
+    """Poisson stream at --qps for --window seconds: 90% training requests
+    (80% C-SVC / 20% nu-SVC, n log-uniform in [500, 20000], d in
+    {32, 128, 512}, density in {0.05, 0.3, 1.0}), 10% predictions of 10,000
+    rows against the newest resident model.  Latency = arrival to result on
+    the host."""
+    import multiprocessing as mp
+    rng = np.random.Generator(np.random.PCG64(1005))
+    gaps = rng.exponential(1.0 / args.qps, size=int(args.qps * args.window * 2) + 16)
+    arrivals = np.cumsum(gaps)
+    arrivals = arrivals[arrivals < args.window]
+    is_pred = rng.random(len(arrivals)) < 0.10
+    n_train = int((~is_pred).sum())
+    t = time.perf_counter()
+    with mp.get_context("fork").Pool(min(16, os.cpu_count() or 1)) as pool:
+        raw = pool.map(_gen_request, range(n_train), chunksize=4)
+    log(f"[bench] {len(arrivals)} requests ({n_train} train) over {args.window:.0f}s at {args.qps} req/s; "
+        f"data generated in {time.perf_counter() - t:.1f}s")
+    probs = [_serving_problem(r) for r in raw]
+    prms = [_serving_param(r) for r in raw]
+    tests = {}
+    for dd in (32, 128, 512):
+        ip, ix, v = datasets.serving_rows(1005 + dd, 10000, dd, 0.3)
+        tests[dd] = ffi.Problem(ip, ix, v, np.zeros(10000), dd)
+    srv = libsvm_hip.Server()
+    # warm-up: one small request end to end (module load, first launches)
+    warm = srv.submit_train(probs[0], prms[0])
+    srv.run_until_idle()
+    srv.release(warm)
+    lat_train, lat_pred, info, slow = [], [], {}, []
+    models = []          # (id, d) of finished, still resident models
+    pending_model = {}   # predict id -> model id
+    ntr = 0
+    nxt = 0
+    t0 = time.perf_counter()
+    last_log = t0
+    done_train = done_pred = 0
+    dev_ms = 0.0
+    iters = 0
+    while True:
+        now = time.perf_counter() - t0
+        while nxt < len(arrivals) and arrivals[nxt] <= now:
+            if is_pred[nxt] and models:
+                mid, dd = models[-1]
+                rid = srv.submit_predict(mid, tests[dd])
+                pending_model[rid] = mid
+                info[rid] = ("predict", arrivals[nxt])
+            else:
+                k = ntr % n_train
+                rid = srv.submit_train(probs[k], prms[k])
+                info[rid] = ("train", arrivals[nxt], k)
+                ntr += 1
+            nxt += 1
+        a, q = srv.step(args.stage_ms)
+        tnow = time.perf_counter() - t0
+        for c in srv.poll():
+            kind = info[c["id"]][0]
+            lat = tnow - info[c["id"]][1]
+            if c["status"] not in (0, 6):  # OK or SVMHIP_ERR_MAXITER
+                raise RuntimeError(f"request {c['id']} failed with status {c['status']}")
+            if kind == "train":
+                lat_train.append(lat)
+                rr = raw[info[c["id"]][2]]
+                slow.append((lat, rr["kind"], rr["n"], rr["d"], rr["density"], rr["C"], c["iters"]))
+                done_train += 1
+                iters += c["iters"]
+                dev_ms += c["device_ms"]
+                models.append((c["id"], raw[info[c["id"]][2]]["d"]))
+                # keep the newest 8 models resident
+                while len(models) > 8:
+                    old = models[0][0]
+                    if old in pending_model.values():
+                        break
+                    srv.release(old)
+                    models.pop(0)
+            else:
+                lat_pred.append(lat)
+                done_pred += 1
+                pending_model.pop(c["id"], None)
+        if time.perf_counter() - last_log > 10:
+            last_log = time.perf_counter()
+            log(f"[bench] t={tnow:.0f}s submitted {nxt} done {done_train}+{done_pred} active {a} queued {q}")
+        if nxt >= len(arrivals) and a == 0 and q == 0:
+            break
+        if a == 0 and q == 0 and nxt < len(arrivals):
+            time.sleep(max(0.0, min(0.005, arrivals[nxt] - (time.perf_counter() - t0))))
+    makespan = time.perf_counter() - t0
+    sst = srv.stats()
+    for row in sorted(slow, reverse=True)[:5]:
+        log("[bench] slow: %.2fs %s n=%d d=%d density=%.2f C=%g iters=%d" % row)
+    lt = np.array(lat_train)
+    lp = np.array(lat_pred) if lat_pred else np.zeros(1)
+    # each result must be the same alone: re-run a sample of requests alone
+    sample = [0, n_train // 3, (2 * n_train) // 3]
+    alone_ok = True
+    for k in sample:
+        s1 = libsvm_hip.Server()
+        a1 = s1.submit_train(probs[k], prms[k])
+        r1 = s1.run_until_idle()[0]
+        c1, _ = s1.train_result(a1, probs[k].l)
+        s1.close()
+        s2 = libsvm_hip.Server()
+        ids2 = [s2.submit_train(probs[j], prms[j]) for j in range(k, min(k + 24, n_train))]
+        r2 = {c["id"]: c for c in s2.run_until_idle()}[ids2[0]]
+        c2, _ = s2.train_result(ids2[0], probs[k].l)
+        s2.close()
+        alone_ok &= bool(np.array_equal(c1, c2) and r1["iters"] == r2["iters"] and r1["rho"] == r2["rho"])
+    out = {
+        "metric": METRIC,
+        "value": done_train / makespan,
+        "unit": "solves/s",
+        "n_gpus": 1, "steps": 1, "warmup": 1,
+        "ms_per_step": 1e3 * makespan,
+        "higher_is_better": True, "scaling": "weak", "vs_baseline": None,
+        "dtype": "f64", "data": "synthetic",
+        "config": {"workload": f"serving stream: Poisson {args.qps} req/s for {args.window:.0f}s, 90% train "
+                               f"(80% C-SVC / 20% nu-SVC, n in [500, 20000], d in {{32,128,512}}), "
+                               f"10% predict of 10,000 rows against a resident model",
+                   "train_requests": done_train, "predict_requests": done_pred,
+                   "train_latency_p50_s": float(np.percentile(lt, 50)),
+                   "train_latency_p99_s": float(np.percentile(lt, 99)),
+                   "predict_latency_p50_s": float(np.percentile(lp, 50)),
+                   "predict_latency_p99_s": float(np.percentile(lp, 99)),
+                   "smo_iters_per_s": iters / makespan, "stage_ms": args.stage_ms,
+                   "device_busy_frac": sst["solve_ms"] / 1e3 / makespan, "stages": sst["launches"],
+                   "results_same_alone_and_in_mix": alone_ok},
+    }
+    if not args.no_cpu:
+        sys.path.insert(0, os.path.join(ROOT, "oracle"))
+        import oracle  # CPU baseline leg only
+        t = time.perf_counter()
+        done, c_it = 0, 0
+        for k in range(n_train):
+            y1 = np.where(probs[k].y == probs[k].y[0], 1.0, -1.0)
+            pk = ffi.Problem(probs[k].indptr, probs[k].indices, probs[k].data, y1, probs[k].n_features)
+            c_it += oracle.train_one(pk, prms[k])["iters"]
+            done += 1
+            if time.perf_counter() - t > args.cpu_seconds:
+                break
+        dt = time.perf_counter() - t
+        out["cpu_baseline"] = {"value": done / dt, "unit": "solves/s", "cores": 1, "kind": "port",
+                               "sample": f"first {done} training requests of the stream, one after another, "
+                                         f"{c_it} iterations in {dt:.1f}s"}
+    srv.close()
+    print(json.dumps(out), flush=True)
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -195,7 +356,10 @@ def main():
     ap.add_argument("--copy-probe-gib", type=float, default=0.0,
                     help="instead of the solve, run a plain 1-pass device copy of this many GiB "
                          "(reference for the FETCH_SIZE / WRITE_SIZE counters)")
-    ap.add_argument("--workload", default="gmm_dense", choices=["gmm_dense", "ovo_cv_grid"])
+    ap.add_argument("--workload", default="gmm_dense", choices=["gmm_dense", "ovo_cv_grid", "serving"])
+    ap.add_argument("--qps", type=float, default=20.0)
+    ap.add_argument("--window", type=float, default=60.0)
+    ap.add_argument("--stage-ms", type=float, default=10.0)
     ap.add_argument("--batch-cache-mb", type=float, default=400.0)
     ap.add_argument("--no-plain", action="store_true")
     args = ap.parse_args()
@@ -210,6 +374,9 @@ def main():
     libsvm_hip.set_quiet(True)
     if args.workload == "ovo_cv_grid":
         bench_ovo_cv_grid(args)
+        return
+    if args.workload == "serving":
+        bench_serving(args)
         return
     n, d = args.n, args.d
     t0 = time.perf_counter()

@@ -249,9 +249,13 @@ int svmhip_server_submit_train(svmhip_server *s, const struct svm_problem *prob,
  * finished training request model_id.  Returns the id or -status. */
 int svmhip_server_submit_predict(svmhip_server *s, int model_id, const struct svm_node *const *x, int n);
 
-/* One round: admit, predict, one stage of every admitted solve.  stage_ms
- * <= 0 uses 10 ms.  *active / *queued (may be null) receive the counts after
- * the round. */
+/* One round.  Waits for the previous round's device work and collects its
+ * results (completions), then admits what fits, launches the admitted
+ * predictions and one stage of every admitted solve, and returns without
+ * waiting for them, so the caller can queue requests while the device
+ * works.  stage_ms <= 0 uses 10 ms.  *active (requests on the device) and
+ * *queued (may be null) receive the counts after the launch; both are 0 once
+ * everything submitted has completed. */
 svmhip_status svmhip_server_step(svmhip_server *s, double stage_ms, int *active, int *queued);
 
 /* Completions since the last call, at most max; returns their number. */
@@ -270,6 +274,10 @@ svmhip_status svmhip_server_release(svmhip_server *s, int id);
 
 /* Device bytes held by admitted requests and resident models. */
 double svmhip_server_bytes_in_use(const svmhip_server *s);
+
+/* Totals since creation: iters (SMO iterations), row_hits / row_misses,
+ * solve_ms (device time of all stages) and launches (stages). */
+svmhip_status svmhip_server_stats(const svmhip_server *s, svmhip_stats *stats);
 
 void svmhip_server_destroy(svmhip_server *s);
 

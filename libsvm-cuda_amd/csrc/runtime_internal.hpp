@@ -214,6 +214,21 @@ struct HostSell {
 // upload, libsvm/cuda_solver.cpp:366); errors go through h->fail.
 void make_host_sell(svmhip_solver *h, const svm_problem *prob, int ktype, HostSell &out);
 
+// Dynamic LDS of a batched launch whose rows have up to maxd features: room
+// for 16 staged query rows (G initialisation), at least one row, at most
+// 64 KiB.
+inline size_t batch_lds_bytes(int maxd) {
+  const size_t one = (size_t)std::max(maxd, 1) * sizeof(float);
+  return std::max(one, std::min<size_t>(16 * one, 64 * 1024));
+}
+
+// Device wall-clock ticks per second (wall_clock64 in kernels).
+inline double device_clock_hz(int dev) {
+  int khz = 0;
+  if (hipDeviceGetAttribute(&khz, hipDeviceAttributeWallClockRate, dev) != hipSuccess || khz <= 0) khz = 100000;
+  return 1e3 * khz;
+}
+
 inline int device_cus(int dev) {
   int cus = 0;
   if (hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, dev) != hipSuccess) cus = 0;

@@ -49,8 +49,9 @@ struct Request {
   double labels[2] = {0, 0};
   std::vector<signed char> y;
   std::vector<double> p, a0;
-  int max_iter = 0, T = 1, iters = 0;
-  bool started = false;
+  int max_iter = 0, T = 1, Tcap = 1, iters = 0;
+  int ginit_next = 0;     // G = p + Q alpha0 is done for variables below this
+  bool started = false, cache_valid = false;
   long long hits = 0, misses = 0;
   double *G = nullptr, *alpha = nullptr, *pv = nullptr, *QD = nullptr;
   int8_t *yd = nullptr, *st = nullptr;
@@ -80,7 +81,7 @@ struct Request {
 struct svmhip_server {
   svmhip_solver *g = nullptr;  // device context: stream, events, allocation helpers
   double budget = 0, used = 0;
-  int team_max = 16;
+  int team_max = 64;
   double cache_cap = kDefaultCacheCap;
   std::vector<Request *> reqs;
   std::deque<int> queue;
@@ -90,6 +91,12 @@ struct svmhip_server {
   int *dItems = nullptr, *dQueue = nullptr;
   smo::Ctl *dCtl = nullptr;
   size_t capV = 0, capI = 0, capC = 0;
+  // the round in flight: its stage's solves and its predictions
+  bool inflight = false, stage_launched = false;
+  std::vector<int> stage, stage_preds;
+  std::vector<smo::View> h_views;  // host sources of the round's uploads, kept until it ends
+  std::vector<int> h_items;
+  svmhip_stats tot{};
   std::string err;
 };
 
@@ -136,16 +143,25 @@ double train_footprint(const Request &q, double cache_cap, long long *arena_floa
   long long af = std::max<long long>(std::min(want, cap), 2 * stride);
   *arena_floats = af;
   const long long S = std::min<long long>(af / stride, l);
-  const double meta = S >= l ? (double)q.T * l : 4.0 * q.T * (l + 3 * S + 2);
+  const double meta = S >= l ? (double)q.Tcap * l : 4.0 * q.Tcap * (l + 3 * S + 2);
   return sell_bytes(o.total, l, o.nslices, o.dense) + 34.0 * nvar + 4.0 * af + meta +
-         2.0 * q.T * sizeof(smo::XPart) + sizeof(smo::Ctl);
+         2.0 * q.Tcap * sizeof(smo::XPart) + sizeof(smo::Ctl);
 }
 
-double est_iter_s(const Request &q) {
+// Estimated device seconds of one SMO iteration of q on one workgroup: two
+// passes over the variables (about 44 bytes each) plus, per kernel-row miss,
+// every stored entry of the rows (4 bytes dense, 8 sparse), at the ~40-60
+// GB/s one workgroup streams; the miss rate is the one measured so far.
+double est_solo_s(const Request &q) {
   const double width = q.rows.nslices > 0 ? (double)q.rows.total / ((double)q.rows.nslices * smo::SLICE) : 1.0;
-  const double solo = 2.5e-9 * (double)q.y.size() * (1.0 + width / 16.0) + 1e-6;
-  return q.T > 1 ? solo / q.T + 6e-6 : solo;
+  const double miss = q.iters > 0 ? std::max(0.05, std::min(2.0, (double)q.misses / q.iters)) : 1.0;
+  const double row_bytes = width * (q.rows.dense ? 4.0 : 8.0);
+  return (double)q.y.size() * (0.7e-9 + miss * row_bytes / 40e9) + 2e-6;
 }
+
+// Two exchanges per iteration make a team worth forming only above this.
+constexpr double kTeamWorthS = 18e-6;
+
 
 void complete(svmhip_server *s, Request &q, int status) {
   q.state = REQ_DONE;
@@ -208,13 +224,13 @@ void admit_train(svmhip_server *s, Request &q) {
   q.full = q.S >= l ? 1 : 0;
   if (q.full) {
     q.S = std::max(l, 1);
-    q.flags = h->dalloc<int8_t>((size_t)q.T * std::max(l, 1), "request cache flags");
+    q.flags = h->dalloc<int8_t>((size_t)q.Tcap * std::max(l, 1), "request cache flags");
     q.pm_stride = 0;
   } else {
     q.pm_stride = (long long)l + 3LL * q.S + 2;
-    q.lru = h->dalloc<int>((size_t)q.T * q.pm_stride, "request LRU bookkeeping");
+    q.lru = h->dalloc<int>((size_t)q.Tcap * q.pm_stride, "request LRU bookkeeping");
   }
-  if (q.T > 1) q.xpart = h->dalloc<smo::XPart>(2 * (size_t)q.T, "request exchange records");
+  if (q.Tcap > 1) q.xpart = h->dalloc<smo::XPart>(2 * (size_t)q.Tcap, "request exchange records");
   smo::View V = rows_view(h, q.rows, q);
   if (l > 0) h->ck(smo_launch_self_dot(&V, q.rows.xsq, h->st), "request x^2");
   // the host copies above read q.hs / st0: wait before they go
@@ -314,15 +330,64 @@ void make_model(svmhip_server *s, Request &q) {
   s->used += q.bytes;
 }
 
+// rho, obj and r from the final alpha and G on the host, in the order of
+// Solver::calculate_rho (libsvm/svm.cpp:1013-1049), Solver_NU::calculate_rho
+// (:1260-1308) and the objective loop of Solver::Solve (:794-803), so the
+// result does not depend on how many workgroups solved the request.
+void host_finalize(Request &q, const std::vector<double> &a, const std::vector<double> &G) {
+  const int n = (int)q.y.size();
+  auto upper = [&](int i) { return a[i] >= (q.y[i] > 0 ? q.Cp : q.Cn); };
+  auto lower = [&](int i) { return a[i] <= 0; };
+  double v = 0;
+  for (int i = 0; i < n; ++i) v += a[i] * (G[i] + q.p[i]);
+  q.obj = v / 2;
+  if (q.kind == SVMHIP_KIND_C) {
+    double ub = INFINITY, lb = -INFINITY, sum_free = 0;
+    int nr_free = 0;
+    for (int i = 0; i < n; ++i) {
+      const double yG = q.y[i] * G[i];
+      if (upper(i)) {
+        if (q.y[i] == -1) ub = std::min(ub, yG);
+        else lb = std::max(lb, yG);
+      } else if (lower(i)) {
+        if (q.y[i] == +1) ub = std::min(ub, yG);
+        else lb = std::max(lb, yG);
+      } else {
+        ++nr_free;
+        sum_free += yG;
+      }
+    }
+    q.rho = nr_free > 0 ? sum_free / nr_free : (ub + lb) / 2;
+    q.r = 0;
+  } else {
+    int nr_free1 = 0, nr_free2 = 0;
+    double ub1 = INFINITY, ub2 = INFINITY, lb1 = -INFINITY, lb2 = -INFINITY, sum1 = 0, sum2 = 0;
+    for (int i = 0; i < n; ++i) {
+      if (q.y[i] == +1) {
+        if (upper(i)) lb1 = std::max(lb1, G[i]);
+        else if (lower(i)) ub1 = std::min(ub1, G[i]);
+        else { ++nr_free1; sum1 += G[i]; }
+      } else {
+        if (upper(i)) lb2 = std::max(lb2, G[i]);
+        else if (lower(i)) ub2 = std::min(ub2, G[i]);
+        else { ++nr_free2; sum2 += G[i]; }
+      }
+    }
+    const double r1 = nr_free1 > 0 ? sum1 / nr_free1 : (ub1 + lb1) / 2;
+    const double r2 = nr_free2 > 0 ? sum2 / nr_free2 : (ub2 + lb2) / 2;
+    q.r = (r1 + r2) / 2;
+    q.rho = (r1 - r2) / 2;
+  }
+}
+
 void finish_train(svmhip_server *s, Request &q, const smo::Ctl &c) {
   svmhip_solver *h = s->g;
   const int nvar = (int)q.y.size();
-  std::vector<double> a(nvar);
+  std::vector<double> a(nvar), G(nvar);
   h->ck(hipMemcpyAsync(a.data(), q.alpha, sizeof(double) * nvar, hipMemcpyDeviceToHost, h->st), "fetch alpha");
+  h->ck(hipMemcpyAsync(G.data(), q.G, sizeof(double) * nvar, hipMemcpyDeviceToHost, h->st), "fetch G");
   h->ck(hipStreamSynchronize(h->st), "fetch alpha");
-  q.rho = c.rho;
-  q.obj = c.obj;
-  q.r = c.r;
+  host_finalize(q, a, G);
   q.coef.assign(nvar, 0.0);
   if (q.svm_type == NU_SVC) {  // libsvm/svm.cpp:1574-1580
     for (int i = 0; i < nvar; ++i) q.coef[i] = a[i] * q.y[i] / q.r;
@@ -364,11 +429,13 @@ void run_predict(svmhip_server *s, Request &q) {
   const int nbt = (n + q.tb - 1) / q.tb;
   for (int b0 = 0; b0 < nbt; b0 += q.chunk)
     h->ck(smo_launch_predict_offset(&A, b0 * q.tb, std::min(q.chunk, nbt - b0), shmem, h->st), "prediction kernel");
-  h->ck(hipMemcpyAsync(q.dec.data(), q.ddec, sizeof(double) * n, hipMemcpyDeviceToHost, h->st), "fetch decisions");
 }
 
+// After the round's synchronisation: decision values to the host.
 void finish_predict(svmhip_server *s, Request &q) {
   svmhip_solver *h = s->g;
+  if (s->reqs[q.model_id]->model.l > 0 && !q.dec.empty())
+    h->ck(hipMemcpy(q.dec.data(), q.ddec, sizeof(double) * q.dec.size(), hipMemcpyDeviceToHost), "fetch decisions");
   free_rows(h, q.rows);
   h->dfree(q.kv);
   h->dfree(q.ddec);
@@ -497,10 +564,14 @@ int svmhip_server_submit_train(svmhip_server *s, const svm_problem *prob, const 
       q->Cp = q->Cn = 1.0;
     }
     q->max_iter = h->opt.max_iter > 0 ? h->opt.max_iter : std::max(10000000, l > INT_MAX / 100 ? INT_MAX : 100 * l);
+    q->ginit_next = 0;
     make_host_sell(h, prob, q->ktype, q->hs);
     if ((size_t)q->hs.d * sizeof(float) > kLdsQueryLimit)
       h->fail(SVMHIP_ERR_ARG, "server: %d features exceed the batched solver's LDS query row", q->hs.d);
-    q->T = l <= 4096 ? 1 : std::max(1, std::min({s->team_max, q->hs.nslices / 2, l / 2048}));
+    // largest team: a power of two, at least four 64-row slices per member
+    q->Tcap = 1;
+    while (2 * q->Tcap <= s->team_max && 2 * q->Tcap * 4 <= q->hs.nslices) q->Tcap *= 2;
+    q->T = 1;
     GUARD_END
   }();
   if (rc != SVMHIP_OK) {
@@ -550,7 +621,62 @@ svmhip_status svmhip_server_step(svmhip_server *s, double stage_ms, int *n_activ
     GUARD_BEGIN
     h->ck(hipSetDevice(h->dev), "hipSetDevice");
     const double tau = (stage_ms > 0 ? stage_ms : kDefaultStageMs) * 1e-3;
-    // admission, first come first served
+    const double clock_hz = device_clock_hz(h->dev);
+    // 1. results of the round in flight
+    if (s->inflight) {
+      h->ck(hipStreamSynchronize(h->st), "server round");
+      s->inflight = false;
+      if (s->stage_launched) {
+        float ms = 0;
+        (void)hipEventElapsedTime(&ms, h->ev[0], h->ev[1]);
+        s->tot.solve_ms += ms;
+        s->tot.launches += 1;
+        std::vector<smo::Ctl> hc(s->stage.size());
+        h->ck(hipMemcpy(hc.data(), s->dCtl, sizeof(smo::Ctl) * s->stage.size(), hipMemcpyDeviceToHost),
+              "fetch server control");
+        std::vector<int> still;
+        std::vector<char> ran(s->reqs.size(), 0);
+        for (size_t k = 0; k < s->stage.size(); ++k) {
+          Request &q = *s->reqs[s->stage[k]];
+          ran[q.id] = 1;
+          const smo::Ctl &c = hc[k];
+          q.device_ms += ms;
+          q.cache_valid = true;
+          if (!q.started && c.stop == smo::STOP_PAUSE && !c.err && c.iter == q.iters &&
+              c.ginit_next < (int)q.y.size() && c.ginit_next > q.ginit_next) {
+            q.ginit_next = c.ginit_next;  // still initialising G
+            still.push_back(q.id);
+            continue;
+          }
+          q.started = true;
+          s->tot.iters += c.iter - q.iters;
+          q.iters = c.iter;
+          q.hits += (long long)c.hits;
+          q.misses += (long long)c.misses;
+          s->tot.row_hits += (long long)c.hits;
+          s->tot.row_misses += (long long)c.misses;
+          if (c.err) {
+            free_train_state(h, q);
+            free_rows(h, q.rows);
+            s->used -= q.bytes;
+            q.bytes = 0;
+            complete(s, q, SVMHIP_ERR_TIMEOUT);
+          } else if (c.stop == smo::STOP_OPTIMAL || c.iter >= q.max_iter) {
+            finish_train(s, q, c);
+          } else {
+            still.push_back(q.id);
+          }
+        }
+        for (int id : s->active)
+          if (!ran[id]) still.push_back(id);
+        s->active.swap(still);
+      }
+      for (int id : s->stage_preds) finish_predict(s, *s->reqs[id]);
+      s->stage.clear();
+      s->stage_preds.clear();
+      s->stage_launched = false;
+    }
+    // 2. admission, first come first served
     while (!s->queue.empty()) {
       Request &q = *s->reqs[s->queue.front()];
       long long af = 0;
@@ -566,28 +692,54 @@ svmhip_status svmhip_server_step(svmhip_server *s, double stage_ms, int *n_activ
       if (q.type == 0) admit_train(s, q);
       else admit_predict(s, q);
     }
-    // predictions
-    std::vector<int> preds;
-    preds.swap(s->predicts);
-    for (int id : preds) run_predict(s, *s->reqs[id]);
-    // one stage of every admitted solve: teams first (as many as fit in the
-    // grid at once), then single workgroups, largest first
-    std::vector<int> stage;
-    int G = 0;
-    float ms = 0;
+    // 3. predictions
+    s->stage_preds.swap(s->predicts);
+    for (int id : s->stage_preds) {
+      Request &q = *s->reqs[id];
+      q.dec.assign(q.rows.l, 0.0);
+      if (s->reqs[q.model_id]->model.l == 0)  // no support vectors: every decision value is -rho
+        for (double &v : q.dec) v = -s->reqs[q.model_id]->rho;
+      else
+        run_predict(s, q);
+    }
+    // 4. one stage of every admitted solve, launched without waiting: teams
+    // first (as many as fit in the grid at once), then single workgroups,
+    // largest first
     if (!s->active.empty()) {
       int maxd = 1;
       for (int id : s->active) maxd = std::max(maxd, s->reqs[id]->rows.d);
-      const size_t shmem = (size_t)maxd * sizeof(float);
+      const size_t shmem = batch_lds_bytes(maxd);
       int per_cu = 0;
       h->ck(smo_batch_capacity(shmem, &per_cu), "occupancy query");
-      G = std::max(1, per_cu) * device_cus(h->dev);
+      const int G = std::max(1, per_cu) * device_cus(h->dev);
+      // team sizes follow the load: each solve's share of the grid in
+      // proportion to its estimated iteration cost, as a power of two up to
+      // its cap; a size changes only on a 2x gain or a 4x excess (each
+      // change restarts that solve's cache)
+      double wsum = 0;
+      for (int id : s->active) wsum += est_solo_s(*s->reqs[id]);
+      for (int id : s->active) {
+        Request &q = *s->reqs[id];
+        const double share = (double)G * est_solo_s(q) / wsum;
+        int t = 1;
+        if (est_solo_s(q) > kTeamWorthS)
+          while (2 * t <= q.Tcap && 2 * t <= share) t *= 2;
+        int nt = q.T;
+        if (!q.started || t >= 2 * q.T || 4 * t <= q.T) nt = t;
+        nt = std::min(nt, std::max(1, G));
+        if (nt != q.T) {
+          q.T = nt;
+          q.cache_valid = false;
+        }
+      }
+      std::vector<int> &stage = s->stage;
+      stage.clear();
       std::vector<int> teams, solos;
       for (int id : s->active) (s->reqs[id]->T > 1 ? teams : solos).push_back(id);
+      std::stable_sort(teams.begin(), teams.end(), [&](int a, int b) { return a < b; });
       int tsum = 0;
       for (int id : teams) {
         Request &q = *s->reqs[id];
-        if (q.T > G) q.T = G;  // never reached on a device with >= 64 resident workgroups
         if (tsum + q.T > G) continue;  // waits for a later stage
         tsum += q.T;
         stage.push_back(id);
@@ -595,8 +747,10 @@ svmhip_status svmhip_server_step(svmhip_server *s, double stage_ms, int *n_activ
       std::stable_sort(solos.begin(), solos.end(),
                        [&](int a, int b) { return s->reqs[a]->y.size() > s->reqs[b]->y.size(); });
       stage.insert(stage.end(), solos.begin(), solos.end());
-      std::vector<smo::View> views(stage.size());
-      std::vector<int> items;
+      std::vector<smo::View> &views = s->h_views;
+      std::vector<int> &items = s->h_items;
+      views.assign(stage.size(), smo::View{});
+      items.clear();
       for (size_t k = 0; k < stage.size(); ++k) {
         Request &q = *s->reqs[stage[k]];
         smo::View V = rows_view(h, q.rows, q);
@@ -608,12 +762,26 @@ svmhip_status svmhip_server_step(svmhip_server *s, double stage_ms, int *n_activ
         V.rows = q.arena; V.row_stride = q.row_stride; V.S = q.S; V.full = q.full;
         V.pm_flags = q.flags; V.pm_lru = q.lru; V.pm_stride = q.pm_stride;
         V.own_cache = 1;
-        V.keep_cache = q.started ? 1 : 0;
+        V.keep_cache = q.cache_valid ? 1 : 0;
         V.resume = q.started ? 1 : 0;
         V.iter0 = q.iters;
-        long long cap = (long long)(tau / est_iter_s(q));
-        cap = std::max<long long>(cap, 64);
-        V.max_iter = (int)std::min<long long>((long long)q.iters + cap, q.max_iter);
+        if (!q.started) {
+          // this stage's share of G = p + Q alpha0: about tau of staged
+          // passes (each query row costs one multiply-add per stored entry
+          // of the owned rows, ~7e10 per second per workgroup)
+          const double width = (double)q.rows.total / std::max(1.0, (double)q.rows.nslices * smo::SLICE);
+          const double row_s = (double)q.rows.l * width / q.T / 7e10 + 1e-7;
+          long long want = std::max<long long>(16, (long long)(tau / row_s));
+          int to = q.ginit_next;
+          for (long long c = 0; to < V.nvar && c < want; ++to)
+            if (q.a0[to] > 0) ++c;
+          // the range ends just before the next nonzero alpha (or at nvar)
+          while (to < V.nvar && !(q.a0[to] > 0)) ++to;
+          V.ginit_from = q.ginit_next;
+          V.ginit_to = to;
+        }
+        V.max_iter = q.max_iter;
+        V.pause_ticks = std::max<long long>(1, (long long)(tau * clock_hz));
         V.nblocks = q.T;
         V.xpart = q.xpart;
         views[k] = V;
@@ -639,45 +807,14 @@ svmhip_status svmhip_server_step(svmhip_server *s, double stage_ms, int *n_activ
       pool.K = 0;
       const int nitems = (int)items.size() / 2;
       h->ck(hipEventRecord(h->ev[0], h->st), "event");
+      pool.lds_floats = (int)(shmem / sizeof(float));
       h->ck(smo_launch_batch(s->dV, s->dItems, nitems, std::min(G, nitems), &pool, shmem, h->st),
             "batched SMO kernel");
       h->ck(hipEventRecord(h->ev[1], h->st), "event");
-      std::vector<smo::Ctl> hc(stage.size());
-      h->ck(hipMemcpyAsync(hc.data(), s->dCtl, sizeof(smo::Ctl) * stage.size(), hipMemcpyDeviceToHost, h->st),
-            "fetch server control");
-      h->ck(hipStreamSynchronize(h->st), "server stage");
-      (void)hipEventElapsedTime(&ms, h->ev[0], h->ev[1]);
-      std::vector<int> still;
-      std::vector<char> ran(s->reqs.size(), 0);
-      for (size_t k = 0; k < stage.size(); ++k) {
-        Request &q = *s->reqs[stage[k]];
-        ran[q.id] = 1;
-        const smo::Ctl &c = hc[k];
-        q.device_ms += ms;
-        q.started = true;
-        q.iters = c.iter;
-        q.hits += (long long)c.hits;
-        q.misses += (long long)c.misses;
-        if (c.err) {
-          free_train_state(h, q);
-          free_rows(h, q.rows);
-          s->used -= q.bytes;
-          q.bytes = 0;
-          complete(s, q, SVMHIP_ERR_TIMEOUT);
-        } else if (c.stop == smo::STOP_OPTIMAL || c.iter >= q.max_iter) {
-          finish_train(s, q, c);
-        } else {
-          still.push_back(q.id);
-        }
-      }
-      for (int id : s->active)
-        if (!ran[id]) still.push_back(id);
-      s->active.swap(still);
-    } else {
-      h->ck(hipStreamSynchronize(h->st), "server predictions");
+      s->stage_launched = true;
     }
-    for (int id : preds) finish_predict(s, *s->reqs[id]);
-    if (n_active) *n_active = (int)s->active.size();
+    s->inflight = s->stage_launched || !s->stage_preds.empty();
+    if (n_active) *n_active = (int)s->active.size() + (int)s->stage_preds.size();
     if (n_queued) *n_queued = (int)s->queue.size();
     GUARD_END
   }();
@@ -737,6 +874,12 @@ svmhip_status svmhip_server_release(svmhip_server *s, int id) {
 }
 
 double svmhip_server_bytes_in_use(const svmhip_server *s) { return s ? s->used : 0.0; }
+
+svmhip_status svmhip_server_stats(const svmhip_server *s, svmhip_stats *stats) {
+  if (!s || !stats) return SVMHIP_ERR_ARG;
+  *stats = s->tot;
+  return SVMHIP_OK;
+}
 
 void svmhip_server_destroy(svmhip_server *s) {
   if (!s) return;

@@ -36,7 +36,7 @@ constexpr double INF = __builtin_huge_val();
 enum : int8_t { AT_LOWER = 0, AT_UPPER = 1, IS_FREE = 2 };   // libsvm/svm_device.cu:13
 enum { KIND_C = 0, KIND_NU = 1 };
 enum { K_LINEAR = 0, K_POLY = 1, K_RBF = 2, K_SIGMOID = 3, K_PRECOMPUTED = 4 };
-enum { STOP_NONE = 0, STOP_OPTIMAL = 1, STOP_MAXITER = 2 };
+enum { STOP_NONE = 0, STOP_OPTIMAL = 1, STOP_MAXITER = 2, STOP_PAUSE = 3 };
 enum { ERR_NONE = 0, ERR_TIMEOUT = 5 };
 
 // Control record.  Written by exactly one thread (the leader's thread 0) at
@@ -49,7 +49,8 @@ struct Ctl {
   int i, j, ip, in;      // selected variables
   int slot_i, slot_j, slot_ip, slot_in;
   int miss_i, miss_j, miss_ip, miss_in;
-  int stop, iter, max_iter, pad0;
+  int stop, iter, max_iter;
+  int ginit_next;        // batched solves: first variable of G initialisation not yet done
   double dai, daj;       // alpha changes of the last step
   double Gmax, Gmax2;    // C rule
   double Gp, Gp2, Gn, Gn2;  // nu rule
@@ -101,6 +102,9 @@ struct View {
   int rank;               // batched solves: this workgroup's owner index (-1: blockIdx.x)
   int resume;             // batched solves: G, alpha, status and QD carry over from an earlier stage
   int iter0;              // batched solves: iterations already done
+  long long pause_ticks;  // batched solves: pause after this many device clock ticks (0 = never)
+  int ginit_from, ginit_to;  // batched solves: variables [from, to) of G = p + Q alpha to add in this
+                             // launch (resume = 0); to < nvar pauses before the first iteration
   int own_cache;          // batched solves: rows/S/full/row_stride and the cache metadata
                           // (nblocks copies, pm_stride apart) are the subproblem's own
   int keep_cache;         // batched solves: the own cache is valid from an earlier stage
@@ -120,6 +124,8 @@ struct BatchPool {
   long long flag_stride, lru_stride;
   int *queue;              // next item (zeroed before each launch)
   int K, by_item;
+  int lds_floats;          // dynamic LDS of the launch, in floats
+  int pad;
 };
 
 // ---------------------------------------------------------------- helpers
@@ -204,7 +210,7 @@ __device__ __forceinline__ void owned_slices(const View &V, int b, int nb, int &
 
 // Stage row q of X densely (by feature column, or by position for
 // PRECOMPUTED) into xs[0..d).  Whole workgroup; ends with a barrier.
-__device__ void stage_row(const View &V, int q, float *xs) {
+__device__ __forceinline__ void stage_row(const View &V, int q, float *xs) {
   const int tid = threadIdx.x, nt = blockDim.x;
   const int len = V.row_len[q];
   const long long base = V.slice_ptr[q / SLICE] + (q % SLICE);
@@ -273,6 +279,46 @@ __device__ __forceinline__ double row_dot(const View &V, const float *xs, int r)
   return acc;
 }
 
+// dot(x_q(m), x_r) for the M rows staged at xs + m * ld, each summed in the
+// same order as row_dot (so each equals row_dot bit for bit): one pass over
+// row r serves M query rows.
+template <int M>
+__device__ __forceinline__ void row_dot_multi(const View &V, const float *xs, int ld, int r, double acc[M]) {
+  const int s = r / SLICE;
+  const long long base = V.slice_ptr[s] + (r % SLICE);
+  const float *vp = V.vals + base;
+#pragma unroll
+  for (int m = 0; m < M; ++m) acc[m] = 0.0;
+  constexpr int U = 8;
+  if (V.dense) {
+    const int len = V.d;
+    int p = 0;
+    for (; p + U <= len; p += U) {
+      float a[U];
+#pragma unroll
+      for (int u = 0; u < U; ++u) a[u] = vp[(long long)(p + u) * SLICE];
+#pragma unroll
+      for (int u = 0; u < U; ++u)
+#pragma unroll
+        for (int m = 0; m < M; ++m) acc[m] = fma((double)a[u], (double)xs[m * ld + p + u], acc[m]);
+    }
+    for (; p < len; ++p) {
+      const double a = vp[(long long)p * SLICE];
+#pragma unroll
+      for (int m = 0; m < M; ++m) acc[m] = fma(a, (double)xs[m * ld + p], acc[m]);
+    }
+  } else {
+    const int len = V.row_len[r];
+    const int *cp = V.cols + base;
+    for (int p = 0; p < len; ++p) {
+      const double a = vp[(long long)p * SLICE];
+      const int c = cp[(long long)p * SLICE];
+#pragma unroll
+      for (int m = 0; m < M; ++m) acc[m] = acc[m] + a * (double)xs[m * ld + c];
+    }
+  }
+}
+
 // K(q, r) in FP64 given dot(x_q, x_r) (Kernel::kernel_*, libsvm/svm.cpp:243-255).
 __device__ __forceinline__ double kernel_f64(const View &V, int q, int r, double dot) {
   switch (V.ktype) {
@@ -317,7 +363,7 @@ struct RedSmem {
 // Block-wide reduction of up to 4 (value, index) lanes; op per slot.
 // kinds: 0 = max_idx, 1 = min_idx, 2 = plain max.  Result in thread 0.
 template <int N>
-__device__ void block_reduce(RedSmem &sm, double (&v)[4], int (&ix)[2], const int (&kinds)[4]) {
+__device__ __forceinline__ void block_reduce(RedSmem &sm, double (&v)[4], int (&ix)[2], const int (&kinds)[4]) {
   const int wave = threadIdx.x / 64, lane = threadIdx.x % 64, nw = blockDim.x / 64;
   int slot_i = 0;
 #pragma unroll
@@ -375,7 +421,7 @@ __device__ __forceinline__ void lru_touch(CacheMeta &M, int s) {
 // Slot holding kernel row `row`; on a miss the least recently used slot other
 // than `exclude` is reassigned (the row is then computed by every workgroup
 // for its owned range in the next pass).  Single thread.
-__device__ int cache_lookup(const View &V, CacheMeta &M, int row, int exclude, int *miss) {
+__device__ __forceinline__ int cache_lookup(const View &V, CacheMeta &M, int row, int exclude, int *miss) {
   if (V.full) {
     *miss = !M.computed[row];
     M.computed[row] = 1;
@@ -511,13 +557,16 @@ struct Pend {
   double ai, aj;
 };
 
+// Slices a wave visits per trip of the pass loops.
+constexpr int PASS_SLICES = 4;
+
 __device__ __forceinline__ bool owns(const View &V, int sb, int se, int v) {
   int r = real_of(V, v);
   return r >= sb * SLICE && r < se * SLICE;
 }
 
 template <int KIND>
-__device__ Partial pass_gradient_select(const View &V, const Snap &c, float *xs, RedSmem &sm, bool update,
+__device__ __forceinline__ Partial pass_gradient_select(const View &V, const Snap &c, float *xs, RedSmem &sm, bool update,
                                         bool select, bool to_part = true, const Pend *pend = nullptr) {
   int sb, se;
   owned_slices(V, my_owner(V), V.nblocks, sb, se);
@@ -533,43 +582,71 @@ __device__ Partial pass_gradient_select(const View &V, const Snap &c, float *xs,
   int ix[2] = {-1, -1};
   const int wave = threadIdx.x / 64, lane = threadIdx.x % 64, nw = blockDim.x / 64;
   const int nv = V.svr ? 2 : 1;
-  for (int s = sb + wave; s < se; s += nw) {
-    int r = s * SLICE + lane;
-    if (r >= V.l) continue;
-    float Kir = 0.f, Kjr = 0.f;
-    if (update) {
-      Kir = V.rows[(long long)c.slot_i * V.row_stride + r];
-      Kjr = V.rows[(long long)c.slot_j * V.row_stride + r];
-    }
-    for (int h = 0; h < nv; ++h) {
-      int k = r + h * V.l;
-      int8_t yk = V.y[k];
-      double g = V.G[k];
+  const int8_t yi = update ? V.y[c.i] : (int8_t)0, yj = update ? V.y[c.j] : (int8_t)0;
+  // PASS_SLICES slices per trip, all loads issued before any use, so a wave
+  // waits for one memory round trip per PASS_SLICES slices rather than per
+  // slice (the selection's ties go to the larger index, so the order of
+  // visits does not matter)
+  for (int s0 = sb + wave; s0 < se; s0 += nw * PASS_SLICES) {
+    float Ki[PASS_SLICES], Kj[PASS_SLICES];
+    double g[PASS_SLICES][2];
+    int8_t yk[PASS_SLICES][2], st[PASS_SLICES][2];
+    bool ok[PASS_SLICES];
+    // loads are unconditional (lanes past the owned rows read a clamped,
+    // valid row and ignore it), so they issue back to back with one wait
+#pragma unroll
+    for (int u = 0; u < PASS_SLICES; ++u) {
+      const int r = (s0 + u * nw) * SLICE + lane;
+      ok[u] = s0 + u * nw < se && r < V.l;
+      const int rc = min(r, V.l - 1);
+      Ki[u] = Kj[u] = 0.f;
       if (update) {
-        float Qik = (V.y[c.i] == yk) ? Kir : -Kir;
-        float Qjk = (V.y[c.j] == yk) ? Kjr : -Kjr;
-        double t1 = (double)Qik * c.dai;
-        double t2 = (double)Qjk * c.daj;
-        g = g + (t1 + t2);
-        V.G[k] = g;
+        Ki[u] = V.rows[(long long)c.slot_i * V.row_stride + rc];
+        Kj[u] = V.rows[(long long)c.slot_j * V.row_stride + rc];
       }
-      if (!select) continue;
-      int8_t st = V.status[k];
-      if (KIND == KIND_C) {
-        if (yk == +1) {
-          if (st != AT_UPPER) max_idx(v[0], ix[0], -g, k);
-          if (st != AT_LOWER) v[1] = fmax(v[1], g);
-        } else {
-          if (st != AT_LOWER) max_idx(v[0], ix[0], g, k);
-          if (st != AT_UPPER) v[1] = fmax(v[1], -g);
+#pragma unroll
+      for (int h = 0; h < 2; ++h) {
+        const int k = rc + (h < nv ? h : 0) * V.l;
+        yk[u][h] = V.y[k];
+        g[u][h] = V.G[k];
+        st[u][h] = select ? V.status[k] : (int8_t)0;
+      }
+    }
+#pragma unroll
+    for (int u = 0; u < PASS_SLICES; ++u) {
+      if (!ok[u]) continue;
+      const int r = (s0 + u * nw) * SLICE + lane;
+#pragma unroll
+      for (int h = 0; h < 2; ++h) {
+        if (h >= nv) continue;
+        const int k = r + h * V.l;
+        double gk = g[u][h];
+        if (update) {
+          float Qik = (yi == yk[u][h]) ? Ki[u] : -Ki[u];
+          float Qjk = (yj == yk[u][h]) ? Kj[u] : -Kj[u];
+          double t1 = (double)Qik * c.dai;
+          double t2 = (double)Qjk * c.daj;
+          gk = gk + (t1 + t2);
+          V.G[k] = gk;
         }
-      } else {
-        if (yk == +1) {
-          if (st != AT_UPPER) max_idx(v[0], ix[0], -g, k);
-          if (st != AT_LOWER) v[2] = fmax(v[2], g);
+        if (!select) continue;
+        const int8_t sk = st[u][h];
+        if (KIND == KIND_C) {
+          if (yk[u][h] == +1) {
+            if (sk != AT_UPPER) max_idx(v[0], ix[0], -gk, k);
+            if (sk != AT_LOWER) v[1] = fmax(v[1], gk);
+          } else {
+            if (sk != AT_LOWER) max_idx(v[0], ix[0], gk, k);
+            if (sk != AT_UPPER) v[1] = fmax(v[1], -gk);
+          }
         } else {
-          if (st != AT_LOWER) max_idx(v[1], ix[1], g, k);
-          if (st != AT_UPPER) v[3] = fmax(v[3], -g);
+          if (yk[u][h] == +1) {
+            if (sk != AT_UPPER) max_idx(v[0], ix[0], -gk, k);
+            if (sk != AT_LOWER) v[2] = fmax(v[2], gk);
+          } else {
+            if (sk != AT_LOWER) max_idx(v[1], ix[1], gk, k);
+            if (sk != AT_UPPER) v[3] = fmax(v[3], -gk);
+          }
         }
       }
     }
@@ -593,7 +670,7 @@ __device__ Partial pass_gradient_select(const View &V, const Snap &c, float *xs,
 // (libsvm/svm.cpp:866-917; nu rule :1123-1172), row i (rows ip / in) computed
 // for the owned range first on a miss.
 template <int KIND>
-__device__ Partial pass_select_j(const View &V, const Snap &c, float *xs, RedSmem &sm, bool to_part = true) {
+__device__ __forceinline__ Partial pass_select_j(const View &V, const Snap &c, float *xs, RedSmem &sm, bool to_part = true) {
   int sb, se;
   owned_slices(V, my_owner(V), V.nblocks, sb, se);
   if (KIND == KIND_C) {
@@ -606,52 +683,81 @@ __device__ Partial pass_select_j(const View &V, const Snap &c, float *xs, RedSme
   int bj = -1;
   const int wave = threadIdx.x / 64, lane = threadIdx.x % 64, nw = blockDim.x / 64;
   const int nv = V.svr ? 2 : 1;
-  for (int s = sb + wave; s < se; s += nw) {
-    int r = s * SLICE + lane;
-    if (r >= V.l) continue;
-    for (int h = 0; h < nv; ++h) {
-      int k = r + h * V.l;
-      int8_t yk = V.y[k];
-      int8_t st = V.status[k];
-      double g = V.G[k];
-      if (KIND == KIND_C) {
-        double gd;
-        if (yk == +1) {
-          if (st == AT_LOWER) continue;
-          gd = c.Gmax + g;
-        } else {
-          if (st == AT_UPPER) continue;
-          gd = c.Gmax - g;
-        }
-        if (gd > 0) {
-          float Kir = V.rows[(long long)c.slot_i * V.row_stride + r];
-          int8_t yi = V.y[c.i];
-          float Qik = (yi == yk) ? Kir : -Kir;
-          double t = (2.0 * yi) * (double)Qik;
-          double quad = (yk == +1) ? (V.QD[c.i] + V.QD[k]) - t : (V.QD[c.i] + V.QD[k]) + t;
-          double od = quad > 0 ? -(gd * gd) / quad : -(gd * gd) / TAU;
-          min_idx(best, bj, od, k);
-        }
-      } else {
-        if (yk == +1) {
-          if (st == AT_LOWER) continue;
-          double gd = c.Gp + g;
+  // rows and scalars of the candidates i (C rule) / i_p, i_n (nu rule)
+  const int qi = KIND == KIND_C ? c.i : c.ip;
+  const int qn = KIND == KIND_C ? -1 : c.in;
+  const long long ri = qi >= 0 ? (long long)(KIND == KIND_C ? c.slot_i : c.slot_ip) * V.row_stride : 0;
+  const long long rn = qn >= 0 ? (long long)c.slot_in * V.row_stride : 0;
+  const int8_t yi = qi >= 0 ? V.y[qi] : (int8_t)1, yn = qn >= 0 ? V.y[qn] : (int8_t)1;
+  const double QDi = qi >= 0 ? V.QD[qi] : 0.0, QDn = qn >= 0 ? V.QD[qn] : 0.0;
+  // PASS_SLICES slices per trip, loads first (ties go to the larger index)
+  for (int s0 = sb + wave; s0 < se; s0 += nw * PASS_SLICES) {
+    float Ki[PASS_SLICES], Kn[PASS_SLICES];
+    double g[PASS_SLICES][2], qd[PASS_SLICES][2];
+    int8_t yk[PASS_SLICES][2], st[PASS_SLICES][2];
+    bool ok[PASS_SLICES];
+    // unconditional loads from a clamped, valid row (see pass_gradient_select)
+#pragma unroll
+    for (int u = 0; u < PASS_SLICES; ++u) {
+      const int r = (s0 + u * nw) * SLICE + lane;
+      ok[u] = s0 + u * nw < se && r < V.l;
+      const int rc = min(r, V.l - 1);
+      Ki[u] = qi >= 0 ? V.rows[ri + rc] : 0.f;
+      Kn[u] = qn >= 0 ? V.rows[rn + rc] : 0.f;
+#pragma unroll
+      for (int h = 0; h < 2; ++h) {
+        const int k = rc + (h < nv ? h : 0) * V.l;
+        yk[u][h] = V.y[k];
+        st[u][h] = V.status[k];
+        g[u][h] = V.G[k];
+        qd[u][h] = V.QD[k];
+      }
+    }
+#pragma unroll
+    for (int u = 0; u < PASS_SLICES; ++u) {
+      if (!ok[u]) continue;
+      const int r = (s0 + u * nw) * SLICE + lane;
+#pragma unroll
+      for (int h = 0; h < 2; ++h) {
+        if (h >= nv) continue;
+        const int k = r + h * V.l;
+        const int8_t y = yk[u][h], sk = st[u][h];
+        const double gk = g[u][h];
+        if (KIND == KIND_C) {
+          double gd;
+          if (y == +1) {
+            if (sk == AT_LOWER) continue;
+            gd = c.Gmax + gk;
+          } else {
+            if (sk == AT_UPPER) continue;
+            gd = c.Gmax - gk;
+          }
           if (gd > 0) {
-            float Kr = V.rows[(long long)c.slot_ip * V.row_stride + r];
-            float Q = (V.y[c.ip] == yk) ? Kr : -Kr;
-            double quad = (V.QD[c.ip] + V.QD[k]) - (double)(2.0f * Q);
+            float Qik = (yi == y) ? Ki[u] : -Ki[u];
+            double t = (2.0 * yi) * (double)Qik;
+            double quad = (y == +1) ? (QDi + qd[u][h]) - t : (QDi + qd[u][h]) + t;
             double od = quad > 0 ? -(gd * gd) / quad : -(gd * gd) / TAU;
             min_idx(best, bj, od, k);
           }
         } else {
-          if (st == AT_UPPER) continue;
-          double gd = c.Gn - g;
-          if (gd > 0) {
-            float Kr = V.rows[(long long)c.slot_in * V.row_stride + r];
-            float Q = (V.y[c.in] == yk) ? Kr : -Kr;
-            double quad = (V.QD[c.in] + V.QD[k]) - (double)(2.0f * Q);
-            double od = quad > 0 ? -(gd * gd) / quad : -(gd * gd) / TAU;
-            min_idx(best, bj, od, k);
+          if (y == +1) {
+            if (sk == AT_LOWER) continue;
+            double gd = c.Gp + gk;
+            if (gd > 0) {
+              float Q = (yi == y) ? Ki[u] : -Ki[u];
+              double quad = (QDi + qd[u][h]) - (double)(2.0f * Q);
+              double od = quad > 0 ? -(gd * gd) / quad : -(gd * gd) / TAU;
+              min_idx(best, bj, od, k);
+            }
+          } else {
+            if (sk == AT_UPPER) continue;
+            double gd = c.Gn - gk;
+            if (gd > 0) {
+              float Q = (yn == y) ? Kn[u] : -Kn[u];
+              double quad = (QDn + qd[u][h]) - (double)(2.0f * Q);
+              double od = quad > 0 ? -(gd * gd) / quad : -(gd * gd) / TAU;
+              min_idx(best, bj, od, k);
+            }
           }
         }
       }
@@ -861,6 +967,7 @@ struct Dec {
   double QDi, QDip, QDin;
   int yi, yip, yin, pad2;
   unsigned long long hits, misses;
+  long long t_start;      // device clock at the start of this launch's share of the solve
 };
 
 static_assert(sizeof(Dec) <= sizeof(((Ctl *)nullptr)->dec_bytes), "Dec must fit in Ctl");
@@ -879,6 +986,7 @@ __device__ __forceinline__ void publish(XPart *slot, const XPart &p, unsigned se
   st_agent(&slot->q0, p.q0);
   st_agent(&slot->qd0, p.qd0); st_agent(&slot->qd1, p.qd1);
   st_agent(&slot->y0, p.y0); st_agent(&slot->y1, p.y1);
+  st_agent(&slot->pad, p.pad);
   // the record's stores complete (are visible at agent scope) before its
   // sequence number is written
   __asm__ volatile("s_waitcnt vmcnt(0)" ::: "memory");
@@ -907,7 +1015,7 @@ struct XSmem {
 // thread, the combined record (reduced values; winners' payloads in g0/a0/q0
 // and, for the nu rule's second argmax, g1/a1).
 template <int KIND, int PH>
-__device__ XPart exchange(const View &V, RedSmem &sm, XSmem &xs, const XPart &mine, unsigned seq) {
+__device__ __forceinline__ XPart exchange(const View &V, RedSmem &sm, XSmem &xs, const XPart &mine, unsigned seq) {
   XPart *buf = V.xpart + (size_t)(seq & 1u) * V.nblocks;
   if (threadIdx.x == 0) {
     publish(&buf[my_owner(V)], mine, seq);
@@ -928,7 +1036,10 @@ __device__ XPart exchange(const View &V, RedSmem &sm, XSmem &xs, const XPart &mi
     q.q0 = ld_acq(&buf[t].q0);
     q.qd0 = ld_acq(&buf[t].qd0); q.qd1 = ld_acq(&buf[t].qd1);
     q.y0 = ld_acq(&buf[t].y0); q.y1 = ld_acq(&buf[t].y1);
+    q.pad = ld_acq(&buf[t].pad);
   }
+  // i records carry each owner's pause vote: every owner sees the same OR
+  const int pause = PH == PH_I ? __syncthreads_or(t < V.nblocks ? q.pad : 0) : 0;
   double v[4] = {q.v0, q.v1, q.v2, q.v3};
   int ix[2] = {q.i0, q.i1};
   if (PH == PH_J) {
@@ -962,8 +1073,14 @@ __device__ XPart exchange(const View &V, RedSmem &sm, XSmem &xs, const XPart &mi
   __syncthreads();
   XPart r = xs.win[0];
   r.seq = xs.err;
+  r.pad = pause;
   __syncthreads();
   return r;
+}
+
+// This owner's vote to pause the solve at the current iteration boundary.
+__device__ __forceinline__ int pause_vote(const View &V, const Dec &d) {
+  return V.pause_ticks > 0 && wall_clock64() - d.t_start >= V.pause_ticks ? 1 : 0;
 }
 
 __device__ __forceinline__ CacheMeta private_meta(const View &V, Dec &d) {
@@ -996,13 +1113,14 @@ __device__ __forceinline__ Snap snap_of(const Dec &d) {
 
 // Decisions after the i exchange (every workgroup, thread 0).
 template <int KIND>
-__device__ void decide_i(const View &V, Dec &d, const XPart &r, bool count_maxiter) {
+__device__ __forceinline__ void decide_i(const View &V, Dec &d, const XPart &r, bool count_maxiter) {
   CacheMeta M = private_meta(V, d);
   int stop = STOP_NONE;
   if (count_maxiter && d.iter >= V.max_iter) stop = STOP_MAXITER;
   if (KIND == KIND_C) {
     d.Gmax = r.v0; d.Gmax2 = r.v1; d.i = r.i0; d.Gi = r.g0; d.ai = r.a0; d.QDi = r.qd0; d.yi = r.y0;
     if (!stop && r.v0 + r.v1 < V.eps) stop = STOP_OPTIMAL;   // libsvm/svm.cpp:919
+    if (!stop && r.pad) stop = STOP_PAUSE;
     if (!stop) d.slot_i = cache_lookup(V, M, real_of(V, r.i0), -1, &d.miss_i);
   } else {
     d.Gp = r.v0; d.Gn = r.v1; d.Gp2 = r.v2; d.Gn2 = r.v3;
@@ -1010,6 +1128,7 @@ __device__ void decide_i(const View &V, Dec &d, const XPart &r, bool count_maxit
     d.Gip = r.g0; d.aip = r.a0; d.Gin = r.g1; d.ain = r.a1;
     d.QDip = r.qd0; d.yip = r.y0; d.QDin = r.qd1; d.yin = r.y1;
     if (!stop && fmax(r.v0 + r.v2, r.v1 + r.v3) < V.eps) stop = STOP_OPTIMAL;  // libsvm/svm.cpp:1174
+    if (!stop && r.pad) stop = STOP_PAUSE;
     if (!stop) {
       int sp = -1;
       d.miss_ip = d.miss_in = 0;
@@ -1023,7 +1142,7 @@ __device__ void decide_i(const View &V, Dec &d, const XPart &r, bool count_maxit
 // Decisions after the j exchange: the pair, the row-j slot and the alpha
 // step, computed identically in every workgroup (thread 0).
 template <int KIND>
-__device__ void decide_j(const View &V, Dec &d, const XPart &r) {
+__device__ __forceinline__ void decide_j(const View &V, Dec &d, const XPart &r) {
   CacheMeta M = private_meta(V, d);
   const int j = r.i0;
   if (j < 0) {  // only reachable with eps <= 0 or NaN gradients

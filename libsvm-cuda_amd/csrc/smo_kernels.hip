@@ -128,7 +128,7 @@ __global__ void k_status(View V) {
 // Record of this workgroup's i-pass winner(s) with their G and alpha
 // (thread 0; the pass's block reduction has synchronised the workgroup).
 template <int KIND>
-__device__ XPart make_i_record(const View &V, const Partial &bp) {
+__device__ __forceinline__ XPart make_i_record(const View &V, const Partial &bp) {
   XPart m = xpart_identity<KIND, PH_I>();
   m.v0 = bp.v0; m.v1 = bp.v1; m.v2 = bp.v2; m.v3 = bp.v3;
   m.i0 = bp.i0; m.i1 = bp.i1;
@@ -143,7 +143,7 @@ __device__ XPart make_i_record(const View &V, const Partial &bp) {
 
 // Record of this workgroup's j-pass winner with Q_ij, G_j and alpha_j.
 template <int KIND>
-__device__ XPart make_j_record(const View &V, const Snap &c, const Partial &bp) {
+__device__ __forceinline__ XPart make_j_record(const View &V, const Snap &c, const Partial &bp) {
   XPart m = xpart_identity<KIND, PH_J>();
   m.v0 = bp.v0;
   m.i0 = bp.i0;
@@ -208,6 +208,7 @@ __device__ __forceinline__ void smo_iterations(const View &V, Dec &d, RedSmem &s
     pd.i = d.i; pd.j = d.j; pd.ai = d.ai_new; pd.aj = d.aj_new;
     Partial bp2 = pass_gradient_select<KIND>(V, c, xs, sm, true, true, false, &pd);
     XPart mine2 = make_i_record<KIND>(V, bp2);
+    mine2.pad = pause_vote(V, d);
     seq = (unsigned)d.epoch + 1u;
     XPart r2 = exchange<KIND, PH_I>(V, sm, xsm, mine2, seq);
     if (threadIdx.x == 0) {
@@ -396,25 +397,61 @@ __device__ __forceinline__ void batch_qd(const View &V) {
   }
 }
 
-// G = p + sum_{alpha_v > 0} alpha_v Q_v in increasing v (libsvm/svm.cpp:555-569),
-// for rows [r0, r1) (r0 a multiple of the slice height).  Ends with a barrier.
-__device__ __forceinline__ void batch_ginit(const View &V, float *xs, int r0, int r1) {
+// Query rows staged per pass of batch_ginit (LDS permitting).
+constexpr int GINIT_M = 16;
+
+// G += sum alpha_v Q_v over the variables v in [from, to) with alpha_v > 0,
+// in increasing v (libsvm/svm.cpp:555-569), for rows [r0, r1) (r0 a
+// multiple of the slice height).  Up to GINIT_M such rows are staged in LDS
+// at a time and each lane's row is read once for all of them; every kernel
+// value is the one kernel_value computes.  Ends with a barrier.
+template <int M>
+__device__ __forceinline__ void ginit_pass(const View &V, float *xs, int ld, const int *list, int cnt, int r0, int r1) {
   const int nv = V.svr ? 2 : 1;
-  for (int v = 0; v < V.nvar; ++v) {
-    if (V.status[v] == AT_LOWER) continue;
-    const int q = real_of(V, v);
-    stage_row(V, q, xs);
-    const double av = V.alpha[v];
-    const int8_t yv = V.y[v];
-    for (int r = r0 + threadIdx.x; r < r1; r += blockDim.x) {
-      // row r of a slice is lane r % 64 of wave (r % 512) / 64 in the SELL layout
-      float K = kernel_value(V, xs, q, r);
+  for (int r = r0 + threadIdx.x; r < r1; r += blockDim.x) {
+    // row r of a slice is lane r % 64 of wave (r % 512) / 64 in the SELL layout
+    double acc[M];
+    row_dot_multi<M>(V, xs, ld, r, acc);
+#pragma unroll
+    for (int m = 0; m < M; ++m) {  // static indices keep acc in registers
+      if (m >= cnt) break;
+      const int v = list[m];
+      const int q = real_of(V, v);
+      const float K = (float)kernel_f64(V, q, r, acc[m]);
+      const double av = V.alpha[v];
+      const int8_t yv = V.y[v];
       for (int h = 0; h < nv; ++h) {
-        int k = r + h * V.l;
-        float Qv = (V.y[k] == yv) ? K : -K;
+        const int k = r + h * V.l;
+        const float Qv = (V.y[k] == yv) ? K : -K;
         V.G[k] = V.G[k] + av * (double)Qv;
       }
     }
+  }
+}
+
+__device__ __forceinline__ void batch_ginit(const View &V, float *xs, int lds_floats, int r0, int r1, int from,
+                                            int to) {
+  __shared__ int list[GINIT_M];
+  __shared__ int s_cnt, s_next;
+  const int fit = lds_floats / max(V.d, 1);
+  const int M = fit >= GINIT_M ? GINIT_M : (fit >= 4 ? 4 : 1);
+  int v = from;
+  for (;;) {
+    if (threadIdx.x == 0) {
+      int c = 0, u = v;
+      for (; u < to && c < M; ++u)
+        if (V.status[u] != AT_LOWER) list[c++] = u;
+      s_cnt = c;
+      s_next = u;
+    }
+    __syncthreads();
+    const int cnt = s_cnt;
+    v = s_next;
+    if (cnt == 0) break;
+    for (int m = 0; m < cnt; ++m) stage_row(V, real_of(V, list[m]), xs + m * V.d);
+    if (M == GINIT_M) ginit_pass<GINIT_M>(V, xs, V.d, list, cnt, r0, r1);
+    else if (M >= 4) ginit_pass<4>(V, xs, V.d, list, cnt, r0, r1);
+    else ginit_pass<1>(V, xs, V.d, list, cnt, r0, r1);
     __syncthreads();
   }
   __syncthreads();
@@ -485,7 +522,7 @@ __device__ __forceinline__ void final_terms(const View &V, int r0, int r1, doubl
 
 // rho, obj and r from the combined terms into Ctl (thread 0).
 template <int KIND>
-__device__ void write_final(Ctl *c, const double f[7], const int n[2]) {
+__device__ __forceinline__ void write_final(Ctl *c, const double f[7], const int n[2]) {
   c->obj = f[6] / 2;
   if (KIND == KIND_C) {
     c->rho = n[0] > 0 ? f[0] / n[0] : (f[1] + f[2]) / 2;
@@ -498,7 +535,7 @@ __device__ void write_final(Ctl *c, const double f[7], const int n[2]) {
   }
 }
 
-__device__ void write_ctl_state(Ctl *c, const Dec &d, int err) {
+__device__ __forceinline__ void write_ctl_state(Ctl *c, const Dec &d, int err) {
   c->iter = d.iter;
   c->stop = d.stop;
   c->err = err;
@@ -507,16 +544,34 @@ __device__ void write_ctl_state(Ctl *c, const Dec &d, int err) {
 }
 
 // One subproblem owned by this workgroup alone.
+// Leaves Ctl as a pause before the first iteration (G initialisation not
+// finished: it continues from V.ginit_to in the next launch).
+__device__ __forceinline__ void write_ginit_pause(const View &V) {
+  Ctl *c = V.ctl;
+  c->iter = V.iter0;
+  c->stop = STOP_PAUSE;
+  c->err = 0;
+  c->hits = 0;
+  c->misses = 0;
+  c->ginit_next = V.ginit_to;
+}
+
 template <int KIND>
-__device__ __forceinline__ void solo_solve(View &V, Dec &d, RedSmem &sm, float *xs) {
+__device__ __forceinline__ void solo_solve(View &V, Dec &d, RedSmem &sm, float *xs, int lds_floats) {
   solo_init_cache(V);
   if (!V.resume) {
-    batch_qd(V);
-    batch_ginit(V, xs, 0, V.l);
+    if (V.ginit_from == 0) batch_qd(V);
+    batch_ginit(V, xs, lds_floats, 0, V.l, V.ginit_from, V.ginit_to);
+    if (V.ginit_to < V.nvar) {
+      if (threadIdx.x == 0) write_ginit_pause(V);
+      __syncthreads();
+      return;
+    }
   }
   if (threadIdx.x == 0) {
     memset(&d, 0, sizeof d);
     d.iter = V.iter0;
+    d.t_start = wall_clock64();
   }
   __syncthreads();
   {
@@ -538,7 +593,10 @@ __device__ __forceinline__ void solo_solve(View &V, Dec &d, RedSmem &sm, float *
     pd.i = d.i; pd.j = d.j; pd.ai = d.ai_new; pd.aj = d.aj_new;
     Partial bp2 = pass_gradient_select<KIND>(V, c, xs, sm, true, true, false, &pd);
     XPart m2 = make_i_record<KIND>(V, bp2);
-    if (threadIdx.x == 0) decide_i<KIND>(V, d, m2, true);
+    if (threadIdx.x == 0) {
+      m2.pad = pause_vote(V, d);
+      decide_i<KIND>(V, d, m2, true);
+    }
     __syncthreads();
   }
   double f[7];
@@ -552,18 +610,25 @@ __device__ __forceinline__ void solo_solve(View &V, Dec &d, RedSmem &sm, float *
 
 // One member (rank V.rank of V.nblocks) of a team owning a subproblem.
 template <int KIND>
-__device__ __forceinline__ void team_solve(View &V, Dec &d, RedSmem &sm, XSmem &xsm, float *xs) {
+__device__ __forceinline__ void team_solve(View &V, Dec &d, RedSmem &sm, XSmem &xsm, float *xs, int lds_floats) {
   int sb, se;
   owned_slices(V, V.rank, V.nblocks, sb, se);
   const int r0 = sb * SLICE, r1 = min(se * SLICE, V.l);
   solo_init_cache(V);
   if (!V.resume) {
-    batch_qd(V);  // every member writes every QD (identical values): row i's QD is read by all
-    batch_ginit(V, xs, r0, r1);
+    // every member writes every QD (identical values): row i's QD is read by all
+    if (V.ginit_from == 0) batch_qd(V);
+    batch_ginit(V, xs, lds_floats, r0, r1, V.ginit_from, V.ginit_to);
+    if (V.ginit_to < V.nvar) {  // every member stops at the same variable
+      if (V.rank == 0 && threadIdx.x == 0) write_ginit_pause(V);
+      __syncthreads();
+      return;
+    }
   }
   if (threadIdx.x == 0) {
     memset(&d, 0, sizeof d);
     d.iter = V.iter0;
+    d.t_start = wall_clock64();
   }
   __syncthreads();
   smo_select_first<KIND>(V, d, sm, xsm, xs);
@@ -642,13 +707,13 @@ __global__ void __launch_bounds__(BLOCK) k_smo_batch(const View *views, const in
     }
     V.pm_stride = 0;
     if (team > 1) {
-      if (V.kind == KIND_C) team_solve<KIND_C>(V, d, sm, xsm, xs);
-      else team_solve<KIND_NU>(V, d, sm, xsm, xs);
+      if (V.kind == KIND_C) team_solve<KIND_C>(V, d, sm, xsm, xs, pool.lds_floats);
+      else team_solve<KIND_NU>(V, d, sm, xsm, xs, pool.lds_floats);
     } else {
       V.nblocks = 1;
       V.rank = 0;
-      if (V.kind == KIND_C) solo_solve<KIND_C>(V, d, sm, xs);
-      else solo_solve<KIND_NU>(V, d, sm, xs);
+      if (V.kind == KIND_C) solo_solve<KIND_C>(V, d, sm, xs, pool.lds_floats);
+      else solo_solve<KIND_NU>(V, d, sm, xs, pool.lds_floats);
     }
     __syncthreads();
   }

@@ -666,6 +666,7 @@ svmhip_status svmhip_batch_solve(svmhip_batch *b, svmhip_subproblem *subs, int n
       V.computed = nullptr;
       V.ctl = dC + k; V.part = nullptr; V.xpart = nullptr; V.xs_global = nullptr;
       V.use_lds = 1; V.nblocks = 1; V.rank = 0; V.resume = 0; V.iter0 = 0;
+      V.ginit_from = 0; V.ginit_to = sp.n_var;
       views[k] = V;
       off += sp.n_var;
     }
@@ -675,8 +676,9 @@ svmhip_status svmhip_batch_solve(svmhip_batch *b, svmhip_subproblem *subs, int n
     h->upload(dY, hY.data(), nv_total, "upload batch y");
     h->upload(dS, hS.data(), nv_total, "upload batch status");
     // one cache arena per resident workgroup
+    const size_t lds = batch_lds_bytes(h->d);
     int per_cu = 0;
-    h->ck(smo_batch_capacity(h->shmem, &per_cu), "occupancy query");
+    h->ck(smo_batch_capacity(lds, &per_cu), "occupancy query");
     const int G = std::max(1, per_cu) * device_cus(h->dev);
     const int K = G;
     // arena size: opt.cache_bytes if set, else an equal share of 70% of the
@@ -700,16 +702,18 @@ svmhip_status svmhip_batch_solve(svmhip_batch *b, svmhip_subproblem *subs, int n
     pool.lru_stride = 4LL * maxn + 2;
     pool.lru = (int *)talloc(sizeof(int) * (size_t)pool.lru_stride * K, "batch LRU bookkeeping");
     pool.queue = (int *)talloc(sizeof(int), "batch queue");
+    pool.lds_floats = (int)(lds / sizeof(float));
     smo::XPart *dX = (smo::XPart *)talloc(sizeof(smo::XPart) * 2 * (size_t)G, "team exchange records");
     int *dItems = (int *)talloc(sizeof(int) * 2 * (size_t)std::max(nsub, G), "batch items");
 
-    // Stages.  Every unfinished subproblem runs until it converges or reaches
-    // this stage's iteration cap, sized so that its estimated device time is
-    // about tau; tau grows 4x each stage.  While more subproblems remain than
-    // the grid has workgroups each is solved by one workgroup (largest first);
-    // once fewer remain, the grid is shared out as teams in proportion to
-    // their sizes.  The cap only decides where a solve pauses: a resumed
-    // subproblem continues the same sequence of working sets.
+    // Stages.  Every unfinished subproblem runs until it converges or, after
+    // about tau of device time, pauses at an iteration boundary (team members
+    // agree through their exchange); tau grows 4x each stage.  While more
+    // subproblems remain than the grid has workgroups each is solved by one
+    // workgroup (largest first); once fewer remain, the grid is shared out as
+    // teams in proportion to their work so far.  Where a solve pauses changes
+    // nothing: a resumed subproblem continues the same sequence of working
+    // sets.
     const int team_max = h->opt.grid_blocks > 0 ? std::min(h->opt.grid_blocks, 64) : 32;
     const int fixed_chunk = h->opt.iters_per_launch;
     std::vector<int> rem(nsub), iters(nsub, 0);
@@ -718,13 +722,9 @@ svmhip_status svmhip_batch_solve(svmhip_batch *b, svmhip_subproblem *subs, int n
     std::vector<smo::Ctl> hC(nsub);
     std::vector<int> T(nsub, 1), items;
     double tau = 0.05;
+    const double clock_hz = device_clock_hz(h->dev);
     int stages = 0;
     float total_ms = 0;
-    auto est_iter_s = [&](int k, int t) {
-      const svmhip_batch::Rowset &rs = b->rowsets[subs[k].rowset];
-      double solo = 2.5e-9 * subs[k].n_var * (1.0 + rs.width / 16.0) + 1e-6;
-      return t > 1 ? solo / t + 6e-6 : solo;
-    };
     while (!rem.empty()) {
       // team sizes
       // team shares in proportion to n_var x (iterations so far + 1): the
@@ -769,9 +769,13 @@ svmhip_status svmhip_batch_solve(svmhip_batch *b, svmhip_subproblem *subs, int n
         if (T[k] > 1) xoff += 2 * T[k];
         V.resume = stages > 0 ? 1 : 0;
         V.iter0 = iters[k];
-        long long cap = fixed_chunk > 0 ? fixed_chunk : (long long)(tau / est_iter_s(k, T[k]));
-        cap = std::max<long long>(cap, 64);
-        V.max_iter = (int)std::min<long long>((long long)iters[k] + cap, true_max[k]);
+        if (fixed_chunk > 0) {  // pause after a fixed number of iterations
+          V.max_iter = (int)std::min<long long>((long long)iters[k] + fixed_chunk, true_max[k]);
+          V.pause_ticks = 0;
+        } else {  // pause on the device clock after about tau
+          V.max_iter = true_max[k];
+          V.pause_ticks = (long long)(tau * clock_hz);
+        }
         for (int r = 0; r < T[k]; ++r) {
           items.push_back(k);
           items.push_back(r);
@@ -785,7 +789,7 @@ svmhip_status svmhip_batch_solve(svmhip_batch *b, svmhip_subproblem *subs, int n
       if (xoff > 0) h->ck(hipMemsetAsync(dX, 0, sizeof(smo::XPart) * xoff, h->st), "clear team records");
       h->ck(hipMemsetAsync(dC, 0, sizeof(smo::Ctl) * nsub, h->st), "clear batch control");
       h->ck(hipEventRecord(h->ev[0], h->st), "event");
-      h->ck(smo_launch_batch(dV, dItems, nitems, std::min(G, nitems), &pool, h->shmem, h->st), "batched SMO kernel");
+      h->ck(smo_launch_batch(dV, dItems, nitems, std::min(G, nitems), &pool, lds, h->st), "batched SMO kernel");
       h->ck(hipEventRecord(h->ev[1], h->st), "event");
       h->ck(hipMemcpyAsync(hC.data(), dC, sizeof(smo::Ctl) * nsub, hipMemcpyDeviceToHost, h->st), "fetch batch control");
       h->ck(hipStreamSynchronize(h->st), "batched solve");

@@ -77,7 +77,7 @@ SVM_HIP_SYMBOLS = [
     "svmhip_server_create", "svmhip_server_submit_train", "svmhip_server_submit_predict",
     "svmhip_server_step", "svmhip_server_poll", "svmhip_server_train_result",
     "svmhip_server_predict_result", "svmhip_server_release", "svmhip_server_bytes_in_use",
-    "svmhip_server_destroy", "svmhip_server_last_error"]
+    "svmhip_server_stats", "svmhip_server_destroy", "svmhip_server_last_error"]
 
 _lib = None
 PRINT_FUNC = C.CFUNCTYPE(None, C.c_char_p)
@@ -158,6 +158,8 @@ def lib():
     L.svmhip_server_release.restype = I
     L.svmhip_server_bytes_in_use.argtypes = [h]
     L.svmhip_server_bytes_in_use.restype = D
+    L.svmhip_server_stats.argtypes = [h, P(svmhip_stats)]
+    L.svmhip_server_stats.restype = I
     L.svmhip_server_destroy.argtypes = [h]
     L.svmhip_server_destroy.restype = None
     L.svmhip_server_last_error.argtypes = [h]
@@ -597,6 +599,11 @@ class Server:
 
     def bytes_in_use(self):
         return lib().svmhip_server_bytes_in_use(self._h)
+
+    def stats(self):
+        st = svmhip_stats()
+        lib().svmhip_server_stats(self._h, C.byref(st))
+        return st.as_dict()
 
     def run_until_idle(self, stage_ms=10.0, max_steps=100000):
         """Steps until nothing is active or queued; returns all completions."""

@@ -117,3 +117,28 @@ def test_server_budget_admission(hip):
     res = tiny.run_until_idle()
     assert [(c["id"], c["status"]) for c in res] == [(rid, 3)]
     tiny.close()
+
+
+@pytest.mark.parametrize("k", [32, 40])  # nu-SVC, d=512 dense / d=128 at 30% density
+def test_server_nu_initialisation_split_over_stages(hip, ora, k):
+    """A nu-SVC request starts from alpha0 with nu*l/2 nonzeros per class, so
+    G = p + Q alpha0 is most of its first work; with short stages the server
+    spreads it over many launches.  The result must not change."""
+    r = datasets.serving_request(k, n_range=(1500, 2500))
+    assert r["kind"] == "nu_svc"
+    prob, prm = request_problem(r), request_param(r)
+    s = hip.Server()
+    rid = s.submit_train(prob, prm)
+    (c,) = s.run_until_idle(stage_ms=0.05)
+    coef, _ = s.train_result(rid, prob.l)
+    launches = s.stats()["launches"]
+    s.close()
+    alone, coef_alone = solve_alone(hip, prob, prm, stage_ms=1000.0)
+    ref = ora.train_one(oracle_view(prob), prm)
+    print(f"  n={r['n']} d={r['d']}: {launches} launches, iters {c['iters']} vs {ref['iters']}", flush=True)
+    assert launches > 5
+    assert c["status"] == 0 and c["iters"] == alone["iters"] == ref["iters"]
+    np.testing.assert_array_equal(coef, coef_alone)
+    np.testing.assert_allclose(coef, ref["alpha"], rtol=0, atol=1e-9)
+    assert abs(c["obj"] - ref["obj"]) <= 1e-9 * abs(ref["obj"])
+    assert abs(c["rho"] - ref["rho"]) <= 1e-9
