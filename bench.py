@@ -60,6 +60,22 @@ def cpu_baseline(prob, prm, seconds):
                       f"-m 100 MB row cache, shrinking on), {dt:.1f} s"}
 
 
+def measured_traffic(n, d, grid):
+    """HBM bytes per k_smo launch from the calibrated FETCH_SIZE / WRITE_SIZE
+    runs kept in profiles/ (valid for the configuration they were taken on:
+    gmm_dense n=100000 d=256 with the default grid); None otherwise."""
+    if (n, d, grid) != (100000, 256, 0):
+        return None
+    path = os.path.join(ROOT, "profiles", "k_smo_gmm_dense_100k_counters.md")
+    try:
+        for line in open(path):
+            if line.strip().startswith("traffic_bytes_per_launch:"):
+                return float(line.split(":")[1])
+    except OSError:
+        pass
+    return None
+
+
 def copy_probe(gib):
     """A plain device-to-device copy of known size through torch's copy
     kernel: its FETCH_SIZE / WRITE_SIZE readings calibrate the counters."""
@@ -417,6 +433,7 @@ def main():
     alg_bytes = (iters * b_iter + misses * b_row) / launches
     launch_s = kern_s / launches
     achieved = alg_bytes / launch_s
+    traffic = measured_traffic(n, d, args.grid)
     last = stats[-1]
     out = {
         "metric": METRIC,
@@ -436,7 +453,7 @@ def main():
                    "rows_computed_per_solve": last["row_misses"], "cache_rows": last["cache_slots"],
                    "grid_blocks": last["grid_blocks"], "obj": last["obj"], "rho": last["rho"]},
         "roofline": {"bound": "hbm", "achieved": achieved / 1e9, "peak": HBM_PEAK / 1e9, "unit": "GB/s",
-                     "frac": achieved / HBM_PEAK, "traffic": None,
+                     "frac": achieved / HBM_PEAK, "traffic": traffic,
                      "kernel": "k_smo", "avg_launch_ms": 1e3 * launch_s,
                      "bytes_per_launch": alg_bytes},
     }
