@@ -378,6 +378,7 @@ def main():
     ap.add_argument("--stage-ms", type=float, default=10.0)
     ap.add_argument("--batch-cache-mb", type=float, default=400.0)
     ap.add_argument("--no-plain", action="store_true")
+    ap.add_argument("--plain-iters", type=int, default=2000)
     args = ap.parse_args()
     if args.copy_probe_gib > 0:
         copy_probe(args.copy_probe_gib)
@@ -457,6 +458,35 @@ def main():
                      "kernel": "k_smo", "avg_launch_ms": 1e3 * launch_s,
                      "bytes_per_launch": alg_bytes},
     }
+    if not args.no_plain:
+        # the reference's shape of the loop on the same kernels: one host call
+        # per phase (select, alpha, gradient, status), each a launch, with the
+        # blocking copy of the stopping test, for a bounded number of
+        # iterations from the same start
+        s.setup_solver(ys, p, a0, st0, Cp, Cn)
+        k_plain = min(args.plain_iters, last["iters"])
+        torch_sync = time.perf_counter()
+        done = 0
+        for _ in range(k_plain):
+            opt, _, _ = s.select_working_set()
+            if opt:
+                break
+            s.compute_alpha()
+            s.update_gradient()
+            s.update_alpha_status()
+            done += 1
+        t_plain = time.perf_counter() - torch_sync
+        # the fused kernel over exactly those iterations
+        s.setup_solver(ys, p, a0, st0, Cp, Cn)
+        t = time.perf_counter()
+        sk = s.solve(max_iter=done)
+        t_fused = time.perf_counter() - t
+        out["config"]["stepwise_path_iters_per_s"] = done / t_plain
+        out["config"]["stepwise_path_sample"] = (f"first {done} iterations of the same solve through "
+                                                 f"svmhip_select_working_set/compute_alpha/update_gradient/"
+                                                 f"update_alpha_status, {t_plain:.2f}s; the persistent kernel "
+                                                 f"over the same {sk['iters']} iterations: {t_fused:.3f}s")
+        out["config"]["speedup_vs_stepwise_path"] = t_plain / t_fused
     if not args.no_cpu:
         out["cpu_baseline"] = cpu_baseline(prob, prm, args.cpu_seconds)
     s.close()
