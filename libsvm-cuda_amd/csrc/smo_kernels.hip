@@ -888,12 +888,29 @@ hipError_t smo_launch_lru_private_init(const View *V, hipStream_t st) {
   return hipGetLastError();
 }
 
+// Launches a grid whose workgroups must all be resident together (they
+// exchange records with bounded spins; no grid-wide hardware barrier is
+// used).  The grid never exceeds the co-resident capacity.  A cooperative
+// launch guarantees residency; the runtime refuses it (hipErrorIllegalState)
+// while another process holds the device's cooperative-launch resources, and
+// then the same grid is launched plainly: on a device not running other
+// work every workgroup is resident at once, and if one is not, the spins of
+// the others end with an error (ERR_TIMEOUT) rather than a hang.
+static hipError_t launch_resident(const void *fn, int grid, void **args, size_t shmem, hipStream_t st) {
+  hipError_t e = hipLaunchCooperativeKernel(fn, dim3(grid), dim3(smo::BLOCK), args, (unsigned)shmem, st);
+  if (e == hipErrorIllegalState) {
+    (void)hipGetLastError();
+    e = hipLaunchKernel(fn, dim3(grid), dim3(smo::BLOCK), args, shmem, st);
+  }
+  return e;
+}
+
 hipError_t smo_launch_persistent(const View *V, int iter_limit, size_t shmem, hipStream_t st) {
   View v = *V;
   void *args[] = {&v, &iter_limit};
   const void *fn = V->kind == smo::KIND_C ? (const void *)smo::k_smo<smo::KIND_C>
                                           : (const void *)smo::k_smo<smo::KIND_NU>;
-  return hipLaunchCooperativeKernel(fn, dim3(V->nblocks), dim3(smo::BLOCK), args, (unsigned)shmem, st);
+  return launch_resident(fn, V->nblocks, args, shmem, st);
 }
 
 // Largest co-resident grid for the persistent kernel at this LDS size.
@@ -916,8 +933,7 @@ hipError_t smo_launch_batch(const View *views, const int *items, int nitems, int
   if (nitems <= 0 || grid <= 0) return hipSuccess;
   smo::BatchPool P = *pool;
   void *args[] = {(void *)&views, (void *)&items, (void *)&nitems, (void *)&P};
-  return hipLaunchCooperativeKernel((const void *)smo::k_smo_batch, dim3(grid), dim3(smo::BLOCK), args,
-                                    (unsigned)shmem, st);
+  return launch_resident((const void *)smo::k_smo_batch, grid, args, shmem, st);
 }
 
 hipError_t smo_batch_capacity(size_t shmem, int *blocks_per_cu) {
