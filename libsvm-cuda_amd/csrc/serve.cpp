@@ -89,6 +89,7 @@ struct svmhip_server {
   std::vector<svmhip_completion> done;
   smo::View *dV = nullptr;
   int *dItems = nullptr, *dQueue = nullptr;
+  unsigned long long *dT0 = nullptr;
   smo::Ctl *dCtl = nullptr;
   size_t capV = 0, capI = 0, capC = 0;
   // the round in flight: its stage's solves and its predictions
@@ -490,6 +491,7 @@ svmhip_status svmhip_server_create(const svmhip_options *opt, double hbm_budget_
     if (h->opt.grid_blocks > 0) s->team_max = std::min(h->opt.grid_blocks, 64);
     if (h->opt.cache_bytes > 0) s->cache_cap = h->opt.cache_bytes;
     s->dQueue = h->dalloc<int>(1, "server queue counter");
+    s->dT0 = h->dalloc<unsigned long long>(1, "server launch start");
     GUARD_END
   }();
   if (rc != SVMHIP_OK) {
@@ -640,6 +642,10 @@ svmhip_status svmhip_server_step(svmhip_server *s, double stage_ms, int *n_activ
           Request &q = *s->reqs[s->stage[k]];
           ran[q.id] = 1;
           const smo::Ctl &c = hc[k];
+          if (c.stop == smo::STOP_NONE && !c.err) {  // left for the next stage
+            still.push_back(q.id);
+            continue;
+          }
           q.device_ms += ms;
           q.cache_valid = true;
           if (!q.started && c.stop == smo::STOP_PAUSE && !c.err && c.iter == q.iters &&
@@ -704,7 +710,7 @@ svmhip_status svmhip_server_step(svmhip_server *s, double stage_ms, int *n_activ
     }
     // 4. one stage of every admitted solve, launched without waiting: teams
     // first (as many as fit in the grid at once), then single workgroups,
-    // largest first
+    // oldest first
     if (!s->active.empty()) {
       int maxd = 1;
       for (int id : s->active) maxd = std::max(maxd, s->reqs[id]->rows.d);
@@ -744,8 +750,9 @@ svmhip_status svmhip_server_step(svmhip_server *s, double stage_ms, int *n_activ
         tsum += q.T;
         stage.push_back(id);
       }
-      std::stable_sort(solos.begin(), solos.end(),
-                       [&](int a, int b) { return s->reqs[a]->y.size() > s->reqs[b]->y.size(); });
+      // oldest first: a stage's deadline leaves the queue's tail for the next
+      // stage, and that tail must not always be the same requests
+      std::stable_sort(solos.begin(), solos.end(), [&](int a, int b) { return a < b; });
       stage.insert(stage.end(), solos.begin(), solos.end());
       std::vector<smo::View> &views = s->h_views;
       std::vector<int> &items = s->h_items;
@@ -801,9 +808,13 @@ svmhip_status svmhip_server_step(svmhip_server *s, double stage_ms, int *n_activ
       h->upload(s->dV, views.data(), views.size(), "upload server views");
       h->upload(s->dItems, items.data(), items.size(), "upload server items");
       h->ck(hipMemsetAsync(s->dQueue, 0, sizeof(int), h->st), "clear server queue");
+      h->ck(hipMemsetAsync(s->dT0, 0, sizeof(unsigned long long), h->st), "clear launch start");
+      h->ck(hipMemsetAsync(s->dCtl, 0, sizeof(smo::Ctl) * stage.size(), h->st), "clear server control");
       smo::BatchPool pool;
       memset(&pool, 0, sizeof pool);
       pool.queue = s->dQueue;
+      pool.t0 = s->dT0;
+      pool.deadline_ticks = std::max<long long>(1, (long long)(tau * clock_hz));
       pool.K = 0;
       const int nitems = (int)items.size() / 2;
       h->ck(hipEventRecord(h->ev[0], h->st), "event");

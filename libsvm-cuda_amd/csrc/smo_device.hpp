@@ -126,6 +126,9 @@ struct BatchPool {
   int K, by_item;
   int lds_floats;          // dynamic LDS of the launch, in floats
   int pad;
+  unsigned long long *t0;  // launch start on the device clock (zeroed before each launch; null = none)
+  long long deadline_ticks;  // > 0: items pause at t0 + deadline_ticks, and single-workgroup items
+                             // claimed after it are left for the next launch (their Ctl stays clear)
 };
 
 // ---------------------------------------------------------------- helpers

@@ -557,7 +557,7 @@ __device__ __forceinline__ void write_ginit_pause(const View &V) {
 }
 
 template <int KIND>
-__device__ __forceinline__ void solo_solve(View &V, Dec &d, RedSmem &sm, float *xs, int lds_floats) {
+__device__ __forceinline__ void solo_solve(View &V, Dec &d, RedSmem &sm, float *xs, int lds_floats, long long t0) {
   solo_init_cache(V);
   if (!V.resume) {
     if (V.ginit_from == 0) batch_qd(V);
@@ -571,7 +571,7 @@ __device__ __forceinline__ void solo_solve(View &V, Dec &d, RedSmem &sm, float *
   if (threadIdx.x == 0) {
     memset(&d, 0, sizeof d);
     d.iter = V.iter0;
-    d.t_start = wall_clock64();
+    d.t_start = t0 > 0 ? t0 : (long long)wall_clock64();
   }
   __syncthreads();
   {
@@ -610,7 +610,8 @@ __device__ __forceinline__ void solo_solve(View &V, Dec &d, RedSmem &sm, float *
 
 // One member (rank V.rank of V.nblocks) of a team owning a subproblem.
 template <int KIND>
-__device__ __forceinline__ void team_solve(View &V, Dec &d, RedSmem &sm, XSmem &xsm, float *xs, int lds_floats) {
+__device__ __forceinline__ void team_solve(View &V, Dec &d, RedSmem &sm, XSmem &xsm, float *xs, int lds_floats,
+                                           long long t0) {
   int sb, se;
   owned_slices(V, V.rank, V.nblocks, sb, se);
   const int r0 = sb * SLICE, r1 = min(se * SLICE, V.l);
@@ -628,7 +629,7 @@ __device__ __forceinline__ void team_solve(View &V, Dec &d, RedSmem &sm, XSmem &
   if (threadIdx.x == 0) {
     memset(&d, 0, sizeof d);
     d.iter = V.iter0;
-    d.t_start = wall_clock64();
+    d.t_start = t0 > 0 ? t0 : (long long)wall_clock64();
   }
   __syncthreads();
   smo_select_first<KIND>(V, d, sm, xsm, xs);
@@ -675,16 +676,40 @@ __global__ void __launch_bounds__(BLOCK) k_smo_batch(const View *views, const in
   __shared__ XSmem xsm;
   __shared__ Dec d;
   __shared__ int s_item;
+  __shared__ long long s_t0;
   extern __shared__ float xs_lds[];
   float *xs = xs_lds;
+  // the launch's start: the first workgroup to arrive records it
+  if (threadIdx.x == 0) {
+    long long t = 0;
+    if (pool.t0) {
+      const unsigned long long now = (unsigned long long)wall_clock64();
+      const unsigned long long old = atomicCAS(pool.t0, 0ull, now);
+      t = (long long)(old ? old : now);
+    }
+    s_t0 = t;
+  }
+  __syncthreads();
+  const long long t0 = s_t0;
+  __shared__ long long s_now;
   for (;;) {
-    if (threadIdx.x == 0) s_item = atomicAdd(pool.queue, 1);
+    if (threadIdx.x == 0) {
+      s_item = atomicAdd(pool.queue, 1);
+      s_now = (long long)wall_clock64();
+    }
     __syncthreads();
     // uniform across the workgroup: keeps the View in scalar registers
     const int q = __builtin_amdgcn_readfirstlane(s_item);
     if (q >= nitems) break;
     const int sub = __builtin_amdgcn_readfirstlane(items[2 * q]);
     View V = views[sub];
+    // past the launch's deadline, single-workgroup items are left untouched
+    // for the next launch (team members are always run: they are queued
+    // first and all claimed at the start)
+    if (pool.deadline_ticks > 0 && V.nblocks <= 1 && s_now - t0 >= pool.deadline_ticks) {  // uniform: thread 0's clock
+      __syncthreads();
+      continue;
+    }
     V.rank = __builtin_amdgcn_readfirstlane(items[2 * q + 1]);
     int team;
     if (V.own_cache) {
@@ -707,13 +732,13 @@ __global__ void __launch_bounds__(BLOCK) k_smo_batch(const View *views, const in
     }
     V.pm_stride = 0;
     if (team > 1) {
-      if (V.kind == KIND_C) team_solve<KIND_C>(V, d, sm, xsm, xs, pool.lds_floats);
-      else team_solve<KIND_NU>(V, d, sm, xsm, xs, pool.lds_floats);
+      if (V.kind == KIND_C) team_solve<KIND_C>(V, d, sm, xsm, xs, pool.lds_floats, t0);
+      else team_solve<KIND_NU>(V, d, sm, xsm, xs, pool.lds_floats, t0);
     } else {
       V.nblocks = 1;
       V.rank = 0;
-      if (V.kind == KIND_C) solo_solve<KIND_C>(V, d, sm, xs, pool.lds_floats);
-      else solo_solve<KIND_NU>(V, d, sm, xs, pool.lds_floats);
+      if (V.kind == KIND_C) solo_solve<KIND_C>(V, d, sm, xs, pool.lds_floats, t0);
+      else solo_solve<KIND_NU>(V, d, sm, xs, pool.lds_floats, t0);
     }
     __syncthreads();
   }

@@ -702,6 +702,7 @@ svmhip_status svmhip_batch_solve(svmhip_batch *b, svmhip_subproblem *subs, int n
     pool.lru_stride = 4LL * maxn + 2;
     pool.lru = (int *)talloc(sizeof(int) * (size_t)pool.lru_stride * K, "batch LRU bookkeeping");
     pool.queue = (int *)talloc(sizeof(int), "batch queue");
+    pool.t0 = (unsigned long long *)talloc(sizeof(unsigned long long), "batch launch start");
     pool.lds_floats = (int)(lds / sizeof(float));
     smo::XPart *dX = (smo::XPart *)talloc(sizeof(smo::XPart) * 2 * (size_t)G, "team exchange records");
     int *dItems = (int *)talloc(sizeof(int) * 2 * (size_t)std::max(nsub, G), "batch items");
@@ -717,6 +718,7 @@ svmhip_status svmhip_batch_solve(svmhip_batch *b, svmhip_subproblem *subs, int n
     const int team_max = h->opt.grid_blocks > 0 ? std::min(h->opt.grid_blocks, 64) : 32;
     const int fixed_chunk = h->opt.iters_per_launch;
     std::vector<int> rem(nsub), iters(nsub, 0);
+    std::vector<char> started(nsub, 0);
     bool all_optimal = true;
     for (int k = 0; k < nsub; ++k) rem[k] = k;
     std::vector<smo::Ctl> hC(nsub);
@@ -767,12 +769,12 @@ svmhip_status svmhip_batch_solve(svmhip_batch *b, svmhip_subproblem *subs, int n
         V.nblocks = T[k];
         V.xpart = T[k] > 1 ? dX + xoff : nullptr;
         if (T[k] > 1) xoff += 2 * T[k];
-        V.resume = stages > 0 ? 1 : 0;
+        V.resume = started[k];
         V.iter0 = iters[k];
         if (fixed_chunk > 0) {  // pause after a fixed number of iterations
           V.max_iter = (int)std::min<long long>((long long)iters[k] + fixed_chunk, true_max[k]);
           V.pause_ticks = 0;
-        } else {  // pause on the device clock after about tau
+        } else {  // pause on the device clock about tau after the launch starts
           V.max_iter = true_max[k];
           V.pause_ticks = (long long)(tau * clock_hz);
         }
@@ -786,6 +788,8 @@ svmhip_status svmhip_batch_solve(svmhip_batch *b, svmhip_subproblem *subs, int n
       h->upload(dV, views.data(), nsub, "upload batch views");
       h->upload(dItems, items.data(), items.size(), "upload batch items");
       h->ck(hipMemsetAsync(pool.queue, 0, sizeof(int), h->st), "clear batch queue");
+      h->ck(hipMemsetAsync(pool.t0, 0, sizeof(unsigned long long), h->st), "clear launch start");
+      pool.deadline_ticks = fixed_chunk > 0 ? 0 : (long long)(tau * clock_hz);
       if (xoff > 0) h->ck(hipMemsetAsync(dX, 0, sizeof(smo::XPart) * xoff, h->st), "clear team records");
       h->ck(hipMemsetAsync(dC, 0, sizeof(smo::Ctl) * nsub, h->st), "clear batch control");
       h->ck(hipEventRecord(h->ev[0], h->st), "event");
@@ -800,6 +804,11 @@ svmhip_status svmhip_batch_solve(svmhip_batch *b, svmhip_subproblem *subs, int n
       std::vector<int> next;
       for (int k : rem) {
         const smo::Ctl &c = hC[k];
+        if (c.stop == smo::STOP_NONE && !c.err) {  // left for the next launch
+          next.push_back(k);
+          continue;
+        }
+        started[k] = 1;
         if (c.err) h->fail(SVMHIP_ERR_TIMEOUT, "subproblem %d: a team member did not respond", k);
         iters[k] = c.iter;
         tot.row_hits += (long long)c.hits;
