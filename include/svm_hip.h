@@ -57,6 +57,10 @@ typedef struct {
 	double cache_bytes;       /* kernel-row cache bytes (0 = param.cache_size MB) */
 	int iters_per_launch;     /* persistent-kernel iterations per launch (0 = all) */
 	int verbose;              /* 1 = print setup and cache statistics to stderr */
+	int speculative_rows;     /* svmhip_solve, C rule, full cache: 1 = on a row-i miss also compute
+				     the rows of the strongest other violators in the same pass over X
+				     (0 = off, the default: slower on the measured configurations);
+				     results are identical either way */
 } svmhip_options;
 
 typedef struct svmhip_solver svmhip_solver;

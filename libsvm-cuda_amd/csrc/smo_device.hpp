@@ -108,7 +108,7 @@ struct View {
   int own_cache;          // batched solves: rows/S/full/row_stride and the cache metadata
                           // (nblocks copies, pm_stride apart) are the subproblem's own
   int keep_cache;         // batched solves: the own cache is valid from an earlier stage
-  int pad_rank;
+  int spec;               // persistent C-rule solves, full cache: speculative rows on an i miss
 };
 
 // Work queue and kernel-row cache arenas of the batched solver.  Item q of
@@ -285,14 +285,14 @@ __device__ __forceinline__ double row_dot(const View &V, const float *xs, int r)
 // dot(x_q(m), x_r) for the M rows staged at xs + m * ld, each summed in the
 // same order as row_dot (so each equals row_dot bit for bit): one pass over
 // row r serves M query rows.
-template <int M>
+template <int M, int U = 8>
 __device__ __forceinline__ void row_dot_multi(const View &V, const float *xs, int ld, int r, double acc[M]) {
   const int s = r / SLICE;
   const long long base = V.slice_ptr[s] + (r % SLICE);
   const float *vp = V.vals + base;
 #pragma unroll
   for (int m = 0; m < M; ++m) acc[m] = 0.0;
-  constexpr int U = 8;
+  // U entries of row r are loaded per trip (one memory round trip each)
   if (V.dense) {
     const int len = V.d;
     int p = 0;
@@ -637,10 +637,16 @@ __device__ __forceinline__ Partial pass_gradient_select(const View &V, const Sna
         if (KIND == KIND_C) {
           if (yk[u][h] == +1) {
             if (sk != AT_UPPER) max_idx(v[0], ix[0], -gk, k);
-            if (sk != AT_LOWER) v[1] = fmax(v[1], gk);
+            if (sk != AT_LOWER) {
+              if (V.spec) max_idx(v[1], ix[1], gk, k);
+              else v[1] = fmax(v[1], gk);
+            }
           } else {
             if (sk != AT_LOWER) max_idx(v[0], ix[0], gk, k);
-            if (sk != AT_UPPER) v[1] = fmax(v[1], -gk);
+            if (sk != AT_UPPER) {
+              if (V.spec) max_idx(v[1], ix[1], -gk, k);
+              else v[1] = fmax(v[1], -gk);
+            }
           }
         } else {
           if (yk[u][h] == +1) {
@@ -657,8 +663,10 @@ __device__ __forceinline__ Partial pass_gradient_select(const View &V, const Sna
   Partial pt;
   if (!select) return pt;
   if (KIND == KIND_C) {
-    const int kinds[4] = {0, 2, 2, 2};
-    block_reduce<2>(sm, v, ix, kinds);
+    // i, and (for speculative rows) the argmax of Gmax2
+    const int kinds_spec[4] = {0, 0, 2, 2}, kinds[4] = {0, 2, 2, 2};
+    if (V.spec) block_reduce<2>(sm, v, ix, kinds_spec);
+    else block_reduce<2>(sm, v, ix, kinds);
   } else {
     const int kinds[4] = {0, 0, 2, 2};
     block_reduce<4>(sm, v, ix, kinds);
@@ -1049,8 +1057,9 @@ __device__ __forceinline__ XPart exchange(const View &V, RedSmem &sm, XSmem &xs,
     const int kinds[4] = {1, 2, 2, 2};
     block_reduce<1>(sm, v, ix, kinds);
   } else if (KIND == KIND_C) {
-    const int kinds[4] = {0, 2, 2, 2};
-    block_reduce<2>(sm, v, ix, kinds);
+    const int kinds_spec[4] = {0, 0, 2, 2}, kinds[4] = {0, 2, 2, 2};
+    if (V.spec) block_reduce<2>(sm, v, ix, kinds_spec);
+    else block_reduce<2>(sm, v, ix, kinds);
   } else {
     const int kinds[4] = {0, 0, 2, 2};
     block_reduce<4>(sm, v, ix, kinds);

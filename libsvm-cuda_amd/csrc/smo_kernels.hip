@@ -181,16 +181,127 @@ __device__ __forceinline__ void smo_select_first(const View &V, Dec &d, RedSmem 
   __syncthreads();
 }
 
+// Speculative kernel rows (C rule, full cache).  When row i misses, the
+// rows of the strongest other violators are computed in the same pass over
+// X: a violator of one iteration is very likely to be selected in a later
+// one, and one pass serves several rows (row_dot_multi).  The candidates are
+// the owners' best I_up and best I_low variables of the i exchange just
+// made, whose records every owner reads alike (they are overwritten only two
+// exchanges later), so every owner marks the same rows as cached.  Every
+// kernel value is the one a miss would compute.
+constexpr int SPEC_ROWS = 8;  // row i plus up to 7 candidates
+constexpr int SPEC_UP = 3, SPEC_LOW = 4;
+
+// Owned parts of the rows in list[0..cnt) (full cache: slot = row).
+__device__ __forceinline__ void compute_rows_part(const View &V, const int *list, int cnt, float *xs, int sb,
+                                                  int se) {
+  for (int m = 0; m < cnt; ++m) stage_row(V, list[m], xs + m * V.d);
+  const int wave = threadIdx.x / 64, lane = threadIdx.x % 64, nw = blockDim.x / 64;
+  for (int sl = sb + wave; sl < se; sl += nw) {
+    const int r = sl * SLICE + lane;
+    if (r >= V.l) continue;
+    double acc[SPEC_ROWS];
+    row_dot_multi<SPEC_ROWS>(V, xs, V.d, r, acc);
+#pragma unroll
+    for (int m = 0; m < SPEC_ROWS; ++m)
+      if (m < cnt) V.rows[(long long)list[m] * V.row_stride + r] = (float)kernel_f64(V, list[m], r, acc[m]);
+  }
+  __syncthreads();
+}
+
+template <int KIND>
+__device__ __forceinline__ void spec_rows(const View &V, Dec &d, RedSmem &sm, float *xs) {
+  __shared__ int list[SPEC_ROWS];
+  __shared__ int s_cnt, s_pick;
+  const XPart *buf = V.xpart + (size_t)((unsigned)d.epoch & 1u) * V.nblocks;
+  const int t = threadIdx.x;
+  double cu = -INF, cl = -INF;
+  int iu = -1, il = -1;
+  if (t < V.nblocks) {
+    iu = ld_acq(&buf[t].i0);
+    cu = ld_acq(&buf[t].v0);
+    il = ld_acq(&buf[t].i1);
+    cl = ld_acq(&buf[t].v1);
+    if (iu < 0 || iu == d.i) cu = -INF;
+    if (il < 0 || il == d.i) cl = -INF;
+  }
+  // each wave keeps its own top candidates (wave reductions only), then
+  // thread 0 merges the waves' lists
+  __shared__ double wv[16][SPEC_UP + SPEC_LOW];
+  __shared__ int wi[16][SPEC_UP + SPEC_LOW];
+  const int wave = t / 64, lane = t % 64, nw = blockDim.x / 64;
+#pragma unroll
+  for (int k = 0; k < SPEC_UP + SPEC_LOW; ++k) {
+    const bool up = k < SPEC_UP;
+    double v = up ? cu : cl;
+    int ix = up ? iu : il;
+    wave_max_idx(v, ix);
+    if (lane == 0) {
+      wv[wave][k] = v;
+      wi[wave][k] = v > -INF ? ix : -1;
+    }
+    if (v > -INF) {
+      if (up && iu == ix) cu = -INF;
+      if (!up && il == ix) cl = -INF;
+    }
+  }
+  __syncthreads();
+  int picks[SPEC_UP + SPEC_LOW];
+  if (threadIdx.x == 0) {
+    for (int side = 0; side < 2; ++side) {
+      const int k0 = side == 0 ? 0 : SPEC_UP, k1 = side == 0 ? SPEC_UP : SPEC_UP + SPEC_LOW;
+      for (int k = k0; k < k1; ++k) {  // the best entry of this side not yet picked
+        double bv = -INF;
+        int bi = -1, bw = -1, bk = -1;
+        for (int w = 0; w < nw; ++w)
+          for (int kk = k0; kk < k1; ++kk)
+            if (wi[w][kk] >= 0 && (wv[w][kk] > bv || (wv[w][kk] == bv && wi[w][kk] > bi))) {
+              bv = wv[w][kk];
+              bi = wi[w][kk];
+              bw = w;
+              bk = kk;
+            }
+        picks[k] = bi;
+        if (bw >= 0) wi[bw][bk] = -1;
+      }
+    }
+  }
+  if (threadIdx.x == 0) {
+    CacheMeta M = private_meta(V, d);
+    int cnt = 0;
+    list[cnt++] = real_of(V, d.i);  // decide_i has marked it
+    for (int k = 0; k < SPEC_UP + SPEC_LOW; ++k) {
+      if (picks[k] < 0) continue;
+      const int row = real_of(V, picks[k]);
+      bool dup = false;
+      for (int m = 0; m < cnt; ++m) dup |= list[m] == row;
+      if (dup || M.computed[row]) continue;
+      M.computed[row] = 1;
+      ++d.misses;
+      list[cnt++] = row;
+    }
+    s_cnt = cnt;
+  }
+  __syncthreads();
+  int sb, se;
+  owned_slices(V, my_owner(V), V.nblocks, sb, se);
+  compute_rows_part(V, list, s_cnt, xs, sb, se);
+  if (threadIdx.x == 0) d.miss_i = 0;
+  __syncthreads();
+}
+
 // The SMO loop across co-resident owners, one iteration per trip:
 //   [select j over owned variables, computing row i on a miss] exchange
 //   [every workgroup: j, row-j slot, alpha step]
 //   [owners store the pair's alpha/status; G update, computing row j on a
 //    miss; select i] exchange
 //   [every workgroup: i, Gmax/Gmax2, stop test, row-i slot]
-template <int KIND>
-__device__ __forceinline__ void smo_iterations(const View &V, Dec &d, RedSmem &sm, XSmem &xsm, float *xs, int iter_limit) {
+template <int KIND, bool SPEC = false>
+__device__ __forceinline__ void smo_iterations(const View &V, Dec &d, RedSmem &sm, XSmem &xsm, float *xs,
+                                               int iter_limit) {
   for (int n = 0; n < iter_limit; ++n) {
     if (d.stop || d.err) break;
+    if (SPEC && KIND == KIND_C && V.spec && d.miss_i) spec_rows<KIND>(V, d, sm, xs);
     Snap c = snap_of(d);
     Partial bp = pass_select_j<KIND>(V, c, xs, sm, false);
     XPart mine = make_j_record<KIND>(V, c, bp);
@@ -234,7 +345,7 @@ __global__ void __launch_bounds__(BLOCK) k_smo(View V, int iter_limit) {
   if (threadIdx.x == 0) d = ctl_dec(V.ctl);
   __syncthreads();
   if (d.phase == 0 && !d.stop && !d.err) smo_select_first<KIND>(V, d, sm, xsm, xs);
-  smo_iterations<KIND>(V, d, sm, xsm, xs, iter_limit);
+  smo_iterations<KIND, true>(V, d, sm, xsm, xs, iter_limit);
   if (blockIdx.x == 0 && threadIdx.x == 0) {
     Ctl *c = V.ctl;
     ctl_dec(c) = d;

@@ -441,6 +441,13 @@ svmhip_status svmhip_solve(svmhip_solver *h, int max_iter, svmhip_stats *stats) 
     }
     smo::View V = h->view();
     V.max_iter = h->host_ctl->max_iter;
+    // speculative rows (C rule, full cache, 8 staged rows in LDS)
+    size_t shmem = h->shmem;
+    if (h->opt.speculative_rows > 0 && h->full && h->use_lds && h->kind == SVMHIP_KIND_C &&
+        h->ktype != PRECOMPUTED && 8 * (size_t)h->d * sizeof(float) <= kLdsQueryLimit) {
+      V.spec = 1;
+      shmem = 8 * (size_t)h->d * sizeof(float);
+    }
     int chunk = h->opt.iters_per_launch > 0 ? h->opt.iters_per_launch : INT_MAX;
     h->ck(hipEventRecord(h->ev[0], h->st), "event");
     if (h->nvar > 0) {
@@ -448,7 +455,7 @@ svmhip_status svmhip_solve(svmhip_solver *h, int max_iter, svmhip_stats *stats) 
       h->stats.launches = 0;
       for (;;) {
         h->ck(hipEventRecord(h->ev[2], h->st), "event");
-        h->ck(smo_launch_persistent(&V, chunk, h->shmem, h->st), "persistent SMO kernel");
+        h->ck(smo_launch_persistent(&V, chunk, shmem, h->st), "persistent SMO kernel");
         h->ck(hipEventRecord(h->ev[3], h->st), "event");
         h->read_ctl();
         float kms = 0;

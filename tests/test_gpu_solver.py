@@ -165,6 +165,31 @@ def test_grid_sizes_agree(hip, grid):
     np.testing.assert_array_equal(dev["alpha_raw"], ref["alpha_raw"])
 
 
+@pytest.mark.parametrize("dense", [True, False])
+@pytest.mark.parametrize("grid", [0, 1])
+def test_speculative_rows_same_iterates(hip, ora, dense, grid):
+    """Rows computed ahead on an i miss (full cache, C rule) change nothing:
+    the same iterates with speculation on and off, and the oracle's."""
+    if dense:
+        X, y = datasets.gmm_dense_matrix(n=5000, d=64, seed=5)
+        prob = ffi.Problem.from_dense(X.astype(np.float64), y, keep_zeros=True)
+        gamma = 1.0 / 64
+    else:
+        prob, d = zipf_problem(4000)
+        gamma = 1.0 / d
+    prm = ffi.make_param(gamma=gamma, C_=1.0, eps=1e-3, shrinking=0)
+    on = hip.solve_one(prob, prm, cache_bytes=2e9, grid_blocks=grid, speculative_rows=1)
+    off = hip.solve_one(prob, prm, cache_bytes=2e9, grid_blocks=grid, speculative_rows=0)
+    ref = ora.train_one(prob, prm)
+    print(f"  iters {on['iters']} rows computed: speculative {on['stats']['row_misses']}, "
+          f"plain {off['stats']['row_misses']}", flush=True)
+    assert on["iters"] == off["iters"] == ref["iters"]
+    np.testing.assert_array_equal(on["alpha_raw"], off["alpha_raw"])
+    assert on["obj"] == off["obj"] and on["rho"] == off["rho"]
+    assert_solution_close(on, ref)
+    assert on["stats"]["row_misses"] >= off["stats"]["row_misses"]
+
+
 def test_max_iter_cap(hip, ora):
     prob, d = zipf_problem(800)
     prm = ffi.make_param(gamma=1.0 / d, C_=1.0, eps=1e-3, shrinking=0)
