@@ -16,6 +16,11 @@ Also reported on the same JSON line:
   cpu_baseline the FP64 CPU oracle (LIBSVM 3.18 restatement) on one host core,
                on the first iterations of the same solve (bounded sample)
 
+Other workloads (--workload): blob_csr (60,000 x 784 CSR, gamma=0.02, C=10,
+4 GB row cache: the plain sparse configuration), ovo_cv_grid (900 solves of
+one-vs-one x 5-fold CV x 4 gammas in batched launches) and serving (a Poisson
+stream of train / predict requests).
+
 Usage: python bench.py [--gpus 1] [--steps K] [--warmup W] [--n N] [--no-cpu]
 """
 import argparse
@@ -372,7 +377,7 @@ def main():
     ap.add_argument("--copy-probe-gib", type=float, default=0.0,
                     help="instead of the solve, run a plain 1-pass device copy of this many GiB "
                          "(reference for the FETCH_SIZE / WRITE_SIZE counters)")
-    ap.add_argument("--workload", default="gmm_dense", choices=["gmm_dense", "ovo_cv_grid", "serving"])
+    ap.add_argument("--workload", default="gmm_dense", choices=["gmm_dense", "blob_csr", "ovo_cv_grid", "serving"])
     ap.add_argument("--qps", type=float, default=20.0)
     ap.add_argument("--window", type=float, default=60.0)
     ap.add_argument("--stage-ms", type=float, default=10.0)
@@ -395,16 +400,34 @@ def main():
     if args.workload == "serving":
         bench_serving(args)
         return
-    n, d = args.n, args.d
     t0 = time.perf_counter()
-    X, y = datasets.gmm_dense_matrix(n=n, d=d, seed=1003)
-    prob = ffi.Problem.from_dense(X.astype(np.float64), y, keep_zeros=True)
+    if args.workload == "blob_csr":
+        # 60,000 x 784 CSR rows (~19% nonzero), RBF gamma=0.02, C=10, a 4 GB
+        # device row cache: the plain sparse-dot configuration
+        n = args.n if args.n != 100000 else 60000
+        ip, ix, v, y, d = datasets.blob_mask_csr(n=n)
+        prob = ffi.Problem(ip, ix, v, y, d)
+        nnz = int(ip[-1])
+        prm = ffi.make_param(gamma=0.02, C_=10.0, eps=1e-3, cache_size=4096.0, shrinking=0)
+        cache_gb = args.cache_gb if args.cache_gb != 64.0 else 4.0
+        b_row = 8.0 * nnz + 12.0 * n
+        workload = (f"blob_mask_csr n={n} d={d} nnz={nnz} C-SVC RBF gamma=0.02 C=10 eps=1e-3, "
+                    f"{cache_gb:g} GB row cache")
+        row_bytes = "rows_computed * (8*nnz + 12*l)"
+    else:
+        n, d = args.n, args.d
+        X, y = datasets.gmm_dense_matrix(n=n, d=d, seed=1003)
+        prob = ffi.Problem.from_dense(X.astype(np.float64), y, keep_zeros=True)
+        prm = ffi.make_param(gamma=1.0 / d, C_=1.0, eps=1e-3, cache_size=100.0, shrinking=0)
+        cache_gb = args.cache_gb
+        b_row = 4.0 * n * d + 8.0 * n
+        workload = f"gmm_dense n={n} d={d} C-SVC RBF gamma=1/{d} C=1 eps=1e-3"
+        row_bytes = "rows_computed * (4*l*d + 8*l)"
     log(f"[bench] data n={n} d={d} built in {time.perf_counter() - t0:.1f}s")
-    prm = ffi.make_param(gamma=1.0 / d, C_=1.0, eps=1e-3, cache_size=100.0, shrinking=0)
     ys, p, a0, Cp, Cn, kind = libsvm_hip.formulate(prob, prm)
     st0 = libsvm_hip.initial_status(ys, a0, Cp, Cn)
     s = libsvm_hip.DeviceSolver(prob, prm, kind=kind, grid_blocks=args.grid,
-                                cache_bytes=args.cache_gb * 2 ** 30)
+                                cache_bytes=cache_gb * 2 ** 30)
     s.setup_rbf_variables()
     log("[bench] problem resident on the device")
 
@@ -430,11 +453,10 @@ def main():
     launches = sum(st["launches"] for st in stats)
     misses = sum(st["row_misses"] for st in stats)
     b_iter = 44.0 * n
-    b_row = 4.0 * n * d + 8.0 * n
     alg_bytes = (iters * b_iter + misses * b_row) / launches
     launch_s = kern_s / launches
     achieved = alg_bytes / launch_s
-    traffic = measured_traffic(n, d, args.grid)
+    traffic = measured_traffic(n, d, args.grid) if args.workload == "gmm_dense" else None
     last = stats[-1]
     out = {
         "metric": METRIC,
@@ -449,14 +471,15 @@ def main():
         "vs_baseline": None,
         "dtype": "f64",
         "data": "synthetic",
-        "config": {"workload": f"gmm_dense n={n} d={d} C-SVC RBF gamma=1/{d} C=1 eps=1e-3",
+        "config": {"workload": workload,
                    "train_time_s": wall / args.steps, "iters_per_solve": last["iters"],
                    "rows_computed_per_solve": last["row_misses"], "cache_rows": last["cache_slots"],
                    "grid_blocks": last["grid_blocks"], "obj": last["obj"], "rho": last["rho"]},
         "roofline": {"bound": "hbm", "achieved": achieved / 1e9, "peak": HBM_PEAK / 1e9, "unit": "GB/s",
                      "frac": achieved / HBM_PEAK, "traffic": traffic,
                      "kernel": "k_smo", "avg_launch_ms": 1e3 * launch_s,
-                     "bytes_per_launch": alg_bytes},
+                     "bytes_per_launch": alg_bytes,
+                     "bytes_formula": f"(iters * 44*l + {row_bytes}) / launches"},
     }
     if not args.no_plain:
         # the reference's shape of the loop on the same kernels: one host call

@@ -242,6 +242,56 @@ def test_predict_and_model_file_roundtrip(hip, ora, tmp_path):
     np.testing.assert_allclose(dec2, dec, atol=1e-6)
 
 
+def certify(hip, prob, prm, r, C, sample=96, seed=0):
+    """Size-independent checks of a C-SVC solution: 0 <= alpha <= C,
+    y'alpha = 0, the eps stopping rule m(alpha) - M(alpha) < eps, and the
+    returned gradient equal to Q alpha + p (p = -1) on a random sample of
+    variables, each recomputed in FP64 from a device kernel row."""
+    a, G, yy = r["alpha_raw"], r["G"], np.where(prob.y > 0, 1, -1)
+    assert np.all(a >= 0) and np.all(a <= C)
+    assert abs(np.dot(a, yy)) < 1e-8 * max(1.0, a.sum())
+    up = ((yy == 1) & (a < C)) | ((yy == -1) & (a > 0))
+    low = ((yy == 1) & (a > 0)) | ((yy == -1) & (a < C))
+    gap = np.max(-yy[up] * G[up]) - np.min(-yy[low] * G[low])
+    assert gap < prm.eps + 1e-9
+    ya = yy * a
+    s = hip.DeviceSolver(prob, prm)
+    s.setup_rbf_variables()
+    rng = np.random.default_rng(seed)
+    ts = np.concatenate([rng.choice(prob.l, sample, replace=False), np.nonzero(a > 0)[0][:8]])
+    worst = 0.0
+    for t in ts:
+        g = -1.0 + yy[t] * np.dot(s.kernel_row(int(t)).astype(np.float64), ya)
+        worst = max(worst, abs(g - G[t]))
+    s.close()
+    print(f"  iters={r['iters']} nSV={int(np.count_nonzero(a))} gap={gap:.3e} "
+          f"max |G - (Q alpha + p)| over {len(ts)} variables = {worst:.2e} "
+          f"solve_ms={r['stats']['solve_ms']:.1f}", flush=True)
+    assert worst <= 1e-9 * max(1.0, a.sum())
+
+
+def test_certificate_blob_csr_60k_4gb_cache(hip):
+    """The plain sparse configuration at full size: 60,000 x 784 CSR rows,
+    RBF gamma=0.02, C=10, eps=1e-3, a 4 GB row cache (LRU under pressure)."""
+    ip, ix, v, y, d = datasets.blob_mask_csr(n=60000)
+    prob = ffi.Problem(ip, ix, v, y, d)
+    prm = ffi.make_param(gamma=0.02, C_=10.0, eps=1e-3, shrinking=0, cache_size=4096.0)
+    r = hip.solve_one(prob, prm, cache_bytes=4 * 2 ** 30)
+    assert r["stats"]["optimal"] == 1
+    assert r["stats"]["row_misses"] > prob.l  # the cache held only part of Q
+    certify(hip, prob, prm, r, 10.0)
+
+
+def test_certificate_gmm_dense_100k(hip):
+    """The n=100k dense RBF configuration at full size (gamma=1/256, C=1)."""
+    X, y = datasets.gmm_dense_matrix(n=100000, d=256, seed=1003)
+    prob = ffi.Problem.from_dense(X.astype(np.float64), y, keep_zeros=True)
+    prm = ffi.make_param(gamma=1.0 / 256, C_=1.0, eps=1e-3, shrinking=0)
+    r = hip.solve_one(prob, prm, cache_bytes=4 * 2 ** 30)
+    assert r["stats"]["optimal"] == 1
+    certify(hip, prob, prm, r, 1.0)
+
+
 def test_kkt_certificate_blob_csr_20k(hip):
     """Size-independent check at a larger size: the returned point is
     feasible and satisfies the eps stopping rule m(alpha) - M(alpha) < eps."""
