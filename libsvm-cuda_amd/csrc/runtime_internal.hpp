@@ -84,7 +84,7 @@ struct svmhip_solver {
   int *d_slot_of_row = nullptr, *d_row_of_slot = nullptr, *d_prev = nullptr, *d_next = nullptr;
   smo::Ctl *d_ctl = nullptr;
   smo::Partial *d_part = nullptr;
-  smo::XPart *d_xpart = nullptr;
+  smo::XWord *d_xw = nullptr;
   int8_t *d_pm_flags = nullptr;
   int *d_pm_lru = nullptr;
   long long pm_stride = 0;
@@ -149,7 +149,7 @@ struct svmhip_solver {
     V.computed = d_computed;
     V.ctl = d_ctl; V.part = d_part; V.xs_global = d_xs;
     V.use_lds = use_lds; V.nblocks = nblocks; V.rank = -1;
-    V.xpart = d_xpart; V.pm_flags = d_pm_flags; V.pm_lru = d_pm_lru; V.pm_stride = pm_stride;
+    V.xw = d_xw; V.pm_flags = d_pm_flags; V.pm_lru = d_pm_lru; V.pm_stride = pm_stride;
     return V;
   }
 

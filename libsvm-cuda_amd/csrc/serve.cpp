@@ -60,7 +60,7 @@ struct Request {
   int S = 0, full = 0;
   int8_t *flags = nullptr;
   int *lru = nullptr;
-  smo::XPart *xpart = nullptr;
+  smo::XWord *xw = nullptr;
   double rho = 0, obj = 0, r = 0;
   std::vector<double> coef;  // signed, all rows
   int nSV = 0;
@@ -146,7 +146,7 @@ double train_footprint(const Request &q, double cache_cap, long long *arena_floa
   const long long S = std::min<long long>(af / stride, l);
   const double meta = S >= l ? (double)q.Tcap * l : 4.0 * q.Tcap * (l + 3 * S + 2);
   return sell_bytes(o.total, l, o.nslices, o.dense) + 34.0 * nvar + 4.0 * af + meta +
-         2.0 * q.Tcap * sizeof(smo::XPart) + sizeof(smo::Ctl);
+         (double)smo::xrec_words(q.Tcap) * sizeof(smo::XWord) + sizeof(smo::Ctl);
 }
 
 // Estimated device seconds of one SMO iteration of q on one workgroup: two
@@ -185,14 +185,14 @@ void complete(svmhip_server *s, Request &q, int status) {
 // Frees the solver state of a train request (keeps rows and model).
 void free_train_state(svmhip_solver *h, Request &q) {
   for (void *p : {(void *)q.G, (void *)q.alpha, (void *)q.pv, (void *)q.QD, (void *)q.yd, (void *)q.st,
-                  (void *)q.arena, (void *)q.flags, (void *)q.lru, (void *)q.xpart})
+                  (void *)q.arena, (void *)q.flags, (void *)q.lru, (void *)q.xw})
     h->dfree(p);
   q.G = q.alpha = q.pv = q.QD = nullptr;
   q.yd = q.st = nullptr;
   q.arena = nullptr;
   q.flags = nullptr;
   q.lru = nullptr;
-  q.xpart = nullptr;
+  q.xw = nullptr;
 }
 
 void admit_train(svmhip_server *s, Request &q) {
@@ -231,7 +231,7 @@ void admit_train(svmhip_server *s, Request &q) {
     q.pm_stride = (long long)l + 3LL * q.S + 2;
     q.lru = h->dalloc<int>((size_t)q.Tcap * q.pm_stride, "request LRU bookkeeping");
   }
-  if (q.Tcap > 1) q.xpart = h->dalloc<smo::XPart>(2 * (size_t)q.Tcap, "request exchange records");
+  if (q.Tcap > 1) q.xw = h->dalloc<smo::XWord>(smo::xrec_words(q.Tcap), "request exchange records");
   smo::View V = rows_view(h, q.rows, q);
   if (l > 0) h->ck(smo_launch_self_dot(&V, q.rows.xsq, h->st), "request x^2");
   // the host copies above read q.hs / st0: wait before they go
@@ -790,7 +790,7 @@ svmhip_status svmhip_server_step(svmhip_server *s, double stage_ms, int *n_activ
         V.max_iter = q.max_iter;
         V.pause_ticks = std::max<long long>(1, (long long)(tau * clock_hz));
         V.nblocks = q.T;
-        V.xpart = q.xpart;
+        V.xw = q.xw;
         views[k] = V;
         for (int r = 0; r < q.T; ++r) {
           items.push_back((int)k);
@@ -803,7 +803,7 @@ svmhip_status svmhip_server_step(svmhip_server *s, double stage_ms, int *n_activ
       for (size_t k = 0; k < stage.size(); ++k) {
         views[k].ctl = s->dCtl + k;
         Request &q = *s->reqs[stage[k]];
-        if (q.T > 1) h->ck(hipMemsetAsync(q.xpart, 0, sizeof(smo::XPart) * 2 * q.T, h->st), "clear team records");
+        if (q.T > 1) h->ck(hipMemsetAsync(q.xw, 0, sizeof(smo::XWord) * smo::xrec_words(q.T), h->st), "clear team records");
       }
       h->upload(s->dV, views.data(), views.size(), "upload server views");
       h->upload(s->dItems, items.data(), items.size(), "upload server items");

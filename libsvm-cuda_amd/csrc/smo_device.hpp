@@ -25,6 +25,7 @@
 #pragma once
 
 #include <hip/hip_runtime.h>
+#include <limits.h>
 #include <stdint.h>
 
 namespace smo {
@@ -93,7 +94,7 @@ struct View {
   // control
   Ctl *ctl;
   Partial *part;
-  struct XPart *xpart;    // persistent kernel: 2 x nblocks exchange records
+  unsigned long long *xw; // persistent kernel / teams: exchange words, two buffers of XW x nblocks
   int8_t *pm_flags;       // persistent kernel, full cache: nblocks x l row flags
   int *pm_lru;            // persistent kernel, LRU cache: nblocks x pm_stride ints
   long long pm_stride;
@@ -161,23 +162,60 @@ __device__ __forceinline__ void min_idx(double &v, int &i, double v2, int i2) {
   if (v2 < v || (v2 == v && i2 > i)) { v = v2; i = i2; }
 }
 
-__device__ __forceinline__ double wave_max(double v) {
-  for (int o = 32; o > 0; o >>= 1) v = fmax(v, __shfl_xor(v, o, 64));
-  return v;
+// Wavefront max / min through DPP lane moves (VALU, no LDS round trips):
+// quad_perm xor 1 and xor 2, row_ror 4 and 8 (every lane of a row of 16 then
+// holds the row's result), row_bcast 15 and 31 (lane 63 then holds the
+// wave's), and a read of lane 63.  Rows masked out of a move keep their own
+// value, which max / min leave unchanged.
+template <int CTRL, int ROWS = 0xf>
+__device__ __forceinline__ int dpp_i(int x) {
+  return __builtin_amdgcn_update_dpp(x, x, CTRL, ROWS, 0xf, false);
 }
+template <int CTRL, int ROWS = 0xf>
+__device__ __forceinline__ double dpp_d(double x) {
+  const int lo = dpp_i<CTRL, ROWS>(__double2loint(x));
+  const int hi = dpp_i<CTRL, ROWS>(__double2hiint(x));
+  return __hiloint2double(hi, lo);
+}
+__device__ __forceinline__ double lane63_d(double x) {
+  return __hiloint2double(__builtin_amdgcn_readlane(__double2hiint(x), 63),
+                          __builtin_amdgcn_readlane(__double2loint(x), 63));
+}
+template <bool MAX>
+__device__ __forceinline__ double dmm(double a, double b) { return MAX ? fmax(a, b) : fmin(a, b); }
+template <bool MAX>
+__device__ __forceinline__ double wave_mm(double x) {
+  x = dmm<MAX>(x, dpp_d<0xB1>(x));        // quad_perm [1,0,3,2]
+  x = dmm<MAX>(x, dpp_d<0x4E>(x));        // quad_perm [2,3,0,1]
+  x = dmm<MAX>(x, dpp_d<0x124>(x));       // row_ror:4
+  x = dmm<MAX>(x, dpp_d<0x128>(x));       // row_ror:8
+  x = dmm<MAX>(x, dpp_d<0x142, 0xa>(x));  // row_bcast:15 into rows 1, 3
+  x = dmm<MAX>(x, dpp_d<0x143, 0xc>(x));  // row_bcast:31 into rows 2, 3
+  return lane63_d(x);
+}
+__device__ __forceinline__ int wave_max_i(int x) {
+  x = max(x, dpp_i<0xB1>(x));
+  x = max(x, dpp_i<0x4E>(x));
+  x = max(x, dpp_i<0x124>(x));
+  x = max(x, dpp_i<0x128>(x));
+  x = max(x, dpp_i<0x142, 0xa>(x));
+  x = max(x, dpp_i<0x143, 0xc>(x));
+  return __builtin_amdgcn_readlane(x, 63);
+}
+__device__ __forceinline__ double wave_max(double v) { return wave_mm<true>(v); }
+// (value, index) arg-max / arg-min with ties to the larger index: the extreme
+// value first, then the largest index among the lanes holding it (the same
+// result as any order of pairwise max_idx / min_idx steps).  Every lane
+// receives the result.
 __device__ __forceinline__ void wave_max_idx(double &v, int &i) {
-  for (int o = 32; o > 0; o >>= 1) {
-    double v2 = __shfl_xor(v, o, 64);
-    int i2 = __shfl_xor(i, o, 64);
-    max_idx(v, i, v2, i2);
-  }
+  const double m = wave_mm<true>(v);
+  i = wave_max_i(v == m ? i : INT_MIN);
+  v = m;
 }
 __device__ __forceinline__ void wave_min_idx(double &v, int &i) {
-  for (int o = 32; o > 0; o >>= 1) {
-    double v2 = __shfl_xor(v, o, 64);
-    int i2 = __shfl_xor(i, o, 64);
-    min_idx(v, i, v2, i2);
-  }
+  const double m = wave_mm<false>(v);
+  i = wave_max_i(v == m ? i : INT_MIN);
+  v = m;
 }
 __device__ __forceinline__ double wave_sum(double v) {
   for (int o = 32; o > 0; o >>= 1) v += __shfl_xor(v, o, 64);
@@ -357,16 +395,22 @@ __device__ __forceinline__ void compute_row_part(const View &V, int q, int slot,
 // ------------------------------------------------------- block reductions
 
 struct RedSmem {
-  double v[16][4];
-  int i[16][2];
+  double v[2][16][4];
+  int i[2][16][2];
   int last;
   unsigned gen;
 };
 
 // Block-wide reduction of up to 4 (value, index) lanes; op per slot.
-// kinds: 0 = max_idx, 1 = min_idx, 2 = plain max.  Result in thread 0.
+// kinds: 0 = max_idx, 1 = min_idx, 2 = plain max.  The result is left in
+// every thread.  One barrier: the waves' results go through sm buffer `buf`,
+// which the caller must not have the next reduction on the same buffer write
+// before a later barrier (the passes use buffer 0 and the exchanges buffer 1,
+// and every pass or exchange has a barrier between two reductions on one
+// buffer).
 template <int N>
-__device__ __forceinline__ void block_reduce(RedSmem &sm, double (&v)[4], int (&ix)[2], const int (&kinds)[4]) {
+__device__ __forceinline__ void block_reduce(RedSmem &sm, double (&v)[4], int (&ix)[2], const int (&kinds)[4],
+                                             int buf = 0) {
   const int wave = threadIdx.x / 64, lane = threadIdx.x % 64, nw = blockDim.x / 64;
   int slot_i = 0;
 #pragma unroll
@@ -377,23 +421,22 @@ __device__ __forceinline__ void block_reduce(RedSmem &sm, double (&v)[4], int (&
   }
   if (lane == 0) {
 #pragma unroll
-    for (int k = 0; k < N; ++k) sm.v[wave][k] = v[k];
-    sm.i[wave][0] = ix[0];
-    sm.i[wave][1] = ix[1];
+    for (int k = 0; k < N; ++k) sm.v[buf][wave][k] = v[k];
+    sm.i[buf][wave][0] = ix[0];
+    sm.i[buf][wave][1] = ix[1];
   }
   __syncthreads();
-  if (threadIdx.x == 0) {
-    for (int w = 1; w < nw; ++w) {
-      int si = 0;
+  // every thread combines the waves' results (broadcast LDS reads)
+  for (int w = 0; w < nw; ++w) {
+    if (w == wave) continue;
+    int si = 0;
 #pragma unroll
-      for (int k = 0; k < N; ++k) {
-        if (kinds[k] == 0) { max_idx(v[k], ix[si], sm.v[w][k], sm.i[w][si]); ++si; }
-        else if (kinds[k] == 1) { min_idx(v[k], ix[si], sm.v[w][k], sm.i[w][si]); ++si; }
-        else v[k] = fmax(v[k], sm.v[w][k]);
-      }
+    for (int k = 0; k < N; ++k) {
+      if (kinds[k] == 0) { max_idx(v[k], ix[si], sm.v[buf][w][k], sm.i[buf][w][si]); ++si; }
+      else if (kinds[k] == 1) { min_idx(v[k], ix[si], sm.v[buf][w][k], sm.i[buf][w][si]); ++si; }
+      else v[k] = fmax(v[k], sm.v[buf][w][k]);
     }
   }
-  __syncthreads();
 }
 
 // --------------------------------------------------------------- the cache
@@ -949,10 +992,19 @@ constexpr long long SPIN_LIMIT = 1LL << 23;
 // same decisions from the same data.  The only data that crosses workgroups
 // are the per-pass partial records below: each carries a workgroup's best
 // candidate together with the scalars the decision needs (G, alpha and, for
-// the j pass, Q_ij of that candidate).  A record is published with
-// agent-scope (write-through) stores, a wait for their completion, and then
-// its sequence number; readers poll the sequence number and then load the
-// record.  No read-modify-write atomics, no L2 write-back or invalidation.
+// the j pass, Q_ij of that candidate).
+//
+// A record travels as 64-bit words, each holding the exchange's sequence
+// number in its high half and 32 bits of the record in its low half.  An
+// aligned 64-bit store is single-copy atomic, so a reader that finds the
+// expected number in every word of a record has all of that exchange's
+// record: the writer issues its stores back to back with no wait and no
+// separate flag, and a reader needs one round trip per poll.  The words are
+// stored and loaded with agent-scope (cross-XCD coherent) accesses; no
+// read-modify-write atomics, no L2 write-back or invalidation.  Word w of
+// owner t in buffer p (the sequence number's parity) is at
+// xw[(p * XW + w) * nblocks + t], so the readers' loads of one word are
+// contiguous across owners.
 
 struct XPart {
   unsigned seq;
@@ -963,6 +1015,124 @@ struct XPart {
   int y0, y1;              // label / sign of the winner(s)
   float q0, fpad;          // j record: Q_ij
 };
+
+using XWord = unsigned long long;
+constexpr int XW = 23;  // words of the largest record format
+
+// Exchange words for two buffers of nb owners.
+__host__ __device__ constexpr size_t xrec_words(int nb) { return 2 * (size_t)XW * (size_t)(nb > 0 ? nb : 0); }
+
+// Record formats: the j record, the i record of the C rule, the i record of
+// the nu rule, and a team's final terms.
+enum { XF_J = 0, XF_IC = 1, XF_INU = 2, XF_FIN = 3 };
+
+template <int F>
+__host__ __device__ constexpr int xf_words() {
+  return F == XF_J ? 11 : F == XF_IC ? 14 : F == XF_INU ? 23 : 18;
+}
+
+__device__ __forceinline__ void xput_d(unsigned *w, int &k, double x) {
+  const unsigned long long b = (unsigned long long)__double_as_longlong(x);
+  w[k++] = (unsigned)b;
+  w[k++] = (unsigned)(b >> 32);
+}
+
+__device__ __forceinline__ double xget_d(const unsigned *w, int &k) {
+  const unsigned long long b = (unsigned long long)w[k] | ((unsigned long long)w[k + 1] << 32);
+  k += 2;
+  return __longlong_as_double((long long)b);
+}
+
+template <int F>
+__device__ __forceinline__ void xpack(const XPart &p, unsigned (&w)[XW]) {
+  int k = 0;
+  w[k++] = (unsigned)p.i0;
+  if (F != XF_J) w[k++] = (unsigned)p.i1;
+  xput_d(w, k, p.v0);
+  if (F != XF_J) xput_d(w, k, p.v1);
+  if (F == XF_INU || F == XF_FIN) { xput_d(w, k, p.v2); xput_d(w, k, p.v3); }
+  xput_d(w, k, p.g0);
+  xput_d(w, k, p.a0);
+  if (F == XF_INU || F == XF_FIN) { xput_d(w, k, p.g1); xput_d(w, k, p.a1); }
+  // the nu rule's i_p has y = +1 and i_n has y = -1 by construction: their
+  // labels are not sent
+  if (F != XF_FIN) xput_d(w, k, p.qd0);
+  if (F == XF_J || F == XF_IC) w[k++] = (unsigned)p.y0;
+  if (F == XF_INU) xput_d(w, k, p.qd1);
+  if (F == XF_J) w[k++] = __float_as_uint(p.q0);
+  if (F == XF_IC || F == XF_INU) w[k++] = (unsigned)p.pad;
+}
+
+template <int F>
+__device__ __forceinline__ void xunpack(const unsigned (&w)[XW], XPart &p) {
+  int k = 0;
+  p.i0 = (int)w[k++];
+  if (F != XF_J) p.i1 = (int)w[k++];
+  p.v0 = xget_d(w, k);
+  if (F != XF_J) p.v1 = xget_d(w, k);
+  if (F == XF_INU || F == XF_FIN) { p.v2 = xget_d(w, k); p.v3 = xget_d(w, k); }
+  p.g0 = xget_d(w, k);
+  p.a0 = xget_d(w, k);
+  if (F == XF_INU || F == XF_FIN) { p.g1 = xget_d(w, k); p.a1 = xget_d(w, k); }
+  if (F != XF_FIN) p.qd0 = xget_d(w, k);
+  if (F == XF_J || F == XF_IC) p.y0 = (int)w[k++];
+  if (F == XF_INU) { p.qd1 = xget_d(w, k); p.y0 = +1; p.y1 = -1; }
+  if (F == XF_J) p.q0 = __uint_as_float(w[k++]);
+  if (F == XF_IC || F == XF_INU) p.pad = (int)w[k++];
+}
+
+__device__ __forceinline__ XWord *xword(XWord *X, int nb, unsigned parity, int w, int t) {
+  return X + ((size_t)(parity * XW + (unsigned)w) * (size_t)nb + (size_t)t);
+}
+
+template <class T>
+__device__ __forceinline__ void st_agent(T *p, T v) {
+  __hip_atomic_store(p, v, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+}
+
+// Owner t's record of exchange `seq` (one thread).
+template <int F>
+__device__ __forceinline__ void xpublish(XWord *X, int nb, int t, const XPart &p, unsigned seq) {
+  unsigned w[XW];
+  xpack<F>(p, w);
+#pragma unroll
+  for (int k = 0; k < xf_words<F>(); ++k)
+    st_agent(xword(X, nb, seq & 1u, k, t), ((XWord)seq << 32) | (XWord)w[k]);
+}
+
+// Waits for owner t's record of exchange `seq` and unpacks it into q (one
+// thread).  The wait polls the record's last word alone (every workgroup
+// polls every record, so polling whole records multiplies the traffic on a
+// few cache lines and slows every reader), then reads the whole record and
+// checks every word.  False if the wait exceeded SPIN_LIMIT polls.
+template <int F>
+__device__ __forceinline__ bool xgather(XWord *X, int nb, int t, unsigned seq, XPart &q) {
+  constexpr int NW = xf_words<F>();
+  unsigned w[XW];
+  long long spins = 0;
+  for (;;) {
+    while ((unsigned)(ld_acq(xword(X, nb, seq & 1u, NW - 1, t)) >> 32) != seq) {
+      if (++spins > SPIN_LIMIT) return false;
+      __builtin_amdgcn_s_sleep(1);
+    }
+    bool ok = true;
+#pragma unroll
+    for (int k = 0; k < NW; ++k) {
+      const XWord x = ld_acq(xword(X, nb, seq & 1u, k, t));
+      w[k] = (unsigned)x;
+      ok &= (unsigned)(x >> 32) == seq;
+    }
+    if (ok) break;
+    if (++spins > SPIN_LIMIT) return false;
+  }
+  xunpack<F>(w, q);
+  return true;
+}
+
+// Payload word w of owner t in buffer `parity` (a completed exchange).
+__device__ __forceinline__ unsigned xpeek(XWord *X, int nb, unsigned parity, int w, int t) {
+  return (unsigned)ld_acq(xword(X, nb, parity, w, t));
+}
 
 // Control state every workgroup holds (identical in all of them); also the
 // record block 0 leaves in Ctl between launches.
@@ -985,25 +1155,6 @@ static_assert(sizeof(Dec) <= sizeof(((Ctl *)nullptr)->dec_bytes), "Dec must fit 
 
 __host__ __device__ __forceinline__ Dec &ctl_dec(Ctl *c) { return *reinterpret_cast<Dec *>(c->dec_bytes); }
 
-template <class T>
-__device__ __forceinline__ void st_agent(T *p, T v) {
-  __hip_atomic_store(p, v, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
-}
-
-__device__ __forceinline__ void publish(XPart *slot, const XPart &p, unsigned seq) {
-  st_agent(&slot->i0, p.i0); st_agent(&slot->i1, p.i1);
-  st_agent(&slot->v0, p.v0); st_agent(&slot->v1, p.v1); st_agent(&slot->v2, p.v2); st_agent(&slot->v3, p.v3);
-  st_agent(&slot->g0, p.g0); st_agent(&slot->a0, p.a0); st_agent(&slot->g1, p.g1); st_agent(&slot->a1, p.a1);
-  st_agent(&slot->q0, p.q0);
-  st_agent(&slot->qd0, p.qd0); st_agent(&slot->qd1, p.qd1);
-  st_agent(&slot->y0, p.y0); st_agent(&slot->y1, p.y1);
-  st_agent(&slot->pad, p.pad);
-  // the record's stores complete (are visible at agent scope) before its
-  // sequence number is written
-  __asm__ volatile("s_waitcnt vmcnt(0)" ::: "memory");
-  st_agent(&slot->seq, seq);
-}
-
 template <int KIND, int PH>
 __device__ __forceinline__ XPart xpart_identity() {
   XPart p;
@@ -1017,8 +1168,7 @@ __device__ __forceinline__ XPart xpart_identity() {
 }
 
 struct XSmem {
-  XPart win[2];
-  int widx[2];
+  XPart win[1];
   int err;
 };
 
@@ -1027,66 +1177,53 @@ struct XSmem {
 // and, for the nu rule's second argmax, g1/a1).
 template <int KIND, int PH>
 __device__ __forceinline__ XPart exchange(const View &V, RedSmem &sm, XSmem &xs, const XPart &mine, unsigned seq) {
-  XPart *buf = V.xpart + (size_t)(seq & 1u) * V.nblocks;
+  constexpr int F = PH == PH_J ? XF_J : (KIND == KIND_C ? XF_IC : XF_INU);
   if (threadIdx.x == 0) {
-    publish(&buf[my_owner(V)], mine, seq);
+    xpublish<F>(V.xw, V.nblocks, my_owner(V), mine, seq);
     xs.err = 0;
   }
   XPart q = xpart_identity<KIND, PH>();
   const int t = threadIdx.x;
-  if (t < V.nblocks) {
-    long long spins = 0;
-    while (ld_acq(&buf[t].seq) != seq) {
-      __builtin_amdgcn_s_sleep(1);
-      if (++spins > SPIN_LIMIT) { xs.err = 1; break; }
-    }
-    __asm__ volatile("" ::: "memory");
-    q.i0 = ld_acq(&buf[t].i0); q.i1 = ld_acq(&buf[t].i1);
-    q.v0 = ld_acq(&buf[t].v0); q.v1 = ld_acq(&buf[t].v1); q.v2 = ld_acq(&buf[t].v2); q.v3 = ld_acq(&buf[t].v3);
-    q.g0 = ld_acq(&buf[t].g0); q.a0 = ld_acq(&buf[t].a0); q.g1 = ld_acq(&buf[t].g1); q.a1 = ld_acq(&buf[t].a1);
-    q.q0 = ld_acq(&buf[t].q0);
-    q.qd0 = ld_acq(&buf[t].qd0); q.qd1 = ld_acq(&buf[t].qd1);
-    q.y0 = ld_acq(&buf[t].y0); q.y1 = ld_acq(&buf[t].y1);
-    q.pad = ld_acq(&buf[t].pad);
-  }
+  if (t < V.nblocks && !xgather<F>(V.xw, V.nblocks, t, seq, q)) xs.err = 1;
   // i records carry each owner's pause vote: every owner sees the same OR
   const int pause = PH == PH_I ? __syncthreads_or(t < V.nblocks ? q.pad : 0) : 0;
   double v[4] = {q.v0, q.v1, q.v2, q.v3};
   int ix[2] = {q.i0, q.i1};
   if (PH == PH_J) {
     const int kinds[4] = {1, 2, 2, 2};
-    block_reduce<1>(sm, v, ix, kinds);
+    block_reduce<1>(sm, v, ix, kinds, 1);
   } else if (KIND == KIND_C) {
     const int kinds_spec[4] = {0, 0, 2, 2}, kinds[4] = {0, 2, 2, 2};
-    if (V.spec) block_reduce<2>(sm, v, ix, kinds_spec);
-    else block_reduce<2>(sm, v, ix, kinds);
+    if (V.spec) block_reduce<2>(sm, v, ix, kinds_spec, 1);
+    else block_reduce<2>(sm, v, ix, kinds, 1);
   } else {
     const int kinds[4] = {0, 0, 2, 2};
-    block_reduce<4>(sm, v, ix, kinds);
+    block_reduce<4>(sm, v, ix, kinds, 1);
   }
-  if (threadIdx.x == 0) {
-    xs.widx[0] = ix[0];
-    xs.widx[1] = ix[1];
-    xs.win[0] = xpart_identity<KIND, PH>();
-    xs.win[1] = xpart_identity<KIND, PH>();
-    xs.win[0].v0 = v[0]; xs.win[0].v1 = v[1]; xs.win[0].v2 = v[2]; xs.win[0].v3 = v[3];
-    xs.win[0].i0 = ix[0]; xs.win[0].i1 = ix[1];
-  }
-  __syncthreads();
-  // the record holding each winning index supplies its payload
+  // every thread holds the reduced values; the record holding each winning
+  // index supplies its payload
   if (t < V.nblocks) {
-    if (ix[0] >= 0 && q.i0 == xs.widx[0]) {
+    if (ix[0] >= 0 && q.i0 == ix[0]) {
       xs.win[0].g0 = q.g0; xs.win[0].a0 = q.a0; xs.win[0].q0 = q.q0; xs.win[0].qd0 = q.qd0; xs.win[0].y0 = q.y0;
     }
-    if (KIND == KIND_NU && PH == PH_I && xs.widx[1] >= 0 && q.i1 == xs.widx[1]) {
+    if (KIND == KIND_NU && PH == PH_I && ix[1] >= 0 && q.i1 == ix[1]) {
       xs.win[0].g1 = q.g1; xs.win[0].a1 = q.a1; xs.win[0].qd1 = q.qd1; xs.win[0].y1 = q.y1;
     }
   }
   __syncthreads();
-  XPart r = xs.win[0];
+  // (xs is written again only in the next exchange, after the next pass's
+  // barrier)
+  XPart r = xpart_identity<KIND, PH>();
+  r.v0 = v[0]; r.v1 = v[1]; r.v2 = v[2]; r.v3 = v[3];
+  r.i0 = ix[0]; r.i1 = ix[1];
+  if (ix[0] >= 0) {
+    r.g0 = xs.win[0].g0; r.a0 = xs.win[0].a0; r.q0 = xs.win[0].q0; r.qd0 = xs.win[0].qd0; r.y0 = xs.win[0].y0;
+  }
+  if (KIND == KIND_NU && PH == PH_I && ix[1] >= 0) {
+    r.g1 = xs.win[0].g1; r.a1 = xs.win[0].a1; r.qd1 = xs.win[0].qd1; r.y1 = xs.win[0].y1;
+  }
   r.seq = xs.err;
   r.pad = pause;
-  __syncthreads();
   return r;
 }
 

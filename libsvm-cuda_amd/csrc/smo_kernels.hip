@@ -213,15 +213,20 @@ template <int KIND>
 __device__ __forceinline__ void spec_rows(const View &V, Dec &d, RedSmem &sm, float *xs) {
   __shared__ int list[SPEC_ROWS];
   __shared__ int s_cnt, s_pick;
-  const XPart *buf = V.xpart + (size_t)((unsigned)d.epoch & 1u) * V.nblocks;
+  const unsigned par = (unsigned)d.epoch & 1u;
   const int t = threadIdx.x;
   double cu = -INF, cl = -INF;
   int iu = -1, il = -1;
   if (t < V.nblocks) {
-    iu = ld_acq(&buf[t].i0);
-    cu = ld_acq(&buf[t].v0);
-    il = ld_acq(&buf[t].i1);
-    cl = ld_acq(&buf[t].v1);
+    // the i record (C rule): i0, i1, v0, v1 in words 0..5
+    unsigned w[XW];
+#pragma unroll
+    for (int k = 0; k < 6; ++k) w[k] = xpeek(V.xw, V.nblocks, par, k, t);
+    int k = 2;
+    iu = (int)w[0];
+    il = (int)w[1];
+    cu = xget_d(w, k);
+    cl = xget_d(w, k);
     if (iu < 0 || iu == d.i) cu = -INF;
     if (il < 0 || il == d.i) cl = -INF;
   }
@@ -751,28 +756,25 @@ __device__ __forceinline__ void team_solve(View &V, Dec &d, RedSmem &sm, XSmem &
   final_terms<KIND>(V, r0, r1, f, n);
   if (threadIdx.x == 0) {
     const unsigned seq = (unsigned)d.epoch + 1u;
-    XPart *buf = V.xpart + (size_t)(seq & 1u) * V.nblocks;
     XPart m = xpart_identity<KIND, PH_I>();
     m.v0 = f[0]; m.v1 = f[1]; m.v2 = f[2]; m.v3 = f[3]; m.g0 = f[4]; m.a0 = f[5]; m.g1 = f[6];
     m.i0 = n[0]; m.i1 = n[1]; m.a1 = d.err ? 1.0 : 0.0;
-    publish(&buf[V.rank], m, seq);
+    xpublish<XF_FIN>(V.xw, V.nblocks, V.rank, m, seq);
     if (V.rank == 0) {
       double F[7] = {0, INF, -INF, 0, INF, -INF, 0};
       int N[2] = {0, 0};
       int err = d.err;
       for (int t = 0; t < V.nblocks && !err; ++t) {
-        long long spins = 0;
-        while (ld_acq(&buf[t].seq) != seq) {
-          __builtin_amdgcn_s_sleep(1);
-          if (++spins > SPIN_LIMIT) { err = 1; break; }
+        XPart q;
+        if (!xgather<XF_FIN>(V.xw, V.nblocks, t, seq, q)) {
+          err = 1;
+          break;
         }
-        if (err) break;
-        __asm__ volatile("" ::: "memory");
-        F[0] += ld_acq(&buf[t].v0); F[1] = fmin(F[1], ld_acq(&buf[t].v1)); F[2] = fmax(F[2], ld_acq(&buf[t].v2));
-        F[3] += ld_acq(&buf[t].v3); F[4] = fmin(F[4], ld_acq(&buf[t].g0)); F[5] = fmax(F[5], ld_acq(&buf[t].a0));
-        F[6] += ld_acq(&buf[t].g1);
-        N[0] += ld_acq(&buf[t].i0); N[1] += ld_acq(&buf[t].i1);
-        if (ld_acq(&buf[t].a1) != 0.0) err = 1;
+        F[0] += q.v0; F[1] = fmin(F[1], q.v1); F[2] = fmax(F[2], q.v2);
+        F[3] += q.v3; F[4] = fmin(F[4], q.g0); F[5] = fmax(F[5], q.a0);
+        F[6] += q.g1;
+        N[0] += q.i0; N[1] += q.i1;
+        if (q.a1 != 0.0) err = 1;
       }
       write_final<KIND>(V.ctl, F, N);
       write_ctl_state(V.ctl, d, err ? (int)ERR_TIMEOUT : 0);

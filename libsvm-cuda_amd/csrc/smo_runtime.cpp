@@ -178,11 +178,14 @@ svmhip_status svmhip_create(const svm_problem *prob, const svm_parameter *param,
     build_sell(h, prob);
     // launch geometry: at most one workgroup per slice, one per CU by default
     int cus = device_cus(h->dev);
-    // default: about six 64-row slices per workgroup, 32..128 workgroups
-    // (fewer workgroups make each exchange cheaper, more make row passes
-    // faster; this balance was measured on 20k-100k row problems)
+    // default: one workgroup per CU once there are two 64-row slices per CU
+    // (row passes then use the whole device); below that about six slices
+    // per workgroup, 32..128 workgroups (fewer workgroups make each exchange
+    // cheaper).  Measured on 10k-100k row problems: at 100k rows 256
+    // workgroups beat 128 by 6%, at 10k rows 64 beat 128 by 4%.
     int want = h->opt.grid_blocks > 0 ? h->opt.grid_blocks
-                                      : std::min(std::min(128, cus), std::max(32, h->nslices / 6));
+               : h->nslices >= 2 * cus ? cus
+                                        : std::min(std::min(128, cus), std::max(32, h->nslices / 6));
     h->nblocks = std::max(1, std::min(want, std::max(h->nslices, 1)));
     size_t qbytes = (size_t)h->d * sizeof(float);
     h->use_lds = qbytes <= kLdsQueryLimit;
@@ -195,7 +198,7 @@ svmhip_status svmhip_create(const svm_problem *prob, const svm_parameter *param,
     if (!h->use_lds) h->d_xs = h->dalloc<float>((size_t)h->nblocks * h->d, "query-row scratch");
     h->d_ctl = h->dalloc<smo::Ctl>(1, "control record");
     h->d_part = h->dalloc<smo::Partial>(2 * (size_t)h->nblocks, "partials");
-    h->d_xpart = h->dalloc<smo::XPart>(2 * (size_t)h->nblocks, "exchange records");
+    h->d_xw = h->dalloc<smo::XWord>(smo::xrec_words(h->nblocks), "exchange records");
     h->ck(hipMemsetAsync(h->d_ctl, 0, sizeof(smo::Ctl), h->st), "clear control record");
     h->ck(hipStreamSynchronize(h->st), "create");
     GUARD_END
@@ -431,7 +434,7 @@ svmhip_status svmhip_solve(svmhip_solver *h, int max_iter, svmhip_stats *stats) 
       h->host_ctl->stop = 0;
       h->host_ctl->err = 0;
       h->ck(hipMemcpyAsync(h->d_ctl, h->host_ctl, sizeof(smo::Ctl), hipMemcpyHostToDevice, h->st), "seed control");
-      h->ck(hipMemsetAsync(h->d_xpart, 0, 2 * sizeof(smo::XPart) * h->nblocks, h->st), "clear exchange records");
+      h->ck(hipMemsetAsync(h->d_xw, 0, sizeof(smo::XWord) * smo::xrec_words(h->nblocks), h->st), "clear exchange records");
       smo::View V0 = h->view();
       if (h->full)
         h->ck(hipMemsetAsync(h->d_pm_flags, 0, (size_t)h->nblocks * std::max(h->l, 1), h->st), "clear cache flags");
@@ -671,7 +674,7 @@ svmhip_status svmhip_batch_solve(svmhip_batch *b, svmhip_subproblem *subs, int n
       V.rows = nullptr; V.S = 0; V.full = 0;
       V.slot_of_row = V.row_of_slot = V.lru_prev = V.lru_next = nullptr;
       V.computed = nullptr;
-      V.ctl = dC + k; V.part = nullptr; V.xpart = nullptr; V.xs_global = nullptr;
+      V.ctl = dC + k; V.part = nullptr; V.xw = nullptr; V.xs_global = nullptr;
       V.use_lds = 1; V.nblocks = 1; V.rank = 0; V.resume = 0; V.iter0 = 0;
       V.ginit_from = 0; V.ginit_to = sp.n_var;
       views[k] = V;
@@ -711,7 +714,7 @@ svmhip_status svmhip_batch_solve(svmhip_batch *b, svmhip_subproblem *subs, int n
     pool.queue = (int *)talloc(sizeof(int), "batch queue");
     pool.t0 = (unsigned long long *)talloc(sizeof(unsigned long long), "batch launch start");
     pool.lds_floats = (int)(lds / sizeof(float));
-    smo::XPart *dX = (smo::XPart *)talloc(sizeof(smo::XPart) * 2 * (size_t)G, "team exchange records");
+    smo::XWord *dX = (smo::XWord *)talloc(sizeof(smo::XWord) * smo::xrec_words(G), "team exchange records");
     int *dItems = (int *)talloc(sizeof(int) * 2 * (size_t)std::max(nsub, G), "batch items");
 
     // Stages.  Every unfinished subproblem runs until it converges or, after
@@ -774,8 +777,8 @@ svmhip_status svmhip_batch_solve(svmhip_batch *b, svmhip_subproblem *subs, int n
       for (int k : ord) {
         smo::View &V = views[k];
         V.nblocks = T[k];
-        V.xpart = T[k] > 1 ? dX + xoff : nullptr;
-        if (T[k] > 1) xoff += 2 * T[k];
+        V.xw = T[k] > 1 ? dX + xoff : nullptr;
+        if (T[k] > 1) xoff += (int)smo::xrec_words(T[k]);
         V.resume = started[k];
         V.iter0 = iters[k];
         if (fixed_chunk > 0) {  // pause after a fixed number of iterations
@@ -797,7 +800,7 @@ svmhip_status svmhip_batch_solve(svmhip_batch *b, svmhip_subproblem *subs, int n
       h->ck(hipMemsetAsync(pool.queue, 0, sizeof(int), h->st), "clear batch queue");
       h->ck(hipMemsetAsync(pool.t0, 0, sizeof(unsigned long long), h->st), "clear launch start");
       pool.deadline_ticks = fixed_chunk > 0 ? 0 : (long long)(tau * clock_hz);
-      if (xoff > 0) h->ck(hipMemsetAsync(dX, 0, sizeof(smo::XPart) * xoff, h->st), "clear team records");
+      if (xoff > 0) h->ck(hipMemsetAsync(dX, 0, sizeof(smo::XWord) * xoff, h->st), "clear team records");
       h->ck(hipMemsetAsync(dC, 0, sizeof(smo::Ctl) * nsub, h->st), "clear batch control");
       h->ck(hipEventRecord(h->ev[0], h->st), "event");
       h->ck(smo_launch_batch(dV, dItems, nitems, std::min(G, nitems), &pool, lds, h->st), "batched SMO kernel");
