@@ -273,8 +273,12 @@ __device__ __forceinline__ void stage_row(const View &V, int q, float *xs) {
 // two floats are exact in FP64, so fma == multiply-then-add here and the sum
 // is LIBSVM's sequential merge-order sum.  Loads are issued in batches of
 // ROW_UNROLL per wave (16 KB in flight per wave) so a row pass is bound by
-// HBM bandwidth rather than by one memory round trip per few elements.
+// HBM bandwidth rather than by one memory round trip per few elements; the
+// rest of a row goes in batches of ROW_TAIL, their loads clamped to the
+// row's stored entries and the entries past the row's length left out of the
+// sum (adding +0.0 leaves the sum unchanged: it is never -0.0).
 constexpr int ROW_UNROLL = 32;
+constexpr int ROW_TAIL = 8;
 
 __device__ __forceinline__ double row_dot(const View &V, const float *xs, int r) {
   const int s = r / SLICE;
@@ -292,11 +296,17 @@ __device__ __forceinline__ double row_dot(const View &V, const float *xs, int r)
 #pragma unroll
       for (int u = 0; u < U; ++u) acc = fma((double)a[u], (double)xs[p + u], acc);
     }
-    for (; p < len; ++p) acc = fma((double)vp[(long long)p * SLICE], (double)xs[p], acc);
+    for (; p < len; p += ROW_TAIL) {  // the rest, in clamped batches
+      float a[ROW_TAIL];
+#pragma unroll
+      for (int u = 0; u < ROW_TAIL; ++u) a[u] = vp[(long long)min(p + u, len - 1) * SLICE];
+#pragma unroll
+      for (int u = 0; u < ROW_TAIL; ++u)
+        if (p + u < len) acc = fma((double)a[u], (double)xs[p + u], acc);
+    }
   } else {
     // walk the slice's common width (rows are zero-padded to it) so the trip
-    // count is uniform across the wave; entries past this row's length are
-    // masked out of the sum
+    // count is uniform across the wave
     const int len = V.row_len[r];
     const int width = (int)((V.slice_ptr[s + 1] - V.slice_ptr[s]) / SLICE);
     const int *cp = V.cols + base;
@@ -315,14 +325,29 @@ __device__ __forceinline__ double row_dot(const View &V, const float *xs, int r)
         acc = acc + t;
       }
     }
-    for (; p < len; ++p) acc = fma((double)vp[(long long)p * SLICE], (double)xs[cp[(long long)p * SLICE]], acc);
+    for (; p < width; p += ROW_TAIL) {  // the rest, in clamped batches
+      float a[ROW_TAIL];
+      int c[ROW_TAIL];
+#pragma unroll
+      for (int u = 0; u < ROW_TAIL; ++u) {
+        const long long q = min(p + u, width - 1);
+        a[u] = vp[q * SLICE];
+        c[u] = cp[q * SLICE];
+      }
+#pragma unroll
+      for (int u = 0; u < ROW_TAIL; ++u) {
+        double t = (p + u < len) ? (double)a[u] * (double)xs[c[u]] : 0.0;
+        acc = acc + t;
+      }
+    }
   }
   return acc;
 }
 
 // dot(x_q(m), x_r) for the M rows staged at xs + m * ld, each summed in the
 // same order as row_dot (so each equals row_dot bit for bit): one pass over
-// row r serves M query rows.
+// row r serves M query rows.  U entries of row r are loaded per trip (one
+// memory round trip each), the last trip clamped like row_dot's.
 template <int M, int U = 8>
 __device__ __forceinline__ void row_dot_multi(const View &V, const float *xs, int ld, int r, double acc[M]) {
   const int s = r / SLICE;
@@ -330,32 +355,39 @@ __device__ __forceinline__ void row_dot_multi(const View &V, const float *xs, in
   const float *vp = V.vals + base;
 #pragma unroll
   for (int m = 0; m < M; ++m) acc[m] = 0.0;
-  // U entries of row r are loaded per trip (one memory round trip each)
   if (V.dense) {
     const int len = V.d;
-    int p = 0;
-    for (; p + U <= len; p += U) {
+    for (int p = 0; p < len; p += U) {
       float a[U];
 #pragma unroll
-      for (int u = 0; u < U; ++u) a[u] = vp[(long long)(p + u) * SLICE];
+      for (int u = 0; u < U; ++u) a[u] = vp[(long long)min(p + u, len - 1) * SLICE];
 #pragma unroll
       for (int u = 0; u < U; ++u)
+        if (p + u < len) {
 #pragma unroll
-        for (int m = 0; m < M; ++m) acc[m] = fma((double)a[u], (double)xs[m * ld + p + u], acc[m]);
-    }
-    for (; p < len; ++p) {
-      const double a = vp[(long long)p * SLICE];
-#pragma unroll
-      for (int m = 0; m < M; ++m) acc[m] = fma(a, (double)xs[m * ld + p], acc[m]);
+          for (int m = 0; m < M; ++m) acc[m] = fma((double)a[u], (double)xs[m * ld + p + u], acc[m]);
+        }
     }
   } else {
     const int len = V.row_len[r];
+    const int width = (int)((V.slice_ptr[s + 1] - V.slice_ptr[s]) / SLICE);
     const int *cp = V.cols + base;
-    for (int p = 0; p < len; ++p) {
-      const double a = vp[(long long)p * SLICE];
-      const int c = cp[(long long)p * SLICE];
+    for (int p = 0; p < width; p += U) {
+      float a[U];
+      int c[U];
 #pragma unroll
-      for (int m = 0; m < M; ++m) acc[m] = acc[m] + a * (double)xs[m * ld + c];
+      for (int u = 0; u < U; ++u) {
+        const long long q = min(p + u, width - 1);
+        a[u] = vp[q * SLICE];
+        c[u] = cp[q * SLICE];
+      }
+#pragma unroll
+      for (int u = 0; u < U; ++u)
+        if (p + u < len) {
+          const double av = a[u];
+#pragma unroll
+          for (int m = 0; m < M; ++m) acc[m] = acc[m] + av * (double)xs[m * ld + c[u]];
+        }
     }
   }
 }

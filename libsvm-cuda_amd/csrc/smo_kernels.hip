@@ -33,9 +33,17 @@ __global__ void __launch_bounds__(BLOCK) k_self_dot(View V, double *xsq) {
   const int len = V.row_len[r];
   const long long base = V.slice_ptr[r / SLICE] + (r % SLICE);
   double acc = 0.0;
-  for (int p = 0; p < len; ++p) {
-    double v = (double)V.vals[base + (long long)p * SLICE];
-    acc = fma(v, v, acc);  // v*v is exact: same as multiply-then-add
+  // 8 entries per trip (one memory round trip each), the last trip's loads
+  // clamped to the row
+  for (int p = 0; p < len; p += 8) {
+    float a[8];
+#pragma unroll
+    for (int u = 0; u < 8; ++u) a[u] = V.vals[base + (long long)min(p + u, len - 1) * SLICE];
+#pragma unroll
+    for (int u = 0; u < 8; ++u) {
+      const double v = (double)a[u];
+      if (p + u < len) acc = fma(v, v, acc);  // v*v is exact: same as multiply-then-add
+    }
   }
   xsq[r] = acc;
 }
@@ -916,13 +924,27 @@ __global__ void __launch_bounds__(BLOCK) k_predict(PredictArgs A, int first_row)
     for (int t = 0; t < TBMAX; ++t) acc[t] = 0.0;
     const int len = S.row_len[r];
     const long long base = S.slice_ptr[r / SLICE] + (r % SLICE);
-    for (int p = 0; p < len; ++p) {
-      const double v = (double)S.vals[base + (long long)p * SLICE];
-      const int c = S.dense ? p : S.cols[base + (long long)p * SLICE];
-      if (c >= d) continue;
+    // PU entries of the support vector per trip (one memory round trip
+    // each), the last trip's loads clamped to the slice's stored width
+    const int width = (int)((S.slice_ptr[r / SLICE + 1] - S.slice_ptr[r / SLICE]) / SLICE);
+    constexpr int PU = 8;
+    for (int p = 0; p < width; p += PU) {
+      float vv[PU];
+      int cc[PU];
 #pragma unroll
-      for (int t = 0; t < TBMAX; ++t)
-        if (t < nt) acc[t] = fma(v, (double)xt[t * d + c], acc[t]);
+      for (int u = 0; u < PU; ++u) {
+        const long long q = min(p + u, width - 1);
+        vv[u] = S.vals[base + q * SLICE];
+        cc[u] = S.dense ? (int)q : S.cols[base + q * SLICE];
+      }
+#pragma unroll
+      for (int u = 0; u < PU; ++u) {
+        if (p + u >= len || cc[u] >= d) continue;
+        const double v = (double)vv[u];
+#pragma unroll
+        for (int t = 0; t < TBMAX; ++t)
+          if (t < nt) acc[t] = fma(v, (double)xt[t * d + cc[u]], acc[t]);
+      }
     }
 #pragma unroll
     for (int t = 0; t < TBMAX; ++t) {
