@@ -1036,7 +1036,10 @@ constexpr long long SPIN_LIMIT = 1LL << 23;
 // read-modify-write atomics, no L2 write-back or invalidation.  Word w of
 // owner t in buffer p (the sequence number's parity) is at
 // xw[(p * XW + w) * nblocks + t], so the readers' loads of one word are
-// contiguous across owners.
+// contiguous across owners.  Every record is written XREP times, to XREP
+// copies of the buffers, and workgroup b reads copy b % XREP: every
+// workgroup reads every record at every exchange, and spreading those reads
+// over copies (on different memory channels) shortens the exchange.
 
 struct XPart {
   unsigned seq;
@@ -1049,10 +1052,12 @@ struct XPart {
 };
 
 using XWord = unsigned long long;
-constexpr int XW = 23;  // words of the largest record format
+constexpr int XW = 23;    // words of the largest record format
+constexpr int XREP = 8;   // copies of the buffers
 
-// Exchange words for two buffers of nb owners.
-__host__ __device__ constexpr size_t xrec_words(int nb) { return 2 * (size_t)XW * (size_t)(nb > 0 ? nb : 0); }
+// Exchange words of one copy of the two buffers of nb owners, and of all copies.
+__host__ __device__ constexpr size_t xcopy_words(int nb) { return 2 * (size_t)XW * (size_t)(nb > 0 ? nb : 0); }
+__host__ __device__ constexpr size_t xrec_words(int nb) { return XREP * xcopy_words(nb); }
 
 // Record formats: the j record, the i record of the C rule, the i record of
 // the nu rule, and a team's final terms.
@@ -1122,14 +1127,21 @@ __device__ __forceinline__ void st_agent(T *p, T v) {
   __hip_atomic_store(p, v, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
 }
 
-// Owner t's record of exchange `seq` (one thread).
+// Owner t's record of exchange `seq`, into every copy (one thread).
 template <int F>
 __device__ __forceinline__ void xpublish(XWord *X, int nb, int t, const XPart &p, unsigned seq) {
   unsigned w[XW];
   xpack<F>(p, w);
+#pragma unroll 1
+  for (int c = 0; c < XREP; ++c)
 #pragma unroll
-  for (int k = 0; k < xf_words<F>(); ++k)
-    st_agent(xword(X, nb, seq & 1u, k, t), ((XWord)seq << 32) | (XWord)w[k]);
+    for (int k = 0; k < xf_words<F>(); ++k)
+      st_agent(xword(X + (size_t)c * xcopy_words(nb), nb, seq & 1u, k, t), ((XWord)seq << 32) | (XWord)w[k]);
+}
+
+// The copy of the buffers this workgroup reads.
+__device__ __forceinline__ XWord *xcopy(XWord *X, int nb) {
+  return X + (size_t)(blockIdx.x % XREP) * xcopy_words(nb);
 }
 
 // Waits for owner t's record of exchange `seq` and unpacks it into q (one
@@ -1142,6 +1154,7 @@ __device__ __forceinline__ bool xgather(XWord *X, int nb, int t, unsigned seq, X
   constexpr int NW = xf_words<F>();
   unsigned w[XW];
   long long spins = 0;
+  X = xcopy(X, nb);
   for (;;) {
     while ((unsigned)(ld_acq(xword(X, nb, seq & 1u, NW - 1, t)) >> 32) != seq) {
       if (++spins > SPIN_LIMIT) return false;
@@ -1163,7 +1176,7 @@ __device__ __forceinline__ bool xgather(XWord *X, int nb, int t, unsigned seq, X
 
 // Payload word w of owner t in buffer `parity` (a completed exchange).
 __device__ __forceinline__ unsigned xpeek(XWord *X, int nb, unsigned parity, int w, int t) {
-  return (unsigned)ld_acq(xword(X, nb, parity, w, t));
+  return (unsigned)ld_acq(xword(xcopy(X, nb), nb, parity, w, t));
 }
 
 // Control state every workgroup holds (identical in all of them); also the
