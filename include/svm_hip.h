@@ -57,10 +57,12 @@ typedef struct {
 	double cache_bytes;       /* kernel-row cache bytes (0 = param.cache_size MB) */
 	int iters_per_launch;     /* persistent-kernel iterations per launch (0 = all) */
 	int verbose;              /* 1 = print setup and cache statistics to stderr */
-	int speculative_rows;     /* svmhip_solve, C rule, full cache: 1 = on a row-i miss also compute
-				     the rows of the strongest other violators in the same pass over X
-				     (0 = off, the default: slower on the measured configurations);
-				     results are identical either way */
+	int speculative_rows;     /* speculative kernel rows: on a row miss also compute the rows of the
+				     strongest other candidates in the same pass over X; results are
+				     identical either way.  svmhip_solve (C rule, full cache): on only
+				     when 1 (slower on the measured single-solve configurations).
+				     Batched and serving solves (single-workgroup C-rule subproblems
+				     with a full cache): on unless -1 */
 } svmhip_options;
 
 typedef struct svmhip_solver svmhip_solver;

@@ -819,6 +819,7 @@ svmhip_status svmhip_server_step(svmhip_server *s, double stage_ms, int *n_activ
       const int nitems = (int)items.size() / 2;
       h->ck(hipEventRecord(h->ev[0], h->st), "event");
       pool.lds_floats = (int)(shmem / sizeof(float));
+      pool.spec = h->opt.speculative_rows >= 0 ? 1 : 0;
       h->ck(smo_launch_batch(s->dV, s->dItems, nitems, std::min(G, nitems), &pool, shmem, h->st),
             "batched SMO kernel");
       h->ck(hipEventRecord(h->ev[1], h->st), "event");

@@ -126,7 +126,8 @@ struct BatchPool {
   int *queue;              // next item (zeroed before each launch)
   int K, by_item;
   int lds_floats;          // dynamic LDS of the launch, in floats
-  int pad;
+  int spec;                // single-workgroup C-rule items with a full cache: on a row miss, also
+                           // compute the rows of the pass's per-wave winners in the same pass
   unsigned long long *t0;  // launch start on the device clock (zeroed before each launch; null = none)
   long long deadline_ticks;  // > 0: items pause at t0 + deadline_ticks, and single-workgroup items
                              // claimed after it are left for the next launch (their Ctl stays clear)

@@ -680,6 +680,55 @@ __device__ __forceinline__ void write_ginit_pause(const View &V) {
   c->ginit_next = V.ginit_to;
 }
 
+// Speculative kernel rows of a single-workgroup solve (C rule, full cache).
+// A batched subproblem computes most of its kernel rows before it stops,
+// each in its own pass over the subproblem's X by one workgroup; so on a row
+// miss the rows of the strongest candidates of the pass just made (the
+// per-wave winners block_reduce leaves in sm buffer 0: I_up and I_low for
+// the i pass, the best gains for the j pass) that are not computed yet go
+// through the same pass (row_dot_multi), up to SPEC_ROWS rows and the LDS
+// room for their staged query rows.  Every value is the one a miss would
+// compute, so the iterates do not change.
+__device__ __forceinline__ void solo_spec_rows(const View &V, Dec &d, RedSmem &sm, float *xs, int lds_floats,
+                                               bool jside) {
+  __shared__ int list[SPEC_ROWS];
+  __shared__ int s_cnt;
+  if (threadIdx.x == 0) {
+    const int cap = min(SPEC_ROWS, lds_floats / max(V.d, 1));
+    const int nw = blockDim.x / 64;
+    CacheMeta M = private_meta(V, d);
+    int cnt = 0;
+    list[cnt++] = real_of(V, jside ? d.j : d.i);  // the decision has marked it
+    while (cnt < cap) {
+      double bv = -INF;
+      int bw = -1, bk = 0;
+      for (int w = 0; w < nw; ++w) {
+        if (sm.i[0][w][0] >= 0) {
+          const double v = jside ? -sm.v[0][w][0] : sm.v[0][w][0];
+          if (v > bv) { bv = v; bw = w; bk = 0; }
+        }
+        if (!jside && sm.i[0][w][1] >= 0 && sm.v[0][w][1] > bv) { bv = sm.v[0][w][1]; bw = w; bk = 1; }
+      }
+      if (bw < 0) break;
+      const int row = real_of(V, sm.i[0][bw][bk]);
+      sm.i[0][bw][bk] = -1;  // taken (the buffer is rewritten by the next pass)
+      if (M.computed[row]) continue;
+      M.computed[row] = 1;
+      ++d.misses;
+      list[cnt++] = row;
+    }
+    s_cnt = cnt;
+  }
+  __syncthreads();
+  if (s_cnt > 1) compute_rows_part(V, list, s_cnt, xs, 0, V.nslices);
+  else compute_row_part(V, list[0], list[0], xs, 0, V.nslices);
+  if (threadIdx.x == 0) {
+    if (jside) d.miss_j = 0;
+    else d.miss_i = 0;
+  }
+  __syncthreads();
+}
+
 template <int KIND>
 __device__ __forceinline__ void solo_solve(View &V, Dec &d, RedSmem &sm, float *xs, int lds_floats, long long t0) {
   solo_init_cache(V);
@@ -706,12 +755,14 @@ __device__ __forceinline__ void solo_solve(View &V, Dec &d, RedSmem &sm, float *
     __syncthreads();
   }
   while (!d.stop) {
+    if (KIND == KIND_C && V.spec && d.miss_i) solo_spec_rows(V, d, sm, xs, lds_floats, false);
     Snap c = snap_of(d);
     Partial bp = pass_select_j<KIND>(V, c, xs, sm, false);
     XPart m = make_j_record<KIND>(V, c, bp);
     if (threadIdx.x == 0) decide_j<KIND>(V, d, m);
     __syncthreads();
     if (d.stop) break;
+    if (KIND == KIND_C && V.spec && d.miss_j) solo_spec_rows(V, d, sm, xs, lds_floats, true);
     c = snap_of(d);
     Pend pd;
     pd.i = d.i; pd.j = d.j; pd.ai = d.ai_new; pd.aj = d.aj_new;
@@ -858,6 +909,8 @@ __global__ void __launch_bounds__(BLOCK) k_smo_batch(const View *views, const in
     } else {
       V.nblocks = 1;
       V.rank = 0;
+      V.spec = pool.spec && V.kind == KIND_C && V.full && V.ktype != K_PRECOMPUTED &&
+               pool.lds_floats >= 2 * V.d ? 1 : 0;
       if (V.kind == KIND_C) solo_solve<KIND_C>(V, d, sm, xs, pool.lds_floats, t0);
       else solo_solve<KIND_NU>(V, d, sm, xs, pool.lds_floats, t0);
     }

@@ -703,6 +703,7 @@ svmhip_status svmhip_batch_solve(svmhip_batch *b, svmhip_subproblem *subs, int n
     // no subproblem needs more than its whole matrix
     arena = std::min<long long>(arena, std::max<long long>((long long)maxn * maxstride, 2 * maxstride));
     smo::BatchPool pool;
+    memset(&pool, 0, sizeof pool);
     pool.K = K;
     pool.by_item = 0;
     pool.arena_floats = arena;
@@ -714,6 +715,7 @@ svmhip_status svmhip_batch_solve(svmhip_batch *b, svmhip_subproblem *subs, int n
     pool.queue = (int *)talloc(sizeof(int), "batch queue");
     pool.t0 = (unsigned long long *)talloc(sizeof(unsigned long long), "batch launch start");
     pool.lds_floats = (int)(lds / sizeof(float));
+    pool.spec = h->opt.speculative_rows >= 0 ? 1 : 0;
     smo::XWord *dX = (smo::XWord *)talloc(sizeof(smo::XWord) * smo::xrec_words(G), "team exchange records");
     int *dItems = (int *)talloc(sizeof(int) * 2 * (size_t)std::max(nsub, G), "batch items");
 

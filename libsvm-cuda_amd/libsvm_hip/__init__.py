@@ -449,15 +449,16 @@ class BatchSolver:
     one launch (one workgroup each): the svmhip_batch_* C-ABI."""
 
     def __init__(self, problem: Problem, param, cache_bytes=0.0, max_iter=0, device=0, team_max=0, chunk=0,
-                 verbose=0):
+                 verbose=0, speculative_rows=0):
         """team_max: largest team of workgroups per subproblem (0 = default,
         1 = one workgroup each); chunk: iterations per subproblem per launch
-        (0 = sized by the scheduler)."""
+        (0 = sized by the scheduler); speculative_rows: -1 turns off the
+        speculative kernel rows of single-workgroup solves."""
         L = lib()
         self.problem = problem
         self.param = param
         opt = svmhip_options(C.sizeof(svmhip_options), device, team_max, 0, max_iter, float(cache_bytes), chunk,
-                             verbose)
+                             verbose, speculative_rows)
         self._h = C.c_void_p()
         rc = L.svmhip_batch_create(C.byref(problem.struct), C.byref(param), C.byref(opt), C.byref(self._h))
         if rc != 0:
@@ -539,9 +540,10 @@ class Server:
     C-SVC / nu-SVC training solves and predictions against their resident
     models, admitted by HBM budget and advanced together one stage per step."""
 
-    def __init__(self, budget_bytes=0.0, cache_bytes=0.0, team_max=0, max_iter=0, device=0):
+    def __init__(self, budget_bytes=0.0, cache_bytes=0.0, team_max=0, max_iter=0, device=0, speculative_rows=0):
         L = lib()
-        opt = svmhip_options(C.sizeof(svmhip_options), device, team_max, 0, max_iter, float(cache_bytes), 0, 0)
+        opt = svmhip_options(C.sizeof(svmhip_options), device, team_max, 0, max_iter, float(cache_bytes), 0, 0,
+                             speculative_rows)
         self._h = C.c_void_p()
         rc = L.svmhip_server_create(C.byref(opt), float(budget_bytes), C.byref(self._h))
         if rc != 0:

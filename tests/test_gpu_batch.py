@@ -118,3 +118,40 @@ def test_predict_batch_matches_oracle(hip, ora, kname, svm_type):
     np.testing.assert_allclose(dec_d, dec_r, rtol=0, atol=1e-4)
     if svm_type != ffi.EPSILON_SVR:
         assert np.mean(lab_d == lab_r) >= 0.999
+
+
+@pytest.mark.parametrize("dense", [False, True])
+def test_batch_speculative_rows_same_results(hip, dense):
+    """Single-workgroup solves with a full cache compute, on a row miss, the
+    rows of the pass's other strong candidates in the same pass; the iterates
+    must be bit-identical to solves without it (speculative_rows=-1)."""
+    if dense:
+        X, y = datasets.gmm_dense_matrix(n=4000, d=64, seed=11)
+        prob = ffi.Problem.from_dense(X.astype(np.float64), y, keep_zeros=True)
+        gammas = [1.0 / 64, 0.1]
+    else:
+        ip, ix, v, y, d = datasets.zipf_binary(n=4000, seed=5)
+        prob = ffi.Problem(ip, ix, v, y, d)
+        gammas = [1.0 / d, 0.05]
+    prm = ffi.make_param(gamma=gammas[0], C_=1.0, eps=1e-3, shrinking=0)
+    rng = np.random.default_rng(1)
+    out = {}
+    for spec in (0, -1):
+        b = hip.BatchSolver(prob, prm, team_max=1, speculative_rows=spec)
+        subs = []
+        for k in range(6):
+            rows = np.sort(rng.choice(prob.l, size=int(rng.integers(800, 3000)), replace=False)) if spec == 0 \
+                else out[0]["rows"][k]
+            sub = prob.subset(rows)
+            yy, p, a0, Cp, Cn, kind = hip.formulate(sub, prm)
+            subs.append((rows, dict(rowset=b.add_rowset(rows), kind=kind, y=yy, p=p, alpha0=a0, Cp=Cp, Cn=Cn,
+                                    gamma=gammas[k % 2], eps=1e-3)))
+        res, st = b.solve([s[1] for s in subs])
+        out[spec] = dict(rows=[s[0] for s in subs], res=res, st=st)
+    print(f"  rows computed: speculative {out[0]['st']['row_misses']}, plain {out[-1]['st']['row_misses']}",
+          flush=True)
+    for a, b_ in zip(out[0]["res"], out[-1]["res"]):
+        assert a["status"] == 0 and b_["status"] == 0
+        assert a["iters"] == b_["iters"]
+        np.testing.assert_array_equal(a["alpha_raw"], b_["alpha_raw"])
+    assert out[0]["st"]["row_misses"] >= out[-1]["st"]["row_misses"]
