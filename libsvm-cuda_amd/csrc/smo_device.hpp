@@ -110,6 +110,8 @@ struct View {
                           // (nblocks copies, pm_stride apart) are the subproblem's own
   int keep_cache;         // batched solves: the own cache is valid from an earlier stage
   int spec;               // persistent C-rule solves, full cache: speculative rows on an i miss
+  unsigned *drain;        // batched solves: workgroups of the launch that found the queue empty
+  int drain_at;           // batched solves: pause once drain reaches this (checked every 32 iterations)
 };
 
 // Work queue and kernel-row cache arenas of the batched solver.  Item q of
@@ -128,6 +130,10 @@ struct BatchPool {
   int lds_floats;          // dynamic LDS of the launch, in floats
   int spec;                // single-workgroup C-rule items with a full cache: on a row miss, also
                            // compute the rows of the pass's per-wave winners in the same pass
+  unsigned *idle;          // workgroups that found the queue empty (zeroed before each launch;
+                           // null = no drain rule)
+  int drain_at;            // running items pause once idle reaches this: the host then re-plans
+                           // the rest of the solves (as teams) for the workgroups that went idle
   unsigned long long *t0;  // launch start on the device clock (zeroed before each launch; null = none)
   long long deadline_ticks;  // > 0: items pause at t0 + deadline_ticks, and single-workgroup items
                              // claimed after it are left for the next launch (their Ctl stays clear)
@@ -1273,9 +1279,13 @@ __device__ __forceinline__ XPart exchange(const View &V, RedSmem &sm, XSmem &xs,
   return r;
 }
 
-// This owner's vote to pause the solve at the current iteration boundary.
+// This owner's vote to pause the solve at the current iteration boundary:
+// the stage's time is up, or (batched launches) enough of the launch's
+// workgroups have run out of items.
 __device__ __forceinline__ int pause_vote(const View &V, const Dec &d) {
-  return V.pause_ticks > 0 && wall_clock64() - d.t_start >= V.pause_ticks ? 1 : 0;
+  if (V.pause_ticks > 0 && wall_clock64() - d.t_start >= V.pause_ticks) return 1;
+  if (V.drain && (d.iter & 31) == 0 && ld_acq(V.drain) >= (unsigned)V.drain_at) return 1;
+  return 0;
 }
 
 __device__ __forceinline__ CacheMeta private_meta(const View &V, Dec &d) {

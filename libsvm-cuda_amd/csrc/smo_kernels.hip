@@ -872,9 +872,14 @@ __global__ void __launch_bounds__(BLOCK) k_smo_batch(const View *views, const in
     __syncthreads();
     // uniform across the workgroup: keeps the View in scalar registers
     const int q = __builtin_amdgcn_readfirstlane(s_item);
-    if (q >= nitems) break;
+    if (q >= nitems) {
+      if (threadIdx.x == 0 && pool.idle) atomicAdd(pool.idle, 1u);
+      break;
+    }
     const int sub = __builtin_amdgcn_readfirstlane(items[2 * q]);
     View V = views[sub];
+    V.drain = pool.idle;
+    V.drain_at = pool.drain_at;
     // past the launch's deadline, single-workgroup items are left untouched
     // for the next launch (team members are always run: they are queued
     // first and all claimed at the start)

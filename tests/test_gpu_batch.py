@@ -154,4 +154,5 @@ def test_batch_speculative_rows_same_results(hip, dense):
         assert a["status"] == 0 and b_["status"] == 0
         assert a["iters"] == b_["iters"]
         np.testing.assert_array_equal(a["alpha_raw"], b_["alpha_raw"])
-    assert out[0]["st"]["row_misses"] >= out[-1]["st"]["row_misses"]
+    # (the rows computed are not compared: where a stage pauses depends on
+    # timing, and a paused solve restarts with an empty row cache)
