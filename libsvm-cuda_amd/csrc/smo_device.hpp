@@ -115,10 +115,14 @@ struct View {
 };
 
 // Work queue and kernel-row cache arenas of the batched solver.  Item q of
-// a launch is (subproblem, rank).  With by_item = 0 every item is a whole
-// subproblem and workgroup b uses arena b; with by_item = 1 item q uses
-// arena q, and the T consecutive items of a team use arenas [q0, q0 + T)
-// as one row cache (each member keeps its own cache metadata).
+// a launch is (subproblem, rank); the items of teams come first.  A team's
+// T consecutive items [q0, q0 + T) use arenas [q0, q0 + T) as one row cache
+// (each member keeps its own cache metadata), and the workgroup that ran
+// member item q keeps arena q for the single-workgroup items it takes
+// afterwards; a workgroup whose first item is a single-workgroup one takes
+// a fresh arena from team_items + (*arena_next)++.  Every workgroup thus
+// holds one arena, no two hold the same, and at most the grid's size are
+// used.
 struct BatchPool {
   float *rows;             // K arenas of arena_floats floats
   long long arena_floats;
@@ -126,7 +130,8 @@ struct BatchPool {
   int *lru;                // K x lru_stride ints (LRU arenas)
   long long flag_stride, lru_stride;
   int *queue;              // next item (zeroed before each launch)
-  int K, by_item;
+  int *arena_next;         // fresh arenas handed out (zeroed before each launch)
+  int K, team_items;       // arenas; items belonging to teams (the first team_items)
   int lds_floats;          // dynamic LDS of the launch, in floats
   int spec;                // single-workgroup C-rule items with a full cache: on a row miss, also
                            // compute the rows of the pass's per-wave winners in the same pass

@@ -864,6 +864,8 @@ __global__ void __launch_bounds__(BLOCK) k_smo_batch(const View *views, const in
   __syncthreads();
   const long long t0 = s_t0;
   __shared__ long long s_now;
+  __shared__ int s_arena;
+  int my_arena = -1;  // the arena this workgroup holds (uniform)
   for (;;) {
     if (threadIdx.x == 0) {
       s_item = atomicAdd(pool.queue, 1);
@@ -895,9 +897,16 @@ __global__ void __launch_bounds__(BLOCK) k_smo_batch(const View *views, const in
       V.pm_flags += (long long)V.rank * V.l;
       V.pm_lru += (long long)V.rank * V.pm_stride;
     } else {
-      const long long own = pool.by_item ? q : (long long)blockIdx.x;
-      const long long first = pool.by_item ? own - V.rank : own;
-      team = pool.by_item ? V.nblocks : 1;
+      if (q < pool.team_items) {
+        my_arena = q;
+      } else if (my_arena < 0) {
+        if (threadIdx.x == 0) s_arena = pool.team_items + atomicAdd(pool.arena_next, 1);
+        __syncthreads();
+        my_arena = __builtin_amdgcn_readfirstlane(s_arena);
+      }
+      const long long own = my_arena;
+      team = q < pool.team_items ? V.nblocks : 1;
+      const long long first = own - (team > 1 ? V.rank : 0);
       V.rows = pool.rows + first * pool.arena_floats;
       V.row_stride = ((long long)V.l + 63) / 64 * 64;
       const long long fit = team * pool.arena_floats / (V.row_stride > 0 ? V.row_stride : 1);

@@ -705,7 +705,6 @@ svmhip_status svmhip_batch_solve(svmhip_batch *b, svmhip_subproblem *subs, int n
     smo::BatchPool pool;
     memset(&pool, 0, sizeof pool);
     pool.K = K;
-    pool.by_item = 0;
     pool.arena_floats = arena;
     pool.rows = (float *)talloc(sizeof(float) * (size_t)arena * K, "batch row-cache arenas");
     pool.flag_stride = maxn;
@@ -713,21 +712,25 @@ svmhip_status svmhip_batch_solve(svmhip_batch *b, svmhip_subproblem *subs, int n
     pool.lru_stride = 4LL * maxn + 2;
     pool.lru = (int *)talloc(sizeof(int) * (size_t)pool.lru_stride * K, "batch LRU bookkeeping");
     pool.queue = (int *)talloc(sizeof(int), "batch queue");
+    pool.arena_next = (int *)talloc(sizeof(int), "batch arena counter");
     pool.t0 = (unsigned long long *)talloc(sizeof(unsigned long long), "batch launch start");
     unsigned *dIdle = (unsigned *)talloc(sizeof(unsigned), "batch idle count");
     pool.lds_floats = (int)(lds / sizeof(float));
     pool.spec = h->opt.speculative_rows >= 0 ? 1 : 0;
     smo::XWord *dX = (smo::XWord *)talloc(sizeof(smo::XWord) * smo::xrec_words(G), "team exchange records");
-    int *dItems = (int *)talloc(sizeof(int) * 2 * (size_t)std::max(nsub, G), "batch items");
+    // items: one per subproblem plus the extra members of teams (at most G)
+    int *dItems = (int *)talloc(sizeof(int) * 2 * ((size_t)nsub + (size_t)G), "batch items");
 
     // Stages.  Every unfinished subproblem runs until it converges or, after
     // about tau of device time, pauses at an iteration boundary (team members
     // agree through their exchange); tau starts at 0.8 s and grows 4x each
     // stage.  A stage also ends once three quarters of its workgroups have
     // found the queue empty (the drain rule).  While more subproblems remain
-    // than the grid has workgroups each is solved by one workgroup (largest
-    // first); once fewer remain, the grid is shared out as teams in
-    // proportion to their work so far.  Where a solve pauses changes
+    // than the grid has workgroups, each is solved by one workgroup (largest
+    // first) except the few whose estimated work would outlast a
+    // workgroup's share, which get small teams on up to half the grid; once
+    // fewer remain, the grid is shared out as teams in proportion to their
+    // work so far.  Where a solve pauses changes
     // nothing: a resumed subproblem continues the same sequence of working
     // sets, but its kernel-row cache starts empty, so stages are long
     // (measured on 900 solves of 3k-14k rows: 0.8 s first stages with the
@@ -741,6 +744,14 @@ svmhip_status svmhip_batch_solve(svmhip_batch *b, svmhip_subproblem *subs, int n
     std::vector<smo::Ctl> hC(nsub);
     std::vector<int> T(nsub, 1), items;
     double tau = 0.8;
+    // team weights: with fewer subproblems than workgroups, n x (iterations
+    // so far + 1); with more, an estimate of a subproblem's whole work,
+    // n^1.5 x (iterations so far + n) (a one-workgroup iteration costs about
+    // n, and larger problems take more iterations)
+    auto work = [&](int k, bool fewer) {
+      const double n = (double)std::max(subs[k].n_var, 1);
+      return fewer ? n * (iters[k] + 1.0) : n * std::sqrt(n) * ((double)iters[k] + n);
+    };
     const double clock_hz = device_clock_hz(h->dev);
     int stages = 0;
     float total_ms = 0;
@@ -748,22 +759,36 @@ svmhip_status svmhip_batch_solve(svmhip_batch *b, svmhip_subproblem *subs, int n
       // team sizes
       // team shares in proportion to n_var x (iterations so far + 1): the
       // subproblems that have run longest are the likeliest to run longest
+      const bool fewer = (int)rem.size() < G;
       double wsum = 0;
-      for (int k : rem) wsum += (double)subs[k].n_var * (iters[k] + 1.0);
+      for (int k : rem) wsum += work(k, fewer);
       int tsum = 0;
       for (int k : rem) {
         const svmhip_batch::Rowset &rs = b->rowsets[subs[k].rowset];
         int t = 1;
-        if ((int)rem.size() < G && team_max > 1) {
-          t = (int)((double)G * subs[k].n_var * (iters[k] + 1.0) / wsum);
-          t = std::max(1, std::min({t, team_max, rs.nslices / 2}));
+        if (team_max > 1) {
+          // fewer subproblems than workgroups: the grid is shared out in
+          // proportion to the weights; more: a subproblem whose estimated
+          // work is more than about 1.5 workgroups' share of the whole gets
+          // a small team (up to 8), the rest run one workgroup each
+          const double share = (double)G * work(k, fewer) / wsum;
+          t = fewer ? (int)share : (int)(share + 0.5);
+          t = std::max(1, std::min({t, fewer ? team_max : std::min(team_max, 8), rs.nslices / 2}));
         }
         T[k] = t;
         tsum += t;
       }
-      // members given to the one-workgroup minimum can push the sum past the
-      // grid: take them back from the largest teams
-      while (tsum > G) {
+      // teams may hold the whole grid when every subproblem fits in it, and
+      // half the grid otherwise (the rest drains the single-workgroup items):
+      // take members back from the largest teams
+      auto team_members = [&]() {
+        int m = 0;
+        for (int k : rem)
+          if (T[k] > 1) m += T[k];
+        return m;
+      };
+      const int team_cap = fewer ? G : G / 2;
+      while ((fewer ? tsum : team_members()) > team_cap) {
         int big = -1;
         for (int k : rem)
           if (T[k] > 1 && (big < 0 || T[k] > T[big])) big = k;
@@ -771,8 +796,8 @@ svmhip_status svmhip_batch_solve(svmhip_batch *b, svmhip_subproblem *subs, int n
         --T[big];
         --tsum;
       }
-      const bool teams = tsum > (int)rem.size();
-      if (teams && tsum > G) h->fail(SVMHIP_ERR_STATE, "team plan exceeds the grid");
+      const int team_items = team_members();
+      if (team_items > G) h->fail(SVMHIP_ERR_STATE, "team plan exceeds the grid");
       // queue order: teams (largest first), then single workgroups (largest first)
       std::vector<int> ord = rem;
       std::stable_sort(ord.begin(), ord.end(), [&](int a, int c) {
@@ -801,8 +826,10 @@ svmhip_status svmhip_batch_solve(svmhip_batch *b, svmhip_subproblem *subs, int n
         }
       }
       const int nitems = (int)items.size() / 2;
-      pool.by_item = teams ? 1 : 0;
+      pool.team_items = team_items;
+      h->ck(hipMemsetAsync(pool.arena_next, 0, sizeof(int), h->st), "clear batch arena counter");
       h->upload(dV, views.data(), nsub, "upload batch views");
+      if (items.size() > 2 * ((size_t)nsub + (size_t)G)) h->fail(SVMHIP_ERR_STATE, "batch item list overflows");
       h->upload(dItems, items.data(), items.size(), "upload batch items");
       h->ck(hipMemsetAsync(pool.queue, 0, sizeof(int), h->st), "clear batch queue");
       h->ck(hipMemsetAsync(pool.t0, 0, sizeof(unsigned long long), h->st), "clear launch start");

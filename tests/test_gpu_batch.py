@@ -156,3 +156,33 @@ def test_batch_speculative_rows_same_results(hip, dense):
         np.testing.assert_array_equal(a["alpha_raw"], b_["alpha_raw"])
     # (the rows computed are not compared: where a stage pauses depends on
     # timing, and a paused solve restarts with an empty row cache)
+
+
+def test_batch_more_subproblems_than_workgroups(hip, ora):
+    """More subproblems than the grid has workgroups (300 > 256 on one
+    MI355X): most run one workgroup each while the few large ones get small
+    teams in the same launch, and workgroups take arenas as they go; every
+    subproblem must still give the oracle's iterates."""
+    ip, ix, v, y, d = datasets.zipf_binary(n=4000, seed=13)
+    prob = ffi.Problem(ip, ix, v, y, d)
+    prm = ffi.make_param(gamma=1.0 / d, C_=1.0, eps=1e-3, shrinking=0)
+    rng = np.random.default_rng(3)
+    b = hip.BatchSolver(prob, prm)
+    cases = []
+    for k in range(300):
+        n = int(rng.integers(2500, 3500)) if k % 30 == 0 else int(rng.integers(60, 200))
+        rows = np.sort(rng.choice(prob.l, size=n, replace=False))
+        sub = prob.subset(rows)
+        if len(np.unique(np.where(sub.y > 0, 1, -1))) < 2:
+            continue
+        yy, p, a0, Cp, Cn, kind = hip.formulate(sub, prm)
+        cases.append((sub, dict(rowset=b.add_rowset(rows), kind=kind, y=yy, p=p, alpha0=a0, Cp=Cp, Cn=Cn,
+                                gamma=prm.gamma, eps=1e-3)))
+    assert len(cases) > 256
+    res, st = b.solve([c[1] for c in cases])
+    print(f"  {len(cases)} subproblems, {st['iters']} iterations, {st['launches']} launches", flush=True)
+    for (sub, _), r in zip(cases, res):
+        ref = ora.train_one(sub, prm)
+        assert r["status"] == 0
+        assert r["iters"] == ref["iters"], (sub.l, r["iters"], ref["iters"])
+        np.testing.assert_allclose(r["alpha_raw"], ref["alpha_raw"], rtol=0, atol=1e-12)
