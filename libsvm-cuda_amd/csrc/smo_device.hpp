@@ -1048,10 +1048,11 @@ constexpr long long SPIN_LIMIT = 1LL << 23;
 // read-modify-write atomics, no L2 write-back or invalidation.  Word w of
 // owner t in buffer p (the sequence number's parity) is at
 // xw[(p * XW + w) * nblocks + t], so the readers' loads of one word are
-// contiguous across owners.  Every record is written XREP times, to XREP
-// copies of the buffers, and workgroup b reads copy b % XREP: every
-// workgroup reads every record at every exchange, and spreading those reads
-// over copies (on different memory channels) shortens the exchange.
+// contiguous across owners.  With 64 or more owners every record is written
+// XREP times, to XREP copies of the buffers, and workgroup b reads copy
+// b % XREP: every workgroup reads every record at every exchange, and
+// spreading those reads over copies (on different memory channels) shortens
+// the exchange.
 
 struct XPart {
   unsigned seq;
@@ -1139,13 +1140,18 @@ __device__ __forceinline__ void st_agent(T *p, T v) {
   __hip_atomic_store(p, v, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
 }
 
-// Owner t's record of exchange `seq`, into every copy (one thread).
+// Copies in use for nb owners: with few owners the reads of one copy are
+// few too, and the extra stores would cost more than they save.
+__device__ __forceinline__ int xcopies(int nb) { return nb >= 64 ? XREP : 1; }
+
+// Owner t's record of exchange `seq`, into every copy in use (one thread).
 template <int F>
 __device__ __forceinline__ void xpublish(XWord *X, int nb, int t, const XPart &p, unsigned seq) {
   unsigned w[XW];
   xpack<F>(p, w);
+  const int nc = xcopies(nb);
 #pragma unroll 1
-  for (int c = 0; c < XREP; ++c)
+  for (int c = 0; c < nc; ++c)
 #pragma unroll
     for (int k = 0; k < xf_words<F>(); ++k)
       st_agent(xword(X + (size_t)c * xcopy_words(nb), nb, seq & 1u, k, t), ((XWord)seq << 32) | (XWord)w[k]);
@@ -1153,7 +1159,7 @@ __device__ __forceinline__ void xpublish(XWord *X, int nb, int t, const XPart &p
 
 // The copy of the buffers this workgroup reads.
 __device__ __forceinline__ XWord *xcopy(XWord *X, int nb) {
-  return X + (size_t)(blockIdx.x % XREP) * xcopy_words(nb);
+  return X + (size_t)(blockIdx.x % xcopies(nb)) * xcopy_words(nb);
 }
 
 // Waits for owner t's record of exchange `seq` and unpacks it into q (one
