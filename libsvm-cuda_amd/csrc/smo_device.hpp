@@ -647,16 +647,20 @@ struct Pend {
   double ai, aj;
 };
 
-// Slices a wave visits per trip of the pass loops.
+// Slices a wave visits per trip of the pass loops, and (for rows that are
+// not SVR) in single-workgroup batched solves, whose waves own whole
+// subproblems: more slices per trip cost the persistent kernel, whose waves
+// own one or two slices, more than they save.
 constexpr int PASS_SLICES = 4;
+constexpr int PASS_SLICES_SOLO = 8;
 
 __device__ __forceinline__ bool owns(const View &V, int sb, int se, int v) {
   int r = real_of(V, v);
   return r >= sb * SLICE && r < se * SLICE;
 }
 
-template <int KIND>
-__device__ __forceinline__ Partial pass_gradient_select(const View &V, const Snap &c, float *xs, RedSmem &sm, bool update,
+template <int KIND, int NV, int PS>
+__device__ __forceinline__ Partial pass_gradient_select_nv(const View &V, const Snap &c, float *xs, RedSmem &sm, bool update,
                                         bool select, bool to_part = true, const Pend *pend = nullptr) {
   int sb, se;
   owned_slices(V, my_owner(V), V.nblocks, sb, se);
@@ -671,44 +675,44 @@ __device__ __forceinline__ Partial pass_gradient_select(const View &V, const Sna
   double v[4] = {-INF, -INF, -INF, -INF};
   int ix[2] = {-1, -1};
   const int wave = threadIdx.x / 64, lane = threadIdx.x % 64, nw = blockDim.x / 64;
-  const int nv = V.svr ? 2 : 1;
   const int8_t yi = update ? V.y[c.i] : (int8_t)0, yj = update ? V.y[c.j] : (int8_t)0;
-  // PASS_SLICES slices per trip, all loads issued before any use, so a wave
-  // waits for one memory round trip per PASS_SLICES slices rather than per
+  // PS slices per trip, all loads issued before any use, so a wave
+  // waits for one memory round trip per PS slices rather than per
   // slice (the selection's ties go to the larger index, so the order of
   // visits does not matter)
-  for (int s0 = sb + wave; s0 < se; s0 += nw * PASS_SLICES) {
-    float Ki[PASS_SLICES], Kj[PASS_SLICES];
-    double g[PASS_SLICES][2];
-    int8_t yk[PASS_SLICES][2], st[PASS_SLICES][2];
-    bool ok[PASS_SLICES];
+  for (int s0 = sb + wave; s0 < se; s0 += nw * PS) {
+    float Ki[PS], Kj[PS];
+    double g[PS][NV];
+    int8_t yk[PS][NV], st[PS][NV];
+    bool ok[PS];
     // loads are unconditional (lanes past the owned rows read a clamped,
     // valid row and ignore it), so they issue back to back with one wait
 #pragma unroll
-    for (int u = 0; u < PASS_SLICES; ++u) {
+    for (int u = 0; u < PS; ++u) {
       const int r = (s0 + u * nw) * SLICE + lane;
       ok[u] = s0 + u * nw < se && r < V.l;
-      const int rc = min(r, V.l - 1);
+      // slots past the wave's owned slices re-read its last owned slice
+      // (lines already being fetched) instead of other owners' rows
+      const int rc = min(min(s0 + u * nw, se - 1) * SLICE + lane, V.l - 1);
       Ki[u] = Kj[u] = 0.f;
       if (update) {
         Ki[u] = V.rows[(long long)c.slot_i * V.row_stride + rc];
         Kj[u] = V.rows[(long long)c.slot_j * V.row_stride + rc];
       }
 #pragma unroll
-      for (int h = 0; h < 2; ++h) {
-        const int k = rc + (h < nv ? h : 0) * V.l;
+      for (int h = 0; h < NV; ++h) {
+        const int k = rc + h * V.l;
         yk[u][h] = V.y[k];
         g[u][h] = V.G[k];
         st[u][h] = select ? V.status[k] : (int8_t)0;
       }
     }
 #pragma unroll
-    for (int u = 0; u < PASS_SLICES; ++u) {
+    for (int u = 0; u < PS; ++u) {
       if (!ok[u]) continue;
       const int r = (s0 + u * nw) * SLICE + lane;
 #pragma unroll
-      for (int h = 0; h < 2; ++h) {
-        if (h >= nv) continue;
+      for (int h = 0; h < NV; ++h) {
         const int k = r + h * V.l;
         double gk = g[u][h];
         if (update) {
@@ -767,8 +771,8 @@ __device__ __forceinline__ Partial pass_gradient_select(const View &V, const Sna
 // Pass "select j": second-order gain over the owned variables
 // (libsvm/svm.cpp:866-917; nu rule :1123-1172), row i (rows ip / in) computed
 // for the owned range first on a miss.
-template <int KIND>
-__device__ __forceinline__ Partial pass_select_j(const View &V, const Snap &c, float *xs, RedSmem &sm, bool to_part = true) {
+template <int KIND, int NV, int PS>
+__device__ __forceinline__ Partial pass_select_j_nv(const View &V, const Snap &c, float *xs, RedSmem &sm, bool to_part = true) {
   int sb, se;
   owned_slices(V, my_owner(V), V.nblocks, sb, se);
   if (KIND == KIND_C) {
@@ -780,7 +784,6 @@ __device__ __forceinline__ Partial pass_select_j(const View &V, const Snap &c, f
   double best = INF;
   int bj = -1;
   const int wave = threadIdx.x / 64, lane = threadIdx.x % 64, nw = blockDim.x / 64;
-  const int nv = V.svr ? 2 : 1;
   // rows and scalars of the candidates i (C rule) / i_p, i_n (nu rule)
   const int qi = KIND == KIND_C ? c.i : c.ip;
   const int qn = KIND == KIND_C ? -1 : c.in;
@@ -788,23 +791,25 @@ __device__ __forceinline__ Partial pass_select_j(const View &V, const Snap &c, f
   const long long rn = qn >= 0 ? (long long)c.slot_in * V.row_stride : 0;
   const int8_t yi = qi >= 0 ? V.y[qi] : (int8_t)1, yn = qn >= 0 ? V.y[qn] : (int8_t)1;
   const double QDi = qi >= 0 ? V.QD[qi] : 0.0, QDn = qn >= 0 ? V.QD[qn] : 0.0;
-  // PASS_SLICES slices per trip, loads first (ties go to the larger index)
-  for (int s0 = sb + wave; s0 < se; s0 += nw * PASS_SLICES) {
-    float Ki[PASS_SLICES], Kn[PASS_SLICES];
-    double g[PASS_SLICES][2], qd[PASS_SLICES][2];
-    int8_t yk[PASS_SLICES][2], st[PASS_SLICES][2];
-    bool ok[PASS_SLICES];
+  // PS slices per trip, loads first (ties go to the larger index)
+  for (int s0 = sb + wave; s0 < se; s0 += nw * PS) {
+    float Ki[PS], Kn[PS];
+    double g[PS][NV], qd[PS][NV];
+    int8_t yk[PS][NV], st[PS][NV];
+    bool ok[PS];
     // unconditional loads from a clamped, valid row (see pass_gradient_select)
 #pragma unroll
-    for (int u = 0; u < PASS_SLICES; ++u) {
+    for (int u = 0; u < PS; ++u) {
       const int r = (s0 + u * nw) * SLICE + lane;
       ok[u] = s0 + u * nw < se && r < V.l;
-      const int rc = min(r, V.l - 1);
+      // slots past the wave's owned slices re-read its last owned slice
+      // (lines already being fetched) instead of other owners' rows
+      const int rc = min(min(s0 + u * nw, se - 1) * SLICE + lane, V.l - 1);
       Ki[u] = qi >= 0 ? V.rows[ri + rc] : 0.f;
       Kn[u] = qn >= 0 ? V.rows[rn + rc] : 0.f;
 #pragma unroll
-      for (int h = 0; h < 2; ++h) {
-        const int k = rc + (h < nv ? h : 0) * V.l;
+      for (int h = 0; h < NV; ++h) {
+        const int k = rc + h * V.l;
         yk[u][h] = V.y[k];
         st[u][h] = V.status[k];
         g[u][h] = V.G[k];
@@ -812,12 +817,11 @@ __device__ __forceinline__ Partial pass_select_j(const View &V, const Snap &c, f
       }
     }
 #pragma unroll
-    for (int u = 0; u < PASS_SLICES; ++u) {
+    for (int u = 0; u < PS; ++u) {
       if (!ok[u]) continue;
       const int r = (s0 + u * nw) * SLICE + lane;
 #pragma unroll
-      for (int h = 0; h < 2; ++h) {
-        if (h >= nv) continue;
+      for (int h = 0; h < NV; ++h) {
         const int k = r + h * V.l;
         const int8_t y = yk[u][h], sk = st[u][h];
         const double gk = g[u][h];
@@ -870,6 +874,24 @@ __device__ __forceinline__ Partial pass_select_j(const View &V, const Snap &c, f
   pt.i0 = ix[0]; pt.i1 = -1; pt.pad0 = pt.pad1 = 0;
   if (to_part && threadIdx.x == 0) V.part[blockIdx.x] = pt;
   return pt;
+}
+
+// The passes by row kind: SVR variables come two per row (both halves of
+// the row are visited together); every other kind has one, which leaves the
+// registers for PS1 slices per trip.
+template <int KIND, int PS1 = PASS_SLICES>
+__device__ __forceinline__ Partial pass_gradient_select(const View &V, const Snap &c, float *xs, RedSmem &sm,
+                                                        bool update, bool select, bool to_part = true,
+                                                        const Pend *pend = nullptr) {
+  if (V.svr) return pass_gradient_select_nv<KIND, 2, PASS_SLICES>(V, c, xs, sm, update, select, to_part, pend);
+  return pass_gradient_select_nv<KIND, 1, PS1>(V, c, xs, sm, update, select, to_part, pend);
+}
+
+template <int KIND, int PS1 = PASS_SLICES>
+__device__ __forceinline__ Partial pass_select_j(const View &V, const Snap &c, float *xs, RedSmem &sm,
+                                                 bool to_part = true) {
+  if (V.svr) return pass_select_j_nv<KIND, 2, PASS_SLICES>(V, c, xs, sm, to_part);
+  return pass_select_j_nv<KIND, 1, PS1>(V, c, xs, sm, to_part);
 }
 
 // Partials are read with agent-scope loads: they were written by other

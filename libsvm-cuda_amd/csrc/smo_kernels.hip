@@ -749,7 +749,7 @@ __device__ __forceinline__ void solo_solve(View &V, Dec &d, RedSmem &sm, float *
   __syncthreads();
   {
     Snap c = snap_of(d);
-    Partial bp = pass_gradient_select<KIND>(V, c, xs, sm, false, true, false, nullptr);
+    Partial bp = pass_gradient_select<KIND, PASS_SLICES_SOLO>(V, c, xs, sm, false, true, false, nullptr);
     XPart m = make_i_record<KIND>(V, bp);
     if (threadIdx.x == 0) { d.phase = 1; decide_i<KIND>(V, d, m, true); }
     __syncthreads();
@@ -757,7 +757,7 @@ __device__ __forceinline__ void solo_solve(View &V, Dec &d, RedSmem &sm, float *
   while (!d.stop) {
     if (KIND == KIND_C && V.spec && d.miss_i) solo_spec_rows(V, d, sm, xs, lds_floats, false);
     Snap c = snap_of(d);
-    Partial bp = pass_select_j<KIND>(V, c, xs, sm, false);
+    Partial bp = pass_select_j<KIND, PASS_SLICES_SOLO>(V, c, xs, sm, false);
     XPart m = make_j_record<KIND>(V, c, bp);
     if (threadIdx.x == 0) decide_j<KIND>(V, d, m);
     __syncthreads();
@@ -766,7 +766,7 @@ __device__ __forceinline__ void solo_solve(View &V, Dec &d, RedSmem &sm, float *
     c = snap_of(d);
     Pend pd;
     pd.i = d.i; pd.j = d.j; pd.ai = d.ai_new; pd.aj = d.aj_new;
-    Partial bp2 = pass_gradient_select<KIND>(V, c, xs, sm, true, true, false, &pd);
+    Partial bp2 = pass_gradient_select<KIND, PASS_SLICES_SOLO>(V, c, xs, sm, true, true, false, &pd);
     XPart m2 = make_i_record<KIND>(V, bp2);
     if (threadIdx.x == 0) {
       m2.pad = pause_vote(V, d);
