@@ -186,3 +186,47 @@ def test_batch_more_subproblems_than_workgroups(hip, ora):
         assert r["status"] == 0
         assert r["iters"] == ref["iters"], (sub.l, r["iters"], ref["iters"])
         np.testing.assert_allclose(r["alpha_raw"], ref["alpha_raw"], rtol=0, atol=1e-12)
+
+
+def test_ovo_cv_gamma_grid_full_size(hip, ora):
+    """The 10-class configuration at full size (n=38,217, d=128): 5-fold CV x
+    4 gammas x 45 one-vs-one pairs = 900 solves in one batched call.  Every
+    solve must converge to a feasible point whose dual objective is below
+    the starting point's (0), and the 20 smallest must give the oracle's
+    iterates (identical iteration count, alpha to 1e-9, objective to 1e-9)."""
+    gammas = [2.0 ** -9, 2.0 ** -7, 2.0 ** -5, 2.0 ** -3]
+    C = 4.0
+    X, y, fold = datasets.clusters_10class()
+    prob = ffi.Problem.from_dense(X.astype(np.float64), y.astype(np.float64), keep_zeros=True)
+    b = hip.BatchSolver(prob, ffi.make_param(gamma=gammas[0], C_=C, eps=1e-3, shrinking=0))
+    jobs, subs = [], []
+    for f in range(int(fold.max()) + 1):
+        train = np.nonzero(fold != f)[0]
+        labels = list(dict.fromkeys(y[train].tolist()))
+        by = {c: train[y[train] == c] for c in labels}
+        for i in range(len(labels)):
+            for j in range(i + 1, len(labels)):
+                rows = np.concatenate([by[labels[i]], by[labels[j]]])
+                yy = np.concatenate([np.ones(len(by[labels[i]])), -np.ones(len(by[labels[j]]))]).astype(np.int8)
+                rid = b.add_rowset(rows)
+                for g in gammas:
+                    jobs.append((rows, yy, g))
+                    subs.append(dict(rowset=rid, kind=hip.KIND_C, y=yy, p=-np.ones(len(rows)),
+                                     alpha0=np.zeros(len(rows)), Cp=C, Cn=C, gamma=g, eps=1e-3))
+    assert len(subs) == 900
+    res, st = b.solve(subs)
+    print(f"  900 solves: {st['iters']} iterations, {st['launches']} launches, {st['solve_ms']:.0f} ms", flush=True)
+    for (rows, yy, g), r in zip(jobs, res):
+        a = r["alpha_raw"]
+        assert r["status"] == 0
+        assert np.all(a >= 0) and np.all(a <= C)
+        assert abs(np.dot(a, yy)) < 1e-8 * max(1.0, a.sum())
+        assert r["obj"] < 0
+    order = np.argsort([len(j[0]) for j in jobs], kind="stable")[:20]
+    for k in order:
+        rows, yy, g = jobs[k]
+        sub = ffi.Problem.from_dense(X[rows].astype(np.float64), yy.astype(np.float64), keep_zeros=True)
+        ref = ora.train_one(sub, ffi.make_param(gamma=g, C_=C, eps=1e-3, shrinking=0))
+        assert res[k]["iters"] == ref["iters"], (len(rows), g)
+        np.testing.assert_allclose(res[k]["alpha_raw"], ref["alpha_raw"], rtol=0, atol=1e-9)
+        assert abs(res[k]["obj"] - ref["obj"]) <= 1e-9 * abs(ref["obj"])
