@@ -369,7 +369,17 @@ __device__ __forceinline__ void row_dot_multi(const View &V, const float *xs, in
   for (int m = 0; m < M; ++m) acc[m] = 0.0;
   if (V.dense) {
     const int len = V.d;
-    for (int p = 0; p < len; p += U) {
+    int p = 0;
+    for (; p + U <= len; p += U) {
+      float a[U];
+#pragma unroll
+      for (int u = 0; u < U; ++u) a[u] = vp[(long long)(p + u) * SLICE];
+#pragma unroll
+      for (int u = 0; u < U; ++u)
+#pragma unroll
+        for (int m = 0; m < M; ++m) acc[m] = fma((double)a[u], (double)xs[m * ld + p + u], acc[m]);
+    }
+    if (p < len) {  // the last, partial trip
       float a[U];
 #pragma unroll
       for (int u = 0; u < U; ++u) a[u] = vp[(long long)min(p + u, len - 1) * SLICE];
