@@ -723,7 +723,7 @@ svmhip_status svmhip_batch_solve(svmhip_batch *b, svmhip_subproblem *subs, int n
 
     // Stages.  Every unfinished subproblem runs until it converges or, after
     // about tau of device time, pauses at an iteration boundary (team members
-    // agree through their exchange); tau starts at 0.8 s and grows 4x each
+    // agree through their exchange); tau starts at 0.4 s and grows 4x each
     // stage.  A stage also ends once three quarters of its workgroups have
     // found the queue empty (the drain rule).  While more subproblems remain
     // than the grid has workgroups, each is solved by one workgroup (largest
@@ -732,9 +732,10 @@ svmhip_status svmhip_batch_solve(svmhip_batch *b, svmhip_subproblem *subs, int n
     // fewer remain, the grid is shared out as teams in proportion to their
     // work so far.  Where a solve pauses changes
     // nothing: a resumed subproblem continues the same sequence of working
-    // sets, but its kernel-row cache starts empty, so stages are long
-    // (measured on 900 solves of 3k-14k rows: 0.8 s first stages with the
-    // drain rule took 5.1 s, 0.05 s first stages 5.3 s).
+    // sets, but its kernel-row cache starts empty, so stages are long; the
+    // first stage is the one that learns which subproblems run long
+    // (measured on 900 solves of 3k-14k rows, one box: first stages of
+    // 0.2 / 0.4 / 0.8 s took 3.83 / 3.59 / 3.93 s).
     const int team_max = h->opt.grid_blocks > 0 ? std::min(h->opt.grid_blocks, 64) : 32;
     const int fixed_chunk = h->opt.iters_per_launch;
     std::vector<int> rem(nsub), iters(nsub, 0);
@@ -743,7 +744,7 @@ svmhip_status svmhip_batch_solve(svmhip_batch *b, svmhip_subproblem *subs, int n
     for (int k = 0; k < nsub; ++k) rem[k] = k;
     std::vector<smo::Ctl> hC(nsub);
     std::vector<int> T(nsub, 1), items;
-    double tau = 0.8;
+    double tau = 0.4;
     // team weights: with fewer subproblems than workgroups, n x (iterations
     // so far + 1); with more, an estimate of a subproblem's whole work,
     // n^1.5 x (iterations so far + n) (a one-workgroup iteration costs about
