@@ -179,13 +179,21 @@ svmhip_status svmhip_create(const svm_problem *prob, const svm_parameter *param,
     // launch geometry: at most one workgroup per slice, one per CU by default
     int cus = device_cus(h->dev);
     // default: one workgroup per CU once there are two 64-row slices per CU
-    // (row passes then use the whole device); below that about six slices
-    // per workgroup, 32..128 workgroups (fewer workgroups make each exchange
-    // cheaper).  Measured on 10k-100k row problems: at 100k rows 256
-    // workgroups beat 128 by 6%, at 10k rows 64 beat 128 by 4%.
-    int want = h->opt.grid_blocks > 0 ? h->opt.grid_blocks
-               : h->nslices >= 2 * cus ? cus
-                                        : std::min(std::min(128, cus), std::max(32, h->nslices / 6));
+    // (row passes then use the whole device); below that the largest power
+    // of two (at least 32) that leaves each workgroup about 2.4 slices:
+    // fewer workgroups make each exchange cheaper, and powers of two spread
+    // evenly over the 8 XCDs.  Measured on dense 256-feature problems: 10k
+    // rows 64 workgroups (79.7 ms; 32: 89.9), 20k and 30k rows 128 (106.9,
+    // 129.2 ms; 256: 111.5, 131.9), 100k rows 256 (6% ahead of 128).
+    int want = h->opt.grid_blocks;
+    if (want <= 0) {
+      if (h->nslices >= 2 * cus) {
+        want = cus;
+      } else {
+        want = 32;
+        while (2 * want <= cus && 2 * want <= h->nslices * 5 / 12) want *= 2;
+      }
+    }
     h->nblocks = std::max(1, std::min(want, std::max(h->nslices, 1)));
     size_t qbytes = (size_t)h->d * sizeof(float);
     h->use_lds = qbytes <= kLdsQueryLimit;
