@@ -289,9 +289,16 @@ __device__ __forceinline__ void stage_row(const View &V, int q, float *xs) {
 // rest of a row goes in batches of ROW_TAIL, their loads clamped to the
 // row's stored entries and the entries past the row's length left out of the
 // sum (adding +0.0 leaves the sum unchanged: it is never -0.0).
+// With PIPE, a sparse row goes in batches of ROW_PIPE with two batches in
+// flight: batch k+1's loads are issued before batch k is summed, so the
+// memory round trips of a row overlap pairwise.  Same sum order, so the same
+// bits; it needs ~50 more registers, which the multi-workgroup passes have
+// and the passes of the batched kernel do not.
 constexpr int ROW_UNROLL = 32;
 constexpr int ROW_TAIL = 8;
+constexpr int ROW_PIPE = 16;
 
+template <bool PIPE = false>
 __device__ __forceinline__ double row_dot(const View &V, const float *xs, int r) {
   const int s = r / SLICE;
   const long long base = V.slice_ptr[s] + (r % SLICE);
@@ -323,6 +330,45 @@ __device__ __forceinline__ double row_dot(const View &V, const float *xs, int r)
     const int width = (int)((V.slice_ptr[s + 1] - V.slice_ptr[s]) / SLICE);
     const int *cp = V.cols + base;
     int p = 0;
+    if (PIPE) {
+      constexpr int B = ROW_PIPE;
+      float a0[B], a1[B];
+      int c0[B], c1[B];
+      auto load = [&](int p0, float(&a)[B], int(&c)[B]) {
+        if (p0 + B <= width) {
+          const float *v = vp + (long long)p0 * SLICE;
+          const int *ci = cp + (long long)p0 * SLICE;
+#pragma unroll
+          for (int u = 0; u < B; ++u) {
+            a[u] = v[u * SLICE];
+            c[u] = ci[u * SLICE];
+          }
+        } else {  // the row's last batch, clamped to its stored entries
+#pragma unroll
+          for (int u = 0; u < B; ++u) {
+            const long long q = min(p0 + u, width - 1);
+            a[u] = vp[q * SLICE];
+            c[u] = cp[q * SLICE];
+          }
+        }
+      };
+      auto sum = [&](int p0, const float(&a)[B], const int(&c)[B]) {
+#pragma unroll
+        for (int u = 0; u < B; ++u) {
+          double t = (p0 + u < len) ? (double)a[u] * (double)xs[c[u]] : 0.0;
+          acc = acc + t;
+        }
+      };
+      if (width > 0) load(0, a0, c0);
+      for (; p < width; p += 2 * B) {
+        if (p + B < width) load(p + B, a1, c1);
+        sum(p, a0, c0);
+        if (p + B >= width) break;
+        if (p + 2 * B < width) load(p + 2 * B, a0, c0);
+        sum(p + B, a1, c1);
+      }
+      return acc;
+    }
     for (; p + U <= width; p += U) {
       float a[U];
       int c[U];
@@ -426,22 +472,24 @@ __device__ __forceinline__ double kernel_f64(const View &V, int q, int r, double
 }
 
 // Cached (float-rounded) kernel value K(q, r); q staged in xs.
+template <bool PIPE = false>
 __device__ __forceinline__ float kernel_value(const View &V, const float *xs, int q, int r) {
   if (V.ktype == K_PRECOMPUTED) {
     int s = V.serial[r];
     return (s >= 0 && s < V.d) ? xs[s] : 0.0f;
   }
-  return (float)kernel_f64(V, q, r, row_dot(V, xs, r));
+  return (float)kernel_f64(V, q, r, row_dot<PIPE>(V, xs, r));
 }
 
 // Materialise the owned part of kernel row q into cache slot `slot`.
+template <bool PIPE = false>
 __device__ __forceinline__ void compute_row_part(const View &V, int q, int slot, float *xs, int sb, int se) {
   stage_row(V, q, xs);
   float *dst = V.rows + (long long)slot * V.row_stride;
   const int wave = threadIdx.x / 64, lane = threadIdx.x % 64, nw = blockDim.x / 64;
   for (int s = sb + wave; s < se; s += nw) {
     int r = s * SLICE + lane;
-    if (r < V.l) dst[r] = kernel_value(V, xs, q, r);
+    if (r < V.l) dst[r] = kernel_value<PIPE>(V, xs, q, r);
   }
   __syncthreads();
 }
@@ -669,7 +717,7 @@ __device__ __forceinline__ bool owns(const View &V, int sb, int se, int v) {
   return r >= sb * SLICE && r < se * SLICE;
 }
 
-template <int KIND, int NV, int PS>
+template <int KIND, int NV, int PS, bool PIPE>
 __device__ __forceinline__ Partial pass_gradient_select_nv(const View &V, const Snap &c, float *xs, RedSmem &sm, bool update,
                                         bool select, bool to_part = true, const Pend *pend = nullptr) {
   int sb, se;
@@ -681,7 +729,7 @@ __device__ __forceinline__ Partial pass_gradient_select_nv(const View &V, const 
     }
     __syncthreads();
   }
-  if (update && c.miss_j) compute_row_part(V, real_of(V, c.j), c.slot_j, xs, sb, se);
+  if (update && c.miss_j) compute_row_part<PIPE>(V, real_of(V, c.j), c.slot_j, xs, sb, se);
   double v[4] = {-INF, -INF, -INF, -INF};
   int ix[2] = {-1, -1};
   const int wave = threadIdx.x / 64, lane = threadIdx.x % 64, nw = blockDim.x / 64;
@@ -781,15 +829,15 @@ __device__ __forceinline__ Partial pass_gradient_select_nv(const View &V, const 
 // Pass "select j": second-order gain over the owned variables
 // (libsvm/svm.cpp:866-917; nu rule :1123-1172), row i (rows ip / in) computed
 // for the owned range first on a miss.
-template <int KIND, int NV, int PS>
+template <int KIND, int NV, int PS, bool PIPE>
 __device__ __forceinline__ Partial pass_select_j_nv(const View &V, const Snap &c, float *xs, RedSmem &sm, bool to_part = true) {
   int sb, se;
   owned_slices(V, my_owner(V), V.nblocks, sb, se);
   if (KIND == KIND_C) {
-    if (c.miss_i) compute_row_part(V, real_of(V, c.i), c.slot_i, xs, sb, se);
+    if (c.miss_i) compute_row_part<PIPE>(V, real_of(V, c.i), c.slot_i, xs, sb, se);
   } else {
-    if (c.ip >= 0 && c.miss_ip) compute_row_part(V, real_of(V, c.ip), c.slot_ip, xs, sb, se);
-    if (c.in >= 0 && c.miss_in) compute_row_part(V, real_of(V, c.in), c.slot_in, xs, sb, se);
+    if (c.ip >= 0 && c.miss_ip) compute_row_part<PIPE>(V, real_of(V, c.ip), c.slot_ip, xs, sb, se);
+    if (c.in >= 0 && c.miss_in) compute_row_part<PIPE>(V, real_of(V, c.in), c.slot_in, xs, sb, se);
   }
   double best = INF;
   int bj = -1;
@@ -888,20 +936,22 @@ __device__ __forceinline__ Partial pass_select_j_nv(const View &V, const Snap &c
 
 // The passes by row kind: SVR variables come two per row (both halves of
 // the row are visited together); every other kind has one, which leaves the
-// registers for PS1 slices per trip.
-template <int KIND, int PS1 = PASS_SLICES>
+// registers for PS1 slices per trip.  PIPE: rows missed here are computed
+// with two batches of loads in flight (row_dot), for kernels with the
+// registers to spare.
+template <int KIND, int PS1 = PASS_SLICES, bool PIPE = false>
 __device__ __forceinline__ Partial pass_gradient_select(const View &V, const Snap &c, float *xs, RedSmem &sm,
                                                         bool update, bool select, bool to_part = true,
                                                         const Pend *pend = nullptr) {
-  if (V.svr) return pass_gradient_select_nv<KIND, 2, PASS_SLICES>(V, c, xs, sm, update, select, to_part, pend);
-  return pass_gradient_select_nv<KIND, 1, PS1>(V, c, xs, sm, update, select, to_part, pend);
+  if (V.svr) return pass_gradient_select_nv<KIND, 2, PASS_SLICES, PIPE>(V, c, xs, sm, update, select, to_part, pend);
+  return pass_gradient_select_nv<KIND, 1, PS1, PIPE>(V, c, xs, sm, update, select, to_part, pend);
 }
 
-template <int KIND, int PS1 = PASS_SLICES>
+template <int KIND, int PS1 = PASS_SLICES, bool PIPE = false>
 __device__ __forceinline__ Partial pass_select_j(const View &V, const Snap &c, float *xs, RedSmem &sm,
                                                  bool to_part = true) {
-  if (V.svr) return pass_select_j_nv<KIND, 2, PASS_SLICES>(V, c, xs, sm, to_part);
-  return pass_select_j_nv<KIND, 1, PS1>(V, c, xs, sm, to_part);
+  if (V.svr) return pass_select_j_nv<KIND, 2, PASS_SLICES, PIPE>(V, c, xs, sm, to_part);
+  return pass_select_j_nv<KIND, 1, PS1, PIPE>(V, c, xs, sm, to_part);
 }
 
 // Partials are read with agent-scope loads: they were written by other

@@ -173,10 +173,10 @@ __device__ __forceinline__ XPart make_j_record(const View &V, const Snap &c, con
 
 // First selection of a solve (or of a resumed solve): i from the current G,
 // exchanged, then the stop test and the row-i slot.
-template <int KIND>
+template <int KIND, bool PIPE = false>
 __device__ __forceinline__ void smo_select_first(const View &V, Dec &d, RedSmem &sm, XSmem &xsm, float *xs) {
   Snap c = snap_of(d);
-  Partial bp = pass_gradient_select<KIND>(V, c, xs, sm, false, true, false, nullptr);
+  Partial bp = pass_gradient_select<KIND, PASS_SLICES, PIPE>(V, c, xs, sm, false, true, false, nullptr);
   XPart mine = make_i_record<KIND>(V, bp);
   const unsigned seq = (unsigned)d.epoch + 1u;
   XPart r = exchange<KIND, PH_I>(V, sm, xsm, mine, seq);
@@ -309,14 +309,14 @@ __device__ __forceinline__ void spec_rows(const View &V, Dec &d, RedSmem &sm, fl
 //   [owners store the pair's alpha/status; G update, computing row j on a
 //    miss; select i] exchange
 //   [every workgroup: i, Gmax/Gmax2, stop test, row-i slot]
-template <int KIND, bool SPEC = false>
+template <int KIND, bool SPEC = false, bool PIPE = false>
 __device__ __forceinline__ void smo_iterations(const View &V, Dec &d, RedSmem &sm, XSmem &xsm, float *xs,
                                                int iter_limit) {
   for (int n = 0; n < iter_limit; ++n) {
     if (d.stop || d.err) break;
     if (SPEC && KIND == KIND_C && V.spec && d.miss_i) spec_rows<KIND>(V, d, sm, xs);
     Snap c = snap_of(d);
-    Partial bp = pass_select_j<KIND>(V, c, xs, sm, false);
+    Partial bp = pass_select_j<KIND, PASS_SLICES, PIPE>(V, c, xs, sm, false);
     XPart mine = make_j_record<KIND>(V, c, bp);
     unsigned seq = (unsigned)d.epoch + 1u;
     XPart r = exchange<KIND, PH_J>(V, sm, xsm, mine, seq);
@@ -330,7 +330,7 @@ __device__ __forceinline__ void smo_iterations(const View &V, Dec &d, RedSmem &s
     c = snap_of(d);
     Pend pd;
     pd.i = d.i; pd.j = d.j; pd.ai = d.ai_new; pd.aj = d.aj_new;
-    Partial bp2 = pass_gradient_select<KIND>(V, c, xs, sm, true, true, false, &pd);
+    Partial bp2 = pass_gradient_select<KIND, PASS_SLICES, PIPE>(V, c, xs, sm, true, true, false, &pd);
     XPart mine2 = make_i_record<KIND>(V, bp2);
     mine2.pad = pause_vote(V, d);
     seq = (unsigned)d.epoch + 1u;
@@ -357,8 +357,8 @@ __global__ void __launch_bounds__(BLOCK) k_smo(View V, int iter_limit) {
   float *xs = V.use_lds ? xs_lds : V.xs_global + (long long)blockIdx.x * V.d;
   if (threadIdx.x == 0) d = ctl_dec(V.ctl);
   __syncthreads();
-  if (d.phase == 0 && !d.stop && !d.err) smo_select_first<KIND>(V, d, sm, xsm, xs);
-  smo_iterations<KIND, true>(V, d, sm, xsm, xs, iter_limit);
+  if (d.phase == 0 && !d.stop && !d.err) smo_select_first<KIND, true>(V, d, sm, xsm, xs);
+  smo_iterations<KIND, true, true>(V, d, sm, xsm, xs, iter_limit);
   if (blockIdx.x == 0 && threadIdx.x == 0) {
     Ctl *c = V.ctl;
     ctl_dec(c) = d;
