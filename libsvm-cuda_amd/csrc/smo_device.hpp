@@ -298,6 +298,50 @@ constexpr int ROW_UNROLL = 32;
 constexpr int ROW_TAIL = 8;
 constexpr int ROW_PIPE = 16;
 
+// Sparse row dot with two batches of ROW_PIPE entries in flight (the PIPE
+// form of row_dot): vp / cp are the row's first value and column in its
+// slice, len its length, width the slice's stored width.
+__device__ __forceinline__ double sparse_dot_pipe(const float *vp, const int *cp, const float *xs, int len, int width) {
+  constexpr int B = ROW_PIPE;
+  double acc = 0.0;
+  float a0[B], a1[B];
+  int c0[B], c1[B];
+  auto load = [&](int p0, float(&a)[B], int(&c)[B]) {
+    if (p0 + B <= width) {
+      const float *v = vp + (long long)p0 * SLICE;
+      const int *ci = cp + (long long)p0 * SLICE;
+#pragma unroll
+      for (int u = 0; u < B; ++u) {
+        a[u] = v[u * SLICE];
+        c[u] = ci[u * SLICE];
+      }
+    } else {  // the row's last batch, clamped to its stored entries
+#pragma unroll
+      for (int u = 0; u < B; ++u) {
+        const long long q = min(p0 + u, width - 1);
+        a[u] = vp[q * SLICE];
+        c[u] = cp[q * SLICE];
+      }
+    }
+  };
+  auto sum = [&](int p0, const float(&a)[B], const int(&c)[B]) {
+#pragma unroll
+    for (int u = 0; u < B; ++u) {
+      double t = (p0 + u < len) ? (double)a[u] * (double)xs[c[u]] : 0.0;
+      acc = acc + t;
+    }
+  };
+  if (width > 0) load(0, a0, c0);
+  for (int p = 0; p < width; p += 2 * B) {
+    if (p + B < width) load(p + B, a1, c1);
+    sum(p, a0, c0);
+    if (p + B >= width) break;
+    if (p + 2 * B < width) load(p + 2 * B, a0, c0);
+    sum(p + B, a1, c1);
+  }
+  return acc;
+}
+
 template <bool PIPE = false>
 __device__ __forceinline__ double row_dot(const View &V, const float *xs, int r) {
   const int s = r / SLICE;
@@ -330,45 +374,7 @@ __device__ __forceinline__ double row_dot(const View &V, const float *xs, int r)
     const int width = (int)((V.slice_ptr[s + 1] - V.slice_ptr[s]) / SLICE);
     const int *cp = V.cols + base;
     int p = 0;
-    if (PIPE) {
-      constexpr int B = ROW_PIPE;
-      float a0[B], a1[B];
-      int c0[B], c1[B];
-      auto load = [&](int p0, float(&a)[B], int(&c)[B]) {
-        if (p0 + B <= width) {
-          const float *v = vp + (long long)p0 * SLICE;
-          const int *ci = cp + (long long)p0 * SLICE;
-#pragma unroll
-          for (int u = 0; u < B; ++u) {
-            a[u] = v[u * SLICE];
-            c[u] = ci[u * SLICE];
-          }
-        } else {  // the row's last batch, clamped to its stored entries
-#pragma unroll
-          for (int u = 0; u < B; ++u) {
-            const long long q = min(p0 + u, width - 1);
-            a[u] = vp[q * SLICE];
-            c[u] = cp[q * SLICE];
-          }
-        }
-      };
-      auto sum = [&](int p0, const float(&a)[B], const int(&c)[B]) {
-#pragma unroll
-        for (int u = 0; u < B; ++u) {
-          double t = (p0 + u < len) ? (double)a[u] * (double)xs[c[u]] : 0.0;
-          acc = acc + t;
-        }
-      };
-      if (width > 0) load(0, a0, c0);
-      for (; p < width; p += 2 * B) {
-        if (p + B < width) load(p + B, a1, c1);
-        sum(p, a0, c0);
-        if (p + B >= width) break;
-        if (p + 2 * B < width) load(p + 2 * B, a0, c0);
-        sum(p + B, a1, c1);
-      }
-      return acc;
-    }
+    if (PIPE) return sparse_dot_pipe(vp, cp, xs, len, width);
     for (; p + U <= width; p += U) {
       float a[U];
       int c[U];

@@ -220,7 +220,7 @@ __device__ __forceinline__ void compute_rows_part(const View &V, const int *list
 template <int KIND>
 __device__ __forceinline__ void spec_rows(const View &V, Dec &d, RedSmem &sm, float *xs) {
   __shared__ int list[SPEC_ROWS];
-  __shared__ int s_cnt, s_pick;
+  __shared__ int s_cnt;
   const unsigned par = (unsigned)d.epoch & 1u;
   const int t = threadIdx.x;
   double cu = -INF, cl = -INF;

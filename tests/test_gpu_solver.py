@@ -100,6 +100,23 @@ def test_csvc_sparse_fused_matches_oracle(hip, ora):
     assert dev["stats"]["optimal"] == 1
 
 
+def test_wide_column_ids_same_iterates(hip, ora):
+    """Feature ids past 65535 (sparse rows read through 32-bit columns) give
+    the same solve as the same rows with narrow ids (16-bit columns) and as
+    the oracle: the id map is increasing, so every dot keeps its merge order."""
+    ip, ix, v, y, d = datasets.zipf_binary(n=1500)
+    narrow = ffi.Problem(ip, ix, v, y, d)
+    wide_ix = ix.astype(np.int64) * 1000 + 7
+    wide = ffi.Problem(ip, wide_ix, v, y, int(wide_ix.max()) + 1)
+    prm = ffi.make_param(gamma=1.0 / d, C_=1.0, eps=1e-3, shrinking=0)
+    ref = ora.train_one(wide, prm)
+    dn = hip.solve_one(narrow, prm)
+    dw = hip.solve_one(wide, prm)
+    assert_solution_close(dw, ref)
+    assert dn["iters"] == dw["iters"]
+    np.testing.assert_array_equal(dn["alpha"], dw["alpha"])
+
+
 def test_csvc_step_api_trace_matches_oracle(hip, ora):
     """The host-driven CudaSolver-style interface picks the same (i, j)."""
     prob, d = zipf_problem(800)
