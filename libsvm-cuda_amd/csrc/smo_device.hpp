@@ -1154,7 +1154,7 @@ struct XPart {
 
 using XWord = unsigned long long;
 constexpr int XW = 23;    // words of the largest record format
-constexpr int XREP = 8;   // copies of the buffers
+constexpr int XREP = 4;   // copies of the buffers (n=100k, 256 owners: 1: 242, 2: 236, 4: 235.5, 8: 240, 16: 249 ms)
 
 // Exchange words of one copy of the two buffers of nb owners, and of all copies.
 __host__ __device__ constexpr size_t xcopy_words(int nb) { return 2 * (size_t)XW * (size_t)(nb > 0 ? nb : 0); }
