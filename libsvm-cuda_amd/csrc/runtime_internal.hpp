@@ -42,6 +42,7 @@ hipError_t smo_launch_pack_rowset(const smo::View *src, const int *rows, int n, 
 namespace svmhip_detail {
 
 constexpr size_t kLdsQueryLimit = 64 * 1024;  // query row staged in LDS up to this size
+constexpr size_t kFlagLdsLimit = 64 * 1024;  // persistent kernel: query row + row-flag bitset in LDS
 
 struct Failure {
   svmhip_status code;

@@ -110,6 +110,9 @@ struct View {
                           // (nblocks copies, pm_stride apart) are the subproblem's own
   int keep_cache;         // batched solves: the own cache is valid from an earlier stage
   int spec;               // persistent C-rule solves, full cache: speculative rows on an i miss
+  int flag_lds;           // persistent solves, full cache: > 0 = the workgroup's row flags live as
+                          // a bitset in dynamic LDS from float offset flag_lds (>= 4, past the
+                          // query row) during the launch; 0 = in pm_flags only
   unsigned *drain;        // batched solves: workgroups of the launch that found the queue empty
   int drain_at;           // batched solves: pause once drain reaches this (checked every 32 iterations)
 };
@@ -554,6 +557,7 @@ __device__ __forceinline__ void block_reduce(RedSmem &sm, double (&v)[4], int (&
 // used by the persistent kernel.
 struct CacheMeta {
   int8_t *computed;                 // full cache: row already materialised
+  unsigned *bits;                   // full cache, flags in LDS: bit r of word r / 32 (else null)
   int *slot_of_row, *row_of_slot, *prev, *next;
   int *head, *tail;
   unsigned long long *hits, *misses;
@@ -576,6 +580,13 @@ __device__ __forceinline__ void lru_touch(CacheMeta &M, int s) {
 // than `exclude` is reassigned (the row is then computed by every workgroup
 // for its owned range in the next pass).  Single thread.
 __device__ __forceinline__ int cache_lookup(const View &V, CacheMeta &M, int row, int exclude, int *miss) {
+  if (V.full && M.bits) {
+    const unsigned w = M.bits[row >> 5], b = 1u << (row & 31);
+    *miss = (w & b) ? 0 : 1;
+    M.bits[row >> 5] = w | b;
+    if (*miss) ++*M.misses; else ++*M.hits;
+    return row;
+  }
   if (V.full) {
     *miss = !M.computed[row];
     M.computed[row] = 1;
@@ -603,6 +614,7 @@ __device__ __forceinline__ int cache_lookup(const View &V, CacheMeta &M, int row
 
 __device__ __forceinline__ CacheMeta shared_meta(const View &V) {
   CacheMeta M;
+  M.bits = nullptr;
   M.computed = V.computed;
   M.slot_of_row = V.slot_of_row; M.row_of_slot = V.row_of_slot; M.prev = V.lru_prev; M.next = V.lru_next;
   M.head = &V.ctl->lru_head; M.tail = &V.ctl->lru_tail;
@@ -1387,10 +1399,15 @@ __device__ __forceinline__ int pause_vote(const View &V, const Dec &d) {
   return 0;
 }
 
+// Dynamic LDS of the launch (every extern __shared__ array starts here).
+extern __shared__ float smo_dyn_lds[];
+
 __device__ __forceinline__ CacheMeta private_meta(const View &V, Dec &d) {
   CacheMeta M;
+  M.bits = nullptr;
   if (V.full) {
     M.computed = V.pm_flags + pm_index(V) * V.l;
+    if (V.flag_lds > 0) M.bits = reinterpret_cast<unsigned *>(smo_dyn_lds + V.flag_lds);
   } else {
     int *base = V.pm_lru + pm_index(V) * V.pm_stride;
     M.slot_of_row = base;

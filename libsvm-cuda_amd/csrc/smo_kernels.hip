@@ -356,9 +356,24 @@ __global__ void __launch_bounds__(BLOCK) k_smo(View V, int iter_limit) {
   extern __shared__ float xs_lds[];
   float *xs = V.use_lds ? xs_lds : V.xs_global + (long long)blockIdx.x * V.d;
   if (threadIdx.x == 0) d = ctl_dec(V.ctl);
+  // full cache: this workgroup's row flags as a bitset in LDS for the launch
+  // (the lookups after every exchange are then LDS reads, not memory round
+  // trips), from and back to its flag bytes
+  unsigned *fbits = reinterpret_cast<unsigned *>(smo_dyn_lds + V.flag_lds);  // as private_meta
+  int8_t *fbytes = V.pm_flags + pm_index(V) * V.l;
+  if (V.flag_lds > 0) {
+    for (int w = threadIdx.x; w < (V.l + 31) / 32; w += blockDim.x) fbits[w] = 0u;
+    __syncthreads();
+    for (int r = threadIdx.x; r < V.l; r += blockDim.x)
+      if (fbytes[r]) atomicOr(&fbits[r >> 5], 1u << (r & 31));
+  }
   __syncthreads();
   if (d.phase == 0 && !d.stop && !d.err) smo_select_first<KIND, true>(V, d, sm, xsm, xs);
   smo_iterations<KIND, true, true>(V, d, sm, xsm, xs, iter_limit);
+  if (V.flag_lds > 0) {
+    __syncthreads();
+    for (int r = threadIdx.x; r < V.l; r += blockDim.x) fbytes[r] = (int8_t)((fbits[r >> 5] >> (r & 31)) & 1u);
+  }
   if (blockIdx.x == 0 && threadIdx.x == 0) {
     Ctl *c = V.ctl;
     ctl_dec(c) = d;

@@ -459,6 +459,19 @@ svmhip_status svmhip_solve(svmhip_solver *h, int max_iter, svmhip_stats *stats) 
       V.spec = 1;
       shmem = 8 * (size_t)h->d * sizeof(float);
     }
+    // full cache: each workgroup's row flags as an LDS bitset past the query
+    // row for the launch (the lookup after every exchange is on the
+    // iteration's critical path), while the grid stays co-resident
+    if (h->full && !V.spec) {
+      const size_t off = std::max<size_t>(4, (shmem / sizeof(float) + 3) / 4 * 4);
+      const size_t need = off * sizeof(float) + (size_t)((h->l + 31) / 32) * sizeof(unsigned);
+      int per_cu = 0;
+      if (need <= kFlagLdsLimit && smo_persistent_capacity((int)h->kind, need, &per_cu) == hipSuccess &&
+          per_cu * device_cus(h->dev) >= h->nblocks) {
+        V.flag_lds = (int)off;
+        shmem = need;
+      }
+    }
     int chunk = h->opt.iters_per_launch > 0 ? h->opt.iters_per_launch : INT_MAX;
     h->ck(hipEventRecord(h->ev[0], h->st), "event");
     if (h->nvar > 0) {
