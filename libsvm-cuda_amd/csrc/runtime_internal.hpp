@@ -223,6 +223,14 @@ inline size_t batch_lds_bytes(int maxd) {
   return std::max(one, std::min<size_t>(16 * one, 64 * 1024));
 }
 
+// Words of the LDS row-flag bitset a batched launch adds past
+// batch_lds_bytes(maxd) for subproblems of up to maxn rows (0 = none: the
+// launch's dynamic LDS stays within 64 KiB).
+inline int batch_flag_words(int maxd, long long maxn) {
+  const long long w = (std::max<long long>(maxn, 1) + 31) / 32;
+  return batch_lds_bytes(maxd) + (size_t)w * sizeof(unsigned) <= 64 * 1024 ? (int)w : 0;
+}
+
 // Device wall-clock ticks per second (wall_clock64 in kernels).
 inline double device_clock_hz(int dev) {
   int khz = 0;

@@ -712,9 +712,14 @@ svmhip_status svmhip_server_step(svmhip_server *s, double stage_ms, int *n_activ
     // first (as many as fit in the grid at once), then single workgroups,
     // oldest first
     if (!s->active.empty()) {
-      int maxd = 1;
-      for (int id : s->active) maxd = std::max(maxd, s->reqs[id]->rows.d);
-      const size_t shmem = batch_lds_bytes(maxd);
+      int maxd = 1, maxl = 1;
+      for (int id : s->active) {
+        maxd = std::max(maxd, s->reqs[id]->rows.d);
+        maxl = std::max(maxl, s->reqs[id]->rows.l);
+      }
+      const size_t shmem0 = batch_lds_bytes(maxd);
+      const int flag_words = batch_flag_words(maxd, maxl);
+      const size_t shmem = shmem0 + (size_t)flag_words * sizeof(unsigned);
       int per_cu = 0;
       h->ck(smo_batch_capacity(shmem, &per_cu), "occupancy query");
       const int G = std::max(1, per_cu) * device_cus(h->dev);
@@ -818,7 +823,8 @@ svmhip_status svmhip_server_step(svmhip_server *s, double stage_ms, int *n_activ
       pool.K = 0;
       const int nitems = (int)items.size() / 2;
       h->ck(hipEventRecord(h->ev[0], h->st), "event");
-      pool.lds_floats = (int)(shmem / sizeof(float));
+      pool.lds_floats = (int)(shmem0 / sizeof(float));
+      pool.flag_words = flag_words;
       pool.spec = h->opt.speculative_rows >= 0 ? 1 : 0;
       h->ck(smo_launch_batch(s->dV, s->dItems, nitems, std::min(G, nitems), &pool, shmem, h->st),
             "batched SMO kernel");

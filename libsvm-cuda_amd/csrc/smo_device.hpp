@@ -145,6 +145,8 @@ struct BatchPool {
   unsigned long long *t0;  // launch start on the device clock (zeroed before each launch; null = none)
   long long deadline_ticks;  // > 0: items pause at t0 + deadline_ticks, and single-workgroup items
                              // claimed after it are left for the next launch (their Ctl stays clear)
+  int flag_words;          // > 0: single-workgroup full-cache items of up to 32 x flag_words rows keep
+                           // their row flags as an LDS bitset of that many words past lds_floats
 };
 
 // ---------------------------------------------------------------- helpers
@@ -579,17 +581,21 @@ __device__ __forceinline__ void lru_touch(CacheMeta &M, int s) {
 // Slot holding kernel row `row`; on a miss the least recently used slot other
 // than `exclude` is reassigned (the row is then computed by every workgroup
 // for its owned range in the next pass).  Single thread.
-__device__ __forceinline__ int cache_lookup(const View &V, CacheMeta &M, int row, int exclude, int *miss) {
-  if (V.full && M.bits) {
+// Full cache: sets row's flag; true if it was already set.
+__device__ __forceinline__ bool flag_test_set(CacheMeta &M, int row) {
+  if (M.bits) {
     const unsigned w = M.bits[row >> 5], b = 1u << (row & 31);
-    *miss = (w & b) ? 0 : 1;
     M.bits[row >> 5] = w | b;
-    if (*miss) ++*M.misses; else ++*M.hits;
-    return row;
+    return (w & b) != 0;
   }
+  const bool was = M.computed[row] != 0;
+  M.computed[row] = 1;
+  return was;
+}
+
+__device__ __forceinline__ int cache_lookup(const View &V, CacheMeta &M, int row, int exclude, int *miss) {
   if (V.full) {
-    *miss = !M.computed[row];
-    M.computed[row] = 1;
+    *miss = flag_test_set(M, row) ? 0 : 1;
     if (*miss) ++*M.misses; else ++*M.hits;
     return row;
   }

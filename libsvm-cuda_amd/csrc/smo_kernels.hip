@@ -288,8 +288,7 @@ __device__ __forceinline__ void spec_rows(const View &V, Dec &d, RedSmem &sm, fl
       const int row = real_of(V, picks[k]);
       bool dup = false;
       for (int m = 0; m < cnt; ++m) dup |= list[m] == row;
-      if (dup || M.computed[row]) continue;
-      M.computed[row] = 1;
+      if (dup || flag_test_set(M, row)) continue;
       ++d.misses;
       list[cnt++] = row;
     }
@@ -727,8 +726,7 @@ __device__ __forceinline__ void solo_spec_rows(const View &V, Dec &d, RedSmem &s
       if (bw < 0) break;
       const int row = real_of(V, sm.i[0][bw][bk]);
       sm.i[0][bw][bk] = -1;  // taken (the buffer is rewritten by the next pass)
-      if (M.computed[row]) continue;
-      M.computed[row] = 1;
+      if (flag_test_set(M, row)) continue;
       ++d.misses;
       list[cnt++] = row;
     }
@@ -747,6 +745,18 @@ __device__ __forceinline__ void solo_spec_rows(const View &V, Dec &d, RedSmem &s
 template <int KIND>
 __device__ __forceinline__ void solo_solve(View &V, Dec &d, RedSmem &sm, float *xs, int lds_floats, long long t0) {
   solo_init_cache(V);
+  // full cache in LDS: the row flags as a bitset for this item (the lookup
+  // after every selection is then no memory round trip), from and back to
+  // the arena's flag bytes
+  unsigned *fbits = reinterpret_cast<unsigned *>(smo_dyn_lds + V.flag_lds);  // as private_meta
+  if (V.flag_lds > 0) {
+    for (int w = threadIdx.x; w < (V.l + 31) / 32; w += blockDim.x) fbits[w] = 0u;
+    __syncthreads();
+    if (V.keep_cache)
+      for (int r = threadIdx.x; r < V.l; r += blockDim.x)
+        if (V.pm_flags[r]) atomicOr(&fbits[r >> 5], 1u << (r & 31));
+    __syncthreads();
+  }
   if (!V.resume) {
     if (V.ginit_from == 0) batch_qd(V);
     batch_ginit(V, xs, lds_floats, 0, V.l, V.ginit_from, V.ginit_to);
@@ -789,6 +799,8 @@ __device__ __forceinline__ void solo_solve(View &V, Dec &d, RedSmem &sm, float *
     }
     __syncthreads();
   }
+  if (V.flag_lds > 0)
+    for (int r = threadIdx.x; r < V.l; r += blockDim.x) V.pm_flags[r] = (int8_t)((fbits[r >> 5] >> (r & 31)) & 1u);
   double f[7];
   int n[2];
   final_terms<KIND>(V, 0, V.l, f, n);
@@ -940,6 +952,7 @@ __global__ void __launch_bounds__(BLOCK) k_smo_batch(const View *views, const in
       V.rank = 0;
       V.spec = pool.spec && V.kind == KIND_C && V.full && V.ktype != K_PRECOMPUTED &&
                pool.lds_floats >= 2 * V.d ? 1 : 0;
+      V.flag_lds = V.full && pool.flag_words > 0 && (V.l + 31) / 32 <= pool.flag_words ? pool.lds_floats : 0;
       if (V.kind == KIND_C) solo_solve<KIND_C>(V, d, sm, xs, pool.lds_floats, t0);
       else solo_solve<KIND_NU>(V, d, sm, xs, pool.lds_floats, t0);
     }

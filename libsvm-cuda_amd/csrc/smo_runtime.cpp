@@ -707,7 +707,9 @@ svmhip_status svmhip_batch_solve(svmhip_batch *b, svmhip_subproblem *subs, int n
     h->upload(dY, hY.data(), nv_total, "upload batch y");
     h->upload(dS, hS.data(), nv_total, "upload batch status");
     // one cache arena per resident workgroup
-    const size_t lds = batch_lds_bytes(h->d);
+    const size_t lds0 = batch_lds_bytes(h->d);
+    const int flag_words = batch_flag_words(h->d, maxn);
+    const size_t lds = lds0 + (size_t)flag_words * sizeof(unsigned);
     int per_cu = 0;
     h->ck(smo_batch_capacity(lds, &per_cu), "occupancy query");
     const int G = std::max(1, per_cu) * device_cus(h->dev);
@@ -736,7 +738,8 @@ svmhip_status svmhip_batch_solve(svmhip_batch *b, svmhip_subproblem *subs, int n
     pool.arena_next = (int *)talloc(sizeof(int), "batch arena counter");
     pool.t0 = (unsigned long long *)talloc(sizeof(unsigned long long), "batch launch start");
     unsigned *dIdle = (unsigned *)talloc(sizeof(unsigned), "batch idle count");
-    pool.lds_floats = (int)(lds / sizeof(float));
+    pool.lds_floats = (int)(lds0 / sizeof(float));
+    pool.flag_words = flag_words;
     pool.spec = h->opt.speculative_rows >= 0 ? 1 : 0;
     smo::XWord *dX = (smo::XWord *)talloc(sizeof(smo::XWord) * smo::xrec_words(G), "team exchange records");
     // items: one per subproblem plus the extra members of teams (at most G)
