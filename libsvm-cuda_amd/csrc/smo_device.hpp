@@ -565,10 +565,10 @@ struct CacheMeta {
   unsigned long long *hits, *misses;
 };
 
-__device__ __forceinline__ void lru_touch(CacheMeta &M, int s) {
-  int head = *M.head;
+// Moves slot s to the front of the LRU list; head, pv and nx are the list's
+// head and s's neighbours as they are before the move.
+__device__ __forceinline__ void lru_touch(CacheMeta &M, int s, int head, int pv, int nx) {
   if (head == s) return;
-  int pv = M.prev[s], nx = M.next[s];
   if (pv >= 0) M.next[pv] = nx;
   if (nx >= 0) M.prev[nx] = pv;
   else *M.tail = pv;
@@ -599,20 +599,25 @@ __device__ __forceinline__ int cache_lookup(const View &V, CacheMeta &M, int row
     if (*miss) ++*M.misses; else ++*M.hits;
     return row;
   }
-  int s = M.slot_of_row[row];
+  // the loads that do not depend on one another are issued together (the
+  // lookup runs on the iteration's critical path, one thread, metadata in
+  // device memory): head and tail with the row's slot, then the slot's
+  // neighbours (and, on a miss, its old row) in one more round trip
+  const int s = M.slot_of_row[row];
+  const int head = *M.head, tail = *M.tail;
   if (s >= 0) {
     *miss = 0;
     ++*M.hits;
-    lru_touch(M, s);
+    lru_touch(M, s, head, M.prev[s], M.next[s]);
     return s;
   }
-  int v = *M.tail;
+  int v = tail;
   if (v == exclude) v = M.prev[v];
-  int old = M.row_of_slot[v];
+  const int old = M.row_of_slot[v], pv = M.prev[v], nx = M.next[v];
   if (old >= 0) M.slot_of_row[old] = -1;
   M.row_of_slot[v] = row;
   M.slot_of_row[row] = v;
-  lru_touch(M, v);
+  lru_touch(M, v, head, pv, nx);
   *M.misses += 1;
   *miss = 1;
   return v;
