@@ -732,6 +732,7 @@ __device__ __forceinline__ Snap snap(const Ctl *c) {
 struct Pend {
   int i, j;
   double ai, aj;
+  int si, sj;  // their alpha_status (decided with the step, so the owner loads no label)
 };
 
 // Slices a wave visits per trip of the pass loops, and (for rows that are
@@ -753,8 +754,8 @@ __device__ __forceinline__ Partial pass_gradient_select_nv(const View &V, const 
   owned_slices(V, my_owner(V), V.nblocks, sb, se);
   if (pend) {
     if (threadIdx.x == 0) {
-      if (owns(V, sb, se, pend->i)) { V.alpha[pend->i] = pend->ai; V.status[pend->i] = status_for(V, pend->i, pend->ai); }
-      if (owns(V, sb, se, pend->j)) { V.alpha[pend->j] = pend->aj; V.status[pend->j] = status_for(V, pend->j, pend->aj); }
+      if (owns(V, sb, se, pend->i)) { V.alpha[pend->i] = pend->ai; V.status[pend->i] = (int8_t)pend->si; }
+      if (owns(V, sb, se, pend->j)) { V.alpha[pend->j] = pend->aj; V.status[pend->j] = (int8_t)pend->sj; }
     }
     __syncthreads();
   }
@@ -1315,7 +1316,7 @@ struct Dec {
   int i, j, ip, in;
   int slot_i, slot_j, slot_ip, slot_in;
   int miss_i, miss_j, miss_ip, miss_in;
-  int pend, err, pad0, pad1;
+  int pend, err, st_i, st_j;   // st_i, st_j: alpha_status of i and j after the pending step
   double ai_new, aj_new, dai, daj;
   double Gmax, Gmax2, Gp, Gn, Gp2, Gn2;
   double Gi, ai, Gip, aip, Gin, ain;
@@ -1500,6 +1501,10 @@ __device__ __forceinline__ void decide_j(const View &V, Dec &d, const XPart &r) 
   d.daj = aj2 - r.a0;
   d.ai_new = ai2;
   d.aj_new = aj2;
+  // Solver::update_alpha_status (libsvm/svm.cpp:425-432) with the labels at hand
+  const double Ci = yi > 0 ? V.Cp : V.Cn, Cj = yj > 0 ? V.Cp : V.Cn;
+  d.st_i = ai2 >= Ci ? AT_UPPER : (ai2 <= 0 ? AT_LOWER : IS_FREE);
+  d.st_j = aj2 >= Cj ? AT_UPPER : (aj2 <= 0 ? AT_LOWER : IS_FREE);
   d.pend = 1;
 }
 

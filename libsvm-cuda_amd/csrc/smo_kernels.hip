@@ -328,7 +328,7 @@ __device__ __forceinline__ void smo_iterations(const View &V, Dec &d, RedSmem &s
     if (d.stop || d.err) break;
     c = snap_of(d);
     Pend pd;
-    pd.i = d.i; pd.j = d.j; pd.ai = d.ai_new; pd.aj = d.aj_new;
+    pd.i = d.i; pd.j = d.j; pd.ai = d.ai_new; pd.aj = d.aj_new; pd.si = d.st_i; pd.sj = d.st_j;
     Partial bp2 = pass_gradient_select<KIND, PASS_SLICES, PIPE>(V, c, xs, sm, true, true, false, &pd);
     XPart mine2 = make_i_record<KIND>(V, bp2);
     mine2.pad = pause_vote(V, d);
@@ -790,7 +790,7 @@ __device__ __forceinline__ void solo_solve(View &V, Dec &d, RedSmem &sm, float *
     if (KIND == KIND_C && V.spec && d.miss_j) solo_spec_rows(V, d, sm, xs, lds_floats, true);
     c = snap_of(d);
     Pend pd;
-    pd.i = d.i; pd.j = d.j; pd.ai = d.ai_new; pd.aj = d.aj_new;
+    pd.i = d.i; pd.j = d.j; pd.ai = d.ai_new; pd.aj = d.aj_new; pd.si = d.st_i; pd.sj = d.st_j;
     Partial bp2 = pass_gradient_select<KIND, PASS_SLICES_SOLO>(V, c, xs, sm, true, true, false, &pd);
     XPart m2 = make_i_record<KIND>(V, bp2);
     if (threadIdx.x == 0) {
