@@ -10,7 +10,9 @@
 
 #include <algorithm>
 #include <climits>
+#include <map>
 #include <string>
+#include <unordered_map>
 #include <vector>
 
 #include "../../include/svm_hip.h"
@@ -110,21 +112,71 @@ struct svmhip_solver {
            hipGetErrorString(e));
     }
   }
+  // Freed blocks kept for reuse, by size class, up to pool_cap bytes (the
+  // server turns it on: its requests come and go between two stages, where
+  // hipMalloc / hipFree would leave the device idle).  A block is reused
+  // only by a later dalloc, whose uploads and kernels follow on the same
+  // stream every use of the block's previous owner.
+  size_t pool_cap = 0, pooled = 0;
+  std::multimap<size_t, void *> pool;
+  std::unordered_map<void *, size_t> block_class;
+  static size_t size_class(size_t b) {
+    if (b <= 65536) return (b + 255) / 256 * 256;
+    size_t p2 = 65536;
+    while (p2 < b) p2 <<= 1;
+    const size_t step = p2 / 16;
+    return (b + step - 1) / step * step;
+  }
+  void release_pool() {
+    for (auto &kv : pool) {
+      block_class.erase(kv.second);
+      auto it = std::find(allocs.begin(), allocs.end(), kv.second);
+      if (it != allocs.end()) allocs.erase(it);
+      (void)hipFree(kv.second);
+    }
+    pool.clear();
+    pooled = 0;
+  }
   template <class T>
   T *dalloc(size_t n, const char *what) {
     void *p = nullptr;
     size_t bytes = std::max<size_t>(n, 1) * sizeof(T);
+    if (pool_cap > 0) {
+      bytes = size_class(bytes);
+      auto it = pool.find(bytes);
+      if (it != pool.end()) {
+        p = it->second;
+        pool.erase(it);
+        pooled -= bytes;
+        return (T *)p;
+      }
+    }
     hipError_t e = hipMalloc(&p, bytes);
+    if (e != hipSuccess && pooled > 0) {  // hand the pooled blocks back and try again
+      (void)hipGetLastError();
+      release_pool();
+      e = hipMalloc(&p, bytes);
+    }
     if (e != hipSuccess) {
       (void)hipGetLastError();
       fail(SVMHIP_ERR_NOMEM, "device allocation of %zu bytes for %s failed: %s", bytes, what,
            hipGetErrorString(e));
     }
     allocs.push_back(p);
+    if (pool_cap > 0) block_class[p] = bytes;
     return (T *)p;
   }
   void dfree(void *p) {
     if (!p) return;
+    auto bc = block_class.find(p);
+    if (bc != block_class.end()) {
+      if (pooled + bc->second <= pool_cap) {
+        pool.emplace(bc->second, p);
+        pooled += bc->second;
+        return;
+      }
+      block_class.erase(bc);
+    }
     auto it = std::find(allocs.begin(), allocs.end(), p);
     if (it != allocs.end()) allocs.erase(it);
     (void)hipFree(p);

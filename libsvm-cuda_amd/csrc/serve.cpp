@@ -488,6 +488,8 @@ svmhip_status svmhip_server_create(const svmhip_options *opt, double hbm_budget_
     size_t freeb = 0, totb = 0;
     h->ck(hipMemGetInfo(&freeb, &totb), "hipMemGetInfo");
     s->budget = hbm_budget_bytes > 0 ? hbm_budget_bytes : 0.8 * (double)freeb;
+    // freed request buffers are kept for reuse, up to 2 GiB or a tenth of the budget
+    h->pool_cap = (size_t)std::min(2.0 * 1024 * 1024 * 1024, 0.1 * s->budget);
     if (h->opt.grid_blocks > 0) s->team_max = std::min(h->opt.grid_blocks, 64);
     if (h->opt.cache_bytes > 0) s->cache_cap = h->opt.cache_bytes;
     s->dQueue = h->dalloc<int>(1, "server queue counter");
