@@ -278,7 +278,9 @@ svmhip_status svmhip_server_predict_result(svmhip_server *s, int id, double *dec
 /* Frees a finished request (for a train request: its resident model). */
 svmhip_status svmhip_server_release(svmhip_server *s, int id);
 
-/* Device bytes held by admitted requests and resident models. */
+/* Device bytes held by admitted requests and resident models (freed request
+   buffers kept for reuse, up to 2 GiB or a tenth of the budget, are not
+   counted). */
 double svmhip_server_bytes_in_use(const svmhip_server *s);
 
 /* Totals since creation: iters (SMO iterations), row_hits / row_misses,
