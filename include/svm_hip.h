@@ -51,7 +51,8 @@ typedef enum { SVMHIP_KIND_C = 0, SVMHIP_KIND_NU = 1 } svmhip_kind;
 typedef struct {
 	size_t struct_size;       /* sizeof(svmhip_options) */
 	int device;               /* HIP device ordinal (default 0) */
-	int grid_blocks;          /* persistent-grid workgroups (0 = autotune) */
+	int grid_blocks;          /* persistent-grid workgroups (0 = the size rule: one per CU on large
+				     problems, fewer on mid-size ones) */
 	int block_threads;        /* threads per workgroup (0 = 512, the only size built) */
 	int max_iter;             /* 0 = LIBSVM rule max(10^7, 100*l) */
 	double cache_bytes;       /* kernel-row cache bytes (0 = param.cache_size MB) */
