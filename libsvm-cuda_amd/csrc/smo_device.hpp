@@ -1361,16 +1361,21 @@ __device__ __forceinline__ XPart exchange(const View &V, RedSmem &sm, XSmem &xs,
   const int t = threadIdx.x;
   if (t < V.nblocks && !xgather<F>(V.xw, V.nblocks, t, seq, q)) xs.err = 1;
   // i records carry each owner's pause vote: every owner sees the same OR
-  const int pause = PH == PH_I ? __syncthreads_or(t < V.nblocks ? q.pad : 0) : 0;
+  // (C rule: the max of reduction slot 2, which its i records leave unused,
+  // so no barrier of its own; nu rule: one barrier)
+  int pause = 0;
+  if (PH == PH_I && KIND != KIND_C) pause = __syncthreads_or(t < V.nblocks ? q.pad : 0);
   double v[4] = {q.v0, q.v1, q.v2, q.v3};
   int ix[2] = {q.i0, q.i1};
   if (PH == PH_J) {
     const int kinds[4] = {1, 2, 2, 2};
     block_reduce<1>(sm, v, ix, kinds, 1);
   } else if (KIND == KIND_C) {
+    v[2] = (t < V.nblocks && q.pad) ? 1.0 : 0.0;
     const int kinds_spec[4] = {0, 0, 2, 2}, kinds[4] = {0, 2, 2, 2};
-    if (V.spec) block_reduce<2>(sm, v, ix, kinds_spec, 1);
-    else block_reduce<2>(sm, v, ix, kinds, 1);
+    if (V.spec) block_reduce<3>(sm, v, ix, kinds_spec, 1);
+    else block_reduce<3>(sm, v, ix, kinds, 1);
+    pause = v[2] > 0.0 ? 1 : 0;
   } else {
     const int kinds[4] = {0, 0, 2, 2};
     block_reduce<4>(sm, v, ix, kinds, 1);
