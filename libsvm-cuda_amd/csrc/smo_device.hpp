@@ -757,7 +757,9 @@ __device__ __forceinline__ Partial pass_gradient_select_nv(const View &V, const 
       if (owns(V, sb, se, pend->i)) { V.alpha[pend->i] = pend->ai; V.status[pend->i] = (int8_t)pend->si; }
       if (owns(V, sb, se, pend->j)) { V.alpha[pend->j] = pend->aj; V.status[pend->j] = (int8_t)pend->sj; }
     }
-    __syncthreads();
+    // PIPE: this pass takes the two statuses from pend (below) rather than
+    // waiting for the stores; later passes read them after other barriers
+    if (!PIPE) __syncthreads();
   }
   if (update && c.miss_j) compute_row_part<PIPE>(V, real_of(V, c.j), c.slot_j, xs, sb, se);
   double v[4] = {-INF, -INF, -INF, -INF};
@@ -812,7 +814,11 @@ __device__ __forceinline__ Partial pass_gradient_select_nv(const View &V, const 
           V.G[k] = gk;
         }
         if (!select) continue;
-        const int8_t sk = st[u][h];
+        int8_t sk = st[u][h];
+        if (PIPE && pend) {
+          if (k == pend->i) sk = (int8_t)pend->si;
+          else if (k == pend->j) sk = (int8_t)pend->sj;
+        }
         if (KIND == KIND_C) {
           if (yk[u][h] == +1) {
             if (sk != AT_UPPER) max_idx(v[0], ix[0], -gk, k);
