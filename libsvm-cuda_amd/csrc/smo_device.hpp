@@ -1356,8 +1356,15 @@ struct XSmem {
 // Publish this workgroup's record and gather everyone's: returns, in every
 // thread, the combined record (reduced values; winners' payloads in g0/a0/q0
 // and, for the nu rule's second argmax, g1/a1).
+//
+// hold (one winner: the j exchange, or the i exchange of the C rule): the
+// result is complete only in the thread that gathered the winning record
+// (thread 0 when there is no winner), and *hold is set in that thread alone;
+// the caller decides there, which spares the barrier of handing the payload
+// over through LDS.
 template <int KIND, int PH>
-__device__ __forceinline__ XPart exchange(const View &V, RedSmem &sm, XSmem &xs, const XPart &mine, unsigned seq) {
+__device__ __forceinline__ XPart exchange(const View &V, RedSmem &sm, XSmem &xs, const XPart &mine, unsigned seq,
+                                          bool *hold = nullptr) {
   constexpr int F = PH == PH_J ? XF_J : (KIND == KIND_C ? XF_IC : XF_INU);
   if (threadIdx.x == 0) {
     xpublish<F>(V.xw, V.nblocks, my_owner(V), mine, seq);
@@ -1385,6 +1392,17 @@ __device__ __forceinline__ XPart exchange(const View &V, RedSmem &sm, XSmem &xs,
   } else {
     const int kinds[4] = {0, 0, 2, 2};
     block_reduce<4>(sm, v, ix, kinds, 1);
+  }
+  if (hold && (PH == PH_J || KIND == KIND_C)) {
+    const bool mine_wins = ix[0] >= 0 ? (t < V.nblocks && q.i0 == ix[0]) : (t == 0);
+    XPart r = xpart_identity<KIND, PH>();
+    r.v0 = v[0]; r.v1 = v[1]; r.v2 = v[2]; r.v3 = v[3];
+    r.i0 = ix[0]; r.i1 = ix[1];
+    if (mine_wins && ix[0] >= 0) { r.g0 = q.g0; r.a0 = q.a0; r.q0 = q.q0; r.qd0 = q.qd0; r.y0 = q.y0; }
+    r.seq = xs.err;  // set before the reduction's barrier
+    r.pad = pause;
+    *hold = mine_wins;
+    return r;
   }
   // every thread holds the reduced values; the record holding each winning
   // index supplies its payload

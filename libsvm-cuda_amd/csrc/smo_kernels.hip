@@ -318,8 +318,10 @@ __device__ __forceinline__ void smo_iterations(const View &V, Dec &d, RedSmem &s
     Partial bp = pass_select_j<KIND, PASS_SLICES, PIPE>(V, c, xs, sm, false);
     XPart mine = make_j_record<KIND>(V, c, bp);
     unsigned seq = (unsigned)d.epoch + 1u;
-    XPart r = exchange<KIND, PH_J>(V, sm, xsm, mine, seq);
-    if (threadIdx.x == 0) {
+    // PIPE (k_smo): the thread holding the winning record decides
+    bool hold = threadIdx.x == 0;
+    XPart r = exchange<KIND, PH_J>(V, sm, xsm, mine, seq, PIPE ? &hold : nullptr);
+    if (hold) {
       d.epoch = (int)seq;
       if (r.seq) d.err = 1;
       else decide_j<KIND>(V, d, r);
@@ -333,8 +335,9 @@ __device__ __forceinline__ void smo_iterations(const View &V, Dec &d, RedSmem &s
     XPart mine2 = make_i_record<KIND>(V, bp2);
     mine2.pad = pause_vote(V, d);
     seq = (unsigned)d.epoch + 1u;
-    XPart r2 = exchange<KIND, PH_I>(V, sm, xsm, mine2, seq);
-    if (threadIdx.x == 0) {
+    bool hold2 = threadIdx.x == 0;
+    XPart r2 = exchange<KIND, PH_I>(V, sm, xsm, mine2, seq, PIPE && KIND == KIND_C ? &hold2 : nullptr);
+    if (hold2) {
       d.epoch = (int)seq;
       d.pend = 0;
       if (r2.seq) d.err = 1;
