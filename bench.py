@@ -13,17 +13,25 @@ Also reported on the same JSON line:
   roofline     the persistent SMO kernel (k_smo): algorithmic bytes per launch
                (iters * 44*l + rows_computed * (4*l*d + 8*l)) / launch time
                measured with HIP events on the solver's stream, vs 8 TB/s
-  cpu_baseline the FP64 CPU oracle (LIBSVM 3.18 restatement) on one host core,
-               on the first iterations of the same solve (bounded sample)
-  cpu_baseline_parallel  (ovo_cv_grid and serving only) the oracle as
-               min(16, host CPUs) independent processes over the same solves
+  cpu_baseline the FP64 CPU oracle (LIBSVM 3.18 restatement) on the host: one
+               full training solve of the same problem (large row cache,
+               missing kernel rows filled by all the cores it states)
+  config.train_time_with_upload_s  svm_train's wall clock for the same solve
+               from host rows: upload and SELL packing, cache allocation, the
+               solve and the copy-out
+  legs         one measurement of each other configuration in the same run:
+               zipf_2000 (the CPU reference path's configuration, GPU and
+               one-core oracle full solves), blob_csr_60k (the plain sparse
+               configuration with a 4 GB row cache), ovo_cv_grid (900 solves of
+               one-vs-one x 5-fold CV x 4 gammas in batched launches) and
+               serving (a Poisson stream at 60 req/s: p50 / p99 latency,
+               solves/s, results identical alone and in the mix)
 
-Other workloads (--workload): blob_csr (60,000 x 784 CSR, gamma=0.02, C=10,
-4 GB row cache: the plain sparse configuration), ovo_cv_grid (900 solves of
-one-vs-one x 5-fold CV x 4 gammas in batched launches) and serving (a Poisson
-stream of train / predict requests).
+Other workloads (--workload) run one configuration alone with its own
+steps: blob_csr, ovo_cv_grid and serving (cpu_baseline_parallel: the
+oracle as min(16, host CPUs) independent processes over the same solves).
 
-Usage: python bench.py [--gpus 1] [--steps K] [--warmup W] [--n N] [--no-cpu]
+Usage: python bench.py [--gpus 1] [--steps K] [--warmup W] [--no-cpu] [--no-legs]
 """
 import argparse
 import json
@@ -100,24 +108,43 @@ def cpu_parallel(make_task, count, seconds):
     return P, done, iters, dt
 
 
-def cpu_baseline(prob, prm, seconds):
-    """Oracle on one core over the first iterations of the same solve."""
+def _oracle():
     sys.path.insert(0, os.path.join(ROOT, "oracle"))
-    import oracle  # CPU baseline leg only
-    p = ffi.make_param(gamma=prm.gamma, C_=prm.C, eps=prm.eps, cache_size=100.0, shrinking=1,
-                       cuda_flag=0)
-    iters = 50
+    import oracle  # CPU baseline legs only
+    return oracle
+
+
+def cpu_threads():
+    """Host threads for the CPU baseline: the box's CPU share (16 at most)."""
+    return max(1, min(16, os.cpu_count() or 1))
+
+
+def cpu_full_solve(prob, prm, cache_mb, threads, shrinking=0):
+    """One full oracle training solve (LIBSVM 3.18 restatement): wall seconds
+    and the result.  threads > 1 fills each missing kernel row with that many
+    threads (the elements are computed as before, so the result is the same)."""
+    oracle = _oracle()
+    p = ffi.make_param(svm_type=prm.svm_type, gamma=prm.gamma, C_=prm.C, nu=prm.nu, eps=prm.eps,
+                       cache_size=cache_mb, shrinking=shrinking, cuda_flag=0)
+    oracle.set_threads(threads)
     t0 = time.perf_counter()
-    oracle.train_one(prob, p, max_iter=iters)
+    r = oracle.train_one(prob, p)
     dt = time.perf_counter() - t0
-    if dt < seconds / 4:
-        iters = max(iters, int(iters * seconds / max(dt, 1e-3)))
-        t0 = time.perf_counter()
-        oracle.train_one(prob, p, max_iter=iters)
-        dt = time.perf_counter() - t0
-    return {"value": iters / dt, "unit": "SMO iters/s", "cores": 1, "kind": "port",
-            "sample": f"first {iters} iterations of the same solve (LIBSVM 3.18 restatement, "
-                      f"-m 100 MB row cache, shrinking on), {dt:.1f} s"}
+    oracle.set_threads(1)
+    return dt, r
+
+
+def cpu_baseline(prob, prm):
+    """The oracle's full training solve of the benchmark problem on the host:
+    a 16 GB row cache (every row it computes stays cached, as on the device)
+    and the box's cores filling missing rows."""
+    P = cpu_threads()
+    dt, r = cpu_full_solve(prob, prm, 16000.0, P)
+    return {"value": r["iters"] / dt, "unit": "SMO iters/s", "cores": P, "kind": "port",
+            "train_time_s": dt, "iters": r["iters"], "obj": r["obj"], "rho": r["rho"],
+            "sample": f"one full training solve of the same problem ({r['iters']} iterations, {dt:.1f} s): "
+                      f"LIBSVM 3.18 restatement, 16 GB row cache (labelled large-cache run), shrinking off, "
+                      f"missing kernel rows filled by {P} threads"}
 
 
 def measured_traffic(n, d, grid):
@@ -166,10 +193,14 @@ def ovo_cv_grid_jobs(X, y, fold, gammas, C):
     return jobs
 
 
-def bench_ovo_cv_grid(args):
+def bench_ovo_cv_grid(args, leg=False):
     """10 classes, ~40k x 128 dense, 5-fold CV x 4 gammas x 45 pairs = 900
     solves per step through the batched solver (one workgroup or a team of
-    workgroups per solve, all solves in flight together)."""
+    workgroups per solve, all solves in flight together).  leg: one warm-up
+    and one timed step, without the plain-path and CPU samples; returns the
+    result instead of printing it."""
+    steps, warmup = (1, 1) if leg else (args.steps, args.warmup)
+    no_plain, no_cpu = (True, True) if leg else (args.no_plain, args.no_cpu)
     gammas = [2.0 ** -9, 2.0 ** -7, 2.0 ** -5, 2.0 ** -3]
     C = 4.0
     X, y, fold = datasets.clusters_10class()
@@ -186,11 +217,11 @@ def bench_ovo_cv_grid(args):
             n = len(rows)
             subs.append(dict(rowset=rid, kind=libsvm_hip.KIND_C, y=yy, p=-np.ones(n), alpha0=np.zeros(n),
                              Cp=C, Cn=C, gamma=g, eps=1e-3))
-    for w in range(args.warmup):
+    for w in range(warmup):
         _, st = b.solve(subs)
         log(f"[bench] warmup {w}: {st['iters']} iters, device {st['solve_ms']:.0f} ms")
     walls, stats = [], []
-    for k in range(args.steps):
+    for k in range(steps):
         t = time.perf_counter()
         res, st = b.solve(subs)
         walls.append(time.perf_counter() - t)
@@ -203,7 +234,7 @@ def bench_ovo_cv_grid(args):
     # on fold 0: its 45 pairs at all 4 gammas, a fifth of the workload
     sample = [(k, g) for k, jb in enumerate(jobs) if jb[0] == 0 for g in gammas]
     p_iters, p_time = 0, 0.0
-    if not args.no_plain:
+    if not no_plain:
         for n_done, (k, g) in enumerate(sample):
             _, _, _, rows, yy = jobs[k]
             sub = prob.subset(rows, y=yy.astype(np.float64))
@@ -220,13 +251,13 @@ def bench_ovo_cv_grid(args):
         "metric": METRIC,
         "value": iters / wall,
         "unit": "SMO iters/s",
-        "n_gpus": 1, "steps": args.steps, "warmup": args.warmup,
-        "ms_per_step": 1e3 * wall / args.steps,
+        "n_gpus": 1, "steps": steps, "warmup": warmup,
+        "ms_per_step": 1e3 * wall / steps,
         "higher_is_better": True, "scaling": "weak", "vs_baseline": None,
         "dtype": "f64", "data": "synthetic",
         "config": {"workload": f"clusters_10class n={prob.l} d=128: 5-fold CV x 4 gammas x 45 OvO pairs, "
                                f"C-SVC C=4 eps=1e-3, {len(subs)} solves per step in one batched launch",
-                   "solves_per_s": len(subs) * args.steps / wall, "train_time_s": wall / args.steps,
+                   "solves_per_s": len(subs) * steps / wall, "train_time_s": wall / steps,
                    "iters_per_step": last["iters"], "rows_computed_per_step": last["row_misses"],
                    "stages_per_step": last["launches"],
                    "plain_path_sample": (f"fold 0, {len(sample)} solves one at a time: {p_iters} iters "
@@ -235,7 +266,7 @@ def bench_ovo_cv_grid(args):
                    "plain_path_train_time_s_extrapolated": (p_time * last["iters"] / p_iters) if p_time else None,
                    "speedup_vs_plain_path": ((iters / wall) / (p_iters / p_time)) if p_time else None},
     }
-    if not args.no_cpu:
+    if not no_cpu:
         sys.path.insert(0, os.path.join(ROOT, "oracle"))
         import oracle  # CPU baseline leg only
         t = time.perf_counter()
@@ -267,7 +298,7 @@ def bench_ovo_cv_grid(args):
             "sample": f"{P} oracle processes over the same fold-0 solves in the same order, one solve "
                       f"per process at a time: {pdone} solves done in {pdt:.1f}s"}
     b.close()
-    print(json.dumps(out), flush=True)
+    return out
 
 
 def _gen_request(k):
@@ -284,23 +315,26 @@ def _serving_param(r):
     return ffi.make_param(svm_type=t, gamma=r["gamma"], C_=r["C"], nu=r["nu"], eps=1e-3, shrinking=0)
 
 
-def bench_serving(args):
+def bench_serving(args, leg=False):
     """Poisson stream at --qps for --window seconds: 90% training requests
     (80% C-SVC / 20% nu-SVC, n log-uniform in [500, 20000], d in
     {32, 128, 512}, density in {0.05, 0.3, 1.0}), 10% predictions of 10,000
     rows against the newest resident model.  Latency = arrival to result on
-    the host."""
+    the host.  leg: 60 req/s for --leg-window seconds without the CPU
+    samples; returns the result instead of printing it."""
     import multiprocessing as mp
+    qps, window = (60.0, args.leg_window) if leg else (args.qps, args.window)
+    no_cpu = True if leg else args.no_cpu
     rng = np.random.Generator(np.random.PCG64(1005))
-    gaps = rng.exponential(1.0 / args.qps, size=int(args.qps * args.window * 2) + 16)
+    gaps = rng.exponential(1.0 / qps, size=int(qps * window * 2) + 16)
     arrivals = np.cumsum(gaps)
-    arrivals = arrivals[arrivals < args.window]
+    arrivals = arrivals[arrivals < window]
     is_pred = rng.random(len(arrivals)) < 0.10
     n_train = int((~is_pred).sum())
     t = time.perf_counter()
     with mp.get_context("fork").Pool(min(16, os.cpu_count() or 1)) as pool:
         raw = pool.map(_gen_request, range(n_train), chunksize=4)
-    log(f"[bench] {len(arrivals)} requests ({n_train} train) over {args.window:.0f}s at {args.qps} req/s; "
+    log(f"[bench] {len(arrivals)} requests ({n_train} train) over {window:.0f}s at {qps} req/s; "
         f"data generated in {time.perf_counter() - t:.1f}s")
     probs = [_serving_problem(r) for r in raw]
     prms = [_serving_param(r) for r in raw]
@@ -399,7 +433,7 @@ def bench_serving(args):
         "ms_per_step": 1e3 * makespan,
         "higher_is_better": True, "scaling": "weak", "vs_baseline": None,
         "dtype": "f64", "data": "synthetic",
-        "config": {"workload": f"serving stream: Poisson {args.qps} req/s for {args.window:.0f}s, 90% train "
+        "config": {"workload": f"serving stream: Poisson {qps} req/s for {window:.0f}s, 90% train "
                                f"(80% C-SVC / 20% nu-SVC, n in [500, 20000], d in {{32,128,512}}), "
                                f"10% predict of 10,000 rows against a resident model",
                    "train_requests": done_train, "predict_requests": done_pred,
@@ -411,7 +445,7 @@ def bench_serving(args):
                    "device_busy_frac": sst["solve_ms"] / 1e3 / makespan, "stages": sst["launches"],
                    "results_same_alone_and_in_mix": alone_ok},
     }
-    if not args.no_cpu:
+    if not no_cpu:
         sys.path.insert(0, os.path.join(ROOT, "oracle"))
         import oracle  # CPU baseline leg only
         t = time.perf_counter()
@@ -442,19 +476,213 @@ def bench_serving(args):
             "sample": f"{P} oracle processes over the stream's training requests in arrival order, one "
                       f"request per process at a time: {pdone} done in {pdt:.1f}s ({piters} iterations)"}
     srv.close()
-    print(json.dumps(out), flush=True)
+    return out
+
+
+def bench_single(args, workload, leg=False):
+    """One persistent-kernel solve per step (gmm_dense: the metric's n=100k
+    configuration; blob_csr: 60,000 x 784 CSR with a 4 GB row cache).  leg:
+    no warm-up, one timed step, no step-wise and CPU samples."""
+    steps, warmup = (1, 0) if leg else (args.steps, args.warmup)
+    t0 = time.perf_counter()
+    if workload == "blob_csr":
+        # 60,000 x 784 CSR rows (~19% nonzero), RBF gamma=0.02, C=10, a 4 GB
+        # device row cache: the plain sparse-dot configuration
+        n = args.n if (args.n != 100000 and not leg) else 60000
+        ip, ix, v, y, d = datasets.blob_mask_csr(n=n)
+        prob = ffi.Problem(ip, ix, v, y, d)
+        nnz = int(ip[-1])
+        prm = ffi.make_param(gamma=0.02, C_=10.0, eps=1e-3, cache_size=4096.0, shrinking=0)
+        cache_gb = args.cache_gb if (args.cache_gb != 64.0 and not leg) else 4.0
+        b_row = 8.0 * nnz + 12.0 * n
+        wl = (f"blob_mask_csr n={n} d={d} nnz={nnz} C-SVC RBF gamma=0.02 C=10 eps=1e-3, "
+              f"{cache_gb:g} GB row cache")
+        row_bytes = "rows_computed * (8*nnz + 12*l)"
+    else:
+        n, d = args.n, args.d
+        X, y = datasets.gmm_dense_matrix(n=n, d=d, seed=1003)
+        prob = ffi.Problem.from_dense(X.astype(np.float64), y, keep_zeros=True)
+        prm = ffi.make_param(gamma=1.0 / d, C_=1.0, eps=1e-3, cache_size=100.0, shrinking=0)
+        cache_gb = args.cache_gb
+        b_row = 4.0 * n * d + 8.0 * n
+        wl = f"gmm_dense n={n} d={d} C-SVC RBF gamma=1/{d} C=1 eps=1e-3"
+        row_bytes = "rows_computed * (4*l*d + 8*l)"
+    log(f"[bench] {workload}: data n={n} d={d} built in {time.perf_counter() - t0:.1f}s")
+    ys, p, a0, Cp, Cn, kind = libsvm_hip.formulate(prob, prm)
+    st0 = libsvm_hip.initial_status(ys, a0, Cp, Cn)
+    s = libsvm_hip.DeviceSolver(prob, prm, kind=kind, grid_blocks=args.grid,
+                                cache_bytes=cache_gb * 2 ** 30)
+    s.setup_rbf_variables()
+    log("[bench] problem resident on the device")
+
+    def one_step():
+        t = time.perf_counter()
+        s.setup_solver(ys, p, a0, st0, Cp, Cn)
+        st = s.solve()
+        return time.perf_counter() - t, st
+
+    for w in range(warmup):
+        dt, st = one_step()
+        log(f"[bench] warmup {w}: {st['iters']} iters in {dt:.3f}s")
+    walls, stats = [], []
+    for k in range(steps):
+        dt, st = one_step()
+        walls.append(dt)
+        stats.append(st)
+        log(f"[bench] step {k}: {st['iters']} iters, {dt:.3f}s wall, kernel {st['kernel_ms']:.1f} ms, "
+            f"rows computed {st['row_misses']}, obj {st['obj']:.10g}, rho {st['rho']:.10g}, "
+            f"device allocations {st['device_allocs']}")
+    iters = sum(st["iters"] for st in stats)
+    wall = sum(walls)
+    kern_s = sum(st["kernel_ms"] for st in stats) / 1e3
+    launches = sum(st["launches"] for st in stats)
+    misses = sum(st["row_misses"] for st in stats)
+    b_iter = 44.0 * n
+    alg_bytes = (iters * b_iter + misses * b_row) / launches
+    launch_s = kern_s / launches
+    achieved = alg_bytes / launch_s
+    traffic = measured_traffic(n, d, args.grid) if workload == "gmm_dense" else None
+    last = stats[-1]
+    out = {
+        "metric": METRIC,
+        "value": iters / wall,
+        "unit": "SMO iters/s",
+        "n_gpus": 1,
+        "steps": steps,
+        "warmup": warmup,
+        "ms_per_step": 1e3 * wall / steps,
+        "higher_is_better": True,
+        "scaling": "weak",
+        "vs_baseline": None,
+        "dtype": "f64",
+        "data": "synthetic",
+        "config": {"workload": wl,
+                   "train_time_s": wall / steps, "iters_per_solve": last["iters"],
+                   "rows_computed_per_solve": last["row_misses"], "cache_rows": last["cache_slots"],
+                   "grid_blocks": last["grid_blocks"], "obj": last["obj"], "rho": last["rho"],
+                   "device_allocs_per_step_after_first": (stats[-1]["device_allocs"] - stats[0]["device_allocs"])
+                   / max(1, steps - 1)},
+        "roofline": {"bound": "hbm", "achieved": achieved / 1e9, "peak": HBM_PEAK / 1e9, "unit": "GB/s",
+                     "frac": achieved / HBM_PEAK, "traffic": traffic,
+                     "kernel": "k_smo", "avg_launch_ms": 1e3 * launch_s,
+                     "bytes_per_launch": alg_bytes,
+                     "bytes_formula": f"(iters * 44*l + {row_bytes}) / launches"},
+    }
+    if not args.no_plain and not leg:
+        # the reference's shape of the loop on the same kernels: one host call
+        # per phase (select, alpha, gradient, status), each a launch, with the
+        # blocking copy of the stopping test, for a bounded number of
+        # iterations from the same start
+        s.setup_solver(ys, p, a0, st0, Cp, Cn)
+        k_plain = min(args.plain_iters, last["iters"])
+        t_start = time.perf_counter()
+        done = 0
+        for _ in range(k_plain):
+            opt, _, _ = s.select_working_set()
+            if opt:
+                break
+            s.compute_alpha()
+            s.update_gradient()
+            s.update_alpha_status()
+            done += 1
+        t_plain = time.perf_counter() - t_start
+        # the fused kernel over exactly those iterations
+        s.setup_solver(ys, p, a0, st0, Cp, Cn)
+        t = time.perf_counter()
+        sk = s.solve(max_iter=done)
+        t_fused = time.perf_counter() - t
+        out["config"]["stepwise_path_iters_per_s"] = done / t_plain
+        out["config"]["stepwise_path_sample"] = (f"first {done} iterations of the same solve through "
+                                                 f"svmhip_select_working_set/compute_alpha/update_gradient/"
+                                                 f"update_alpha_status, {t_plain:.2f}s; the persistent kernel "
+                                                 f"over the same {sk['iters']} iterations: {t_fused:.3f}s")
+        out["config"]["speedup_vs_stepwise_path"] = t_plain / t_fused
+    s.close()
+    if not leg:
+        # svm_train's own wall clock from the host rows (upload, SELL packing,
+        # cache allocation, solve, copy-out, model), the row cache sized by
+        # svm_parameter.cache_size
+        prm_t = ffi.make_param(gamma=prm.gamma, C_=prm.C, eps=prm.eps, shrinking=0,
+                               cache_size=cache_gb * 1024.0)
+        t = time.perf_counter()
+        m = libsvm_hip.svm_train(prob, prm_t)
+        out["config"]["train_time_with_upload_s"] = time.perf_counter() - t
+        del m
+        log(f"[bench] svm_train from host rows: {out['config']['train_time_with_upload_s']:.3f}s")
+    if not args.no_cpu and not leg:
+        t = time.perf_counter()
+        out["cpu_baseline"] = cpu_baseline(prob, prm)
+        log(f"[bench] CPU oracle full solve: {time.perf_counter() - t:.1f}s")
+    return out
+
+
+def leg_zipf(args):
+    """The CPU reference path's configuration (zipf_binary n=2000, d=123,
+    RBF gamma=1/123, C=1, eps=1e-3, shrinking on, 100 MB cache): svm_train
+    on the device from host rows, and the oracle's full solve on one core."""
+    ip, ix, v, y, d = datasets.zipf_binary(n=2000)
+    prob = ffi.Problem(ip, ix, v, y, d)
+    prm = ffi.make_param(gamma=1.0 / 123, C_=1.0, eps=1e-3, cache_size=100.0, shrinking=1)
+    libsvm_hip.svm_train(prob, prm)  # first call: module load, first launches
+    t = time.perf_counter()
+    m = libsvm_hip.svm_train(prob, prm)
+    gpu_s = time.perf_counter() - t
+    nsv = m.arrays()["l"]
+    dev = libsvm_hip.solve_one(prob, prm)
+    out = {"workload": "zipf_binary n=2000 d=123 C-SVC RBF gamma=1/123 C=1 eps=1e-3, 100 MB cache",
+           "gpu_svm_train_s": gpu_s, "gpu_iters": dev["iters"], "gpu_obj": dev["obj"], "gpu_rho": dev["rho"],
+           "nSV": nsv}
+    if not args.no_cpu:
+        for shr in (1, 0):
+            dt, r = cpu_full_solve(prob, prm, 100.0, 1, shrinking=shr)
+            key = "cpu_shrinking" if shr else "cpu_no_shrinking"
+            out[key] = {"train_time_s": dt, "iters": r["iters"], "obj": r["obj"], "rho": r["rho"], "cores": 1,
+                        "obj_rel_err_vs_gpu": abs(r["obj"] - dev["obj"]) / abs(r["obj"]),
+                        "rho_abs_err_vs_gpu": abs(r["rho"] - dev["rho"])}
+    log(f"[bench] zipf_2000: {out}")
+    return out
+
+
+def leg_blob(args):
+    o = bench_single(args, "blob_csr", leg=True)
+    c, r = o["config"], o["roofline"]
+    out = {"workload": c["workload"], "iters": c["iters_per_solve"], "train_time_s": c["train_time_s"],
+           "iters_per_s": o["value"], "rows_computed": c["rows_computed_per_solve"], "obj": c["obj"],
+           "rho": c["rho"], "k_smo_ms": r["avg_launch_ms"], "roofline_frac": r["frac"],
+           "roofline_achieved_GBs": r["achieved"], "bytes_formula": r["bytes_formula"]}
+    log(f"[bench] blob_csr_60k: {out}")
+    return out
+
+
+def leg_ovo(args):
+    o = bench_ovo_cv_grid(args, leg=True)
+    c = o["config"]
+    out = {"workload": c["workload"], "solves_per_s": c["solves_per_s"], "train_time_s": c["train_time_s"],
+           "iters_per_s": o["value"], "iters": c["iters_per_step"], "rows_computed": c["rows_computed_per_step"],
+           "stages": c["stages_per_step"]}
+    log(f"[bench] ovo_cv_grid: {out}")
+    return out
+
+
+def leg_serving(args):
+    o = bench_serving(args, leg=True)
+    out = dict(o["config"])
+    out["solves_per_s"] = o["value"]
+    log(f"[bench] serving: {out}")
+    return out
 
 
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
-    ap.add_argument("--steps", type=int, default=3)
-    ap.add_argument("--warmup", type=int, default=1)
+    ap.add_argument("--steps", type=int, default=20)
+    ap.add_argument("--warmup", type=int, default=3)
     ap.add_argument("--n", type=int, default=100000)
     ap.add_argument("--d", type=int, default=256)
     ap.add_argument("--grid", type=int, default=0)
     ap.add_argument("--cache-gb", type=float, default=64.0)
     ap.add_argument("--no-cpu", action="store_true")
+    ap.add_argument("--no-legs", action="store_true", help="the n=100k solve only")
     ap.add_argument("--cpu-seconds", type=float, default=20.0)
     ap.add_argument("--copy-probe-gib", type=float, default=0.0,
                     help="instead of the solve, run a plain 1-pass device copy of this many GiB "
@@ -462,6 +690,7 @@ def main():
     ap.add_argument("--workload", default="gmm_dense", choices=["gmm_dense", "blob_csr", "ovo_cv_grid", "serving"])
     ap.add_argument("--qps", type=float, default=20.0)
     ap.add_argument("--window", type=float, default=60.0)
+    ap.add_argument("--leg-window", type=float, default=15.0, help="window of the serving leg (60 req/s)")
     ap.add_argument("--stage-ms", type=float, default=10.0)
     ap.add_argument("--batch-cache-mb", type=float, default=400.0)
     ap.add_argument("--no-plain", action="store_true")
@@ -477,124 +706,15 @@ def main():
 
     libsvm_hip.set_quiet(True)
     if args.workload == "ovo_cv_grid":
-        bench_ovo_cv_grid(args)
+        print(json.dumps(bench_ovo_cv_grid(args)), flush=True)
         return
     if args.workload == "serving":
-        bench_serving(args)
+        print(json.dumps(bench_serving(args)), flush=True)
         return
-    t0 = time.perf_counter()
-    if args.workload == "blob_csr":
-        # 60,000 x 784 CSR rows (~19% nonzero), RBF gamma=0.02, C=10, a 4 GB
-        # device row cache: the plain sparse-dot configuration
-        n = args.n if args.n != 100000 else 60000
-        ip, ix, v, y, d = datasets.blob_mask_csr(n=n)
-        prob = ffi.Problem(ip, ix, v, y, d)
-        nnz = int(ip[-1])
-        prm = ffi.make_param(gamma=0.02, C_=10.0, eps=1e-3, cache_size=4096.0, shrinking=0)
-        cache_gb = args.cache_gb if args.cache_gb != 64.0 else 4.0
-        b_row = 8.0 * nnz + 12.0 * n
-        workload = (f"blob_mask_csr n={n} d={d} nnz={nnz} C-SVC RBF gamma=0.02 C=10 eps=1e-3, "
-                    f"{cache_gb:g} GB row cache")
-        row_bytes = "rows_computed * (8*nnz + 12*l)"
-    else:
-        n, d = args.n, args.d
-        X, y = datasets.gmm_dense_matrix(n=n, d=d, seed=1003)
-        prob = ffi.Problem.from_dense(X.astype(np.float64), y, keep_zeros=True)
-        prm = ffi.make_param(gamma=1.0 / d, C_=1.0, eps=1e-3, cache_size=100.0, shrinking=0)
-        cache_gb = args.cache_gb
-        b_row = 4.0 * n * d + 8.0 * n
-        workload = f"gmm_dense n={n} d={d} C-SVC RBF gamma=1/{d} C=1 eps=1e-3"
-        row_bytes = "rows_computed * (4*l*d + 8*l)"
-    log(f"[bench] data n={n} d={d} built in {time.perf_counter() - t0:.1f}s")
-    ys, p, a0, Cp, Cn, kind = libsvm_hip.formulate(prob, prm)
-    st0 = libsvm_hip.initial_status(ys, a0, Cp, Cn)
-    s = libsvm_hip.DeviceSolver(prob, prm, kind=kind, grid_blocks=args.grid,
-                                cache_bytes=cache_gb * 2 ** 30)
-    s.setup_rbf_variables()
-    log("[bench] problem resident on the device")
-
-    def one_step():
-        t = time.perf_counter()
-        s.setup_solver(ys, p, a0, st0, Cp, Cn)
-        st = s.solve()
-        return time.perf_counter() - t, st
-
-    for w in range(args.warmup):
-        dt, st = one_step()
-        log(f"[bench] warmup {w}: {st['iters']} iters in {dt:.2f}s")
-    walls, stats = [], []
-    for k in range(args.steps):
-        dt, st = one_step()
-        walls.append(dt)
-        stats.append(st)
-        log(f"[bench] step {k}: {st['iters']} iters, {dt:.3f}s wall, kernel {st['kernel_ms']:.1f} ms, "
-            f"rows computed {st['row_misses']}, obj {st['obj']:.10g}, rho {st['rho']:.10g}")
-    iters = sum(st["iters"] for st in stats)
-    wall = sum(walls)
-    kern_s = sum(st["kernel_ms"] for st in stats) / 1e3
-    launches = sum(st["launches"] for st in stats)
-    misses = sum(st["row_misses"] for st in stats)
-    b_iter = 44.0 * n
-    alg_bytes = (iters * b_iter + misses * b_row) / launches
-    launch_s = kern_s / launches
-    achieved = alg_bytes / launch_s
-    traffic = measured_traffic(n, d, args.grid) if args.workload == "gmm_dense" else None
-    last = stats[-1]
-    out = {
-        "metric": METRIC,
-        "value": iters / wall,
-        "unit": "SMO iters/s",
-        "n_gpus": 1,
-        "steps": args.steps,
-        "warmup": args.warmup,
-        "ms_per_step": 1e3 * wall / args.steps,
-        "higher_is_better": True,
-        "scaling": "weak",
-        "vs_baseline": None,
-        "dtype": "f64",
-        "data": "synthetic",
-        "config": {"workload": workload,
-                   "train_time_s": wall / args.steps, "iters_per_solve": last["iters"],
-                   "rows_computed_per_solve": last["row_misses"], "cache_rows": last["cache_slots"],
-                   "grid_blocks": last["grid_blocks"], "obj": last["obj"], "rho": last["rho"]},
-        "roofline": {"bound": "hbm", "achieved": achieved / 1e9, "peak": HBM_PEAK / 1e9, "unit": "GB/s",
-                     "frac": achieved / HBM_PEAK, "traffic": traffic,
-                     "kernel": "k_smo", "avg_launch_ms": 1e3 * launch_s,
-                     "bytes_per_launch": alg_bytes,
-                     "bytes_formula": f"(iters * 44*l + {row_bytes}) / launches"},
-    }
-    if not args.no_plain:
-        # the reference's shape of the loop on the same kernels: one host call
-        # per phase (select, alpha, gradient, status), each a launch, with the
-        # blocking copy of the stopping test, for a bounded number of
-        # iterations from the same start
-        s.setup_solver(ys, p, a0, st0, Cp, Cn)
-        k_plain = min(args.plain_iters, last["iters"])
-        torch_sync = time.perf_counter()
-        done = 0
-        for _ in range(k_plain):
-            opt, _, _ = s.select_working_set()
-            if opt:
-                break
-            s.compute_alpha()
-            s.update_gradient()
-            s.update_alpha_status()
-            done += 1
-        t_plain = time.perf_counter() - torch_sync
-        # the fused kernel over exactly those iterations
-        s.setup_solver(ys, p, a0, st0, Cp, Cn)
-        t = time.perf_counter()
-        sk = s.solve(max_iter=done)
-        t_fused = time.perf_counter() - t
-        out["config"]["stepwise_path_iters_per_s"] = done / t_plain
-        out["config"]["stepwise_path_sample"] = (f"first {done} iterations of the same solve through "
-                                                 f"svmhip_select_working_set/compute_alpha/update_gradient/"
-                                                 f"update_alpha_status, {t_plain:.2f}s; the persistent kernel "
-                                                 f"over the same {sk['iters']} iterations: {t_fused:.3f}s")
-        out["config"]["speedup_vs_stepwise_path"] = t_plain / t_fused
-    if not args.no_cpu:
-        out["cpu_baseline"] = cpu_baseline(prob, prm, args.cpu_seconds)
-    s.close()
+    out = bench_single(args, args.workload)
+    if args.workload == "gmm_dense" and not args.no_legs:
+        out["legs"] = {"zipf_2000": leg_zipf(args), "blob_csr_60k": leg_blob(args),
+                       "ovo_cv_grid": leg_ovo(args), "serving": leg_serving(args)}
     print(json.dumps(out), flush=True)
 
 

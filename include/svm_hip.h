@@ -54,7 +54,7 @@ typedef struct {
 	int grid_blocks;          /* persistent-grid workgroups (0 = the size rule: one per CU on large
 				     problems, fewer on mid-size ones) */
 	int block_threads;        /* threads per workgroup (0 = 512, the only size built) */
-	int max_iter;             /* 0 = LIBSVM rule max(10^7, 100*l) */
+	int max_iter;             /* 0 = LIBSVM rule max(10^7, 100*n_var), n_var = 2l for regression */
 	double cache_bytes;       /* kernel-row cache bytes (0 = param.cache_size MB) */
 	int iters_per_launch;     /* persistent-kernel iterations per launch (0 = all) */
 	int verbose;              /* 1 = print setup and cache statistics to stderr */
@@ -83,6 +83,9 @@ typedef struct {
 	double setup_ms;          /* device time of G initialisation */
 	int cache_slots;          /* rows the cache holds */
 	int grid_blocks;          /* workgroups used by the persistent kernel */
+	long long device_allocs;  /* device allocations made by the handle since svmhip_create (a
+				     repeated svmhip_setup + svmhip_solve over the same rows adds none
+				     once the first solve has sized the buffers) */
 } svmhip_stats;
 
 /* CudaSolver(prob, param, quiet) / CudaSolverNU(...)
@@ -102,9 +105,11 @@ svmhip_status svmhip_setup_rbf(svmhip_solver *h, int l);
  * alpha and alpha_status, computes QD, initialises G = p + Q alpha on the
  * device in FP64 and sizes the kernel-row cache.  Calling it again on the
  * same handle discards the previous solve's state and starts a new one over
- * the already uploaded rows (labels, C and alpha may change).  n_var is l for
- * classification / one-class and 2l for regression (SVR variable i+l shares
- * kernel row i). */
+ * the already uploaded rows (labels, C and alpha may change); the solver
+ * vectors, the kernel-row cache and its bookkeeping stay allocated and are
+ * reused when the new solve fits them (the cache's rows are invalidated).
+ * n_var is l for classification / one-class and 2l for regression (SVR
+ * variable i+l shares kernel row i). */
 svmhip_status svmhip_setup(svmhip_solver *h, const signed char *y, const double *G, const double *alpha,
 			   const char *alpha_status, double Cp, double Cn, int n_var);
 

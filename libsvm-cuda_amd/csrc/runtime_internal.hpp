@@ -22,6 +22,7 @@ extern "C" {
 hipError_t smo_launch_self_dot(const smo::View *V, double *xsq, hipStream_t st);
 hipError_t smo_launch_qd(const smo::View *V, hipStream_t st);
 hipError_t smo_launch_ginit_row(const smo::View *V, int v, size_t shmem, hipStream_t st);
+hipError_t smo_launch_ginit_multi(const smo::View *V, const int *list, int count, size_t shmem, hipStream_t st);
 hipError_t smo_launch_select_i(const smo::View *V, size_t shmem, hipStream_t st);
 hipError_t smo_launch_select_j(const smo::View *V, size_t shmem, hipStream_t st);
 hipError_t smo_launch_alpha(const smo::View *V, hipStream_t st);
@@ -31,6 +32,7 @@ hipError_t smo_launch_persistent(const smo::View *V, int iter_limit, size_t shme
 hipError_t smo_launch_lru_private_init(const smo::View *V, hipStream_t st);
 hipError_t smo_persistent_capacity(int kind, size_t shmem, int *blocks_per_cu);
 hipError_t smo_launch_finalize(const smo::View *V, hipStream_t st);
+hipError_t smo_launch_finalize_views(const smo::View *views, const int *idx, int n, hipStream_t st);
 hipError_t smo_launch_kernel_row(const smo::View *V, int q, float *out, size_t shmem, hipStream_t st);
 hipError_t smo_launch_batch(const smo::View *views, const int *items, int nitems, int grid, const smo::BatchPool *pool,
                             size_t shmem, hipStream_t st);
@@ -93,6 +95,12 @@ struct svmhip_solver {
   long long pm_stride = 0;
   bool shared_meta_dirty = false;  // the persistent kernel overwrote cache slots
   std::vector<void *> allocs;
+  long long n_allocs = 0;         // hipMalloc calls made for this handle (all time)
+  // capacities (elements) of the per-solve buffers svmhip_setup keeps from
+  // one solve to the next on the same handle
+  size_t cap_nvar = 0, cap_rows = 0, cap_flags = 0, cap_pm_flags = 0, cap_lru_map = 0, cap_lru_slots = 0,
+         cap_pm_lru = 0, cap_list = 0;
+  int *d_list = nullptr;          // G initialisation: variables with alpha != 0
   svmhip_stats stats{};
   std::string err;
 
@@ -163,8 +171,22 @@ struct svmhip_solver {
            hipGetErrorString(e));
     }
     allocs.push_back(p);
+    ++n_allocs;
     if (pool_cap > 0) block_class[p] = bytes;
     return (T *)p;
+  }
+  // Buffer p of at least n elements: kept when its capacity (cap) suffices,
+  // else freed and allocated anew.
+  template <class T>
+  T *ensure(T *&p, size_t &cap, size_t n, const char *what) {
+    n = std::max<size_t>(n, 1);
+    if (p && cap >= n) return p;
+    dfree(p);
+    p = nullptr;
+    cap = 0;
+    p = dalloc<T>(n, what);
+    cap = n;
+    return p;
   }
   void dfree(void *p) {
     if (!p) return;
@@ -206,17 +228,13 @@ struct svmhip_solver {
     return V;
   }
 
-  // Frees what svmhip_setup allocated, keeping the uploaded problem, so one
-  // handle can run several solves over the same rows.
-  void release_solver_state() {
+  // Ends the current solve before a new svmhip_setup on the same rows.  The
+  // per-solve buffers (solver vectors, kernel-row cache, cache bookkeeping)
+  // stay allocated and are reused when the next solve fits them: freeing
+  // and re-allocating a cache of tens of GB per solve costs more than the
+  // solve.
+  void end_solve() {
     (void)hipStreamSynchronize(st);
-    for (void **p : {(void **)&d_G, (void **)&d_alpha, (void **)&d_p, (void **)&d_QD, (void **)&d_y,
-                     (void **)&d_status, (void **)&d_rows, (void **)&d_computed, (void **)&d_slot_of_row,
-                     (void **)&d_row_of_slot, (void **)&d_prev, (void **)&d_next, (void **)&d_pm_flags,
-                     (void **)&d_pm_lru}) {
-      dfree(*p);
-      *p = nullptr;
-    }
     setup_done = false;
     have_j = false;
     stats = svmhip_stats{};

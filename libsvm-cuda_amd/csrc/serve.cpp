@@ -71,6 +71,7 @@ struct Request {
   int mcount[2] = {0, 0};
   // predict
   int model_id = -1;
+  double mlabels[2] = {0, 0};  // the model's two labels, copied at submission (the model may be released first)
   std::vector<double> dec;
   double *ddec = nullptr, *kv = nullptr;
   int tb = 1, chunk = 1;
@@ -88,10 +89,10 @@ struct svmhip_server {
   std::vector<int> active, predicts;
   std::vector<svmhip_completion> done;
   smo::View *dV = nullptr;
-  int *dItems = nullptr, *dQueue = nullptr;
+  int *dItems = nullptr, *dQueue = nullptr, *dFin = nullptr;
   unsigned long long *dT0 = nullptr;
   smo::Ctl *dCtl = nullptr;
-  size_t capV = 0, capI = 0, capC = 0;
+  size_t capV = 0, capI = 0, capC = 0, capF = 0;
   // the round in flight: its stage's solves and its predictions
   bool inflight = false, stage_launched = false;
   std::vector<int> stage, stage_preds;
@@ -331,64 +332,19 @@ void make_model(svmhip_server *s, Request &q) {
   s->used += q.bytes;
 }
 
-// rho, obj and r from the final alpha and G on the host, in the order of
-// Solver::calculate_rho (libsvm/svm.cpp:1013-1049), Solver_NU::calculate_rho
-// (:1260-1308) and the objective loop of Solver::Solve (:794-803), so the
-// result does not depend on how many workgroups solved the request.
-void host_finalize(Request &q, const std::vector<double> &a, const std::vector<double> &G) {
-  const int n = (int)q.y.size();
-  auto upper = [&](int i) { return a[i] >= (q.y[i] > 0 ? q.Cp : q.Cn); };
-  auto lower = [&](int i) { return a[i] <= 0; };
-  double v = 0;
-  for (int i = 0; i < n; ++i) v += a[i] * (G[i] + q.p[i]);
-  q.obj = v / 2;
-  if (q.kind == SVMHIP_KIND_C) {
-    double ub = INFINITY, lb = -INFINITY, sum_free = 0;
-    int nr_free = 0;
-    for (int i = 0; i < n; ++i) {
-      const double yG = q.y[i] * G[i];
-      if (upper(i)) {
-        if (q.y[i] == -1) ub = std::min(ub, yG);
-        else lb = std::max(lb, yG);
-      } else if (lower(i)) {
-        if (q.y[i] == +1) ub = std::min(ub, yG);
-        else lb = std::max(lb, yG);
-      } else {
-        ++nr_free;
-        sum_free += yG;
-      }
-    }
-    q.rho = nr_free > 0 ? sum_free / nr_free : (ub + lb) / 2;
-    q.r = 0;
-  } else {
-    int nr_free1 = 0, nr_free2 = 0;
-    double ub1 = INFINITY, ub2 = INFINITY, lb1 = -INFINITY, lb2 = -INFINITY, sum1 = 0, sum2 = 0;
-    for (int i = 0; i < n; ++i) {
-      if (q.y[i] == +1) {
-        if (upper(i)) lb1 = std::max(lb1, G[i]);
-        else if (lower(i)) ub1 = std::min(ub1, G[i]);
-        else { ++nr_free1; sum1 += G[i]; }
-      } else {
-        if (upper(i)) lb2 = std::max(lb2, G[i]);
-        else if (lower(i)) ub2 = std::min(ub2, G[i]);
-        else { ++nr_free2; sum2 += G[i]; }
-      }
-    }
-    const double r1 = nr_free1 > 0 ? sum1 / nr_free1 : (ub1 + lb1) / 2;
-    const double r2 = nr_free2 > 0 ? sum2 / nr_free2 : (ub2 + lb2) / 2;
-    q.r = (r1 + r2) / 2;
-    q.rho = (r1 - r2) / 2;
-  }
-}
-
+// A finished solve: rho, obj and r as k_finalize_views left them in c (FP64
+// on the device, Solver::calculate_rho, libsvm/svm.cpp:1013-1049,
+// Solver_NU::calculate_rho, :1260-1308, and the objective of :794-803), the
+// signed coefficients from the final alpha, and the resident model.
 void finish_train(svmhip_server *s, Request &q, const smo::Ctl &c) {
   svmhip_solver *h = s->g;
   const int nvar = (int)q.y.size();
-  std::vector<double> a(nvar), G(nvar);
+  std::vector<double> a(nvar);
   h->ck(hipMemcpyAsync(a.data(), q.alpha, sizeof(double) * nvar, hipMemcpyDeviceToHost, h->st), "fetch alpha");
-  h->ck(hipMemcpyAsync(G.data(), q.G, sizeof(double) * nvar, hipMemcpyDeviceToHost, h->st), "fetch G");
   h->ck(hipStreamSynchronize(h->st), "fetch alpha");
-  host_finalize(q, a, G);
+  q.rho = c.rho;
+  q.obj = c.obj;
+  q.r = c.r;
   q.coef.assign(nvar, 0.0);
   if (q.svm_type == NU_SVC) {  // libsvm/svm.cpp:1574-1580
     for (int i = 0; i < nvar; ++i) q.coef[i] = a[i] * q.y[i] / q.r;
@@ -603,6 +559,8 @@ int svmhip_server_submit_predict(svmhip_server *s, int model_id, const svm_node 
     svm_problem pt{n, dummy.data(), const_cast<svm_node **>(x)};
     q->type = 1;
     q->model_id = model_id;
+    q->mlabels[0] = m.labels[0];
+    q->mlabels[1] = m.labels[1];
     make_host_sell(h, &pt, m.ktype, q->hs);
     if ((size_t)std::max(q->hs.d, m.model.d) * sizeof(float) > 16384 * sizeof(float))
       h->fail(SVMHIP_ERR_ARG, "server: prediction rows wider than 16384 features");
@@ -638,7 +596,7 @@ svmhip_status svmhip_server_step(svmhip_server *s, double stage_ms, int *n_activ
         std::vector<smo::Ctl> hc(s->stage.size());
         h->ck(hipMemcpy(hc.data(), s->dCtl, sizeof(smo::Ctl) * s->stage.size(), hipMemcpyDeviceToHost),
               "fetch server control");
-        std::vector<int> still;
+        std::vector<int> still, fin;  // fin: stage positions of the solves that finished
         std::vector<char> ran(s->reqs.size(), 0);
         for (size_t k = 0; k < s->stage.size(); ++k) {
           Request &q = *s->reqs[s->stage[k]];
@@ -670,10 +628,19 @@ svmhip_status svmhip_server_step(svmhip_server *s, double stage_ms, int *n_activ
             q.bytes = 0;
             complete(s, q, SVMHIP_ERR_TIMEOUT);
           } else if (c.stop == smo::STOP_OPTIMAL || c.iter >= q.max_iter) {
-            finish_train(s, q, c);
+            fin.push_back((int)k);
           } else {
             still.push_back(q.id);
           }
+        }
+        if (!fin.empty()) {  // rho / obj / r of the finished solves on the device
+          s->dFin = grow(h, s->dFin, s->capF, fin.size(), "server finalize list");
+          h->upload(s->dFin, fin.data(), fin.size(), "upload finalize list");
+          h->ck(smo_launch_finalize_views(s->dV, s->dFin, (int)fin.size(), h->st), "finalize kernel");
+          h->ck(hipMemcpyAsync(hc.data(), s->dCtl, sizeof(smo::Ctl) * s->stage.size(), hipMemcpyDeviceToHost, h->st),
+                "fetch server control");
+          h->ck(hipStreamSynchronize(h->st), "finalize");
+          for (int k : fin) finish_train(s, *s->reqs[s->stage[k]], hc[k]);
         }
         for (int id : s->active)
           if (!ran[id]) still.push_back(id);
@@ -870,10 +837,9 @@ svmhip_status svmhip_server_predict_result(svmhip_server *s, int id, double *dec
   if (q.type != 1) return SVMHIP_ERR_ARG;
   if (q.state != REQ_DONE) return SVMHIP_ERR_STATE;
   if (n != (int)q.dec.size()) return SVMHIP_ERR_ARG;
-  const Request &m = *s->reqs[q.model_id];
   for (int t = 0; t < n; ++t) {
     if (dec_values) dec_values[t] = q.dec[t];
-    if (labels) labels[t] = q.dec[t] > 0 ? m.labels[0] : m.labels[1];
+    if (labels) labels[t] = q.dec[t] > 0 ? q.mlabels[0] : q.mlabels[1];
   }
   return SVMHIP_OK;
 }

@@ -598,6 +598,32 @@ __device__ __forceinline__ void batch_ginit(const View &V, float *xs, int lds_fl
   __syncthreads();
 }
 
+// G_k += alpha_v Q_vk for every v in list[0..count) (the variables with
+// alpha_v != 0 in increasing v, libsvm/svm.cpp:555-569) over the rows of this
+// workgroup's block of BLOCK rows, one launch for the whole list: the
+// single-solve form of batch_ginit.  Up to GINIT_M query rows (as the
+// dynamic LDS holds, lds_floats) are staged per pass, each lane's row is
+// read once per pass for all of them, and every G_k still accumulates in
+// increasing v with the kernel values kernel_value computes, so G is
+// bit-identical to one k_ginit_row launch per v.
+__global__ void __launch_bounds__(BLOCK) k_ginit_multi(View V, const int *list, int count, int lds_floats) {
+  extern __shared__ float xs_lds[];
+  __shared__ int sl[GINIT_M];
+  const int r0 = blockIdx.x * BLOCK, r1 = min(V.l, r0 + BLOCK);
+  const int fit = lds_floats / max(V.d, 1);
+  const int M = fit >= GINIT_M ? GINIT_M : (fit >= 4 ? 4 : 1);
+  for (int c0 = 0; c0 < count; c0 += M) {
+    const int cnt = min(M, count - c0);
+    if (threadIdx.x < cnt) sl[threadIdx.x] = list[c0 + threadIdx.x];
+    __syncthreads();
+    for (int m = 0; m < cnt; ++m) stage_row(V, real_of(V, sl[m]), xs_lds + m * V.d);
+    if (M == GINIT_M) ginit_pass<GINIT_M>(V, xs_lds, V.d, sl, cnt, r0, r1);
+    else if (M == 4) ginit_pass<4>(V, xs_lds, V.d, sl, cnt, r0, r1);
+    else ginit_pass<1>(V, xs_lds, V.d, sl, cnt, r0, r1);
+    __syncthreads();  // sl and the staged rows are rewritten by the next pass
+  }
+}
+
 // Terms of rho / obj / r over the variables of rows [r0, r1), reduced over
 // the workgroup into thread 0's f[0..6] (s1, ub1, lb1, s2, ub2, lb2, obj) and
 // n[0..1] (Solver::calculate_rho, libsvm/svm.cpp:1013-1049; Solver_NU::calculate_rho, :1260-1308;
@@ -673,6 +699,25 @@ __device__ __forceinline__ void write_final(Ctl *c, const double f[7], const int
     double r2 = n[1] > 0 ? f[3] / n[1] : (f[4] + f[5]) / 2;
     c->r = (r1 + r2) / 2;
     c->rho = (r1 - r2) / 2;
+  }
+}
+
+// rho, obj and r in FP64 of the finished subproblems views[idx[b]], one
+// workgroup each, into each view's Ctl.  The terms are reduced by one
+// workgroup of fixed shape over all of a subproblem's variables, so the
+// result does not depend on how many workgroups solved it (the serving
+// runtime's team sizes change from stage to stage).
+__global__ void __launch_bounds__(BLOCK) k_finalize_views(const View *views, const int *idx, int n) {
+  if ((int)blockIdx.x >= n) return;
+  const View V = views[idx[blockIdx.x]];
+  double f[7];
+  int nn[2];
+  if (V.kind == KIND_C) {
+    final_terms<KIND_C>(V, 0, V.l, f, nn);
+    if (threadIdx.x == 0) write_final<KIND_C>(V.ctl, f, nn);
+  } else {
+    final_terms<KIND_NU>(V, 0, V.l, f, nn);
+    if (threadIdx.x == 0) write_final<KIND_NU>(V.ctl, f, nn);
   }
 }
 
@@ -1107,6 +1152,20 @@ hipError_t smo_launch_ginit_row(const View *V, int v, size_t shmem, hipStream_t 
   return hipGetLastError();
 }
 
+// shmem = 0: as many staged rows (16, 4 or 1) as 64 KiB of LDS holds.
+hipError_t smo_launch_ginit_multi(const View *V, const int *list, int count, size_t shmem, hipStream_t st) {
+  if (count <= 0 || V->l <= 0) return hipSuccess;
+  const size_t row = (size_t)std::max(V->d, 1) * sizeof(float);
+  if (shmem == 0) {
+    const int M = 16 * row <= 65536 ? 16 : (4 * row <= 65536 ? 4 : 1);
+    shmem = (size_t)M * row;
+  }
+  if (shmem < row || shmem > 65536) return hipErrorInvalidValue;
+  hipLaunchKernelGGL(smo::k_ginit_multi, dim3(nb_for(V->l, smo::BLOCK)), dim3(smo::BLOCK), shmem, st, *V, list,
+                     count, (int)(shmem / sizeof(float)));
+  return hipGetLastError();
+}
+
 hipError_t smo_launch_select_i(const View *V, size_t shmem, hipStream_t st) {
   if (V->kind == smo::KIND_C)
     hipLaunchKernelGGL(smo::k_select_i<smo::KIND_C>, dim3(V->nblocks), dim3(smo::BLOCK), shmem, st, *V);
@@ -1176,6 +1235,12 @@ hipError_t smo_persistent_capacity(int kind, size_t shmem, int *blocks_per_cu) {
   const void *fn = kind == smo::KIND_C ? (const void *)smo::k_smo<smo::KIND_C>
                                        : (const void *)smo::k_smo<smo::KIND_NU>;
   return hipOccupancyMaxActiveBlocksPerMultiprocessor(blocks_per_cu, fn, smo::BLOCK, shmem);
+}
+
+hipError_t smo_launch_finalize_views(const View *views, const int *idx, int n, hipStream_t st) {
+  if (n <= 0) return hipSuccess;
+  hipLaunchKernelGGL(smo::k_finalize_views, dim3(n), dim3(smo::BLOCK), 0, st, views, idx, n);
+  return hipGetLastError();
 }
 
 hipError_t smo_launch_finalize(const View *V, hipStream_t st) {

@@ -240,7 +240,7 @@ svmhip_status svmhip_setup(svmhip_solver *h, const signed char *y, const double 
   if (n_var != h->l && n_var != 2 * h->l) h->fail(SVMHIP_ERR_ARG, "setup: n_var=%d must be l or 2l (l=%d)", n_var, h->l);
   if (n_var == 2 * h->l && h->l > 0 && h->svm_type != EPSILON_SVR && h->svm_type != NU_SVR)
     h->fail(SVMHIP_ERR_ARG, "setup: n_var=2l is only valid for regression");
-  if (h->setup_done) h->release_solver_state();  // re-setup reuses the uploaded rows
+  if (h->setup_done) h->end_solve();  // a new solve over the uploaded rows, reusing the buffers
   if (!(Cp > 0) || !(Cn > 0)) h->fail(SVMHIP_ERR_ARG, "setup: Cp, Cn must be > 0");
   h->ck(hipSetDevice(h->dev), "hipSetDevice");
   if (!h->rbf_ready) {
@@ -253,12 +253,21 @@ svmhip_status svmhip_setup(svmhip_solver *h, const signed char *y, const double 
   for (int v = 0; v < n_var; ++v)
     if (y[v] != 1 && y[v] != -1) h->fail(SVMHIP_ERR_ARG, "setup: y[%d]=%d is not +1/-1", v, (int)y[v]);
   const int nv = n_var;
-  h->d_G = h->dalloc<double>(nv, "G");
-  h->d_alpha = h->dalloc<double>(nv, "alpha");
-  h->d_p = h->dalloc<double>(nv, "p");
-  h->d_QD = h->dalloc<double>(nv, "QD");
-  h->d_y = h->dalloc<int8_t>(nv, "y");
-  h->d_status = h->dalloc<int8_t>(nv, "status");
+  if (h->cap_nvar < (size_t)std::max(nv, 1)) {  // the six solver vectors share one capacity
+    for (void **p : {(void **)&h->d_G, (void **)&h->d_alpha, (void **)&h->d_p, (void **)&h->d_QD, (void **)&h->d_y,
+                     (void **)&h->d_status}) {
+      h->dfree(*p);
+      *p = nullptr;
+    }
+    h->cap_nvar = 0;
+    h->d_G = h->dalloc<double>(nv, "G");
+    h->d_alpha = h->dalloc<double>(nv, "alpha");
+    h->d_p = h->dalloc<double>(nv, "p");
+    h->d_QD = h->dalloc<double>(nv, "QD");
+    h->d_y = h->dalloc<int8_t>(nv, "y");
+    h->d_status = h->dalloc<int8_t>(nv, "status");
+    h->cap_nvar = (size_t)std::max(nv, 1);
+  }
   h->upload(h->d_G, G, nv, "upload G");
   h->upload(h->d_p, G, nv, "upload p");  // Solver::Solve passes G = p here
   h->upload(h->d_alpha, alpha, nv, "upload alpha");
@@ -266,14 +275,16 @@ svmhip_status svmhip_setup(svmhip_solver *h, const signed char *y, const double 
   h->upload(h->d_status, (const int8_t *)alpha_status, nv, "upload status");
 
   // kernel-row cache: S rows of l floats (64-bit sized; the reference's
-  // int byte count overflows at 2 GB, libsvm/cuda_solver.cpp:278)
+  // int byte count overflows at 2 GB, libsvm/cuda_solver.cpp:278).  The
+  // cache a previous solve on this handle left allocated counts as free.
   double bytes = h->opt.cache_bytes > 0 ? h->opt.cache_bytes : h->param_cache_mb * 1048576.0;
   h->row_stride = ((long long)h->l + 63) / 64 * 64;
   long long rowb = std::max<long long>(h->row_stride, 1) * (long long)sizeof(float);
   long long want = (long long)std::floor(bytes / (double)rowb);
   size_t freeb = 0, totb = 0;
   if (hipMemGetInfo(&freeb, &totb) == hipSuccess) {
-    long long cap = (long long)((double)freeb * 0.85) / rowb;
+    const double avail = (double)freeb + (double)h->cap_rows * sizeof(float);
+    long long cap = (long long)(avail * 0.85) / rowb;
     want = std::min(want, cap);
   }
   want = std::max<long long>(want, 2);
@@ -284,31 +295,54 @@ svmhip_status svmhip_setup(svmhip_solver *h, const signed char *y, const double 
     h->full = 0;
     h->S = (int)want;
   }
-  h->d_rows = h->dalloc<float>((size_t)h->S * (size_t)h->row_stride, "kernel-row cache");
+  h->ensure(h->d_rows, h->cap_rows, (size_t)h->S * (size_t)h->row_stride, "kernel-row cache");
   if (h->full) {
-    h->d_computed = h->dalloc<int8_t>(std::max(h->l, 1), "cache flags");
-    h->d_pm_flags = h->dalloc<int8_t>((size_t)h->nblocks * std::max(h->l, 1), "private cache flags");
+    h->ensure(h->d_computed, h->cap_flags, (size_t)std::max(h->l, 1), "cache flags");
+    h->ensure(h->d_pm_flags, h->cap_pm_flags, (size_t)h->nblocks * std::max(h->l, 1), "private cache flags");
   } else {
-    h->d_slot_of_row = h->dalloc<int>(h->l, "cache map");
-    h->d_row_of_slot = h->dalloc<int>(h->S, "cache map");
-    h->d_prev = h->dalloc<int>(h->S, "LRU links");
-    h->d_next = h->dalloc<int>(h->S, "LRU links");
+    h->ensure(h->d_slot_of_row, h->cap_lru_map, (size_t)h->l, "cache map");
+    if (h->cap_lru_slots < (size_t)h->S) {  // row_of_slot, prev, next share one capacity
+      for (void **p : {(void **)&h->d_row_of_slot, (void **)&h->d_prev, (void **)&h->d_next}) {
+        h->dfree(*p);
+        *p = nullptr;
+      }
+      h->cap_lru_slots = 0;
+      h->d_row_of_slot = h->dalloc<int>(h->S, "cache map");
+      h->d_prev = h->dalloc<int>(h->S, "LRU links");
+      h->d_next = h->dalloc<int>(h->S, "LRU links");
+      h->cap_lru_slots = (size_t)h->S;
+    }
     h->pm_stride = ((long long)h->l + 3LL * h->S + 2 + 31) / 32 * 32;
-    h->d_pm_lru = h->dalloc<int>((size_t)h->nblocks * (size_t)h->pm_stride, "private LRU bookkeeping");
+    h->ensure(h->d_pm_lru, h->cap_pm_lru, (size_t)h->nblocks * (size_t)h->pm_stride, "private LRU bookkeeping");
   }
   memset(h->host_ctl, 0, sizeof(smo::Ctl));
   reset_shared_cache(h);
   int mi = h->opt.max_iter;
-  if (mi <= 0) mi = std::max(10000000, h->l > INT_MAX / 100 ? INT_MAX : 100 * h->l);  // svm.cpp:577
+  // Solver::Solve's cap over its own l, the number of variables (2l for
+  // regression), libsvm/svm.cpp:577
+  if (mi <= 0) mi = std::max(10000000, nv > INT_MAX / 100 ? INT_MAX : 100 * nv);
   h->host_ctl->max_iter = mi;
   h->ck(hipMemcpyAsync(h->d_ctl, h->host_ctl, sizeof(smo::Ctl), hipMemcpyHostToDevice, h->st), "init control");
 
   smo::View V = h->view();
   h->ck(hipEventRecord(h->ev[2], h->st), "event");
   if (nv > 0) h->ck(smo_launch_qd(&V, h->st), "QD kernel");
-  // G = p + sum_{alpha_v > 0} alpha_v Q_v, in increasing v (svm.cpp:555-569)
+  // G = p + sum_{alpha_v > 0} alpha_v Q_v, in increasing v (svm.cpp:555-569):
+  // one launch over the list of such v, several rows staged per pass over X
+  // (each G_k still accumulates in increasing v); rows too wide for LDS go
+  // one launch per v
+  std::vector<int> list;
   for (int v = 0; v < nv; ++v)
-    if (alpha_status[v] != smo::AT_LOWER) h->ck(smo_launch_ginit_row(&V, v, h->shmem, h->st), "G init kernel");
+    if (alpha_status[v] != smo::AT_LOWER) list.push_back(v);
+  if (!list.empty()) {
+    if (h->use_lds) {
+      h->ensure(h->d_list, h->cap_list, list.size(), "G initialisation list");
+      h->upload(h->d_list, list.data(), list.size(), "upload G initialisation list");
+      h->ck(smo_launch_ginit_multi(&V, h->d_list, (int)list.size(), 0, h->st), "G init kernel");
+    } else {
+      for (int v : list) h->ck(smo_launch_ginit_row(&V, v, h->shmem, h->st), "G init kernel");
+    }
+  }
   h->ck(hipEventRecord(h->ev[3], h->st), "event");
   h->ck(hipStreamSynchronize(h->st), "setup");
   float ms = 0;
@@ -417,6 +451,7 @@ svmhip_status svmhip_finalize(svmhip_solver *h, svmhip_stats *stats) {
   h->stats.iters = h->host_ctl->iter;
   h->stats.row_hits = (long long)h->host_ctl->hits;
   h->stats.row_misses = (long long)h->host_ctl->misses;
+  h->stats.device_allocs = h->n_allocs;
   if (stats) *stats = h->stats;
   GUARD_END
 }
@@ -501,12 +536,14 @@ svmhip_status svmhip_solve(svmhip_solver *h, int max_iter, svmhip_stats *stats) 
     h->stats.optimal = h->host_ctl->stop == smo::STOP_OPTIMAL || h->nvar == 0;
     GUARD_END
   }();
+  h->stats.device_allocs = h->n_allocs;
   if (stats) *stats = h->stats;
   return rc;
 }
 
 svmhip_status svmhip_get_stats(svmhip_solver *h, svmhip_stats *stats) {
   if (!h || !stats) return SVMHIP_ERR_ARG;
+  h->stats.device_allocs = h->n_allocs;
   *stats = h->stats;
   return SVMHIP_OK;
 }
@@ -688,7 +725,9 @@ svmhip_status svmhip_batch_solve(svmhip_batch *b, svmhip_subproblem *subs, int n
       V.serial = nullptr;
       V.gamma = sp.gamma;
       V.nvar = sp.n_var; V.svr = sp.n_var == 2 * rs.n ? 1 : 0; V.kind = sp.kind;
-      true_max[k] = h->opt.max_iter > 0 ? h->opt.max_iter : std::max(10000000, rs.n > INT_MAX / 100 ? INT_MAX : 100 * rs.n);
+      // over the subproblem's variables, as Solver::Solve (libsvm/svm.cpp:577)
+      true_max[k] = h->opt.max_iter > 0 ? h->opt.max_iter
+                                        : std::max(10000000, sp.n_var > INT_MAX / 100 ? INT_MAX : 100 * sp.n_var);
       V.max_iter = true_max[k];
       V.Cp = sp.Cp; V.Cn = sp.Cn; V.eps = sp.eps;
       V.G = dG + off; V.alpha = dA + off; V.p = dP + off; V.QD = dQ + off; V.y = dY + off; V.status = dS + off;

@@ -47,6 +47,7 @@ def lib():
         L = C.CDLL(LIB_PATH)
         L.ora_set_options.argtypes = [C.POINTER(ora_options)]
         L.ora_set_verbose.argtypes = [C.c_int]
+        L.ora_set_threads.argtypes = [C.c_int]
         L.ora_train_one.argtypes = [C.POINTER(ffi.svm_problem), C.POINTER(ffi.svm_parameter),
                                     C.c_double, C.c_double, C.POINTER(ora_result)]
         L.ora_kernel_row.argtypes = [C.POINTER(ffi.svm_problem), C.POINTER(ffi.svm_parameter),
@@ -70,6 +71,12 @@ def lib():
                                                C.c_int, C.POINTER(C.c_double)]
         _lib = L
     return _lib
+
+
+def set_threads(n):
+    """Threads that fill a missing kernel row (default 1, LIBSVM's one
+    thread); results do not depend on it."""
+    lib().ora_set_threads(int(n))
 
 
 def train_one(problem: ffi.Problem, param, Cp=None, Cn=None, max_iter=0, q_double=False,

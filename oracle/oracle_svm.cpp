@@ -74,6 +74,11 @@ struct ora_result {
 
 static ora_options g_opt = {0, 0, 0, nullptr};
 
+/* Threads used to fill a missing kernel row (1 = LIBSVM's single thread).
+ * Each element of a row is computed independently and identically, so the
+ * thread count never changes a result; it only shortens full-size runs. */
+static int g_threads = 1;
+
 static void print_to_stdout(const char *s) { fputs(s, stdout); fflush(stdout); }
 static void print_nothing(const char *) {}
 static void (*g_print)(const char *) = &print_nothing;
@@ -328,6 +333,7 @@ public:
 	{
 		Qf *d;
 		int from = cache_.fetch(i, &d, len);
+#pragma omp parallel for schedule(static) num_threads(g_threads) if (g_threads > 1 && len - from > 2048)
 		for (int j = from; j < len; j++)
 			d[j] = (Qf)(y_[i] * y_[j] * k_.eval(i, j));
 		return d;
@@ -372,8 +378,10 @@ public:
 	{
 		Qf *d;
 		int ri = real_[i];
-		if (cache_.fetch(ri, &d, l_) < l_)
+		if (cache_.fetch(ri, &d, l_) < l_) {
+#pragma omp parallel for schedule(static) num_threads(g_threads) if (g_threads > 1 && l_ > 2048)
 			for (int j = 0; j < l_; j++) d[j] = (Qf)k_.eval(ri, j);
+		}
 		Qf *out = buf_[next_].data();
 		next_ = 1 - next_;
 		schar si = sign_[i];
@@ -1186,6 +1194,8 @@ void ora_set_options(const ora_options *opt)
 }
 
 void ora_set_verbose(int verbose) { g_print = verbose ? &print_to_stdout : &print_nothing; }
+
+void ora_set_threads(int threads) { g_threads = threads > 1 ? threads : 1; }
 
 /* One svm_train_one solve with raw outputs.  Caller owns out->alpha
  * (length prob->l) and may pass alpha_raw / G of length n_var (l or 2l)

@@ -142,3 +142,25 @@ def test_server_nu_initialisation_split_over_stages(hip, ora, k):
     np.testing.assert_allclose(coef, ref["alpha"], rtol=0, atol=1e-9)
     assert abs(c["obj"] - ref["obj"]) <= 1e-9 * abs(ref["obj"])
     assert abs(c["rho"] - ref["rho"]) <= 1e-9
+
+
+def test_server_release_model_before_fetching_prediction(hip):
+    """A finished prediction stays readable after its model is released: the
+    request keeps the model's two labels."""
+    r = datasets.serving_request(5, n_range=(600, 1200))
+    prob, prm = request_problem(r), request_param(r)
+    s = hip.Server()
+    mid = s.submit_train(prob, prm)
+    s.run_until_idle()
+    ip, ix, v = datasets.serving_rows(77, 300, r["d"], r["density"])
+    test = ffi.Problem(ip, ix, v, np.zeros(300), r["d"])
+    pid = s.submit_predict(mid, test)
+    s.run_until_idle()
+    dec0, lab0 = s.predict_result(pid, 300)
+    s.release(mid)
+    dec1, lab1 = s.predict_result(pid, 300)
+    np.testing.assert_array_equal(dec0, dec1)
+    np.testing.assert_array_equal(lab0, lab1)
+    assert set(np.unique(lab1)) <= set(np.unique(prob.y))
+    s.release(pid)
+    s.close()

@@ -7,6 +7,8 @@ evaluates exactly the oracle's arithmetic (FP64 G/alpha, float kernel rows
 summed in LIBSVM's order) the iteration counts and the working-set sequence
 are expected to be identical too, and are asserted.
 """
+import os
+
 import numpy as np
 import pytest
 
@@ -287,26 +289,63 @@ def certify(hip, prob, prm, r, C, sample=96, seed=0):
     assert worst <= 1e-9 * max(1.0, a.sum())
 
 
-def test_certificate_blob_csr_60k_4gb_cache(hip):
+GOLDEN = os.path.join(os.path.dirname(os.path.abspath(__file__)), "golden")
+
+
+def compare_fullsize(dev, name, C):
+    """The device solve against the oracle's full-size results (fixture
+    written by tests/golden/make_fullsize_fixtures.py): the reference-exact
+    variant (Q cached as float) must be matched iteration for iteration with
+    identical support vectors, objective within 1e-6 relative, rho within
+    1e-5 and alpha within 1e-6 C; the error against the pure-FP64 variant is
+    printed and held to the same objective / rho tolerances."""
+    fx = np.load(os.path.join(GOLDEN, f"fullsize_{name}.npz"))
+    a = dev["alpha_raw"]
+    sv = np.nonzero(a > 0)[0]
+    for variant in ("float", "double"):
+        obj, rho, it = float(fx[f"{variant}_obj"]), float(fx[f"{variant}_rho"]), int(fx[f"{variant}_iters"])
+        same_sv = np.array_equal(sv, fx[f"{variant}_sv"])
+        da = np.max(np.abs(a[fx[f"{variant}_sv"]] - fx[f"{variant}_alpha_sv"])) if same_sv else float("nan")
+        print(f"  {name} vs oracle ({variant} Q): iters {dev['iters']} / {it}, obj rel err "
+              f"{abs(dev['obj'] - obj) / abs(obj):.2e}, rho abs err {abs(dev['rho'] - rho):.2e}, "
+              f"SV sets identical: {same_sv} ({len(sv)} / {len(fx[f'{variant}_sv'])}), max |alpha diff| {da:.2e}",
+              flush=True)
+        assert abs(dev["obj"] - obj) <= 1e-6 * abs(obj)
+        assert abs(dev["rho"] - rho) <= 1e-5
+        if variant == "float":
+            assert dev["iters"] == it
+            assert same_sv
+            assert da <= 1e-6 * C
+
+
+def test_fullsize_blob_csr_60k_4gb_cache(hip):
     """The plain sparse configuration at full size: 60,000 x 784 CSR rows,
-    RBF gamma=0.02, C=10, eps=1e-3, a 4 GB row cache (LRU under pressure)."""
+    RBF gamma=0.02, C=10, eps=1e-3, a 4 GB row cache (LRU under pressure),
+    against the oracle's full-size solve and the solution certificate."""
     ip, ix, v, y, d = datasets.blob_mask_csr(n=60000)
     prob = ffi.Problem(ip, ix, v, y, d)
     prm = ffi.make_param(gamma=0.02, C_=10.0, eps=1e-3, shrinking=0, cache_size=4096.0)
     r = hip.solve_one(prob, prm, cache_bytes=4 * 2 ** 30)
     assert r["stats"]["optimal"] == 1
     assert r["stats"]["row_misses"] > prob.l  # the cache held only part of Q
+    compare_fullsize(r, "blob_csr_60k", 10.0)
     certify(hip, prob, prm, r, 10.0)
 
 
-def test_certificate_gmm_dense_100k(hip):
-    """The n=100k dense RBF configuration at full size (gamma=1/256, C=1)."""
+def test_fullsize_gmm_dense_100k(hip):
+    """The n=100k dense RBF configuration at full size (gamma=1/256, C=1),
+    with every row cached and with a 4 GB LRU cache, against the oracle's
+    full-size solve and the solution certificate."""
     X, y = datasets.gmm_dense_matrix(n=100000, d=256, seed=1003)
     prob = ffi.Problem.from_dense(X.astype(np.float64), y, keep_zeros=True)
     prm = ffi.make_param(gamma=1.0 / 256, C_=1.0, eps=1e-3, shrinking=0)
-    r = hip.solve_one(prob, prm, cache_bytes=4 * 2 ** 30)
-    assert r["stats"]["optimal"] == 1
-    certify(hip, prob, prm, r, 1.0)
+    full = hip.solve_one(prob, prm, cache_bytes=64 * 2 ** 30)
+    assert full["stats"]["optimal"] == 1
+    compare_fullsize(full, "gmm_dense_100k", 1.0)
+    lru = hip.solve_one(prob, prm, cache_bytes=4 * 2 ** 30)
+    assert lru["iters"] == full["iters"]
+    np.testing.assert_array_equal(lru["alpha_raw"], full["alpha_raw"])
+    certify(hip, prob, prm, lru, 1.0)
 
 
 def test_kkt_certificate_blob_csr_20k(hip):
@@ -325,3 +364,51 @@ def test_kkt_certificate_blob_csr_20k(hip):
     M_low = np.min(-yy[low] * G[low])
     print(f"  iters={r['iters']} gap={m_up - M_low:.3e} solve_ms={r['stats']['solve_ms']:.1f}", flush=True)
     assert m_up - M_low < 1e-3 + 1e-9
+
+
+def test_repeated_solves_reuse_buffers(hip, ora):
+    """setup + solve ten times on one handle: the first solve sizes the
+    buffers, the next nine allocate nothing on the device and give
+    bit-identical results (the oracle's)."""
+    prob, d = zipf_problem(1500)
+    prm = ffi.make_param(gamma=1.0 / d, C_=1.0, eps=1e-3, shrinking=0)
+    y, p, a0, Cp, Cn, kind = hip.formulate(prob, prm)
+    st0 = hip.initial_status(y, a0, Cp, Cn)
+    ref = ora.train_one(prob, prm)
+    for cache in (1e9, 200 * 4 * 1536):  # full cache, then an LRU cache of 200 rows
+        s = hip.DeviceSolver(prob, prm, kind=kind, cache_bytes=cache)
+        s.setup_rbf_variables()
+        allocs, first = [], None
+        for k in range(10):
+            s.setup_solver(y, p, a0, st0, Cp, Cn)
+            st = s.solve()
+            G, a, _ = s.fetch_vectors()
+            allocs.append(st["device_allocs"])
+            if first is None:
+                first = (st["iters"], a.copy(), st["obj"], st["rho"])
+            else:
+                assert st["iters"] == first[0]
+                np.testing.assert_array_equal(a, first[1])
+                assert st["obj"] == first[2] and st["rho"] == first[3]
+        s.close()
+        print(f"  cache={cache:g}: device allocations after each solve {allocs}", flush=True)
+        assert allocs[1:] == [allocs[0]] * 9
+        assert first[0] == ref["iters"]
+        assert abs(first[2] - ref["obj"]) <= 1e-9 * abs(ref["obj"])
+
+
+@pytest.mark.parametrize("svm_type", [ffi.NU_SVC, ffi.NU_SVR])
+def test_nu_ginit_many_rows_matches_oracle(hip, ora, svm_type):
+    """nu formulations start with ~nu*l nonzero alphas: G = p + Q alpha0 is
+    built by one multi-row launch; the iterates stay the oracle's."""
+    reg = svm_type == ffi.NU_SVR
+    X, yc = datasets.gmm_dense_matrix(n=6000, d=40, seed=11)
+    if reg:
+        t = (X[:, 0] * 0.5 + np.sin(X[:, 1])).astype(np.float32).astype(np.float64)
+        prob = ffi.Problem.from_dense(X.astype(np.float64), t, keep_zeros=True)
+    else:
+        prob = ffi.Problem.from_dense(X.astype(np.float64), yc, keep_zeros=True)
+    prm = ffi.make_param(svm_type=svm_type, gamma=1.0 / 40, C_=1.0, nu=0.5, shrinking=0)
+    ref = ora.train_one(prob, prm)
+    dev = hip.solve_one(prob, prm)
+    assert_solution_close(dev, ref)

@@ -218,3 +218,32 @@ def test_planted_mistake_is_caught(ora):
     prm = ffi.make_param(gamma=0.5, C_=1.5, eps=1e-10, shrinking=0)
     r = ora.train_one(prob, prm, q_double=True)
     assert abs(r["obj"] - f) > 1e-7 * abs(f)
+
+
+def test_fullsize_fixtures_are_feasible_and_consistent():
+    """The committed full-size oracle results (tests/golden/fullsize_*.npz)
+    are feasible points of their problems: 0 < alpha_sv <= C, y'alpha = 0,
+    the two oracle variants (Q cached as float / as double) agree on the
+    objective to 1e-9 relative, and the support-vector index lists are
+    sorted, unique and in range."""
+    import os
+    from libsvm_hip import datasets
+    golden = os.path.join(os.path.dirname(os.path.abspath(__file__)), "golden")
+    cases = {"gmm_dense_100k": (lambda: datasets.gmm_dense_matrix(n=100000, d=256, seed=1003)[1], 1.0),
+             "blob_csr_60k": (lambda: datasets.blob_mask_csr(n=60000)[3], 10.0)}
+    seen = 0
+    for name, (labels, C) in cases.items():
+        path = os.path.join(golden, f"fullsize_{name}.npz")
+        if not os.path.exists(path):
+            continue
+        seen += 1
+        fx = np.load(path)
+        y = np.where(labels() > 0, 1.0, -1.0)
+        for v in ("float", "double"):
+            sv, a = fx[f"{v}_sv"], fx[f"{v}_alpha_sv"]
+            assert len(sv) == int(fx[f"{v}_nSV"]) and np.all(np.diff(sv) > 0)
+            assert sv[0] >= 0 and sv[-1] < len(y)
+            assert np.all(a > 0) and np.all(a <= C)
+            assert abs(np.dot(y[sv], a)) <= 1e-9 * a.sum()
+        assert abs(float(fx["float_obj"]) - float(fx["double_obj"])) <= 1e-9 * abs(float(fx["float_obj"]))
+    assert seen >= 1
