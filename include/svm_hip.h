@@ -214,10 +214,15 @@ const char *svmhip_batch_last_error(const svmhip_batch *b);
  * reference only has on the CPU.  dec_values receives n x npair values
  * (npair = nr_class*(nr_class-1)/2, or 1 for one-class / regression),
  * labels (may be null) the predicted label per row as svm_predict returns
- * it.  Kernel values use FP64 arithmetic on float32 features; PRECOMPUTED
- * kernels and rows wider than 16384 features return SVMHIP_ERR_ARG. */
+ * it.  Kernel values are computed in FP64 from the caller's FP64 features as
+ * Kernel::k_function computes them (libsvm/svm.cpp:327-384), and each
+ * decision value is summed in svm_predict_values' order.  PRECOMPUTED
+ * kernels and rows wider than the prediction kernel's LDS allows (about
+ * 19,000 features) return SVMHIP_ERR_ARG.  Runs on the device's null stream
+ * and returns when the results are on the host. */
 svmhip_status svmhip_predict_batch(const struct svm_model *model, const struct svm_node *const *x, int n,
 				   const svmhip_options *opt, double *dec_values, double *labels);
+
 
 /* ---------------------------------------------------------------------------
  * Serving.  No counterpart in the reference (one solve per process, host

@@ -1,0 +1,296 @@
+// Batched prediction on the device: svmhip_predict_batch, the batched form of
+// svm_predict_values (libsvm/svm.cpp:2569-2643, which the reference runs on
+// the CPU only).
+//
+// It packs the model's support vectors and the test rows on the host in the
+// SELL-64 layout with the caller's FP64 feature values (svm_predict_values
+// works on the svm_node doubles; only training truncates features to float,
+// as the reference uploads them), copies them to device memory, runs
+// k_predict and copies the decision values back.
+
+#include "runtime_internal.hpp"
+
+namespace {
+
+// Rows in SELL-64 with FP64 values (host side), as smo_device.hpp lays them
+// out: element p of row 64s+lane at vals[sp[s] + 64p + lane].
+struct Sell64 {
+  int l = 0, d = 1, nslices = 0, dense = 0;
+  std::vector<long long> sp;  // nslices + 1
+  std::vector<double> vals;
+  std::vector<int> cols;      // empty when dense
+  std::vector<int> rl;        // stored entries per row
+};
+
+bool pack_sell64(const svm_node *const *x, int n, Sell64 &o, std::string &err) {
+  std::vector<int> len(std::max(n, 0));
+  long long nnz = 0;
+  int maxidx = -1;
+  for (int r = 0; r < n; ++r) {
+    const svm_node *p = x[r];
+    if (!p) {
+      err = "row " + std::to_string(r) + " is null";
+      return false;
+    }
+    int k = 0, prev = INT_MIN;
+    for (; p[k].index != -1; ++k) {
+      if (p[k].index < 0 || p[k].index <= prev) {
+        err = "row " + std::to_string(r) + ": feature indices must be >= 0 and ascending";
+        return false;
+      }
+      prev = p[k].index;
+      maxidx = std::max(maxidx, p[k].index);
+    }
+    len[r] = k;
+    nnz += k;
+  }
+  o.l = n;
+  o.d = std::max(maxidx + 1, 1);
+  o.dense = (n > 0 && (double)nnz >= 0.5 * (double)n * (double)o.d) ? 1 : 0;
+  o.nslices = (n + smo::SLICE - 1) / smo::SLICE;
+  o.sp.assign(o.nslices + 1, 0);
+  o.rl.assign(n, 0);
+  for (int s = 0; s < o.nslices; ++s) {
+    int w = 0;
+    for (int r = s * smo::SLICE; r < std::min(n, (s + 1) * smo::SLICE); ++r) {
+      o.rl[r] = o.dense ? o.d : len[r];
+      w = std::max(w, o.rl[r]);
+    }
+    o.sp[s + 1] = o.sp[s] + (long long)w * smo::SLICE;
+  }
+  o.vals.assign(std::max<long long>(o.sp[o.nslices], 1), 0.0);
+  o.cols.clear();
+  if (!o.dense) o.cols.assign(std::max<long long>(o.sp[o.nslices], 1), 0);
+  for (int r = 0; r < n; ++r) {
+    const long long base = o.sp[r / smo::SLICE] + (r % smo::SLICE);
+    for (int k = 0; k < len[r]; ++k) {
+      if (o.dense) {
+        o.vals[base + (long long)x[r][k].index * smo::SLICE] = x[r][k].value;
+      } else {
+        o.vals[base + (long long)k * smo::SLICE] = x[r][k].value;
+        o.cols[base + (long long)k * smo::SLICE] = x[r][k].index;
+      }
+    }
+  }
+  return true;
+}
+
+// Device memory of one prediction: hipMalloc'd blocks freed at the end, or
+// consecutive 256-byte aligned pieces of the no-alloc scratch area.
+struct Arena {
+  bool scratch = false;
+  char *base = nullptr;
+  size_t cap = 0, used = 0;
+  std::vector<void *> owned;
+  void *take(size_t bytes) {
+    bytes = std::max<size_t>((bytes + 255) / 256 * 256, 256);
+    if (scratch) {
+      if (used + bytes > cap) return nullptr;
+      void *p = base + used;
+      used += bytes;
+      return p;
+    }
+    void *p = nullptr;
+    if (hipMalloc(&p, bytes) != hipSuccess) {
+      (void)hipGetLastError();
+      return nullptr;
+    }
+    owned.push_back(p);
+    return p;
+  }
+  static size_t rounded(size_t bytes) { return std::max<size_t>((bytes + 255) / 256 * 256, 256); }
+  ~Arena() {
+    for (void *p : owned) (void)hipFree(p);
+  }
+};
+
+struct PredictFail {
+  svmhip_status code;
+};
+
+[[noreturn]] void pfail(svmhip_status code, const char *fmt, ...) {
+  char buf[512];
+  va_list ap;
+  va_start(ap, fmt);
+  vsnprintf(buf, sizeof buf, fmt, ap);
+  va_end(ap);
+  fprintf(stderr, "svmhip_predict: %s\n", buf);
+  throw PredictFail{code};
+}
+
+void pck(hipError_t e, const char *what) {
+  if (e != hipSuccess) {
+    (void)hipGetLastError();
+    pfail(e == hipErrorOutOfMemory ? SVMHIP_ERR_NOMEM : SVMHIP_ERR_DEVICE, "%s: %s", what, hipGetErrorString(e));
+  }
+}
+
+template <class T>
+T *put(Arena &ar, const T *src, size_t n, const char *what) {
+  T *p = (T *)ar.take(std::max<size_t>(n, 1) * sizeof(T));
+  if (!p)
+    pfail(SVMHIP_ERR_NOMEM, "%s: %zu bytes do not fit the %s", what, std::max<size_t>(n, 1) * sizeof(T),
+          ar.scratch ? "no-alloc scratch area" : "device");
+  if (n) pck(hipMemcpy(p, src, n * sizeof(T), hipMemcpyHostToDevice), what);
+  return p;
+}
+
+// Device bytes of rows [r0, r1) of a packed operand (slice-aligned r0).
+size_t rows_bytes(const Sell64 &R, int s0, int s1) {
+  const size_t vals = (size_t)(R.sp[s1] - R.sp[s0]);
+  const int r0 = s0 * smo::SLICE, r1 = std::min(R.l, s1 * smo::SLICE);
+  return Arena::rounded(8 * (size_t)(s1 - s0 + 1)) + Arena::rounded(8 * vals) +
+         (R.dense ? 0 : Arena::rounded(4 * vals)) + Arena::rounded(4 * (size_t)(r1 - r0));
+}
+
+// Uploads slices [s0, s1) of R and describes them as one PRows operand.
+smo::PRows upload_rows(Arena &ar, const Sell64 &R, int s0, int s1) {
+  const int r0 = s0 * smo::SLICE, r1 = std::min(R.l, s1 * smo::SLICE);
+  std::vector<long long> sp(s1 - s0 + 1);
+  for (int s = s0; s <= s1; ++s) sp[s - s0] = R.sp[s] - R.sp[s0];
+  const size_t nv = (size_t)(R.sp[s1] - R.sp[s0]);
+  smo::PRows P;
+  memset(&P, 0, sizeof P);
+  P.l = r1 - r0;
+  P.d = R.d;
+  P.nslices = s1 - s0;
+  P.dense = R.dense;
+  P.f64 = 1;
+  P.slice_ptr = put(ar, sp.data(), sp.size(), "slice pointers");
+  P.vals = put(ar, R.vals.data() + R.sp[s0], nv, "feature values");
+  P.cols = R.dense ? nullptr : put(ar, R.cols.data() + R.sp[s0], nv, "feature columns");
+  P.row_len = put(ar, R.rl.data() + r0, (size_t)(r1 - r0), "row lengths");
+  return P;
+}
+
+// The prediction both entry points run; ar provides the device memory.
+svmhip_status predict_run(const svm_model *model, const svm_node *const *x, int n, const svmhip_options *opt,
+                          double *dec_values, double *labels, Arena &ar) {
+  if (!model || (n > 0 && (!x || !dec_values))) return SVMHIP_ERR_ARG;
+  if (model->param.kernel_type == PRECOMPUTED) return SVMHIP_ERR_ARG;
+  const int st_type = model->param.svm_type;
+  const bool reg = st_type == ONE_CLASS || st_type == EPSILON_SVR || st_type == NU_SVR;
+  const int k = model->nr_class;
+  const int npair = reg ? 1 : k * (k - 1) / 2;
+  if (n == 0) return SVMHIP_OK;
+  const int nsv = model->l;
+  try {
+    if (nsv == 0) {  // no support vectors: every decision value is -rho
+      for (int t = 0; t < n; ++t)
+        for (int q = 0; q < npair; ++q) dec_values[(long long)t * npair + q] = -model->rho[q];
+    } else {
+      if (svmhip_device_count() <= 0) pfail(SVMHIP_ERR_DEVICE, "no HIP device available");
+      const int dev = opt ? opt->device : 0;
+      pck(hipSetDevice(dev), "hipSetDevice");
+      Sell64 S, T;
+      std::string err;
+      if (!pack_sell64(model->SV, nsv, S, err)) pfail(SVMHIP_ERR_ARG, "support vectors: %s", err.c_str());
+      if (!pack_sell64(x, n, T, err)) pfail(SVMHIP_ERR_ARG, "test rows: %s", err.c_str());
+      const int ld = std::max(S.d, T.d);
+      size_t shmem = 0;
+      // dynamic LDS: the device's per-workgroup limit less the kernel's own
+      const int tb = smo_predict_tb(ld, (size_t)device_max_lds(dev) - 1024, &shmem);
+      if (tb < 1) pfail(SVMHIP_ERR_ARG, "rows of %d features are too wide for the prediction kernel", ld);
+      const int ncoef = reg ? 1 : k - 1;
+      std::vector<double> coef((size_t)ncoef * nsv);
+      for (int c = 0; c < ncoef; ++c)
+        for (int s = 0; s < nsv; ++s) coef[(size_t)c * nsv + s] = model->sv_coef[c][s];
+      std::vector<int> cstart(std::max(k, 1), 0), ccount(std::max(k, 1), 0);
+      if (!reg) {
+        for (int c = 0; c < k; ++c) ccount[c] = model->nSV[c];
+        for (int c = 1; c < k; ++c) cstart[c] = cstart[c - 1] + ccount[c - 1];
+      }
+      smo::PredictArgs A;
+      memset(&A, 0, sizeof A);
+      A.sv = upload_rows(ar, S, 0, S.nslices);
+      A.coef = put(ar, coef.data(), coef.size(), "coefficients");
+      A.rho = put(ar, model->rho, (size_t)npair, "rho");
+      A.cls_start = put(ar, cstart.data(), cstart.size(), "class starts");
+      A.cls_count = put(ar, ccount.data(), ccount.size(), "class counts");
+      A.ktype = model->param.kernel_type;
+      A.degree = model->param.degree;
+      A.gamma = model->param.gamma;
+      A.coef0 = model->param.coef0;
+      A.k = k;
+      A.npair = npair;
+      A.tb = tb;
+      A.regression_like = reg ? 1 : 0;
+      A.ld = ld;
+      // test rows in chunks of whole slices: all of them when memory is
+      // allocated, as many as the scratch area has room for otherwise
+      const size_t mark = ar.used;
+      for (int s0 = 0; s0 < T.nslices;) {
+        int s1 = T.nslices;
+        if (ar.scratch) {
+          auto need = [&](int e) {
+            const int rows = std::min(T.l, e * smo::SLICE) - s0 * smo::SLICE;
+            return rows_bytes(T, s0, e) + Arena::rounded(8 * (size_t)rows * npair);
+          };
+          const size_t room = ar.cap - mark;
+          if (need(s0 + 1) > room)
+            pfail(SVMHIP_ERR_NOMEM,
+                  "the model (%d support vectors of %d features) leaves no room for one slice of test rows in "
+                  "the %zu-byte no-alloc scratch area",
+                  nsv, S.d, ar.cap);
+          int lo = s0 + 1, hi = T.nslices;  // largest e with need(e) <= room
+          while (lo < hi) {
+            const int mid = (lo + hi + 1) / 2;
+            if (need(mid) <= room) lo = mid;
+            else hi = mid - 1;
+          }
+          s1 = lo;
+          ar.used = mark;
+        }
+        const int r0 = s0 * smo::SLICE, r1 = std::min(T.l, s1 * smo::SLICE);
+        A.tv = upload_rows(ar, T, s0, s1);
+        A.dec = (double *)ar.take(sizeof(double) * (size_t)(r1 - r0) * npair);
+        if (!A.dec) pfail(SVMHIP_ERR_NOMEM, "decision values do not fit");
+        const int nb = (r1 - r0 + tb - 1) / tb;
+        pck(smo_launch_predict_offset(&A, 0, nb, shmem, 0), "prediction kernel");
+        pck(hipMemcpy(dec_values + (long long)r0 * npair, A.dec, sizeof(double) * (size_t)(r1 - r0) * npair,
+                      hipMemcpyDeviceToHost),
+            "fetch decision values");
+        s0 = s1;
+      }
+    }
+  } catch (const PredictFail &f) {
+    return f.code;
+  } catch (const std::bad_alloc &) {
+    fprintf(stderr, "svmhip_predict: host allocation failed\n");
+    return SVMHIP_ERR_NOMEM;
+  }
+  if (labels) {
+    std::vector<int> vote(std::max(k, 1));
+    for (int t = 0; t < n; ++t) {
+      const double *dv = dec_values + (long long)t * npair;
+      if (reg) {
+        labels[t] = st_type == ONE_CLASS ? (dv[0] > 0 ? 1 : -1) : dv[0];
+        continue;
+      }
+      std::fill(vote.begin(), vote.end(), 0);
+      int q = 0;
+      for (int a = 0; a < k; ++a)
+        for (int b = a + 1; b < k; ++b, ++q) {
+          if (dv[q] > 0) ++vote[a];
+          else ++vote[b];
+        }
+      int best = 0;
+      for (int c = 1; c < k; ++c)
+        if (vote[c] > vote[best]) best = c;
+      labels[t] = model->label[best];
+    }
+  }
+  return SVMHIP_OK;
+}
+
+}  // namespace
+
+extern "C" {
+
+svmhip_status svmhip_predict_batch(const svm_model *model, const svm_node *const *x, int n, const svmhip_options *opt,
+                                   double *dec_values, double *labels) {
+  Arena ar;
+  return predict_run(model, x, n, opt, dec_values, labels, ar);
+}
+
+}  // extern "C"

@@ -39,6 +39,7 @@ hipError_t smo_launch_batch(const smo::View *views, const int *items, int nitems
 hipError_t smo_batch_capacity(size_t shmem, int *blocks_per_cu);
 hipError_t smo_launch_predict_offset(const smo::PredictArgs *A, int first_row, int nblocks, size_t shmem,
                                     hipStream_t st);
+int smo_predict_tb(int ld, size_t lds_bytes, size_t *shmem);
 hipError_t smo_launch_pack_rowset(const smo::View *src, const int *rows, int n, const long long *dst_slice_ptr,
                                   float *dst_vals, int *dst_cols, int *dst_len, double *dst_xsq, hipStream_t st);
 }
@@ -306,6 +307,16 @@ inline double device_clock_hz(int dev) {
   int khz = 0;
   if (hipDeviceGetAttribute(&khz, hipDeviceAttributeWallClockRate, dev) != hipSuccess || khz <= 0) khz = 100000;
   return 1e3 * khz;
+}
+
+// Dynamic LDS one workgroup may use (bytes).
+inline int device_max_lds(int dev) {
+  int v = 0;
+  if (hipDeviceGetAttribute(&v, hipDeviceAttributeMaxSharedMemoryPerBlock, dev) != hipSuccess || v <= 0) {
+    (void)hipGetLastError();
+    v = 64 * 1024;
+  }
+  return v;
 }
 
 inline int device_cus(int dev) {

@@ -73,8 +73,9 @@ struct Request {
   int model_id = -1;
   double mlabels[2] = {0, 0};  // the model's two labels, copied at submission (the model may be released first)
   std::vector<double> dec;
-  double *ddec = nullptr, *kv = nullptr;
-  int tb = 1, chunk = 1;
+  double *ddec = nullptr;
+  int tb = 1;
+  size_t shmem = 0;
 };
 
 }  // namespace
@@ -107,6 +108,16 @@ namespace {
 void free_rows(svmhip_solver *h, DevRows &R) {
   for (void *p : {(void *)R.sp, (void *)R.vals, (void *)R.cols, (void *)R.len, (void *)R.xsq}) h->dfree(p);
   R = DevRows{};
+}
+
+// A resident model's or a predict request's rows as a prediction operand
+// (float values: the rows as training uploads them).
+smo::PRows prows(const DevRows &R) {
+  smo::PRows P;
+  memset(&P, 0, sizeof P);
+  P.l = R.l; P.d = R.d; P.nslices = R.nslices; P.dense = R.dense; P.f64 = 0;
+  P.slice_ptr = R.sp; P.vals = R.vals; P.cols = R.cols; P.row_len = R.len;
+  return P;
 }
 
 smo::View rows_view(const svmhip_solver *h, const DevRows &R, const Request &q) {
@@ -249,14 +260,10 @@ void admit_predict(svmhip_server *s, Request &q) {
   upload_rows(h, q.hs, q.rows);
   const int n = q.hs.l;
   const int d = std::max(q.rows.d, m.model.d);
-  q.tb = std::max(1, std::min(16, 16384 / d));
-  const int nbt = (n + q.tb - 1) / q.tb;
-  q.chunk = std::max(1, std::min(nbt, device_cus(h->dev)));
-  q.kv = h->dalloc<double>((size_t)q.chunk * q.tb * std::max(m.model.l, 1), "prediction scratch");
+  q.tb = smo_predict_tb(d, device_max_lds(h->dev) - 1024, &q.shmem);
+  if (q.tb < 1) h->fail(SVMHIP_ERR_ARG, "server: prediction rows of %d features are too wide", d);
   q.ddec = h->dalloc<double>(std::max(n, 1), "prediction output");
-  q.bytes = q.rows.bytes + 8.0 * q.chunk * q.tb * std::max(m.model.l, 1) + 8.0 * n;
-  smo::View V = rows_view(h, q.rows, m);
-  if (n > 0) h->ck(smo_launch_self_dot(&V, q.rows.xsq, h->st), "test rows x^2");
+  q.bytes = q.rows.bytes + 8.0 * n;
   h->ck(hipStreamSynchronize(h->st), "admit");
   q.hs = HostSell{};
   s->used += q.bytes;
@@ -266,11 +273,8 @@ void admit_predict(svmhip_server *s, Request &q) {
 
 double predict_footprint(const svmhip_server *s, const Request &q) {
   const Request &m = *s->reqs[q.model_id];
-  const int d = std::max(q.hs.d, m.model.d);
-  const int tb = std::max(1, std::min(16, 16384 / d));
-  const int chunk = std::max(1, std::min((q.hs.l + tb - 1) / tb, device_cus(s->g->dev)));
-  return sell_bytes(q.hs.total, q.hs.l, q.hs.nslices, q.hs.dense) + 8.0 * chunk * tb * std::max(m.model.l, 1) +
-         8.0 * q.hs.l;
+  (void)m;
+  return sell_bytes(q.hs.total, q.hs.l, q.hs.nslices, q.hs.dense) + 8.0 * q.hs.l;
 }
 
 // Packs the support vectors (class +1 first, then -1, each in row order, as
@@ -369,8 +373,12 @@ void run_predict(svmhip_server *s, Request &q) {
   }
   smo::PredictArgs A;
   memset(&A, 0, sizeof A);
-  A.sv = rows_view(h, m.model, m);
-  A.tv = rows_view(h, q.rows, m);
+  A.sv = prows(m.model);
+  A.tv = prows(q.rows);
+  A.ktype = m.ktype;
+  A.degree = m.degree;
+  A.gamma = m.gamma;
+  A.coef0 = m.coef0;
   A.coef = m.mcoef;
   A.rho = m.mrho;
   A.cls_start = m.mcs;
@@ -379,13 +387,10 @@ void run_predict(svmhip_server *s, Request &q) {
   A.npair = 1;
   A.tb = q.tb;
   A.regression_like = 0;
-  A.kv = q.kv;
+  A.ld = std::max(q.rows.d, m.model.d);
   A.dec = q.ddec;
-  const int d = std::max(q.rows.d, m.model.d);
-  const size_t shmem = (size_t)q.tb * d * sizeof(float);
   const int nbt = (n + q.tb - 1) / q.tb;
-  for (int b0 = 0; b0 < nbt; b0 += q.chunk)
-    h->ck(smo_launch_predict_offset(&A, b0 * q.tb, std::min(q.chunk, nbt - b0), shmem, h->st), "prediction kernel");
+  h->ck(smo_launch_predict_offset(&A, 0, nbt, q.shmem, h->st), "prediction kernel");
 }
 
 // After the round's synchronisation: decision values to the host.
@@ -394,9 +399,8 @@ void finish_predict(svmhip_server *s, Request &q) {
   if (s->reqs[q.model_id]->model.l > 0 && !q.dec.empty())
     h->ck(hipMemcpy(q.dec.data(), q.ddec, sizeof(double) * q.dec.size(), hipMemcpyDeviceToHost), "fetch decisions");
   free_rows(h, q.rows);
-  h->dfree(q.kv);
   h->dfree(q.ddec);
-  q.kv = q.ddec = nullptr;
+  q.ddec = nullptr;
   s->used -= q.bytes;
   q.bytes = 0;
   complete(s, q, SVMHIP_OK);
@@ -562,8 +566,8 @@ int svmhip_server_submit_predict(svmhip_server *s, int model_id, const svm_node 
     q->mlabels[0] = m.labels[0];
     q->mlabels[1] = m.labels[1];
     make_host_sell(h, &pt, m.ktype, q->hs);
-    if ((size_t)std::max(q->hs.d, m.model.d) * sizeof(float) > 16384 * sizeof(float))
-      h->fail(SVMHIP_ERR_ARG, "server: prediction rows wider than 16384 features");
+    if (smo_predict_tb(std::max(q->hs.d, m.model.d), device_max_lds(h->dev) - 1024, nullptr) < 1)
+      h->fail(SVMHIP_ERR_ARG, "server: prediction rows of %d features are too wide", std::max(q->hs.d, m.model.d));
     GUARD_END
   }();
   if (rc != SVMHIP_OK) {

@@ -1537,16 +1537,30 @@ __device__ __forceinline__ void decide_j(const View &V, Dec &d, const XPart &r) 
   d.pend = 1;
 }
 
-// Arguments of the batched prediction kernel.
+// One operand of the prediction kernel: rows in SELL-64 (the solver's
+// layout) with float (f64 = 0: the serving runtime's resident models, stored
+// as training uploads them) or double (f64 = 1: svmhip_predict_batch, the
+// caller's FP64 features as svm_predict_values uses them) values.
+struct PRows {
+  int l, d, nslices, dense, f64;
+  const long long *slice_ptr;
+  const void *vals;
+  const int *cols;         // null when dense
+  const int *row_len;
+};
+
+// Arguments of the prediction kernel.
 struct PredictArgs {
-  View sv;                 // support vectors (l = nSV), kernel parameters
-  View tv;                 // test rows
+  PRows sv;                // support vectors (l = nSV)
+  PRows tv;                // test rows
+  int ktype, degree;
+  double gamma, coef0;
   const double *coef;      // [(k-1) x nsv]
   const double *rho;       // [npair]
   const int *cls_start;    // [k] first SV of each class (classification)
   const int *cls_count;    // [k]
   int k, npair, tb, regression_like;
-  double *kv;              // scratch: gridDim x tb x nsv
+  int ld;                  // doubles per test row staged in LDS (>= both operands' d)
   double *dec;             // [ntest x npair]
 };
 

@@ -980,118 +980,3 @@ void svmhip_batch_destroy(svmhip_batch *b) {
 }
 
 }  // extern "C"
-
-extern "C" {
-
-svmhip_status svmhip_predict_batch(const svm_model *model, const svm_node *const *x, int n, const svmhip_options *opt,
-                                   double *dec_values, double *labels) {
-  if (!model || (n > 0 && (!x || !dec_values))) return SVMHIP_ERR_ARG;
-  if (model->param.kernel_type == PRECOMPUTED) return SVMHIP_ERR_ARG;
-  const int st_type = model->param.svm_type;
-  const bool reg = st_type == ONE_CLASS || st_type == EPSILON_SVR || st_type == NU_SVR;
-  const int k = model->nr_class;
-  const int npair = reg ? 1 : k * (k - 1) / 2;
-  if (n == 0) return SVMHIP_OK;
-  const int nsv = model->l;
-  if (nsv == 0) {  // no support vectors: every decision value is -rho
-    for (int t = 0; t < n; ++t)
-      for (int q = 0; q < npair; ++q) dec_values[(long long)t * npair + q] = -model->rho[q];
-  } else {
-    std::vector<double> dummy_sv(nsv, 0.0), dummy_t(n, 0.0);
-    svm_problem ps{nsv, dummy_sv.data(), model->SV};
-    svm_problem pt{n, dummy_t.data(), const_cast<svm_node **>(x)};
-    svm_parameter prm = model->param;
-    svmhip_solver *hs = nullptr, *ht = nullptr;
-    svmhip_status rc = svmhip_create(&ps, &prm, SVMHIP_KIND_C, opt, &hs);
-    if (rc != SVMHIP_OK) return rc;
-    rc = svmhip_create(&pt, &prm, SVMHIP_KIND_C, opt, &ht);
-    if (rc != SVMHIP_OK) {
-      svmhip_destroy(hs);
-      return rc;
-    }
-    svmhip_solver *h = hs;
-    rc = [&]() -> svmhip_status {
-      GUARD_BEGIN
-      svmhip_status s1 = svmhip_setup_rbf(hs, nsv);
-      if (s1 != SVMHIP_OK) throw Failure{s1};
-      s1 = svmhip_setup_rbf(ht, n);
-      if (s1 != SVMHIP_OK) throw Failure{s1};
-      h->ck(hipStreamSynchronize(ht->st), "test rows");
-      const int d = std::max(hs->d, ht->d);
-      if (d > 16384) h->fail(SVMHIP_ERR_ARG, "prediction rows wider than 16384 features");
-      const int tb = std::max(1, std::min(16, 16384 / d));
-      const int ncoef = reg ? 1 : k - 1;
-      std::vector<double> coef((size_t)ncoef * nsv);
-      for (int c = 0; c < ncoef; ++c)
-        for (int s = 0; s < nsv; ++s) coef[(size_t)c * nsv + s] = model->sv_coef[c][s];
-      std::vector<int> cstart(std::max(k, 1), 0), ccount(std::max(k, 1), 0);
-      if (!reg) {
-        for (int c = 0; c < k; ++c) ccount[c] = model->nSV[c];
-        for (int c = 1; c < k; ++c) cstart[c] = cstart[c - 1] + ccount[c - 1];
-      }
-      double *d_coef = h->dalloc<double>(coef.size(), "coefficients");
-      double *d_rho = h->dalloc<double>(npair, "rho");
-      int *d_cs = h->dalloc<int>(cstart.size(), "class starts");
-      int *d_cc = h->dalloc<int>(ccount.size(), "class counts");
-      h->upload(d_coef, coef.data(), coef.size(), "upload coefficients");
-      h->upload(d_rho, model->rho, (size_t)npair, "upload rho");
-      h->upload(d_cs, cstart.data(), cstart.size(), "upload class starts");
-      h->upload(d_cc, ccount.data(), ccount.size(), "upload class counts");
-      const int nblocks_total = (n + tb - 1) / tb;
-      const int chunk = std::max(1, std::min(nblocks_total, 4 * device_cus(h->dev)));
-      double *d_kv = h->dalloc<double>((size_t)chunk * tb * nsv, "kernel-value scratch");
-      double *d_dec = h->dalloc<double>((size_t)n * npair, "decision values");
-      smo::PredictArgs A;
-      memset(&A, 0, sizeof A);
-      A.sv = hs->view();
-      A.tv = ht->view();
-      A.coef = d_coef;
-      A.rho = d_rho;
-      A.cls_start = d_cs;
-      A.cls_count = d_cc;
-      A.k = k;
-      A.npair = npair;
-      A.tb = tb;
-      A.regression_like = reg ? 1 : 0;
-      A.kv = d_kv;
-      const size_t shmem = (size_t)tb * d * sizeof(float);
-      A.dec = d_dec;
-      // the kv scratch holds `chunk` workgroups: launch the test rows in
-      // stream-ordered chunks that reuse it
-      for (int b0 = 0; b0 < nblocks_total; b0 += chunk)
-        h->ck(smo_launch_predict_offset(&A, b0 * tb, std::min(chunk, nblocks_total - b0), shmem, h->st),
-              "prediction kernel");
-      std::vector<double> dec((size_t)n * npair);
-      h->ck(hipMemcpyAsync(dec.data(), d_dec, sizeof(double) * dec.size(), hipMemcpyDeviceToHost, h->st), "fetch decisions");
-      h->ck(hipStreamSynchronize(h->st), "prediction");
-      memcpy(dec_values, dec.data(), sizeof(double) * dec.size());
-      GUARD_END
-    }();
-    svmhip_destroy(ht);
-    svmhip_destroy(hs);
-    if (rc != SVMHIP_OK) return rc;
-  }
-  if (labels) {
-    for (int t = 0; t < n; ++t) {
-      const double *dv = dec_values + (long long)t * npair;
-      if (reg) {
-        labels[t] = st_type == ONE_CLASS ? (dv[0] > 0 ? 1 : -1) : dv[0];
-        continue;
-      }
-      std::vector<int> vote(k, 0);
-      int q = 0;
-      for (int a = 0; a < k; ++a)
-        for (int b = a + 1; b < k; ++b, ++q) {
-          if (dv[q] > 0) ++vote[a];
-          else ++vote[b];
-        }
-      int best = 0;
-      for (int c = 1; c < k; ++c)
-        if (vote[c] > vote[best]) best = c;
-      labels[t] = model->label[best];
-    }
-  }
-  return SVMHIP_OK;
-}
-
-}  // extern "C"

@@ -400,21 +400,7 @@ class Model:
     def predict_batch(self, problem: Problem, rows=None):
         """Decision values and labels of many rows on the device
         (svmhip_predict_batch)."""
-        m = self.ptr.contents
-        k = m.nr_class
-        ndec = 1 if m.param.svm_type in (ONE_CLASS, EPSILON_SVR, NU_SVR) else k * (k - 1) // 2
-        rows = np.arange(problem.l) if rows is None else np.asarray(rows)
-        ptrs = (C.POINTER(ffi.svm_node) * max(len(rows), 1))()
-        for n, r in enumerate(rows):
-            ptrs[n] = problem.row_nodes(int(r))
-        dec = np.zeros((len(rows), ndec))
-        lab = np.zeros(len(rows))
-        opt = svmhip_options(C.sizeof(svmhip_options), 0, 0, 0, 0, 0.0, 0, 0)
-        rc = lib().svmhip_predict_batch(self.ptr, ptrs, len(rows), C.byref(opt), _ptr(dec, C.c_double),
-                                        _ptr(lab, C.c_double))
-        if rc != 0:
-            raise SolverError(f"svmhip_predict_batch: {STATUS_NAMES.get(rc, rc)}")
-        return lab, dec
+        return predict_batch(self.ptr, problem, rows)
 
     def save(self, path):
         if lib().svm_save_model(path.encode(), self.ptr) != 0:
@@ -426,6 +412,26 @@ class Model:
                 lib().svm_free_and_destroy_model(C.byref(self.ptr))
         except Exception:
             pass
+
+
+def predict_batch(model_ptr, problem: Problem, rows=None):
+    """Decision values and labels of problem's rows (all, or `rows`) against
+    the struct svm_model* model_ptr, on the device."""
+    m = model_ptr.contents
+    k = m.nr_class
+    ndec = 1 if m.param.svm_type in (ONE_CLASS, EPSILON_SVR, NU_SVR) else k * (k - 1) // 2
+    rows = np.arange(problem.l) if rows is None else np.asarray(rows)
+    ptrs = (C.POINTER(ffi.svm_node) * max(len(rows), 1))()
+    for n, r in enumerate(rows):
+        ptrs[n] = problem.row_nodes(int(r))
+    dec = np.zeros((len(rows), ndec))
+    lab = np.zeros(len(rows))
+    opt = svmhip_options(C.sizeof(svmhip_options), 0, 0, 0, 0, 0.0, 0, 0)
+    rc = lib().svmhip_predict_batch(model_ptr, ptrs, len(rows), C.byref(opt), _ptr(dec, C.c_double),
+                                    _ptr(lab, C.c_double))
+    if rc != 0:
+        raise SolverError(f"svmhip_predict_batch: {STATUS_NAMES.get(rc, rc)}")
+    return lab, dec
 
 
 def svm_train(problem: Problem, param):

@@ -230,3 +230,38 @@ def test_ovo_cv_gamma_grid_full_size(hip, ora):
         assert res[k]["iters"] == ref["iters"], (len(rows), g)
         np.testing.assert_allclose(res[k]["alpha_raw"], ref["alpha_raw"], rtol=0, atol=1e-9)
         assert abs(res[k]["obj"] - ref["obj"]) <= 1e-9 * abs(ref["obj"])
+
+
+@pytest.mark.parametrize("dense", [True, False])
+@pytest.mark.parametrize("kname", ["rbf", "linear", "poly", "sigmoid"])
+def test_predict_batch_fp64_features_match_oracle(hip, ora, kname, dense):
+    """Features that are not float32 values, in the model's support vectors
+    and in the test rows: the device predicts from the FP64 features as
+    svm_predict_values does (kernel values in k_function's order, each
+    decision value summed in the reference's order), so its decision values
+    equal the oracle's to 1e-10 relative (device exp / tanh against glibc's
+    aside, the arithmetic is the same; sparse RBF rows add the terms in
+    another order)."""
+    kt = {"rbf": ffi.RBF, "poly": ffi.POLY, "linear": ffi.LINEAR, "sigmoid": ffi.SIGMOID}[kname]
+    rng = np.random.default_rng(21)
+    n, d = 900, 40
+
+    def rows(m):
+        X = rng.standard_normal((m, d)) * 1.37
+        if not dense:
+            X[rng.random((m, d)) < 0.7] = 0.0
+        return X
+
+    X = rows(n)
+    assert np.any(X.astype(np.float32).astype(np.float64) != X)
+    y = np.where(X[:, 0] + 0.5 * X[:, 1] > 0, 1.0, -1.0)
+    prob = ffi.Problem.from_dense(X, y)
+    prm = ffi.make_param(kernel_type=kt, gamma=1.0 / d, coef0=0.1, degree=2, C_=2.0, shrinking=0, cuda_flag=0)
+    m = ora.svm_train(prob, prm)
+    test = ffi.Problem.from_dense(rows(700), np.zeros(700))
+    lab_r, dec_r = m.predict_values(test)
+    lab_d, dec_d = hip.predict_batch(m.ptr, test)
+    err = np.max(np.abs(dec_d - dec_r))
+    print(f"  {kname} dense={dense}: nSV={m.arrays()['l']} max |dec - oracle| = {err:.2e}", flush=True)
+    assert err <= 1e-10 * max(1.0, np.max(np.abs(dec_r)))
+    np.testing.assert_array_equal(lab_d, lab_r)

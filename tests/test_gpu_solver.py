@@ -297,8 +297,11 @@ def compare_fullsize(dev, name, C):
     written by tests/golden/make_fullsize_fixtures.py): the reference-exact
     variant (Q cached as float) must be matched iteration for iteration with
     identical support vectors, objective within 1e-6 relative, rho within
-    1e-5 and alpha within 1e-6 C; the error against the pure-FP64 variant is
-    printed and held to the same objective / rho tolerances."""
+    1e-5 and alpha within 1e-6 C.  The error against the pure-FP64 variant
+    (Q cached as double) is printed; it takes another path through the
+    eps = 1e-3 stopping band, so its objective is held to 1e-6 relative and
+    its rho to the two oracle variants' own difference (8e-5 on the 60k
+    sparse problem) or 1e-5, whichever is larger."""
     fx = np.load(os.path.join(GOLDEN, f"fullsize_{name}.npz"))
     a = dev["alpha_raw"]
     sv = np.nonzero(a > 0)[0]
@@ -311,7 +314,8 @@ def compare_fullsize(dev, name, C):
               f"SV sets identical: {same_sv} ({len(sv)} / {len(fx[f'{variant}_sv'])}), max |alpha diff| {da:.2e}",
               flush=True)
         assert abs(dev["obj"] - obj) <= 1e-6 * abs(obj)
-        assert abs(dev["rho"] - rho) <= 1e-5
+        band = abs(float(fx["float_rho"]) - float(fx["double_rho"]))
+        assert abs(dev["rho"] - rho) <= (1e-5 if variant == "float" else max(1e-5, 1.5 * band))
         if variant == "float":
             assert dev["iters"] == it
             assert same_sv

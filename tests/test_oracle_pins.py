@@ -224,7 +224,8 @@ def test_fullsize_fixtures_are_feasible_and_consistent():
     """The committed full-size oracle results (tests/golden/fullsize_*.npz)
     are feasible points of their problems: 0 < alpha_sv <= C, y'alpha = 0,
     the two oracle variants (Q cached as float / as double) agree on the
-    objective to 1e-9 relative, and the support-vector index lists are
+    objective to 1e-6 relative (the accuracy bar; they take different paths
+    through the eps stopping band), and the support-vector index lists are
     sorted, unique and in range."""
     import os
     from libsvm_hip import datasets
@@ -245,5 +246,5 @@ def test_fullsize_fixtures_are_feasible_and_consistent():
             assert sv[0] >= 0 and sv[-1] < len(y)
             assert np.all(a > 0) and np.all(a <= C)
             assert abs(np.dot(y[sv], a)) <= 1e-9 * a.sum()
-        assert abs(float(fx["float_obj"]) - float(fx["double_obj"])) <= 1e-9 * abs(float(fx["float_obj"]))
+        assert abs(float(fx["float_obj"]) - float(fx["double_obj"])) <= 1e-6 * abs(float(fx["float_obj"]))
     assert seen >= 1
