@@ -223,6 +223,26 @@ const char *svmhip_batch_last_error(const svmhip_batch *b);
 svmhip_status svmhip_predict_batch(const struct svm_model *model, const struct svm_node *const *x, int n,
 				   const svmhip_options *opt, double *dec_values, double *labels);
 
+/* Bytes of device scratch held by svmhip_predict_batch_noalloc. */
+#define SVMHIP_NOALLOC_SCRATCH_BYTES (128u << 20)
+
+/* svmhip_predict_batch without device allocations.  No counterpart in the
+ * reference: for callers that cannot call hipMalloc (tight serving loops,
+ * harnesses that forbid allocation).  Same arguments, same kernels, same
+ * results as svmhip_predict_batch; the support vectors, coefficients, test
+ * rows and decision values are staged in a device scratch area of
+ * SVMHIP_NOALLOC_SCRATCH_BYTES (128 MiB) that the library holds on each
+ * device it runs this on until the process ends, the test rows in as many
+ * chunks as needed.  A model whose support vectors (8 bytes per stored
+ * feature, plus 4 for its column when sparse), coefficients and one 64-row
+ * slice of test rows do not fit in it returns SVMHIP_ERR_NOMEM, with the
+ * sizes on stderr.  All calls on one device share its scratch area and keep
+ * nothing in it between calls: the library serialises them with a lock, and
+ * device work of the same process that overlaps a call and touches the
+ * scratch area (there is none in this library) would race with it.  Host
+ * memory is used for packing the rows. */
+svmhip_status svmhip_predict_batch_noalloc(const struct svm_model *model, const struct svm_node *const *x, int n,
+					   const svmhip_options *opt, double *dec_values, double *labels);
 
 /* ---------------------------------------------------------------------------
  * Serving.  No counterpart in the reference (one solve per process, host

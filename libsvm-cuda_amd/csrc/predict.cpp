@@ -1,12 +1,15 @@
 // Batched prediction on the device: svmhip_predict_batch, the batched form of
 // svm_predict_values (libsvm/svm.cpp:2569-2643, which the reference runs on
-// the CPU only).
+// the CPU only), and svmhip_predict_batch_noalloc, the same computation
+// staged in a library-owned device scratch area instead of allocations.
 //
-// It packs the model's support vectors and the test rows on the host in the
+// Both pack the model's support vectors and the test rows on the host in the
 // SELL-64 layout with the caller's FP64 feature values (svm_predict_values
 // works on the svm_node doubles; only training truncates features to float,
-// as the reference uploads them), copies them to device memory, runs
-// k_predict and copies the decision values back.
+// as the reference uploads them), copy them to device memory, run k_predict
+// and copy the decision values back.  The two differ only in where the
+// device memory comes from, so their results are identical.
+#include <mutex>
 
 #include "runtime_internal.hpp"
 
@@ -283,6 +286,15 @@ svmhip_status predict_run(const svm_model *model, const svm_node *const *x, int 
   return SVMHIP_OK;
 }
 
+// The no-alloc variant's device memory: one array per device the library
+// runs on, held until the process ends.  Host code must not pass `scratch`
+// itself to a kernel or copy (in host code it names a host-side stand-in):
+// its device address, per device, comes from hipGetSymbolAddress.
+constexpr int kMaxDevices = 64;
+static __device__ __align__(256) unsigned char scratch[SVMHIP_NOALLOC_SCRATCH_BYTES];
+std::mutex g_scratch_mu;            // serialises the no-alloc calls: they share each device's array
+void *g_scratch_ptr[kMaxDevices];   // device address of scratch on each device (null: not looked up yet)
+
 }  // namespace
 
 extern "C" {
@@ -290,6 +302,35 @@ extern "C" {
 svmhip_status svmhip_predict_batch(const svm_model *model, const svm_node *const *x, int n, const svmhip_options *opt,
                                    double *dec_values, double *labels) {
   Arena ar;
+  return predict_run(model, x, n, opt, dec_values, labels, ar);
+}
+
+svmhip_status svmhip_predict_batch_noalloc(const svm_model *model, const svm_node *const *x, int n,
+                                           const svmhip_options *opt, double *dec_values, double *labels) {
+  const int dev = opt ? opt->device : 0;
+  if (dev < 0 || dev >= kMaxDevices) return SVMHIP_ERR_ARG;
+  std::lock_guard<std::mutex> lock(g_scratch_mu);
+  if (!g_scratch_ptr[dev] && model && model->l > 0 && n > 0) {
+    // once per device, with that device current
+    if (svmhip_device_count() <= dev) return SVMHIP_ERR_DEVICE;
+    if (hipSetDevice(dev) != hipSuccess) {
+      (void)hipGetLastError();
+      return SVMHIP_ERR_DEVICE;
+    }
+    void *p = nullptr;
+    const hipError_t e = hipGetSymbolAddress(&p, HIP_SYMBOL(scratch));
+    if (e != hipSuccess || !p) {
+      (void)hipGetLastError();
+      fprintf(stderr, "svmhip_predict_batch_noalloc: no device address for the scratch area: %s\n",
+              hipGetErrorString(e));
+      return SVMHIP_ERR_DEVICE;
+    }
+    g_scratch_ptr[dev] = p;
+  }
+  Arena ar;
+  ar.scratch = true;
+  ar.base = (char *)g_scratch_ptr[dev];
+  ar.cap = SVMHIP_NOALLOC_SCRATCH_BYTES;
   return predict_run(model, x, n, opt, dec_values, labels, ar);
 }
 

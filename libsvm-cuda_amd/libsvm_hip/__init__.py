@@ -74,6 +74,7 @@ SVM_HIP_SYMBOLS = [
     "svmhip_solve", "svmhip_finalize", "svmhip_get_stats", "svmhip_kernel_row", "svmhip_destroy",
     "svmhip_last_error", "svmhip_device_count", "svmhip_batch_create", "svmhip_batch_add_rowset",
     "svmhip_batch_solve", "svmhip_batch_destroy", "svmhip_batch_last_error", "svmhip_predict_batch",
+    "svmhip_predict_batch_noalloc",
     "svmhip_server_create", "svmhip_server_submit_train", "svmhip_server_submit_predict",
     "svmhip_server_step", "svmhip_server_poll", "svmhip_server_train_result",
     "svmhip_server_predict_result", "svmhip_server_release", "svmhip_server_bytes_in_use",
@@ -140,6 +141,9 @@ def lib():
     L.svmhip_batch_last_error.restype = C.c_char_p
     L.svmhip_predict_batch.argtypes = [P(ffi.svm_model), P(P(ffi.svm_node)), I, P(svmhip_options), P(D), P(D)]
     L.svmhip_predict_batch.restype = I
+    L.svmhip_predict_batch_noalloc.argtypes = [P(ffi.svm_model), P(P(ffi.svm_node)), I, P(svmhip_options), P(D),
+                                               P(D)]
+    L.svmhip_predict_batch_noalloc.restype = I
     L.svmhip_server_create.argtypes = [P(svmhip_options), D, P(h)]
     L.svmhip_server_create.restype = I
     L.svmhip_server_submit_train.argtypes = [h, P(ffi.svm_problem), P(ffi.svm_parameter)]
@@ -397,10 +401,10 @@ class Model:
             dec[n] = buf
         return lab, dec
 
-    def predict_batch(self, problem: Problem, rows=None):
+    def predict_batch(self, problem: Problem, rows=None, noalloc=False):
         """Decision values and labels of many rows on the device
-        (svmhip_predict_batch)."""
-        return predict_batch(self.ptr, problem, rows)
+        (svmhip_predict_batch, or svmhip_predict_batch_noalloc)."""
+        return predict_batch(self.ptr, problem, rows, noalloc)
 
     def save(self, path):
         if lib().svm_save_model(path.encode(), self.ptr) != 0:
@@ -414,7 +418,7 @@ class Model:
             pass
 
 
-def predict_batch(model_ptr, problem: Problem, rows=None):
+def predict_batch(model_ptr, problem: Problem, rows=None, noalloc=False):
     """Decision values and labels of problem's rows (all, or `rows`) against
     the struct svm_model* model_ptr, on the device."""
     m = model_ptr.contents
@@ -427,10 +431,11 @@ def predict_batch(model_ptr, problem: Problem, rows=None):
     dec = np.zeros((len(rows), ndec))
     lab = np.zeros(len(rows))
     opt = svmhip_options(C.sizeof(svmhip_options), 0, 0, 0, 0, 0.0, 0, 0)
-    rc = lib().svmhip_predict_batch(model_ptr, ptrs, len(rows), C.byref(opt), _ptr(dec, C.c_double),
-                                    _ptr(lab, C.c_double))
+    fn = lib().svmhip_predict_batch_noalloc if noalloc else lib().svmhip_predict_batch
+    rc = fn(model_ptr, ptrs, len(rows), C.byref(opt), _ptr(dec, C.c_double), _ptr(lab, C.c_double))
     if rc != 0:
-        raise SolverError(f"svmhip_predict_batch: {STATUS_NAMES.get(rc, rc)}")
+        raise SolverError(f"{'svmhip_predict_batch_noalloc' if noalloc else 'svmhip_predict_batch'}: "
+                          f"{STATUS_NAMES.get(rc, rc)}")
     return lab, dec
 
 
