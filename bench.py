@@ -28,8 +28,9 @@ Also reported on the same JSON line:
                solves/s, results identical alone and in the mix)
 
 Other workloads (--workload) run one configuration alone with its own
-steps: blob_csr, ovo_cv_grid and serving (cpu_baseline_parallel: the
-oracle as min(16, host CPUs) independent processes over the same solves).
+steps: blob_csr, ovo_cv_grid, serving (cpu_baseline_parallel: the oracle as
+min(16, host CPUs) independent processes over the same solves) and predict
+(10,000 rows against a serving-size model).
 
 Usage: python bench.py [--gpus 1] [--steps K] [--warmup W] [--no-cpu] [--no-legs]
 """
@@ -510,8 +511,9 @@ def bench_single(args, workload, leg=False):
     log(f"[bench] {workload}: data n={n} d={d} built in {time.perf_counter() - t0:.1f}s")
     ys, p, a0, Cp, Cn, kind = libsvm_hip.formulate(prob, prm)
     st0 = libsvm_hip.initial_status(ys, a0, Cp, Cn)
+    mfma = 1 if (args.mfma and workload == "gmm_dense") else 0
     s = libsvm_hip.DeviceSolver(prob, prm, kind=kind, grid_blocks=args.grid,
-                                cache_bytes=cache_gb * 2 ** 30)
+                                cache_bytes=cache_gb * 2 ** 30, mfma_rows=mfma)
     s.setup_rbf_variables()
     log("[bench] problem resident on the device")
 
@@ -530,8 +532,8 @@ def bench_single(args, workload, leg=False):
         walls.append(dt)
         stats.append(st)
         log(f"[bench] step {k}: {st['iters']} iters, {dt:.3f}s wall, kernel {st['kernel_ms']:.1f} ms, "
-            f"rows computed {st['row_misses']}, obj {st['obj']:.10g}, rho {st['rho']:.10g}, "
-            f"device allocations {st['device_allocs']}")
+            f"gram {st['gram_ms']:.1f} ms, rows computed {st['row_misses']}, obj {st['obj']:.12g}, "
+            f"rho {st['rho']:.12g}, device allocations {st['device_allocs']}")
     iters = sum(st["iters"] for st in stats)
     wall = sum(walls)
     kern_s = sum(st["kernel_ms"] for st in stats) / 1e3
@@ -616,6 +618,40 @@ def bench_single(args, workload, leg=False):
     return out
 
 
+def bench_predict(args):
+    """Decision values of 10,000 test rows (d=512, CSR density 0.3, the
+    serving stream's prediction shape) against the model of serving request
+    333 (C-SVC, n=19,189 dense d=512 rows, C=8), through
+    svmhip_predict_batch: one step = one call, host packing and upload
+    included.  roofline: k_predict's FP64 kernel evaluations,
+    3 * d flops per (test row, support vector) pair (subtract, multiply,
+    add), against 78.6 TFLOP/s (FP64)."""
+    r = datasets.serving_request(333)
+    prob = _serving_problem(r)
+    prm = _serving_param(r)
+    m = libsvm_hip.svm_train(prob, prm)
+    nsv = m.arrays()["l"]
+    ip, ix, v = datasets.serving_rows(1005 + r["d"], 10000, r["d"], 0.3)
+    test = ffi.Problem(ip, ix, v, np.zeros(10000), r["d"])
+    for w in range(args.warmup):
+        m.predict_batch(test)
+    walls = []
+    for k in range(args.steps):
+        t = time.perf_counter()
+        m.predict_batch(test)
+        walls.append(time.perf_counter() - t)
+        log(f"[bench] predict step {k}: {walls[-1] * 1e3:.1f} ms")
+    wall = sum(walls) / len(walls)
+    flops = 3.0 * r["d"] * nsv * 10000
+    return {"metric": METRIC, "value": 10000 / wall, "unit": "decision values/s", "n_gpus": 1,
+            "steps": args.steps, "warmup": args.warmup, "ms_per_step": 1e3 * wall, "higher_is_better": True,
+            "scaling": "weak", "vs_baseline": None, "dtype": "f64", "data": "synthetic",
+            "config": {"workload": f"predict 10,000 rows d={r['d']} density 0.3 against {nsv} support vectors "
+                                   f"(serving request 333: C-SVC n={r['n']} dense d={r['d']} C={r['C']})",
+                       "nSV": nsv, "kernel_flops_per_call": flops,
+                       "flops_formula": "3 * d * nSV * rows (RBF: subtract, multiply, add per feature)"}}
+
+
 def leg_zipf(args):
     """The CPU reference path's configuration (zipf_binary n=2000, d=123,
     RBF gamma=1/123, C=1, eps=1e-3, shrinking on, 100 MB cache): svm_train
@@ -683,11 +719,14 @@ def main():
     ap.add_argument("--cache-gb", type=float, default=64.0)
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--no-legs", action="store_true", help="the n=100k solve only")
+    ap.add_argument("--mfma", action="store_true",
+                    help="gmm_dense in Gram-tile mode (whole kernel matrix on the matrix cores)")
     ap.add_argument("--cpu-seconds", type=float, default=20.0)
     ap.add_argument("--copy-probe-gib", type=float, default=0.0,
                     help="instead of the solve, run a plain 1-pass device copy of this many GiB "
                          "(reference for the FETCH_SIZE / WRITE_SIZE counters)")
-    ap.add_argument("--workload", default="gmm_dense", choices=["gmm_dense", "blob_csr", "ovo_cv_grid", "serving"])
+    ap.add_argument("--workload", default="gmm_dense",
+                    choices=["gmm_dense", "blob_csr", "ovo_cv_grid", "serving", "predict"])
     ap.add_argument("--qps", type=float, default=20.0)
     ap.add_argument("--window", type=float, default=60.0)
     ap.add_argument("--leg-window", type=float, default=15.0, help="window of the serving leg (60 req/s)")
@@ -710,6 +749,9 @@ def main():
         return
     if args.workload == "serving":
         print(json.dumps(bench_serving(args)), flush=True)
+        return
+    if args.workload == "predict":
+        print(json.dumps(bench_predict(args)), flush=True)
         return
     out = bench_single(args, args.workload)
     if args.workload == "gmm_dense" and not args.no_legs:

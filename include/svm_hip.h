@@ -64,6 +64,16 @@ typedef struct {
 				     when 1 (slower on the measured single-solve configurations).
 				     Batched and serving solves (single-workgroup C-rule subproblems
 				     with a full cache): on unless -1 */
+	int mfma_rows;            /* svmhip_solve on dense rows (not PRECOMPUTED) whose whole kernel
+				     matrix fits the row cache: 1 = compute all of it at the start of
+				     each solve as Gram tiles on the matrix cores (FP16 hi/lo split
+				     products, FP32 accumulation; kernel values within ~2e-6 relative of
+				     the exact ones, median 5e-8), then run the SMO loop on cache hits;
+				     0 = exact rows (FP64 dots) on demand, the oracle's arithmetic */
+	double hbm_fraction;      /* batched solves: at most this fraction of the free device memory
+				     goes to kernel-row cache arenas (0 = 0.7); svm_train and
+				     svm_cross_validation take it from the environment variable
+				     SVMHIP_HBM_FRACTION */
 } svmhip_options;
 
 typedef struct svmhip_solver svmhip_solver;
@@ -83,6 +93,8 @@ typedef struct {
 	double setup_ms;          /* device time of G initialisation */
 	int cache_slots;          /* rows the cache holds */
 	int grid_blocks;          /* workgroups used by the persistent kernel */
+	double gram_ms;           /* device time of the Gram-tile kernel matrix (mfma_rows = 1; 0 if the
+				     solve computed rows on demand) */
 	long long device_allocs;  /* device allocations made by the handle since svmhip_create (a
 				     repeated svmhip_setup + svmhip_solve over the same rows adds none
 				     once the first solve has sized the buffers) */
@@ -147,7 +159,9 @@ svmhip_status svmhip_finalize(svmhip_solver *h, svmhip_stats *stats);
 svmhip_status svmhip_get_stats(svmhip_solver *h, svmhip_stats *stats);
 
 /* Device kernel row K(i, 0..l-1) as the solver caches it (float, no label
- * signs), for testing the row evaluator.  out has l floats (host memory). */
+ * signs), for testing the row evaluator: the exact row, or after a solve in
+ * Gram-tile mode (mfma_rows = 1) the row the Gram tiles left in the cache.
+ * out has l floats (host memory). */
 svmhip_status svmhip_kernel_row(svmhip_solver *h, int i, float *out);
 
 /* Destructor (libsvm/cuda_solver.cpp:450-454) without the device reset. */
@@ -164,7 +178,8 @@ const char *svmhip_last_error(const svmhip_solver *h);
  * alpha/G/kernel-row cache.  Options: grid_blocks = largest team (0 = 32,
  * 1 = one workgroup per subproblem); iters_per_launch = iterations per
  * subproblem per launch (0 = sized by the scheduler); cache_bytes = arena
- * per workgroup (0 = an equal share of 70% of free device memory).
+ * per workgroup (0 = an equal share of hbm_fraction, default 70%, of free
+ * device memory); verbose = 1 prints the arena size chosen.
  * ------------------------------------------------------------------------- */
 
 typedef struct svmhip_batch svmhip_batch;

@@ -40,6 +40,9 @@ hipError_t smo_batch_capacity(size_t shmem, int *blocks_per_cu);
 hipError_t smo_launch_predict_offset(const smo::PredictArgs *A, int first_row, int nblocks, size_t shmem,
                                     hipStream_t st);
 int smo_predict_tb(int ld, size_t lds_bytes, size_t *shmem);
+hipError_t smo_launch_split_f16(const smo::View *V, _Float16 *xs, int *bad, hipStream_t st);
+hipError_t smo_launch_gram_f16(const smo::View *V, const _Float16 *xs, hipStream_t st);
+size_t smo_split_f16_bytes(int l, int d);
 hipError_t smo_launch_pack_rowset(const smo::View *src, const int *rows, int n, const long long *dst_slice_ptr,
                                   float *dst_vals, int *dst_cols, int *dst_len, double *dst_xsq, hipStream_t st);
 }
@@ -59,7 +62,7 @@ using namespace svmhip_detail;
 struct svmhip_solver {
   int dev = 0;
   hipStream_t st = nullptr;
-  hipEvent_t ev[4] = {nullptr, nullptr, nullptr, nullptr};
+  hipEvent_t ev[6] = {nullptr, nullptr, nullptr, nullptr, nullptr, nullptr};
   svmhip_kind kind = SVMHIP_KIND_C;
   svmhip_options opt{};
   int svm_type = C_SVC;
@@ -102,6 +105,14 @@ struct svmhip_solver {
   size_t cap_nvar = 0, cap_rows = 0, cap_flags = 0, cap_pm_flags = 0, cap_lru_map = 0, cap_lru_slots = 0,
          cap_pm_lru = 0, cap_list = 0;
   int *d_list = nullptr;          // G initialisation: variables with alpha != 0
+  // Gram-tile mode (opt.mfma_rows): the FP16 hi/lo copy of X (made once per
+  // handle), whether X fits FP16's range, and whether this solve's cache
+  // already holds the whole matrix
+  _Float16 *d_xsplit = nullptr;
+  int *d_flag = nullptr;
+  int split_state = 0;            // 0 = not made, 1 = ready, -1 = X out of FP16 range (exact rows)
+  bool gram_done = false;
+  long long gram_rows = 0;        // kernel rows the Gram-tile kernel computed in this solve
   svmhip_stats stats{};
   std::string err;
 

@@ -21,6 +21,9 @@
 //                   solves one-vs-one pairs and folds one after another)
 //   k_pack_rowset   copies a row subset into its own SELL arrays
 //   k_predict       decision values of many rows     (svm_predict_values, svm.cpp:2569-2643)
+#include <stdlib.h>
+#include <string.h>
+
 #include "smo_device.hpp"
 
 namespace smo {
@@ -1318,8 +1321,20 @@ hipError_t smo_launch_lru_private_init(const View *V, hipStream_t st) {
 // while another process holds the device's cooperative-launch resources, and
 // then the same grid is launched plainly: on a device not running other
 // work every workgroup is resident at once, and if one is not, the spins of
-// the others end with an error (ERR_TIMEOUT) rather than a hang.
+// the others end with an error (ERR_TIMEOUT) rather than a hang.  With the
+// environment variable SVMHIP_LAUNCH=plain every such grid is launched
+// plainly (same grid, same code): profilers that mishandle cooperative
+// launches can then trace the solver.
+static bool plain_launches() {
+  static const bool plain = [] {
+    const char *e = getenv("SVMHIP_LAUNCH");
+    return e && strcmp(e, "plain") == 0;
+  }();
+  return plain;
+}
+
 static hipError_t launch_resident(const void *fn, int grid, void **args, size_t shmem, hipStream_t st) {
+  if (plain_launches()) return hipLaunchKernel(fn, dim3(grid), dim3(smo::BLOCK), args, shmem, st);
   hipError_t e = hipLaunchCooperativeKernel(fn, dim3(grid), dim3(smo::BLOCK), args, (unsigned)shmem, st);
   if (e == hipErrorIllegalState) {
     (void)hipGetLastError();

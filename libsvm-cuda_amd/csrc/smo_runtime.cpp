@@ -124,6 +124,24 @@ static void reset_shared_cache(svmhip_solver *h) {
 }
 
 
+// Gram-tile mode: makes the FP16 hi/lo copy of X the first time (false if
+// X leaves FP16's range, in which case the solve computes exact rows).
+static bool gram_prepare(svmhip_solver *h) {
+  if (h->split_state == 0) {
+    h->d_xsplit = h->dalloc<_Float16>(smo_split_f16_bytes(h->l, h->d) / sizeof(_Float16), "FP16 copy of the rows");
+    h->d_flag = h->dalloc<int>(1, "range flag");
+    h->ck(hipMemsetAsync(h->d_flag, 0, sizeof(int), h->st), "clear range flag");
+    smo::View V = h->view();
+    h->ck(smo_launch_split_f16(&V, h->d_xsplit, h->d_flag, h->st), "FP16 split kernel");
+    int bad = 0;
+    h->ck(hipMemcpyAsync(&bad, h->d_flag, sizeof(int), hipMemcpyDeviceToHost, h->st), "read range flag");
+    h->ck(hipStreamSynchronize(h->st), "FP16 split");
+    h->split_state = bad ? -1 : 1;
+    if (bad && h->opt.verbose) fprintf(stderr, "svmhip: features beyond FP16 range, exact rows instead of Gram tiles\n");
+  }
+  return h->split_state == 1;
+}
+
 extern "C" {
 
 int svmhip_device_count(void) {
@@ -352,6 +370,8 @@ svmhip_status svmhip_setup(svmhip_solver *h, const signed char *y, const double 
   h->stats.grid_blocks = h->nblocks;
   h->setup_done = true;
   h->have_j = false;
+  h->gram_done = false;
+  h->gram_rows = 0;
   if (h->opt.verbose)
     fprintf(stderr, "svmhip: l=%d d=%d %s, n_var=%d, cache %d rows%s, grid %d x 512\n", h->l, h->d,
             h->dense ? "dense" : "sparse", nv, h->S, h->full ? " (all)" : "", h->nblocks);
@@ -450,7 +470,7 @@ svmhip_status svmhip_finalize(svmhip_solver *h, svmhip_stats *stats) {
   h->stats.r = h->host_ctl->r;
   h->stats.iters = h->host_ctl->iter;
   h->stats.row_hits = (long long)h->host_ctl->hits;
-  h->stats.row_misses = (long long)h->host_ctl->misses;
+  h->stats.row_misses = (long long)h->host_ctl->misses + h->gram_rows;
   h->stats.device_allocs = h->n_allocs;
   if (stats) *stats = h->stats;
   GUARD_END
@@ -468,8 +488,13 @@ svmhip_status svmhip_solve(svmhip_solver *h, int max_iter, svmhip_stats *stats) 
             "set max_iter");
     }
     h->read_ctl();
+    // Gram-tile mode: the whole kernel matrix on the matrix cores, once per
+    // setup, when the rows are dense and every row fits the cache
+    const bool use_gram = h->opt.mfma_rows == 1 && h->full && h->dense && h->ktype != PRECOMPUTED && h->l > 0 &&
+                          h->nvar > 0 && gram_prepare(h);
     // persistent-kernel state: start from the current alpha / G / status with
-    // every workgroup's private cache bookkeeping empty
+    // every workgroup's private cache bookkeeping empty (Gram-tile mode: every
+    // row present)
     {
       smo::Dec &dc = smo::ctl_dec(h->host_ctl);
       memset(&dc, 0, sizeof dc);
@@ -480,7 +505,8 @@ svmhip_status svmhip_solve(svmhip_solver *h, int max_iter, svmhip_stats *stats) 
       h->ck(hipMemsetAsync(h->d_xw, 0, sizeof(smo::XWord) * smo::xrec_words(h->nblocks), h->st), "clear exchange records");
       smo::View V0 = h->view();
       if (h->full)
-        h->ck(hipMemsetAsync(h->d_pm_flags, 0, (size_t)h->nblocks * std::max(h->l, 1), h->st), "clear cache flags");
+        h->ck(hipMemsetAsync(h->d_pm_flags, use_gram ? 1 : 0, (size_t)h->nblocks * std::max(h->l, 1), h->st),
+              use_gram ? "mark every row cached" : "clear cache flags");
       else
         h->ck(smo_launch_lru_private_init(&V0, h->st), "init private LRU");
       h->shared_meta_dirty = true;
@@ -509,6 +535,19 @@ svmhip_status svmhip_solve(svmhip_solver *h, int max_iter, svmhip_stats *stats) 
     }
     int chunk = h->opt.iters_per_launch > 0 ? h->opt.iters_per_launch : INT_MAX;
     h->ck(hipEventRecord(h->ev[0], h->st), "event");
+    h->stats.gram_ms = 0;
+    if (use_gram && !h->gram_done) {
+      h->ck(hipEventRecord(h->ev[4], h->st), "event");
+      smo::View Vg = h->view();
+      h->ck(smo_launch_gram_f16(&Vg, h->d_xsplit, h->st), "Gram-tile kernel");
+      h->ck(hipEventRecord(h->ev[5], h->st), "event");
+      h->ck(hipEventSynchronize(h->ev[5]), "Gram-tile kernel");
+      float gms = 0;
+      (void)hipEventElapsedTime(&gms, h->ev[4], h->ev[5]);
+      h->stats.gram_ms = gms;
+      h->gram_done = true;
+      h->gram_rows = h->l;
+    }
     if (h->nvar > 0) {
       h->stats.kernel_ms = 0;
       h->stats.launches = 0;
@@ -556,6 +595,13 @@ svmhip_status svmhip_kernel_row(svmhip_solver *h, int i, float *out) {
   if (!h->rbf_ready) {
     svmhip_status s = svmhip_setup_rbf(h, h->l);
     if (s != SVMHIP_OK) throw Failure{s};
+  }
+  if (h->gram_done) {  // Gram-tile mode: the row as the cache holds it
+    h->ck(hipMemcpyAsync(out, h->d_rows + (long long)i * h->row_stride, sizeof(float) * h->l, hipMemcpyDeviceToHost,
+                         h->st),
+          "copy cached row");
+    h->ck(hipStreamSynchronize(h->st), "kernel row");
+    return SVMHIP_OK;
   }
   float *buf = h->dalloc<float>(h->l, "row buffer");
   smo::View V = h->view();
@@ -759,11 +805,15 @@ svmhip_status svmhip_batch_solve(svmhip_batch *b, svmhip_subproblem *subs, int n
     long long maxstride = ((long long)maxn + 63) / 64 * 64;
     size_t freeb = 0, totb = 0;
     h->ck(hipMemGetInfo(&freeb, &totb), "hipMemGetInfo");
-    const long long share = (long long)(0.7 * (double)freeb / sizeof(float)) / K;
+    const double frac = h->opt.hbm_fraction > 0 ? std::min(h->opt.hbm_fraction, 0.95) : 0.7;
+    const long long share = (long long)(frac * (double)freeb / sizeof(float)) / K;
     long long arena = h->opt.cache_bytes > 0 ? (long long)(h->opt.cache_bytes / sizeof(float)) : share;
     arena = std::max<long long>(std::min(arena, std::max<long long>(share, 2 * maxstride)), 2 * maxstride);
     // no subproblem needs more than its whole matrix
     arena = std::min<long long>(arena, std::max<long long>((long long)maxn * maxstride, 2 * maxstride));
+    if (h->opt.verbose)
+      fprintf(stderr, "[svmhip] batch: %d kernel-row arenas of %.1f MB (%.1f GB of %.1f GB free, fraction %.2f)\n", K,
+              arena * 4.0 / 1048576.0, arena * 4.0 * K / 1073741824.0, freeb / 1073741824.0, frac);
     smo::BatchPool pool;
     memset(&pool, 0, sizeof pool);
     pool.K = K;

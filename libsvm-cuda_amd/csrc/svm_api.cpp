@@ -224,6 +224,8 @@ svmhip_options device_options() {
   opt.struct_size = sizeof opt;
   const char *e = getenv("SVMHIP_VERBOSE");
   opt.verbose = e && *e == '1';
+  const char *f = getenv("SVMHIP_HBM_FRACTION");
+  if (f && *f) opt.hbm_fraction = atof(f);
   return opt;
 }
 
@@ -717,9 +719,87 @@ double svm_predict(const svm_model *model, const svm_node *x) {
   return svm_predict_values(model, x, dec.data());
 }
 
+// Platt's sigmoid 1 / (1 + exp(A f + B)), written so that 1 - p keeps its
+// precision (libsvm/svm.cpp:1885-1893).
+static double platt_probability(double f, double A, double B) {
+  const double t = f * A + B;
+  if (t >= 0) return exp(-t) / (1.0 + exp(-t));
+  return 1.0 / (1 + exp(t));
+}
+
+// Pairwise coupling: class probabilities p from the pairwise ones r
+// (r[i*k+j] = P(i | i or j)), method 2 of Wu, Lin & Weng, "Probability
+// estimates for multi-class classification by pairwise coupling", JMLR 5
+// (2004) 975-1005: p minimises sum_t sum_{j != t} (r_jt p_t - r_tj p_j)^2 on
+// the simplex, by the paper's coordinate fixed-point iteration with
+// LIBSVM's stopping rule (libsvm/svm.cpp:1896-1957).
+static void couple_pairwise(int k, const std::vector<double> &r, double *p) {
+  std::vector<double> Q((size_t)k * k), Qp(k);
+  for (int t = 0; t < k; ++t) {
+    p[t] = 1.0 / k;
+    double qtt = 0;
+    for (int j = 0; j < t; ++j) {
+      qtt += r[(size_t)j * k + t] * r[(size_t)j * k + t];
+      Q[(size_t)t * k + j] = Q[(size_t)j * k + t];
+    }
+    for (int j = t + 1; j < k; ++j) {
+      qtt += r[(size_t)j * k + t] * r[(size_t)j * k + t];
+      Q[(size_t)t * k + j] = -r[(size_t)j * k + t] * r[(size_t)t * k + j];
+    }
+    Q[(size_t)t * k + t] = qtt;
+  }
+  const int max_iter = std::max(100, k);
+  const double eps = 0.005 / k;
+  int it = 0;
+  for (; it < max_iter; ++it) {
+    double pQp = 0;  // recomputed each round for accuracy
+    for (int t = 0; t < k; ++t) {
+      double v = 0;
+      for (int j = 0; j < k; ++j) v += Q[(size_t)t * k + j] * p[j];
+      Qp[t] = v;
+      pQp += p[t] * Qp[t];
+    }
+    double worst = 0;
+    for (int t = 0; t < k; ++t) worst = std::max(worst, fabs(Qp[t] - pQp));
+    if (worst < eps) break;
+    for (int t = 0; t < k; ++t) {
+      const double qtt = Q[(size_t)t * k + t];
+      const double diff = (-Qp[t] + pQp) / qtt;
+      p[t] += diff;
+      pQp = (pQp + diff * (diff * qtt + 2 * Qp[t])) / (1 + diff) / (1 + diff);
+      for (int j = 0; j < k; ++j) {
+        Qp[j] = (Qp[j] + diff * Q[(size_t)t * k + j]) / (1 + diff);
+        p[j] /= (1 + diff);
+      }
+    }
+  }
+  if (it >= max_iter) info("Exceeds max_iter in multiclass_prob\n");
+}
+
+// Class probabilities of x from a model that carries Platt parameters
+// (probA/probB, e.g. a model file written by LIBSVM with -b 1), as
+// libsvm/svm.cpp:2660-2697 computes them; other models predict as
+// svm_predict and leave prob_estimates untouched.  Host code: one row.
 double svm_predict_probability(const svm_model *model, const svm_node *x, double *prob_estimates) {
-  (void)prob_estimates;  // probability models are not trained by this library
-  return svm_predict(model, x);
+  const int st = model->param.svm_type;
+  if (!((st == C_SVC || st == NU_SVC) && model->probA && model->probB)) return svm_predict(model, x);
+  const int k = model->nr_class;
+  std::vector<double> dec(std::max(1, k * (k - 1) / 2)), r((size_t)k * k, 0.0);
+  svm_predict_values(model, x, dec.data());
+  const double min_prob = 1e-7;
+  int q = 0;
+  for (int i = 0; i < k; ++i)
+    for (int j = i + 1; j < k; ++j, ++q) {
+      const double pij = std::min(std::max(platt_probability(dec[q], model->probA[q], model->probB[q]), min_prob),
+                                  1 - min_prob);
+      r[(size_t)i * k + j] = pij;
+      r[(size_t)j * k + i] = 1 - pij;
+    }
+  couple_pairwise(k, r, prob_estimates);
+  int best = 0;
+  for (int i = 1; i < k; ++i)
+    if (prob_estimates[i] > prob_estimates[best]) best = i;
+  return model->label[best];
 }
 
 // Model file writer, libsvm/svm.cpp:2709-2801 (same keys and formats).

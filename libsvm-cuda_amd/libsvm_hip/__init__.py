@@ -32,7 +32,8 @@ STATUS_NAMES = {0: "OK", 1: "ERR_ARG", 2: "ERR_DEVICE", 3: "ERR_NOMEM", 4: "ERR_
 class svmhip_options(C.Structure):
     _fields_ = [("struct_size", C.c_size_t), ("device", C.c_int), ("grid_blocks", C.c_int),
                 ("block_threads", C.c_int), ("max_iter", C.c_int), ("cache_bytes", C.c_double),
-                ("iters_per_launch", C.c_int), ("verbose", C.c_int), ("speculative_rows", C.c_int)]
+                ("iters_per_launch", C.c_int), ("verbose", C.c_int), ("speculative_rows", C.c_int),
+                ("mfma_rows", C.c_int), ("hbm_fraction", C.c_double)]
 
 
 class svmhip_stats(C.Structure):
@@ -40,7 +41,7 @@ class svmhip_stats(C.Structure):
                 ("r", C.c_double), ("row_hits", C.c_longlong), ("row_misses", C.c_longlong),
                 ("solve_ms", C.c_double), ("kernel_ms", C.c_double), ("launches", C.c_int),
                 ("setup_ms", C.c_double), ("cache_slots", C.c_int),
-                ("grid_blocks", C.c_int), ("device_allocs", C.c_longlong)]
+                ("grid_blocks", C.c_int), ("gram_ms", C.c_double), ("device_allocs", C.c_longlong)]
 
     def as_dict(self):
         return {k: getattr(self, k) for k, _ in self._fields_}
@@ -200,12 +201,12 @@ class DeviceSolver:
     Method names follow the reference's CudaSolver (libsvm/cuda_solver.h:224-244)."""
 
     def __init__(self, problem: Problem, param, kind=KIND_C, grid_blocks=0, max_iter=0,
-                 cache_bytes=0.0, iters_per_launch=0, device=0, verbose=0, speculative_rows=0):
+                 cache_bytes=0.0, iters_per_launch=0, device=0, verbose=0, speculative_rows=0, mfma_rows=0):
         L = lib()
         self.problem = problem  # keeps the node buffers alive
         self.param = param
         opt = svmhip_options(C.sizeof(svmhip_options), device, grid_blocks, 0, max_iter,
-                             float(cache_bytes), iters_per_launch, verbose, speculative_rows)
+                             float(cache_bytes), iters_per_launch, verbose, speculative_rows, mfma_rows)
         self._h = C.c_void_p()
         rc = L.svmhip_create(C.byref(problem.struct), C.byref(param), kind, C.byref(opt),
                              C.byref(self._h))

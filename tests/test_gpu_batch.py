@@ -2,6 +2,8 @@
 one workgroup or a team of them solves it and however many launches it is
 split over, must give the same iterates as the oracle on the same row subset
 (identical iteration count, alpha to 1e-9, objective to 1e-9)."""
+import ctypes as C
+
 import numpy as np
 import pytest
 
@@ -191,9 +193,11 @@ def test_batch_more_subproblems_than_workgroups(hip, ora):
 def test_ovo_cv_gamma_grid_full_size(hip, ora):
     """The 10-class configuration at full size (n=38,217, d=128): 5-fold CV x
     4 gammas x 45 one-vs-one pairs = 900 solves in one batched call.  Every
-    solve must converge to a feasible point whose dual objective is below
-    the starting point's (0), and the 20 smallest must give the oracle's
-    iterates (identical iteration count, alpha to 1e-9, objective to 1e-9)."""
+    solve must converge to a feasible point and match the oracle's
+    full-size results (tests/golden/fullsize_ovo_cv_grid.npz, reference-exact
+    variant): identical iteration count and support-vector set, objective
+    within 1e-6 relative, rho within 1e-5; the 5 smallest are also compared
+    with a live oracle solve (alpha to 1e-9)."""
     gammas = [2.0 ** -9, 2.0 ** -7, 2.0 ** -5, 2.0 ** -3]
     C = 4.0
     X, y, fold = datasets.clusters_10class()
@@ -222,7 +226,24 @@ def test_ovo_cv_gamma_grid_full_size(hip, ora):
         assert np.all(a >= 0) and np.all(a <= C)
         assert abs(np.dot(a, yy)) < 1e-8 * max(1.0, a.sum())
         assert r["obj"] < 0
-    order = np.argsort([len(j[0]) for j in jobs], kind="stable")[:20]
+    import hashlib
+    import os
+    fx = np.load(os.path.join(os.path.dirname(os.path.abspath(__file__)), "golden", "fullsize_ovo_cv_grid.npz"))
+    worst = {"obj": 0.0, "rho": 0.0, "obj_fp64": 0.0, "rho_fp64": 0.0}
+    for k, r in enumerate(res):
+        sv = np.nonzero(r["alpha_raw"] > 0)[0].astype(np.int32)
+        dig = np.frombuffer(hashlib.sha256(sv.tobytes()).digest()[:8], dtype=np.uint64)[0]
+        assert r["iters"] == fx["float_iters"][k], k
+        assert len(sv) == fx["float_nSV"][k] and dig == fx["float_sv_digest"][k], k
+        eo = abs(r["obj"] - fx["float_obj"][k]) / abs(fx["float_obj"][k])
+        er = abs(r["rho"] - fx["float_rho"][k])
+        assert eo <= 1e-6 and er <= 1e-5, k
+        worst["obj"], worst["rho"] = max(worst["obj"], eo), max(worst["rho"], er)
+        worst["obj_fp64"] = max(worst["obj_fp64"], abs(r["obj"] - fx["double_obj"][k]) / abs(fx["double_obj"][k]))
+        worst["rho_fp64"] = max(worst["rho_fp64"], abs(r["rho"] - fx["double_rho"][k]))
+    print(f"  900 solves vs the oracle: worst obj rel err {worst['obj']:.1e}, rho err {worst['rho']:.1e} "
+          f"(reference-exact); {worst['obj_fp64']:.1e} / {worst['rho_fp64']:.1e} (pure FP64)", flush=True)
+    order = np.argsort([len(j[0]) for j in jobs], kind="stable")[:5]
     for k in order:
         rows, yy, g = jobs[k]
         sub = ffi.Problem.from_dense(X[rows].astype(np.float64), yy.astype(np.float64), keep_zeros=True)
@@ -265,3 +286,55 @@ def test_predict_batch_fp64_features_match_oracle(hip, ora, kname, dense):
     print(f"  {kname} dense={dense}: nSV={m.arrays()['l']} max |dec - oracle| = {err:.2e}", flush=True)
     assert err <= 1e-10 * max(1.0, np.max(np.abs(dec_r)))
     np.testing.assert_array_equal(lab_d, lab_r)
+
+
+@pytest.mark.parametrize("kname", ["rbf", "poly", "linear"])
+def test_predict_batch_matches_sklearn_decision_function(hip, kname):
+    """The device decision values of a model trained by scikit-learn's
+    bundled LIBSVM (tests/golden/sklearn_multiclass.npz, 4 classes) equal
+    sklearn's decision_function to 1e-12."""
+    import os
+    import sys
+    sys.path.insert(0, os.path.dirname(os.path.abspath(__file__)))
+    from model_util import ArrayModel
+    G = np.load(os.path.join(os.path.dirname(os.path.abspath(__file__)), "golden", "sklearn_multiclass.npz"))
+    kt = {"rbf": ffi.RBF, "poly": ffi.POLY, "linear": ffi.LINEAR}[kname]
+    prm = ffi.make_param(kernel_type=kt, gamma=0.2, coef0=0.5, degree=2, C_=2.0)
+    sup = G[f"{kname}_support"]
+    am = ArrayModel(prm, G["X"][sup], G[f"{kname}_dual_coef"], -G[f"{kname}_intercept"], label=[1, 2, 3, 4],
+                    nSV=G[f"{kname}_n_support"])
+    test = ffi.Problem.from_dense(G["Xt"], np.zeros(len(G["Xt"])))
+    lab, dec = hip.predict_batch(am.ptr, test)
+    np.testing.assert_allclose(dec, G[f"{kname}_dec"], rtol=0, atol=1e-12)
+    np.testing.assert_array_equal(lab, G[f"{kname}_pred"])
+
+
+def test_batch_hbm_fraction_caps_arenas_same_results(hip):
+    """A small hbm_fraction (kernel-row arenas from 0.2% of free memory,
+    LRU under pressure) changes no result."""
+    X, y, fold = datasets.clusters_10class(scale=0.08, seed=2)
+    prob = ffi.Problem.from_dense(X.astype(np.float64), y.astype(np.float64))
+    prm = ffi.make_param(gamma=2.0 ** -7, C_=4.0, eps=1e-3, shrinking=0)
+    subs, rows_of = [], []
+    b_def = hip.BatchSolver(prob, prm)
+    L = hip.lib()
+    opt = hip.svmhip_options(C.sizeof(hip.svmhip_options), 0, 0, 0, 0, 0.0, 0, 0, 0, 0, 0.002)
+    h = C.c_void_p()
+    assert L.svmhip_batch_create(C.byref(prob.struct), C.byref(prm), C.byref(opt), C.byref(h)) == 0
+    b_small = hip.BatchSolver.__new__(hip.BatchSolver)
+    b_small._h, b_small.problem, b_small.param = h, prob, prm
+    for c1, c2 in [(1, 2), (3, 7), (5, 9)]:
+        rows = np.nonzero((y == c1) | (y == c2))[0]
+        yy = np.where(y[rows] == c1, 1, -1).astype(np.int8)
+        rid1, rid2 = b_def.add_rowset(rows), b_small.add_rowset(rows)
+        assert rid1 == rid2
+        subs.append(dict(rowset=rid1, kind=hip.KIND_C, y=yy, p=-np.ones(len(rows)), alpha0=np.zeros(len(rows)),
+                         Cp=4.0, Cn=4.0, gamma=2.0 ** -7, eps=1e-3))
+    r1, s1 = b_def.solve(subs)
+    r2, s2 = b_small.solve(subs)
+    assert s2["row_misses"] >= s1["row_misses"]
+    for a, b in zip(r1, r2):
+        assert a["iters"] == b["iters"]
+        np.testing.assert_array_equal(a["alpha_raw"], b["alpha_raw"])
+    b_def.close()
+    b_small.close()

@@ -164,3 +164,62 @@ def test_server_release_model_before_fetching_prediction(hip):
     assert set(np.unique(lab1)) <= set(np.unique(prob.y))
     s.release(pid)
     s.close()
+
+
+def test_server_serving_slice_at_real_sizes(hip):
+    """A fixed slice of the serving stream at its real sizes (n up to 19,974,
+    dense d=512 rows, nu-SVC requests, C up to 8; ids in
+    tests/golden/fullsize_serving_slice.npz) solved together in one server,
+    with a 10,000-row prediction submitted against the dense d=512 model as
+    soon as it is resident (while the others still train).  Every result
+    must be bit-identical to the same request solved alone, and equal the
+    oracle's (reference-exact variant): same iteration count and support
+    vectors, objective within 1e-6 relative, rho within 1e-5, coefficients
+    within 1e-6 C; decision values within 1e-9 of the oracle's."""
+    import os
+    fx = np.load(os.path.join(os.path.dirname(os.path.abspath(__file__)), "golden", "fullsize_serving_slice.npz"))
+    ids = [int(k) for k in fx["ids"]]
+    pm = int(fx["pred_model"])
+    reqs = {k: datasets.serving_request(k) for k in ids}
+    probs = {k: request_problem(reqs[k]) for k in ids}
+    prms = {k: request_param(reqs[k]) for k in ids}
+    r = reqs[pm]
+    ip, ix, v = datasets.serving_rows(1005 + r["d"], 10000, r["d"], 0.3)
+    test = ffi.Problem(ip, ix, v, np.zeros(10000), r["d"])
+    s = hip.Server()
+    rid = {k: s.submit_train(probs[k], prms[k]) for k in ids}
+    comps, pid = {}, None
+    for _ in range(200000):
+        a, q = s.step(10.0)
+        for c in s.poll():
+            comps[c["id"]] = c
+            if c["id"] == rid[pm]:
+                pid = s.submit_predict(rid[pm], test)
+        if a == 0 and q == 0 and pid is not None and pid in comps:
+            break
+    assert len(comps) == len(ids) + 1
+    dec, lab = s.predict_result(pid, 10000)
+    err = np.max(np.abs(dec - fx["pred_dec"]))
+    print(f"  prediction of 10,000 rows against request {pm}: max |dec - oracle| = {err:.2e}", flush=True)
+    assert err <= 1e-9 * max(1.0, np.max(np.abs(fx["pred_dec"])))
+    np.testing.assert_array_equal(lab, np.where(fx["pred_dec"] > 0, fx["pred_labels"][0], fx["pred_labels"][1]))
+    for k in ids:
+        c = comps[rid[k]]
+        coef, _ = s.train_result(rid[k], probs[k].l)
+        alone, coef_alone = solve_alone(hip, probs[k], prms[k], stage_ms=10.0)
+        sv = np.nonzero(coef != 0)[0]
+        it, obj, rho = int(fx[f"float_{k}_iters"]), float(fx[f"float_{k}_obj"]), float(fx[f"float_{k}_rho"])
+        rr = reqs[k]
+        print(f"  request {k}: {rr['kind']} n={rr['n']} d={rr['d']} dens={rr['density']} C={rr['C']}: iters "
+              f"{c['iters']} (oracle {it}), obj rel err {abs(c['obj'] - obj) / abs(obj):.1e}, rho err "
+              f"{abs(c['rho'] - rho):.1e}, nSV {len(sv)}; pure-FP64 oracle obj rel err "
+              f"{abs(c['obj'] - float(fx[f'double_{k}_obj'])) / abs(obj):.1e}", flush=True)
+        assert c["status"] == 0 and alone["status"] == 0
+        assert c["iters"] == alone["iters"] == it
+        np.testing.assert_array_equal(coef, coef_alone)
+        assert c["rho"] == alone["rho"] and c["obj"] == alone["obj"]
+        assert abs(c["obj"] - obj) <= 1e-6 * abs(obj)
+        assert abs(c["rho"] - rho) <= 1e-5
+        np.testing.assert_array_equal(sv, fx[f"float_{k}_sv"])
+        assert np.max(np.abs(coef[sv] - fx[f"float_{k}_coef_sv"])) <= 1e-6 * rr["C"]
+    s.close()

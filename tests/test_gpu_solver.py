@@ -416,3 +416,69 @@ def test_nu_ginit_many_rows_matches_oracle(hip, ora, svm_type):
     ref = ora.train_one(prob, prm)
     dev = hip.solve_one(prob, prm)
     assert_solution_close(dev, ref)
+
+
+@pytest.mark.parametrize("kname", ["rbf", "linear", "poly"])
+def test_mfma_gram_rows_close_to_exact(hip, ora, kname):
+    """Gram-tile mode (mfma_rows = 1): the kernel rows the MFMA tiles leave
+    in the cache, on sizes that are not multiples of the tile (n = 3,001
+    rows, d = 100 features), against the oracle's FP64 rows.  FP16 hi/lo
+    products with FP32 accumulation make an error in the dot product of
+    order 1e-7 of |x| |y|, so each value must be within
+    2e-5 |K| + 4e-6 |dK/d(dot)| |x| |y| of the FP64 one; for RBF the median
+    relative error must stay within a few float roundings (5e-7)."""
+    X, y = datasets.gmm_dense_matrix(n=3001, d=100, seed=17)
+    X64 = X.astype(np.float64)
+    prob = ffi.Problem.from_dense(X64, y, keep_zeros=True)
+    g, c0 = 1.0 / 100, 0.5
+    prm = ffi.make_param(kernel_type=KERNELS[kname], gamma=g, coef0=c0, degree=2, C_=1.0, shrinking=0)
+    yv, p, a0, Cp, Cn, kind = hip.formulate(prob, prm)
+    s = hip.DeviceSolver(prob, prm, kind=kind, cache_bytes=1e9, mfma_rows=1)
+    s.setup_rbf_variables()
+    s.setup_solver(yv, p, a0, hip.initial_status(yv, a0, Cp, Cn), Cp, Cn)
+    st = s.solve(max_iter=3)
+    assert st["gram_ms"] > 0 and st["row_misses"] == prob.l
+    norms = np.sqrt((X64 ** 2).sum(axis=1))
+    rel, worst = [], 0.0
+    for i in (0, 1, 127, 128, 1500, 2999, 3000):
+        dev = s.kernel_row(i).astype(np.float64)
+        ref = ora.kernel_row(prob, prm, i, f64=True)
+        dot = X64 @ X64[i]
+        dkd = {"linear": np.ones_like(dot), "poly": 2 * g * (g * dot + c0), "rbf": 2 * g * ref}[kname]
+        bound = 2e-5 * np.abs(ref) + 4e-6 * np.abs(dkd) * norms * norms[i]
+        err = np.abs(dev - ref)
+        worst = max(worst, float(np.max(err / bound)))
+        assert np.all(err <= bound), (kname, i)
+        rel.append(err / np.maximum(np.abs(ref), 1e-30))
+    rel = np.concatenate(rel)
+    print(f"  {kname}: Gram-tile rows vs FP64: median rel err {np.median(rel):.1e}, worst err / bound {worst:.2f}, "
+          f"gram {st['gram_ms']:.2f} ms", flush=True)
+    if kname == "rbf":
+        assert np.median(rel) <= 5e-7
+    s.close()
+
+
+def test_mfma_gram_fullsize_gmm_dense_100k(hip):
+    """The n=100k dense configuration in Gram-tile mode (the configuration's
+    precision variant): the whole 100k x 100k kernel matrix on the matrix
+    cores, then the SMO loop on cache hits.  Against the oracle's full-size
+    solves: objective within 1e-6 relative and rho within 1e-5 of both
+    variants, the support-vector sets differing in at most 1% of their
+    members (printed)."""
+    fx = np.load(os.path.join(GOLDEN, "fullsize_gmm_dense_100k.npz"))
+    X, y = datasets.gmm_dense_matrix(n=100000, d=256, seed=1003)
+    prob = ffi.Problem.from_dense(X.astype(np.float64), y, keep_zeros=True)
+    prm = ffi.make_param(gamma=1.0 / 256, C_=1.0, eps=1e-3, shrinking=0)
+    r = hip.solve_one(prob, prm, cache_bytes=64 * 2 ** 30, mfma_rows=1)
+    st = r["stats"]
+    assert st["optimal"] == 1 and st["gram_ms"] > 0
+    sv = np.nonzero(r["alpha_raw"] > 0)[0]
+    for v in ("float", "double"):
+        obj, rho, fsv = float(fx[f"{v}_obj"]), float(fx[f"{v}_rho"]), fx[f"{v}_sv"]
+        diff = len(np.setxor1d(sv, fsv))
+        print(f"  Gram-tile mode vs oracle ({v} Q): iters {r['iters']} / {int(fx[f'{v}_iters'])}, obj rel err "
+              f"{abs(r['obj'] - obj) / abs(obj):.2e}, rho abs err {abs(r['rho'] - rho):.2e}, SV sets differ in "
+              f"{diff} of {len(fsv)}; gram {st['gram_ms']:.1f} ms, loop {st['kernel_ms']:.1f} ms", flush=True)
+        assert abs(r["obj"] - obj) <= 1e-6 * abs(obj)
+        assert abs(r["rho"] - rho) <= 1e-5
+        assert diff <= 0.01 * len(fsv)
