@@ -14,7 +14,8 @@
 //                features as [hi x 8][lo x 8] (32 bytes), zero padded to a
 //                multiple of 16 features.
 //   k_gram_f16   one 128 x 128 tile of K per workgroup (4 waves of 64 x 64,
-//                each 2 x 2 MFMA blocks of 32 x 32): per 16-feature step,
+//                each 2 x 2 MFMA blocks of 32 x 32), tiles of the upper
+//                triangle only (K is symmetric): per 16-feature step,
 //                hi*hi + hi*lo + lo*hi on v_mfma_f32_32x32x16_f16 with FP32
 //                accumulation (the dropped lo*lo term is below 2^-22 of the
 //                product).  Then K(q, r) from the dot as the solver defines
@@ -71,19 +72,43 @@ __global__ void __launch_bounds__(GBLOCK) k_split_f16(View V, _Float16 *xs, int 
   if (b) atomicOr(bad, 1);
 }
 
-// Tiles in XCD order: consecutive workgroup ids go round-robin over the 8
-// XCDs, so the tile of workgroup b is taken from XCD (b mod 8)'s contiguous
-// share of the tile list; each XCD's L2 then serves a band of q rows.
-__device__ __forceinline__ long long xcd_tile(long long b, long long ntiles) {
-  const long long per = (ntiles + 7) / 8;
-  const long long t = (b % 8) * per + b / 8;
-  return t;
+// Tile t of the upper triangle (tq <= tr) of an nt x nt tile grid, rows of
+// the triangle in order: tq is the largest with tq (2 nt - tq + 1) / 2 <= t.
+__device__ __forceinline__ void tri_tile(long long t, int nt, int &tq, int &tr) {
+  const double a = 2.0 * nt + 1.0;
+  int q = (int)((a - sqrt(a * a - 8.0 * (double)t)) / 2.0);
+  q = max(0, min(q, nt - 1));
+  while (q > 0 && (long long)q * (2LL * nt - q + 1) / 2 > t) --q;
+  while (q + 1 < nt && (long long)(q + 1) * (2LL * nt - q) / 2 <= t) ++q;
+  tq = q;
+  tr = q + (int)(t - (long long)q * (2LL * nt - q + 1) / 2);
 }
 
-__global__ void __launch_bounds__(GBLOCK) k_gram_f16(View V, const _Float16 *xs, long long ntiles, int ntr) {
-  const long long tile = xcd_tile(blockIdx.x, ntiles);
-  if (tile >= ntiles) return;
-  const int tq = (int)(tile / ntr), tr = (int)(tile % ntr);
+// Kernel value from the dot product (FP64 arithmetic, FP32 exp for RBF).
+__device__ __forceinline__ float gram_value(const View &V, double xq, double xr, float acc) {
+  const double dot = (double)acc;
+  switch (V.ktype) {
+    case K_LINEAR: return (float)dot;
+    case K_POLY: return (float)powi(V.gamma * dot + V.coef0, V.degree);
+    case K_RBF: return expf((float)(-V.gamma * ((xq + xr) - 2 * dot)));
+    case K_SIGMOID: return (float)tanh(V.gamma * dot + V.coef0);
+    default: return 0.0f;
+  }
+}
+
+// K is symmetric: only tiles with tq <= tr are computed, in XCD order
+// (consecutive workgroup ids go round-robin over the 8 XCDs, so workgroup b
+// takes a tile from XCD (b mod 8)'s contiguous share of the list, and each
+// XCD's L2 serves a band of q rows).  A tile off the diagonal is written to
+// rows q (K(q, r)) and, transposed through LDS, to rows r (K(r, q)); every
+// store is a run of consecutive floats of one cache row.
+__global__ void __launch_bounds__(GBLOCK) k_gram_f16(View V, const _Float16 *xs, long long ntiles, int nt) {
+  __shared__ float tp[4][64][65];  // each wave's 64 x 64 block, transposed on the way out
+  const long long per = (ntiles + 7) / 8;
+  const long long tile = (blockIdx.x % 8) * per + blockIdx.x / 8;
+  if (tile >= ntiles) return;  // uniform over the workgroup
+  int tq, tr;
+  tri_tile(tile, nt, tq, tr);
   const int wave = threadIdx.x / 64, lane = threadIdx.x % 64;
   const int q0 = tq * GT + (wave / 2) * 64, r0 = tr * GT + (wave % 2) * 64;
   const long long rh = split_row_halves(V.d);
@@ -107,7 +132,9 @@ __global__ void __launch_bounds__(GBLOCK) k_gram_f16(View V, const _Float16 *xs,
   const int steps = (V.d + 15) / 16;
   for (int s = 0; s < steps; ++s) {
     // lane holds features 16s + 8 (lane / 32) .. + 7 of its row: group
-    // 2s + lane / 32, i.e. half8 index 4s + 2 (lane / 32) (hi) and + 1 (lo)
+    // 2s + lane / 32, i.e. half8 index 4s + 2 (lane / 32) (hi) and + 1 (lo);
+    // the four blocks' products of one term go back to back, so no two
+    // consecutive MFMAs accumulate into the same registers
     half8 ah[2], al[2], bh[2], bl[2];
 #pragma unroll
     for (int b = 0; b < 2; ++b) {
@@ -119,36 +146,41 @@ __global__ void __launch_bounds__(GBLOCK) k_gram_f16(View V, const _Float16 *xs,
 #pragma unroll
     for (int i = 0; i < 2; ++i)
 #pragma unroll
-      for (int j = 0; j < 2; ++j) {
-        acc[i][j] = __builtin_amdgcn_mfma_f32_32x32x16_f16(ah[i], bh[j], acc[i][j], 0, 0, 0);
-        acc[i][j] = __builtin_amdgcn_mfma_f32_32x32x16_f16(ah[i], bl[j], acc[i][j], 0, 0, 0);
-        acc[i][j] = __builtin_amdgcn_mfma_f32_32x32x16_f16(al[i], bh[j], acc[i][j], 0, 0, 0);
-      }
+      for (int j = 0; j < 2; ++j) acc[i][j] = __builtin_amdgcn_mfma_f32_32x32x16_f16(ah[i], bh[j], acc[i][j], 0, 0, 0);
+#pragma unroll
+    for (int i = 0; i < 2; ++i)
+#pragma unroll
+      for (int j = 0; j < 2; ++j) acc[i][j] = __builtin_amdgcn_mfma_f32_32x32x16_f16(ah[i], bl[j], acc[i][j], 0, 0, 0);
+#pragma unroll
+    for (int i = 0; i < 2; ++i)
+#pragma unroll
+      for (int j = 0; j < 2; ++j) acc[i][j] = __builtin_amdgcn_mfma_f32_32x32x16_f16(al[i], bh[j], acc[i][j], 0, 0, 0);
   }
   // D[m][n] of block (i, j) in lane element e: m = 8 (e / 4) + 4 (lane / 32)
-  // + e % 4, n = lane % 32
+  // + e % 4, n = lane % 32.  Rows q: K(q, r), lanes on consecutive r; the
+  // values also go to LDS for the transposed store.
+  const bool mirror = tq != tr;
 #pragma unroll
   for (int j = 0; j < 2; ++j) {
-    const int r = r0 + j * 32 + lane % 32;
-    if (r >= l) continue;
-    const double xr = V.xsq[r];
+    const int n = j * 32 + lane % 32, r = r0 + n;
+    const double xr = V.xsq[min(r, l - 1)];
 #pragma unroll
     for (int i = 0; i < 2; ++i)
 #pragma unroll
       for (int e = 0; e < 16; ++e) {
-        const int q = q0 + i * 32 + 8 * (e / 4) + 4 * (lane / 32) + e % 4;
-        if (q >= l) continue;
-        const double dot = (double)acc[i][j][e];
-        float K;
-        switch (V.ktype) {
-          case K_LINEAR: K = (float)dot; break;
-          case K_POLY: K = (float)powi(V.gamma * dot + V.coef0, V.degree); break;
-          case K_RBF: K = expf((float)(-V.gamma * ((V.xsq[q] + xr) - 2 * dot))); break;
-          case K_SIGMOID: K = (float)tanh(V.gamma * dot + V.coef0); break;
-          default: K = 0.0f;
-        }
-        V.rows[(long long)q * V.row_stride + r] = K;
+        const int m = i * 32 + 8 * (e / 4) + 4 * (lane / 32) + e % 4, q = q0 + m;
+        const float K = gram_value(V, V.xsq[min(q, l - 1)], xr, acc[i][j][e]);
+        if (q < l && r < l) V.rows[(long long)q * V.row_stride + r] = K;
+        tp[wave][m][n] = K;
       }
+  }
+  if (!mirror) return;  // uniform over the workgroup
+  __syncthreads();
+  // rows r: K(r, q) = K(q, r), lanes on consecutive q
+  const int q = q0 + lane;
+  for (int n = 0; n < 64; ++n) {
+    const int r = r0 + n;
+    if (r < l && q < l) V.rows[(long long)r * V.row_stride + q] = tp[wave][lane][n];
   }
 }
 
@@ -167,7 +199,7 @@ hipError_t smo_launch_split_f16(const smo::View *V, _Float16 *xs, int *bad, hipS
 
 hipError_t smo_launch_gram_f16(const smo::View *V, const _Float16 *xs, hipStream_t st) {
   const int nt = (V->l + smo::GT - 1) / smo::GT;
-  const long long ntiles = (long long)nt * nt;
+  const long long ntiles = (long long)nt * (nt + 1) / 2;  // the upper triangle of tiles
   if (ntiles <= 0) return hipSuccess;
   const long long grid = (ntiles + 7) / 8 * 8;  // whole rounds of the XCD order
   if (grid > 0x7fffffffLL) return hipErrorInvalidValue;
