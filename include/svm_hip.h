@@ -51,8 +51,9 @@ typedef enum { SVMHIP_KIND_C = 0, SVMHIP_KIND_NU = 1 } svmhip_kind;
 typedef struct {
 	size_t struct_size;       /* sizeof(svmhip_options) */
 	int device;               /* HIP device ordinal (default 0) */
-	int grid_blocks;          /* persistent-grid workgroups (0 = the size rule: one per CU on large
-				     problems, fewer on mid-size ones) */
+	int grid_blocks;          /* persistent-grid workgroups (0 = the launch table written by the
+				     autotuner, libsvm-cuda_amd/tune/launch_table.txt, else the size
+				     rule: one per CU on large problems, fewer on mid-size ones) */
 	int block_threads;        /* threads per workgroup (0 = 512, the only size built) */
 	int max_iter;             /* 0 = LIBSVM rule max(10^7, 100*n_var), n_var = 2l for regression */
 	double cache_bytes;       /* kernel-row cache bytes (0 = param.cache_size MB) */
@@ -70,6 +71,9 @@ typedef struct {
 				     products, FP32 accumulation; kernel values within ~2e-6 relative of
 				     the exact ones, median 5e-8), then run the SMO loop on cache hits;
 				     0 = exact rows (FP64 dots) on demand, the oracle's arithmetic */
+	int exchange_copies;      /* svmhip_solve: copies of the per-iteration exchange records, 1..4
+				     (0 = the launch table, else 4 with 64 or more workgroups and 1
+				     below); results are identical for every value */
 	double hbm_fraction;      /* batched solves: at most this fraction of the free device memory
 				     goes to kernel-row cache arenas (0 = 0.7); svm_train and
 				     svm_cross_validation take it from the environment variable

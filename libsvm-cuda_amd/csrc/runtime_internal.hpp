@@ -79,6 +79,7 @@ struct svmhip_solver {
   int S = 0, full = 0;
   long long row_stride = 0;
   int nblocks = 1;
+  int xrep = 0;                   // exchange-record copies of the persistent kernel (0 = by grid size)
   size_t shmem = 0;
   int use_lds = 1;
   bool rbf_ready = false, setup_done = false, have_j = false;
@@ -235,7 +236,7 @@ struct svmhip_solver {
     V.slot_of_row = d_slot_of_row; V.row_of_slot = d_row_of_slot; V.lru_prev = d_prev; V.lru_next = d_next;
     V.computed = d_computed;
     V.ctl = d_ctl; V.part = d_part; V.xs_global = d_xs;
-    V.use_lds = use_lds; V.nblocks = nblocks; V.rank = -1;
+    V.use_lds = use_lds; V.nblocks = nblocks; V.rank = -1; V.xrep = xrep;
     V.xw = d_xw; V.pm_flags = d_pm_flags; V.pm_lru = d_pm_lru; V.pm_stride = pm_stride;
     return V;
   }

@@ -110,6 +110,7 @@ struct View {
                           // (nblocks copies, pm_stride apart) are the subproblem's own
   int keep_cache;         // batched solves: the own cache is valid from an earlier stage
   int spec;               // persistent C-rule solves, full cache: speculative rows on an i miss
+  int xrep;               // persistent solves: copies of the exchange records (0 = by the grid size)
   int flag_lds;           // persistent solves, full cache: > 0 = the workgroup's row flags live as
                           // a bitset in dynamic LDS from float offset flag_lds (>= 4, past the
                           // query row) during the launch; 0 = in pm_flags only
@@ -1258,16 +1259,20 @@ __device__ __forceinline__ void st_agent(T *p, T v) {
   __hip_atomic_store(p, v, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
 }
 
-// Copies in use for nb owners: with few owners the reads of one copy are
-// few too, and the extra stores would cost more than they save.
-__device__ __forceinline__ int xcopies(int nb) { return nb >= 64 ? XREP : 1; }
+// Copies in use for nb owners (xrep > 0: that many, at most XREP): by
+// default XREP with 64 or more owners; with few owners the reads of one copy
+// are few too, and the extra stores would cost more than they save.
+__device__ __forceinline__ int xcopies(int nb, int xrep) {
+  if (xrep > 0) return xrep < XREP ? xrep : XREP;
+  return nb >= 64 ? XREP : 1;
+}
 
 // Owner t's record of exchange `seq`, into every copy in use (one thread).
 template <int F>
-__device__ __forceinline__ void xpublish(XWord *X, int nb, int t, const XPart &p, unsigned seq) {
+__device__ __forceinline__ void xpublish(XWord *X, int nb, int t, const XPart &p, unsigned seq, int xrep = 0) {
   unsigned w[XW];
   xpack<F>(p, w);
-  const int nc = xcopies(nb);
+  const int nc = xcopies(nb, xrep);
 #pragma unroll 1
   for (int c = 0; c < nc; ++c)
 #pragma unroll
@@ -1276,8 +1281,8 @@ __device__ __forceinline__ void xpublish(XWord *X, int nb, int t, const XPart &p
 }
 
 // The copy of the buffers this workgroup reads.
-__device__ __forceinline__ XWord *xcopy(XWord *X, int nb) {
-  return X + (size_t)(blockIdx.x % xcopies(nb)) * xcopy_words(nb);
+__device__ __forceinline__ XWord *xcopy(XWord *X, int nb, int xrep) {
+  return X + (size_t)(blockIdx.x % xcopies(nb, xrep)) * xcopy_words(nb);
 }
 
 // Waits for owner t's record of exchange `seq` and unpacks it into q (one
@@ -1286,11 +1291,11 @@ __device__ __forceinline__ XWord *xcopy(XWord *X, int nb) {
 // few cache lines and slows every reader), then reads the whole record and
 // checks every word.  False if the wait exceeded SPIN_LIMIT polls.
 template <int F>
-__device__ __forceinline__ bool xgather(XWord *X, int nb, int t, unsigned seq, XPart &q) {
+__device__ __forceinline__ bool xgather(XWord *X, int nb, int t, unsigned seq, XPart &q, int xrep = 0) {
   constexpr int NW = xf_words<F>();
   unsigned w[XW];
   long long spins = 0;
-  X = xcopy(X, nb);
+  X = xcopy(X, nb, xrep);
   for (;;) {
     while ((unsigned)(ld_acq(xword(X, nb, seq & 1u, NW - 1, t)) >> 32) != seq) {
       if (++spins > SPIN_LIMIT) return false;
@@ -1311,8 +1316,8 @@ __device__ __forceinline__ bool xgather(XWord *X, int nb, int t, unsigned seq, X
 }
 
 // Payload word w of owner t in buffer `parity` (a completed exchange).
-__device__ __forceinline__ unsigned xpeek(XWord *X, int nb, unsigned parity, int w, int t) {
-  return (unsigned)ld_acq(xword(xcopy(X, nb), nb, parity, w, t));
+__device__ __forceinline__ unsigned xpeek(XWord *X, int nb, unsigned parity, int w, int t, int xrep = 0) {
+  return (unsigned)ld_acq(xword(xcopy(X, nb, xrep), nb, parity, w, t));
 }
 
 // Control state every workgroup holds (identical in all of them); also the
@@ -1367,12 +1372,12 @@ __device__ __forceinline__ XPart exchange(const View &V, RedSmem &sm, XSmem &xs,
                                           bool *hold = nullptr) {
   constexpr int F = PH == PH_J ? XF_J : (KIND == KIND_C ? XF_IC : XF_INU);
   if (threadIdx.x == 0) {
-    xpublish<F>(V.xw, V.nblocks, my_owner(V), mine, seq);
+    xpublish<F>(V.xw, V.nblocks, my_owner(V), mine, seq, V.xrep);
     xs.err = 0;
   }
   XPart q = xpart_identity<KIND, PH>();
   const int t = threadIdx.x;
-  if (t < V.nblocks && !xgather<F>(V.xw, V.nblocks, t, seq, q)) xs.err = 1;
+  if (t < V.nblocks && !xgather<F>(V.xw, V.nblocks, t, seq, q, V.xrep)) xs.err = 1;
   // i records carry each owner's pause vote: every owner sees the same OR
   // (C rule: the max of reduction slot 2, which its i records leave unused,
   // so no barrier of its own; nu rule: one barrier)

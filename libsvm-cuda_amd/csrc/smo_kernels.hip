@@ -232,7 +232,7 @@ __device__ __forceinline__ void spec_rows(const View &V, Dec &d, RedSmem &sm, fl
     // the i record (C rule): i0, i1, v0, v1 in words 0..5
     unsigned w[XW];
 #pragma unroll
-    for (int k = 0; k < 6; ++k) w[k] = xpeek(V.xw, V.nblocks, par, k, t);
+    for (int k = 0; k < 6; ++k) w[k] = xpeek(V.xw, V.nblocks, par, k, t, V.xrep);
     int k = 2;
     iu = (int)w[0];
     il = (int)w[1];

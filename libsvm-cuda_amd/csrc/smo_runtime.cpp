@@ -6,7 +6,70 @@
 // svm_node rows to the SELL-64 layout on upload, and drives either the
 // step-wise kernels (mirroring CudaSolver's method-per-phase interface) or
 // the persistent cooperative kernel (svmhip_solve).
+#include <dlfcn.h>
+
 #include "runtime_internal.hpp"
+
+// ------------------------------------------------------- launch table
+//
+// Launch geometry chosen by the autotuner (libsvm-cuda_amd/tune/autotune.py,
+// which times whole solves over a sweep of grid sizes and exchange-record
+// copies on the device and writes the fastest per problem shape).  The
+// table is a text file, tune/launch_table.txt beside the library's lib/
+// directory (or the file SVMHIP_TUNE_TABLE names), one row per shape range:
+//     mode full dense l_min l_max d_min d_max grid xrep
+// mode "exact" (kernel rows on demand) or "gram" (mfma_rows = 1); full = 1
+// when the whole kernel matrix fits the row cache.  Without the file, or
+// for a shape no row covers, the fixed size rule in svmhip_create applies.
+struct TuneRow {
+  int gram, full, dense;
+  long long lmin, lmax;
+  int dmin, dmax, grid, xrep;
+};
+
+static const std::vector<TuneRow> &tune_table() {
+  static const std::vector<TuneRow> rows = [] {
+    std::vector<TuneRow> out;
+    std::string path;
+    const char *env = getenv("SVMHIP_TUNE_TABLE");
+    if (env && *env) {
+      path = env;
+    } else {
+      Dl_info info;
+      if (dladdr((void *)&tune_table, &info) && info.dli_fname) {
+        std::string so = info.dli_fname;
+        const size_t slash = so.rfind('/');
+        path = (slash == std::string::npos ? std::string(".") : so.substr(0, slash)) + "/../tune/launch_table.txt";
+      }
+    }
+    FILE *f = path.empty() ? nullptr : fopen(path.c_str(), "r");
+    if (!f) return out;
+    char line[256];
+    while (fgets(line, sizeof line, f)) {
+      if (line[0] == '#') continue;
+      char mode[16];
+      TuneRow r;
+      if (sscanf(line, "%15s %d %d %lld %lld %d %d %d %d", mode, &r.full, &r.dense, &r.lmin, &r.lmax, &r.dmin,
+                 &r.dmax, &r.grid, &r.xrep) == 9 &&
+          r.grid > 0) {
+        r.gram = strcmp(mode, "gram") == 0;
+        out.push_back(r);
+      }
+    }
+    fclose(f);
+    return out;
+  }();
+  return rows;
+}
+
+// The table's row for a problem shape, or null.
+static const TuneRow *tuned_launch(int gram, int full, int dense, int l, int d) {
+  for (const TuneRow &r : tune_table())
+    if (r.gram == gram && r.full == full && r.dense == dense && r.lmin <= l && l <= r.lmax && r.dmin <= d &&
+        d <= r.dmax)
+      return &r;
+  return nullptr;
+}
 
 // ------------------------------------------------------------- upload
 
@@ -204,6 +267,14 @@ svmhip_status svmhip_create(const svm_problem *prob, const svm_parameter *param,
     // rows 64 workgroups (79.7 ms; 32: 89.9), 20k and 30k rows 128 (106.9,
     // 129.2 ms; 256: 111.5, 131.9), 100k rows 256 (6% ahead of 128).
     int want = h->opt.grid_blocks;
+    // the cache regime the solves will likely run in (the cache is sized at
+    // setup; this uses the same byte budget)
+    const double cache_b = h->opt.cache_bytes > 0 ? h->opt.cache_bytes : h->param_cache_mb * 1048576.0;
+    const int full_cache = cache_b >= 4.0 * (double)h->l * (double)(((long long)h->l + 63) / 64 * 64);
+    const int gram_mode = h->opt.mfma_rows == 1 && h->dense && h->ktype != PRECOMPUTED && full_cache;
+    const TuneRow *tr = tuned_launch(gram_mode, full_cache, h->dense, h->l, h->d);
+    if (want <= 0 && tr) want = tr->grid;
+    h->xrep = h->opt.exchange_copies > 0 ? std::min(h->opt.exchange_copies, smo::XREP) : (tr ? tr->xrep : 0);
     if (want <= 0) {
       if (h->nslices >= 2 * cus) {
         want = cus;
@@ -373,8 +444,8 @@ svmhip_status svmhip_setup(svmhip_solver *h, const signed char *y, const double 
   h->gram_done = false;
   h->gram_rows = 0;
   if (h->opt.verbose)
-    fprintf(stderr, "svmhip: l=%d d=%d %s, n_var=%d, cache %d rows%s, grid %d x 512\n", h->l, h->d,
-            h->dense ? "dense" : "sparse", nv, h->S, h->full ? " (all)" : "", h->nblocks);
+    fprintf(stderr, "svmhip: l=%d d=%d %s, n_var=%d, cache %d rows%s, grid %d x 512, exchange copies %d\n", h->l,
+            h->d, h->dense ? "dense" : "sparse", nv, h->S, h->full ? " (all)" : "", h->nblocks, h->xrep);
   GUARD_END
 }
 
@@ -781,7 +852,7 @@ svmhip_status svmhip_batch_solve(svmhip_batch *b, svmhip_subproblem *subs, int n
       V.slot_of_row = V.row_of_slot = V.lru_prev = V.lru_next = nullptr;
       V.computed = nullptr;
       V.ctl = dC + k; V.part = nullptr; V.xw = nullptr; V.xs_global = nullptr;
-      V.use_lds = 1; V.nblocks = 1; V.rank = 0; V.resume = 0; V.iter0 = 0;
+      V.use_lds = 1; V.nblocks = 1; V.rank = 0; V.resume = 0; V.iter0 = 0; V.xrep = 0;
       V.ginit_from = 0; V.ginit_to = sp.n_var;
       views[k] = V;
       off += sp.n_var;

@@ -33,7 +33,7 @@ class svmhip_options(C.Structure):
     _fields_ = [("struct_size", C.c_size_t), ("device", C.c_int), ("grid_blocks", C.c_int),
                 ("block_threads", C.c_int), ("max_iter", C.c_int), ("cache_bytes", C.c_double),
                 ("iters_per_launch", C.c_int), ("verbose", C.c_int), ("speculative_rows", C.c_int),
-                ("mfma_rows", C.c_int), ("hbm_fraction", C.c_double)]
+                ("mfma_rows", C.c_int), ("exchange_copies", C.c_int), ("hbm_fraction", C.c_double)]
 
 
 class svmhip_stats(C.Structure):
@@ -201,12 +201,14 @@ class DeviceSolver:
     Method names follow the reference's CudaSolver (libsvm/cuda_solver.h:224-244)."""
 
     def __init__(self, problem: Problem, param, kind=KIND_C, grid_blocks=0, max_iter=0,
-                 cache_bytes=0.0, iters_per_launch=0, device=0, verbose=0, speculative_rows=0, mfma_rows=0):
+                 cache_bytes=0.0, iters_per_launch=0, device=0, verbose=0, speculative_rows=0, mfma_rows=0,
+                 exchange_copies=0):
         L = lib()
         self.problem = problem  # keeps the node buffers alive
         self.param = param
         opt = svmhip_options(C.sizeof(svmhip_options), device, grid_blocks, 0, max_iter,
-                             float(cache_bytes), iters_per_launch, verbose, speculative_rows, mfma_rows)
+                             float(cache_bytes), iters_per_launch, verbose, speculative_rows, mfma_rows,
+                             exchange_copies)
         self._h = C.c_void_p()
         rc = L.svmhip_create(C.byref(problem.struct), C.byref(param), kind, C.byref(opt),
                              C.byref(self._h))

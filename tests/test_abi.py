@@ -136,3 +136,30 @@ def test_datasets_deterministic_and_float32_valued():
     assert X.dtype == np.float32 and set(np.unique(yy)) == {-1.0, 1.0}
     Xc, yc, fold = datasets.clusters_10class()
     assert 36000 <= len(yc) <= 44000 and set(np.unique(fold)) == set(range(5))
+
+
+def test_launch_table_parses_and_covers_the_shapes():
+    """libsvm-cuda_amd/tune/launch_table.txt (written by tune/autotune.py):
+    every row has the nine fields svmhip_create reads, grids are powers of
+    two within 1..512, copies within 1..4, and for each mode and row kind the
+    size ranges leave no gap from 1 row upwards."""
+    path = os.path.join(ROOT, "libsvm-cuda_amd", "tune", "launch_table.txt")
+    rows = []
+    for line in open(path):
+        if line.startswith("#") or not line.strip():
+            continue
+        f = line.split("#")[0].split()
+        assert len(f) == 9, line
+        mode, full, dense, lmin, lmax, dmin, dmax, grid, xrep = f[0], *map(int, f[1:])
+        assert mode in ("exact", "gram") and full in (0, 1) and dense in (0, 1)
+        assert 1 <= grid <= 512 and grid & (grid - 1) == 0 and 1 <= xrep <= 4
+        rows.append((mode, full, dense, dmin, dmax, lmin, lmax))
+    groups = {}
+    for mode, full, dense, dmin, dmax, lmin, lmax in rows:
+        groups.setdefault((mode, full, dense, dmin, dmax), []).append((lmin, lmax))
+    assert ("gram", 1, 1, 65, 2 ** 31 - 1) in groups and ("exact", 1, 1, 65, 2 ** 31 - 1) in groups
+    for key, rs in groups.items():
+        rs.sort()
+        assert rs[0][0] == 1 and rs[-1][1] == 2 ** 31 - 1, key
+        for (a0, a1), (b0, b1) in zip(rs, rs[1:]):
+            assert b0 == a1 + 1, key

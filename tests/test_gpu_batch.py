@@ -318,7 +318,7 @@ def test_batch_hbm_fraction_caps_arenas_same_results(hip):
     subs, rows_of = [], []
     b_def = hip.BatchSolver(prob, prm)
     L = hip.lib()
-    opt = hip.svmhip_options(C.sizeof(hip.svmhip_options), 0, 0, 0, 0, 0.0, 0, 0, 0, 0, 0.002)
+    opt = hip.svmhip_options(C.sizeof(hip.svmhip_options), 0, 0, 0, 0, 0.0, 0, 0, 0, 0, 0, 0.002)
     h = C.c_void_p()
     assert L.svmhip_batch_create(C.byref(prob.struct), C.byref(prm), C.byref(opt), C.byref(h)) == 0
     b_small = hip.BatchSolver.__new__(hip.BatchSolver)
