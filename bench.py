@@ -534,6 +534,7 @@ def bench_single(args, workload, leg=False):
         log(f"[bench] step {k}: {st['iters']} iters, {dt:.3f}s wall, kernel {st['kernel_ms']:.1f} ms, "
             f"gram {st['gram_ms']:.1f} ms, rows computed {st['row_misses']}, obj {st['obj']:.12g}, "
             f"rho {st['rho']:.12g}, device allocations {st['device_allocs']}")
+    _, alpha_dev, _ = s.fetch_vectors()  # the last timed solve's alpha
     iters = sum(st["iters"] for st in stats)
     wall = sum(walls)
     kern_s = sum(st["kernel_ms"] for st in stats) / 1e3
@@ -599,7 +600,12 @@ def bench_single(args, workload, leg=False):
                                                  f"update_alpha_status, {t_plain:.2f}s; the persistent kernel "
                                                  f"over the same {sk['iters']} iterations: {t_fused:.3f}s")
         out["config"]["speedup_vs_stepwise_path"] = t_plain / t_fused
+    if workload == "gmm_dense" and not leg:
+        out["config"]["error_vs_oracle"] = oracle_errors(last["obj"], last["rho"], alpha_dev, n, d)
     s.close()
+    if workload == "gmm_dense" and not leg and not args.mfma:
+        out["config"]["precision_variant"] = bench_precision_variant(args, prob, prm, ys, p, a0, st0, Cp, Cn, kind,
+                                                                     cache_gb, steps, warmup)
     if not leg:
         # svm_train's own wall clock from the host rows (upload, SELL packing,
         # cache allocation, solve, copy-out, model), the row cache sized by
@@ -650,6 +656,72 @@ def bench_predict(args):
                                    f"(serving request 333: C-SVC n={r['n']} dense d={r['d']} C={r['C']})",
                        "nSV": nsv, "kernel_flops_per_call": flops,
                        "flops_formula": "3 * d * nSV * rows (RBF: subtract, multiply, add per feature)"}}
+
+
+MFMA_PEAK_F16 = 2.5e15  # MI355X dense FP16 matrix peak, flop/s
+
+
+def oracle_errors(dev_obj, dev_rho, dev_alpha, n, d):
+    """Errors of a device solve of the n=100k dense problem against the
+    oracle's full-size results (tests/golden/fullsize_gmm_dense_100k.npz):
+    objective (relative), rho (absolute), support-vector set difference."""
+    path = os.path.join(ROOT, "tests", "golden", "fullsize_gmm_dense_100k.npz")
+    if (n, d) != (100000, 256) or not os.path.exists(path):
+        return None
+    fx = np.load(path)
+    sv = np.nonzero(dev_alpha > 0)[0]
+    out = {}
+    for v, name in (("float", "reference_exact"), ("double", "pure_fp64")):
+        out[name] = {"obj_rel_err": abs(dev_obj - float(fx[f"{v}_obj"])) / abs(float(fx[f"{v}_obj"])),
+                     "rho_abs_err": abs(dev_rho - float(fx[f"{v}_rho"])),
+                     "sv_set_symmetric_difference": int(len(np.setxor1d(sv, fx[f"{v}_sv"]))),
+                     "oracle_nSV": int(fx[f"{v}_nSV"]), "oracle_iters": int(fx[f"{v}_iters"])}
+    return out
+
+
+def bench_precision_variant(args, prob, prm, ys, p, a0, st0, Cp, Cn, kind, cache_gb, steps, warmup):
+    """The configuration's precision variant on the same problem: the whole
+    kernel matrix as FP16-split MFMA Gram tiles (mfma_rows = 1), then the
+    SMO loop on cache hits; G, alpha, the gain, rho and the objective stay
+    FP64.  Same steps as the headline, same problem resident in HBM."""
+    s = libsvm_hip.DeviceSolver(prob, prm, kind=kind, grid_blocks=args.grid, cache_bytes=cache_gb * 2 ** 30,
+                                mfma_rows=1)
+    s.setup_rbf_variables()
+    for w in range(warmup):
+        s.setup_solver(ys, p, a0, st0, Cp, Cn)
+        s.solve()
+    walls, stats = [], []
+    for k in range(steps):
+        t = time.perf_counter()
+        s.setup_solver(ys, p, a0, st0, Cp, Cn)
+        st = s.solve()
+        walls.append(time.perf_counter() - t)
+        stats.append(st)
+        log(f"[bench] precision variant step {k}: {st['iters']} iters, {walls[-1]:.3f}s wall, gram {st['gram_ms']:.1f} "
+            f"ms, loop {st['kernel_ms']:.1f} ms, grid {st['grid_blocks']}")
+    _, alpha, _ = s.fetch_vectors()
+    s.close()
+    last = stats[-1]
+    n, d = prob.l, prob.n_features
+    wall = sum(walls)
+    gram_s = sum(x["gram_ms"] for x in stats) / 1e3 / steps
+    loop_s = sum(x["kernel_ms"] for x in stats) / 1e3 / steps
+    flops = 2.0 * n * n * d
+    loop_bytes = last["iters"] * 44.0 * n
+    return {"workload": f"gmm_dense n={n} d={d}, kernel matrix as FP16 hi/lo-split MFMA Gram tiles (three "
+                        f"products, FP32 accumulation), SMO loop in FP64 on cache hits",
+            "value": sum(x["iters"] for x in stats) / wall, "unit": "SMO iters/s", "ms_per_step": 1e3 * wall / steps,
+            "train_time_s": wall / steps, "iters_per_solve": last["iters"], "obj": last["obj"], "rho": last["rho"],
+            "grid_blocks": last["grid_blocks"], "gram_ms": 1e3 * gram_s, "k_smo_ms": 1e3 * loop_s,
+            "dtype": "f16x3 MFMA / f32 accumulation (kernel rows), f64 (G, alpha, gain, rho)",
+            "error_vs_oracle": oracle_errors(last["obj"], last["rho"], alpha, n, d),
+            "roofline_gram": {"bound": "mfma", "kernel": "k_gram_f16", "achieved": flops / gram_s / 1e12,
+                              "peak": MFMA_PEAK_F16 / 1e12, "unit": "TFLOP/s", "frac": flops / gram_s / MFMA_PEAK_F16,
+                              "flops_formula": "2 * l^2 * d (every kernel value's dot product; the hi/lo products "
+                                               "and the symmetric half are not counted)"},
+            "roofline_loop": {"bound": "hbm", "kernel": "k_smo", "achieved": loop_bytes / loop_s / 1e9,
+                              "peak": HBM_PEAK / 1e9, "unit": "GB/s", "frac": loop_bytes / loop_s / HBM_PEAK,
+                              "bytes_formula": "iters * 44*l (cache hits only)"}}
 
 
 def leg_zipf(args):

@@ -482,3 +482,15 @@ def test_mfma_gram_fullsize_gmm_dense_100k(hip):
         assert abs(r["obj"] - obj) <= 1e-6 * abs(obj)
         assert abs(r["rho"] - rho) <= 1e-5
         assert diff <= 0.01 * len(fsv)
+
+
+@pytest.mark.parametrize("copies", [1, 2, 3, 4])
+def test_exchange_copies_agree(hip, copies):
+    """Any number of exchange-record copies (the autotuner's second
+    parameter) gives bit-identical iterates."""
+    prob, d = zipf_problem(4000)
+    prm = ffi.make_param(gamma=1.0 / d, C_=1.0, eps=1e-3, shrinking=0)
+    ref = hip.solve_one(prob, prm, grid_blocks=64, exchange_copies=1)
+    dev = hip.solve_one(prob, prm, grid_blocks=64, exchange_copies=copies)
+    assert dev["iters"] == ref["iters"]
+    np.testing.assert_array_equal(dev["alpha_raw"], ref["alpha_raw"])
