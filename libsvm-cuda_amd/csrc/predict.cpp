@@ -9,7 +9,9 @@
 // as the reference uploads them), copy them to device memory, run k_predict
 // and copy the decision values back.  The two differ only in where the
 // device memory comes from, so their results are identical.
+#include <memory>
 #include <mutex>
+#include <thread>
 
 #include "runtime_internal.hpp"
 
@@ -19,33 +21,67 @@ namespace {
 // out: element p of row 64s+lane at vals[sp[s] + 64p + lane].
 struct Sell64 {
   int l = 0, d = 1, nslices = 0, dense = 0;
-  std::vector<long long> sp;  // nslices + 1
-  std::vector<double> vals;
-  std::vector<int> cols;      // empty when dense
-  std::vector<int> rl;        // stored entries per row
+  std::vector<long long> sp;     // nslices + 1
+  std::unique_ptr<double[]> vals;
+  std::unique_ptr<int[]> cols;   // null when dense
+  std::vector<int> rl;           // stored entries per row
 };
+
+// f(i0, i1) over [0, n) split among host threads when the work (n items of
+// `cost` each) is large enough to pay for them.
+template <class F>
+void par_ranges(int n, double cost, F f) {
+  const unsigned hw = std::max(1u, std::thread::hardware_concurrency());
+  const int nt = (int)std::min<double>({(double)std::min(hw, 16u), (double)n, n * cost / 2e5});
+  if (nt <= 1) {
+    f(0, n);
+    return;
+  }
+  std::vector<std::thread> th;
+  for (int t = 0; t < nt; ++t) {
+    const int i0 = (int)((long long)n * t / nt), i1 = (int)((long long)n * (t + 1) / nt);
+    th.emplace_back([=, &f] { f(i0, i1); });
+  }
+  for (auto &x : th) x.join();
+}
 
 bool pack_sell64(const svm_node *const *x, int n, Sell64 &o, std::string &err) {
   std::vector<int> len(std::max(n, 0));
-  long long nnz = 0;
-  int maxidx = -1;
-  for (int r = 0; r < n; ++r) {
-    const svm_node *p = x[r];
-    if (!p) {
+  for (int r = 0; r < n; ++r)
+    if (!x[r]) {
       err = "row " + std::to_string(r) + " is null";
       return false;
     }
-    int k = 0, prev = INT_MIN;
-    for (; p[k].index != -1; ++k) {
-      if (p[k].index < 0 || p[k].index <= prev) {
-        err = "row " + std::to_string(r) + ": feature indices must be >= 0 and ascending";
-        return false;
+  // row lengths and index checks, in parallel over rows
+  std::vector<int> bad(n > 0 ? 1 : 0, -1);
+  std::mutex mu;
+  long long nnz = 0;
+  int maxidx = -1;
+  par_ranges(n, 64.0, [&](int r0, int r1) {
+    long long z = 0;
+    int mx = -1, b = -1;
+    for (int r = r0; r < r1 && b < 0; ++r) {
+      const svm_node *p = x[r];
+      int k = 0, prev = INT_MIN;
+      for (; p[k].index != -1; ++k) {
+        if (p[k].index < 0 || p[k].index <= prev) {
+          b = r;
+          break;
+        }
+        prev = p[k].index;
       }
-      prev = p[k].index;
-      maxidx = std::max(maxidx, p[k].index);
+      mx = std::max(mx, prev);
+      len[r] = k;
+      z += k;
     }
-    len[r] = k;
-    nnz += k;
+    std::lock_guard<std::mutex> g(mu);
+    nnz += z;
+    maxidx = std::max(maxidx, mx);
+    if (b >= 0 && (bad[0] < 0 || b < bad[0])) bad[0] = b;
+  });
+  if (n > 0 && bad[0] >= 0) {
+    err = "row " + std::to_string(bad[0]) + ": feature indices must be >= 0 and ascending";
+    return false;
   }
   o.l = n;
   o.d = std::max(maxidx + 1, 1);
@@ -61,20 +97,27 @@ bool pack_sell64(const svm_node *const *x, int n, Sell64 &o, std::string &err) {
     }
     o.sp[s + 1] = o.sp[s] + (long long)w * smo::SLICE;
   }
-  o.vals.assign(std::max<long long>(o.sp[o.nslices], 1), 0.0);
-  o.cols.clear();
-  if (!o.dense) o.cols.assign(std::max<long long>(o.sp[o.nslices], 1), 0);
-  for (int r = 0; r < n; ++r) {
-    const long long base = o.sp[r / smo::SLICE] + (r % smo::SLICE);
-    for (int k = 0; k < len[r]; ++k) {
-      if (o.dense) {
-        o.vals[base + (long long)x[r][k].index * smo::SLICE] = x[r][k].value;
-      } else {
-        o.vals[base + (long long)k * smo::SLICE] = x[r][k].value;
-        o.cols[base + (long long)k * smo::SLICE] = x[r][k].index;
+  const long long total = std::max<long long>(o.sp[o.nslices], 1);
+  o.vals.reset(new double[total]);
+  o.cols.reset(o.dense ? nullptr : new int[total]);
+  if (o.sp[o.nslices] == 0) o.vals[0] = 0.0;
+  // each thread zeroes and fills whole slices
+  const double per_slice = o.nslices ? (double)o.sp[o.nslices] / o.nslices : 0.0;
+  par_ranges(o.nslices, per_slice, [&](int s0, int s1) {
+    std::fill(o.vals.get() + o.sp[s0], o.vals.get() + o.sp[s1], 0.0);
+    if (!o.dense) std::fill(o.cols.get() + o.sp[s0], o.cols.get() + o.sp[s1], 0);
+    for (int r = s0 * smo::SLICE; r < std::min(n, s1 * smo::SLICE); ++r) {
+      const long long base = o.sp[r / smo::SLICE] + (r % smo::SLICE);
+      for (int k = 0; k < len[r]; ++k) {
+        if (o.dense) {
+          o.vals[base + (long long)x[r][k].index * smo::SLICE] = x[r][k].value;
+        } else {
+          o.vals[base + (long long)k * smo::SLICE] = x[r][k].value;
+          o.cols[base + (long long)k * smo::SLICE] = x[r][k].index;
+        }
       }
     }
-  }
+  });
   return true;
 }
 
@@ -160,8 +203,8 @@ smo::PRows upload_rows(Arena &ar, const Sell64 &R, int s0, int s1) {
   P.dense = R.dense;
   P.f64 = 1;
   P.slice_ptr = put(ar, sp.data(), sp.size(), "slice pointers");
-  P.vals = put(ar, R.vals.data() + R.sp[s0], nv, "feature values");
-  P.cols = R.dense ? nullptr : put(ar, R.cols.data() + R.sp[s0], nv, "feature columns");
+  P.vals = put(ar, R.vals.get() + R.sp[s0], nv, "feature values");
+  P.cols = R.dense ? nullptr : put(ar, R.cols.get() + R.sp[s0], nv, "feature columns");
   P.row_len = put(ar, R.rl.data() + r0, (size_t)(r1 - r0), "row lengths");
   return P;
 }
@@ -190,9 +233,8 @@ svmhip_status predict_run(const svm_model *model, const svm_node *const *x, int 
       if (!pack_sell64(model->SV, nsv, S, err)) pfail(SVMHIP_ERR_ARG, "support vectors: %s", err.c_str());
       if (!pack_sell64(x, n, T, err)) pfail(SVMHIP_ERR_ARG, "test rows: %s", err.c_str());
       const int ld = std::max(S.d, T.d);
-      size_t shmem = 0;
       // dynamic LDS: the device's per-workgroup limit less the kernel's own
-      const int tb = smo_predict_tb(ld, (size_t)device_max_lds(dev) - 1024, &shmem);
+      const int tb = smo_predict_tb(ld, (size_t)device_max_lds(dev) - 1024, nullptr);
       if (tb < 1) pfail(SVMHIP_ERR_ARG, "rows of %d features are too wide for the prediction kernel", ld);
       const int ncoef = reg ? 1 : k - 1;
       std::vector<double> coef((size_t)ncoef * nsv);
@@ -248,8 +290,7 @@ svmhip_status predict_run(const svm_model *model, const svm_node *const *x, int 
         A.tv = upload_rows(ar, T, s0, s1);
         A.dec = (double *)ar.take(sizeof(double) * (size_t)(r1 - r0) * npair);
         if (!A.dec) pfail(SVMHIP_ERR_NOMEM, "decision values do not fit");
-        const int nb = (r1 - r0 + tb - 1) / tb;
-        pck(smo_launch_predict_offset(&A, 0, nb, shmem, 0), "prediction kernel");
+        pck(smo_launch_predict(&A, r1 - r0, device_cus(dev), 0), "prediction kernel");
         pck(hipMemcpy(dec_values + (long long)r0 * npair, A.dec, sizeof(double) * (size_t)(r1 - r0) * npair,
                       hipMemcpyDeviceToHost),
             "fetch decision values");

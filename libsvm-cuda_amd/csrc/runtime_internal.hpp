@@ -37,8 +37,7 @@ hipError_t smo_launch_kernel_row(const smo::View *V, int q, float *out, size_t s
 hipError_t smo_launch_batch(const smo::View *views, const int *items, int nitems, int grid, const smo::BatchPool *pool,
                             size_t shmem, hipStream_t st);
 hipError_t smo_batch_capacity(size_t shmem, int *blocks_per_cu);
-hipError_t smo_launch_predict_offset(const smo::PredictArgs *A, int first_row, int nblocks, size_t shmem,
-                                    hipStream_t st);
+hipError_t smo_launch_predict(const smo::PredictArgs *A, int nrows, int cus, hipStream_t st);
 int smo_predict_tb(int ld, size_t lds_bytes, size_t *shmem);
 hipError_t smo_launch_split_f16(const smo::View *V, _Float16 *xs, int *bad, hipStream_t st);
 hipError_t smo_launch_gram_f16(const smo::View *V, const _Float16 *xs, hipStream_t st);

@@ -74,8 +74,7 @@ struct Request {
   double mlabels[2] = {0, 0};  // the model's two labels, copied at submission (the model may be released first)
   std::vector<double> dec;
   double *ddec = nullptr;
-  int tb = 1;
-  size_t shmem = 0;
+  int tb = 1;  // most test rows per workgroup of the prediction kernel
 };
 
 }  // namespace
@@ -260,7 +259,7 @@ void admit_predict(svmhip_server *s, Request &q) {
   upload_rows(h, q.hs, q.rows);
   const int n = q.hs.l;
   const int d = std::max(q.rows.d, m.model.d);
-  q.tb = smo_predict_tb(d, device_max_lds(h->dev) - 1024, &q.shmem);
+  q.tb = smo_predict_tb(d, device_max_lds(h->dev) - 1024, nullptr);
   if (q.tb < 1) h->fail(SVMHIP_ERR_ARG, "server: prediction rows of %d features are too wide", d);
   q.ddec = h->dalloc<double>(std::max(n, 1), "prediction output");
   q.bytes = q.rows.bytes + 8.0 * n;
@@ -389,8 +388,7 @@ void run_predict(svmhip_server *s, Request &q) {
   A.regression_like = 0;
   A.ld = std::max(q.rows.d, m.model.d);
   A.dec = q.ddec;
-  const int nbt = (n + q.tb - 1) / q.tb;
-  h->ck(smo_launch_predict_offset(&A, 0, nbt, q.shmem, h->st), "prediction kernel");
+  h->ck(smo_launch_predict(&A, n, device_cus(h->dev), h->st), "prediction kernel");
 }
 
 // After the round's synchronisation: decision values to the host.

@@ -20,7 +20,6 @@
 //                   team of them (no reference counterpart: the reference
 //                   solves one-vs-one pairs and folds one after another)
 //   k_pack_rowset   copies a row subset into its own SELL arrays
-//   k_predict       decision values of many rows     (svm_predict_values, svm.cpp:2569-2643)
 #include <stdlib.h>
 #include <string.h>
 
@@ -558,7 +557,7 @@ __device__ __forceinline__ void ginit_pass(const View &V, float *xs, int ld, con
     row_dot_multi<M>(V, xs, ld, r, acc);
 #pragma unroll
     for (int m = 0; m < M; ++m) {  // static indices keep acc in registers
-      if (m >= cnt) break;
+      if (m >= cnt) continue;
       const int v = list[m];
       const int q = real_of(V, v);
       const float K = (float)kernel_f64(V, q, r, acc[m]);
@@ -1040,202 +1039,6 @@ __global__ void __launch_bounds__(BLOCK) k_pack_rowset(View src, const int *rows
 
 
 
-// ------------------------------------------------------------- prediction
-//
-// Decision values of test rows against a model (svm_predict_values,
-// libsvm/svm.cpp:2569-2643).  A workgroup stages tb test rows densely in LDS
-// as doubles, then walks the support vectors in blocks of BLOCK, one per
-// thread (SELL-64 rows, so a wave's loads are coalesced):
-//   1. each thread evaluates K(t, s) for its support vector s and every
-//      staged test row t in FP64, as Kernel::k_function does
-//      (libsvm/svm.cpp:327-384): the dot product summed in increasing feature
-//      order, and for RBF sum_k (x_k - y_k)^2 directly (a dense-stored
-//      support vector visits every feature in order, which is k_function's
-//      merge sequence term for term; a sparse one adds
-//      (x_k - y_k)^2 - y_k^2 over its own features to the test row's squared
-//      norm), each product rounded before it is added;
-//   2. the values of the block go to LDS, and each (test row, pair) sum is
-//      advanced by one thread over the block's support vectors of the pair's
-//      two classes in index order.  The same thread owns that sum for every
-//      block, so it is summed in svm_predict_values' order: the first
-//      class's support vectors, then the second's.
-// Up to PJOBS sums per thread live in registers; beyond that (many classes)
-// they accumulate in dec itself, still one owner per sum.
-constexpr int PJOBS = 4;
-constexpr int PTB = 16;  // test rows per workgroup at most
-
-template <class VT>
-__device__ __forceinline__ double pval(const PRows &R, long long i) {
-  return (double)static_cast<const VT *>(R.vals)[i];
-}
-
-__device__ __forceinline__ double pred_pair_sum(const PredictArgs &A, const double *kt, int sb, int se, int pr,
-                                                double acc) {
-  // kt[s - sb] = K(t, s) for s in [sb, se)
-  if (A.regression_like) {
-    for (int s = sb; s < se; ++s) acc = acc + A.coef[s] * kt[s - sb];
-    return acc;
-  }
-  int a = 0, rem = pr;  // pair index -> classes (a, b), a < b, in LIBSVM's order
-  while (rem >= A.k - 1 - a) { rem -= A.k - 1 - a; ++a; }
-  const int b = a + 1 + rem;
-  const int nsv = A.sv.l;
-  const double *ca = A.coef + (long long)(b - 1) * nsv, *cb = A.coef + (long long)a * nsv;
-  int lo = max(sb, A.cls_start[a]), hi = min(se, A.cls_start[a] + A.cls_count[a]);
-  for (int s = lo; s < hi; ++s) acc = acc + ca[s] * kt[s - sb];
-  lo = max(sb, A.cls_start[b]);
-  hi = min(se, A.cls_start[b] + A.cls_count[b]);
-  for (int s = lo; s < hi; ++s) acc = acc + cb[s] * kt[s - sb];
-  return acc;
-}
-
-template <class VT>
-__global__ void __launch_bounds__(BLOCK) k_predict(PredictArgs A, int first_row) {
-  extern __shared__ double plds[];  // [tb][ld] test rows, then [tb][BLOCK] kernel values
-  __shared__ double tsq[PTB];
-  const PRows &S = A.sv, &T = A.tv;
-  const int ld = A.ld, tid = threadIdx.x;
-  const int t0 = first_row + blockIdx.x * A.tb;
-  const int nt = min(A.tb, T.l - t0);
-  if (nt <= 0) return;
-  double *xt = plds, *kb = plds + (size_t)A.tb * ld;
-  for (int e = tid; e < A.tb * ld; e += BLOCK) xt[e] = 0.0;
-  __syncthreads();
-  for (int t = 0; t < nt; ++t) {
-    const int q = t0 + t;
-    const int len = T.row_len[q];
-    const long long base = T.slice_ptr[q / SLICE] + (q % SLICE);
-    for (int p = tid; p < len; p += BLOCK) {
-      const int c = T.dense ? p : T.cols[base + (long long)p * SLICE];
-      xt[t * ld + c] = pval<VT>(T, base + (long long)p * SLICE);
-    }
-  }
-  __syncthreads();
-  if (tid < nt) {  // squared norms of the test rows, summed in feature order
-    double s2 = 0.0;
-    for (int k = 0; k < ld; ++k) {
-      const double x = xt[tid * ld + k];
-      s2 = s2 + x * x;
-    }
-    tsq[tid] = s2;
-  }
-  __syncthreads();
-  const bool rbf = A.ktype == K_RBF;
-  const int njobs = nt * A.npair;
-  const bool in_regs = njobs <= PJOBS * BLOCK;
-  double jacc[PJOBS];
-#pragma unroll
-  for (int u = 0; u < PJOBS; ++u) jacc[u] = 0.0;
-  for (int sb = 0; sb < S.l; sb += BLOCK) {
-    const int s = sb + tid;
-    if (s < S.l) {
-      double acc[PTB];
-#pragma unroll
-      for (int t = 0; t < PTB; ++t) acc[t] = 0.0;
-      const int len = S.row_len[s];
-      const long long base = S.slice_ptr[s / SLICE] + (s % SLICE);
-      constexpr int U = 8;
-      if (S.dense) {
-        // every feature in order: stored zeros and the test rows' zeros add
-        // +0 terms, which change no sum
-        for (int k0 = 0; k0 < ld; k0 += U) {
-          double v[U];
-#pragma unroll
-          for (int u = 0; u < U; ++u) v[u] = pval<VT>(S, base + (long long)min(k0 + u, max(len - 1, 0)) * SLICE);
-#pragma unroll
-          for (int u = 0; u < U; ++u) {
-            const int k = k0 + u;
-            if (k >= ld) break;
-            const double sv = k < len ? v[u] : 0.0;
-#pragma unroll
-            for (int t = 0; t < PTB; ++t) {
-              if (t >= nt) break;
-              const double x = xt[t * ld + k];
-              if (rbf) {
-                const double dd = sv - x;
-                acc[t] = acc[t] + dd * dd;
-              } else {
-                acc[t] = acc[t] + sv * x;
-              }
-            }
-          }
-        }
-      } else {
-        const int width = (int)((S.slice_ptr[s / SLICE + 1] - S.slice_ptr[s / SLICE]) / SLICE);
-        for (int p0 = 0; p0 < width; p0 += U) {
-          double v[U];
-          int c[U];
-#pragma unroll
-          for (int u = 0; u < U; ++u) {
-            const long long q = min(p0 + u, width - 1);
-            v[u] = pval<VT>(S, base + q * SLICE);
-            c[u] = S.cols[base + q * SLICE];
-          }
-#pragma unroll
-          for (int u = 0; u < U; ++u) {
-            if (p0 + u >= len) break;
-#pragma unroll
-            for (int t = 0; t < PTB; ++t) {
-              if (t >= nt) break;
-              const double x = xt[t * ld + c[u]];
-              if (rbf) {
-                const double dd = v[u] - x;
-                acc[t] = acc[t] + (dd * dd - x * x);
-              } else {
-                acc[t] = acc[t] + v[u] * x;
-              }
-            }
-          }
-        }
-        if (rbf) {
-#pragma unroll
-          for (int t = 0; t < PTB; ++t) acc[t] = tsq[t < nt ? t : 0] + acc[t];
-        }
-      }
-#pragma unroll
-      for (int t = 0; t < PTB; ++t) {
-        if (t >= nt) break;
-        double K;
-        switch (A.ktype) {
-          case K_LINEAR: K = acc[t]; break;
-          case K_POLY: K = powi(A.gamma * acc[t] + A.coef0, A.degree); break;
-          case K_RBF: K = exp(-A.gamma * acc[t]); break;
-          case K_SIGMOID: K = tanh(A.gamma * acc[t] + A.coef0); break;
-          default: K = 0.0;
-        }
-        kb[t * BLOCK + tid] = K;
-      }
-    }
-    __syncthreads();
-    const int se = min(S.l, sb + BLOCK);
-    if (in_regs) {
-#pragma unroll
-      for (int u = 0; u < PJOBS; ++u) {
-        const int j = tid + u * BLOCK;
-        if (j < njobs) jacc[u] = pred_pair_sum(A, kb + (j / A.npair) * BLOCK, sb, se, j % A.npair, jacc[u]);
-      }
-    } else {
-      for (int j = tid; j < njobs; j += BLOCK) {
-        double *dp = A.dec + (long long)(t0 + j / A.npair) * A.npair + j % A.npair;
-        *dp = pred_pair_sum(A, kb + (j / A.npair) * BLOCK, sb, se, j % A.npair, sb == 0 ? 0.0 : *dp);
-      }
-    }
-    __syncthreads();  // kb is rewritten by the next block
-  }
-  if (in_regs) {
-#pragma unroll
-    for (int u = 0; u < PJOBS; ++u) {
-      const int j = tid + u * BLOCK;
-      if (j < njobs) A.dec[(long long)(t0 + j / A.npair) * A.npair + j % A.npair] = jacc[u] - A.rho[j % A.npair];
-    }
-  } else {
-    for (int j = tid; j < njobs; j += BLOCK) {
-      double *dp = A.dec + (long long)(t0 + j / A.npair) * A.npair + j % A.npair;
-      *dp = (S.l > 0 ? *dp : 0.0) - A.rho[j % A.npair];
-    }
-  }
-}
-
 }  // namespace smo
 
 // ------------------------------------------------------------ launchers
@@ -1391,32 +1194,6 @@ hipError_t smo_launch_pack_rowset(const View *src, const int *rows, int n, const
   hipLaunchKernelGGL(smo::k_pack_rowset, dim3((n + smo::BLOCK - 1) / smo::BLOCK), dim3(smo::BLOCK), 0, st, *src, rows, n,
                      dst_slice_ptr, dst_vals, dst_cols, dst_len, dst_xsq);
   return hipGetLastError();
-}
-
-hipError_t smo_launch_predict_offset(const smo::PredictArgs *A, int first_row, int nblocks, size_t shmem,
-                                    hipStream_t st) {
-  if (nblocks <= 0) return hipSuccess;
-  if (A->sv.f64 != A->tv.f64 || A->tb < 1 || A->tb > smo::PTB) return hipErrorInvalidValue;
-  if (A->sv.f64) {
-    (void)hipFuncSetAttribute((const void *)smo::k_predict<double>, hipFuncAttributeMaxDynamicSharedMemorySize,
-                              (int)shmem);
-    hipLaunchKernelGGL(smo::k_predict<double>, dim3(nblocks), dim3(smo::BLOCK), shmem, st, *A, first_row);
-  } else {
-    (void)hipFuncSetAttribute((const void *)smo::k_predict<float>, hipFuncAttributeMaxDynamicSharedMemorySize,
-                              (int)shmem);
-    hipLaunchKernelGGL(smo::k_predict<float>, dim3(nblocks), dim3(smo::BLOCK), shmem, st, *A, first_row);
-  }
-  return hipGetLastError();
-}
-
-// Test rows per workgroup of the prediction kernel for staged rows of ld
-// doubles, within lds_bytes of dynamic LDS (0 if even one does not fit),
-// and the dynamic LDS that takes.
-int smo_predict_tb(int ld, size_t lds_bytes, size_t *shmem) {
-  const size_t per = (size_t)(ld + smo::BLOCK) * sizeof(double);
-  int tb = (int)std::min<size_t>(smo::PTB, lds_bytes / per);
-  if (shmem) *shmem = (size_t)std::max(tb, 0) * per;
-  return tb;
 }
 
 hipError_t smo_launch_kernel_row(const View *V, int q, float *out, size_t shmem, hipStream_t st) {

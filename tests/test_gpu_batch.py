@@ -288,6 +288,43 @@ def test_predict_batch_fp64_features_match_oracle(hip, ora, kname, dense):
     np.testing.assert_array_equal(lab_d, lab_r)
 
 
+@pytest.mark.parametrize("dense", [True, False])
+def test_predict_batch_row_counts_across_launch_split(hip, ora, dense):
+    """Row counts below, at and past whole rounds of 16-row workgroups over
+    the device's compute units: the rows of the last partial round run at a
+    smaller rows-per-workgroup, and a row's decision value does not depend on
+    it (bit-identical to the same row predicted in a batch of 3 or alone),
+    and matches the oracle to 1e-10 relative."""
+    rng = np.random.default_rng(5)
+    d = 48
+
+    def rows(m):
+        X = rng.standard_normal((m, d))
+        if not dense:
+            X[rng.random((m, d)) < 0.6] = 0.0
+        return X
+
+    X = rows(1500)
+    y = np.where(X[:, 0] - 0.3 * X[:, 2] > 0, 1.0, -1.0)
+    prm = ffi.make_param(gamma=0.5 / d, C_=4.0, shrinking=0, cuda_flag=0)
+    m = ora.svm_train(ffi.Problem.from_dense(X, y), prm)
+    T = rows(16 * 256 * 3 + 1001)
+    for n in (1, 3, 255, 16 * 256 + 37, T.shape[0]):
+        test = ffi.Problem.from_dense(T[:n], np.zeros(n))
+        _, dec_d = hip.predict_batch(m.ptr, test)
+        _, dec_r = m.predict_values(test)
+        dec_d, dec_r = dec_d.ravel(), dec_r.ravel()
+        assert np.max(np.abs(dec_d - dec_r)) <= 1e-10 * max(1.0, np.max(np.abs(dec_r))), n
+        if n == 1:
+            first = dec_d[:1].copy()
+        elif n == 3:
+            np.testing.assert_array_equal(dec_d[:1], first)
+            first3 = dec_d[:3].copy()
+        else:
+            np.testing.assert_array_equal(dec_d[:3], first3)
+        print(f"  dense={dense} rows={n}: max |dec - oracle| = {np.max(np.abs(dec_d - dec_r)):.2e}", flush=True)
+
+
 @pytest.mark.parametrize("kname", ["rbf", "poly", "linear"])
 def test_predict_batch_matches_sklearn_decision_function(hip, kname):
     """The device decision values of a model trained by scikit-learn's
