@@ -22,8 +22,15 @@
  *     coefficients), brute-force QP on tiny problems, KKT certificates,
  *     and the two-point closed form.
  *   kernel functions : pinned against closed forms (numpy).
- *   predict / model I/O : pinned against sklearn decision_function and
- *     round trips.
+ *   one-vs-one training (class order, SV packing, sv_coef layout, rho
+ *     order) : pinned against sklearn SVC support_, dual_coef_ and
+ *     intercept_ on a 4-class set.
+ *   predict_values : pinned against sklearn decision_function.
+ *   model I/O : the library's reader and writer are pinned by a hand-written
+ *     model file in LIBSVM 3.18's format (tests/golden/poly_3class.model).
+ *   full-size runs of both Q variants : committed as fixtures by
+ *     tests/golden/make_fullsize_fixtures.py and checked for feasibility and
+ *     consistency.
  *   cross-validation fold assignment (glibc rand()) : parity unpinned.
  */
 #include <math.h>
