@@ -70,7 +70,13 @@ typedef struct {
 				     each solve as Gram tiles on the matrix cores (FP16 hi/lo split
 				     products, FP32 accumulation; kernel values within ~2e-6 relative of
 				     the exact ones, median 5e-8), then run the SMO loop on cache hits;
-				     0 = exact rows (FP64 dots) on demand, the oracle's arithmetic */
+				     0 = exact rows (FP64 dots) on demand, the oracle's arithmetic.
+				     svmhip_predict_batch, models with one coefficient per support
+				     vector (two classes, regression, one-class): 1 = dot products on
+				     the same FP16 hi/lo Gram tiles, kernel values and sums in FP64
+				     (decision values within ~1e-6 relative); other models, values out
+				     of FP16's range and svmhip_predict_batch_noalloc run the FP64
+				     kernel */
 	int exchange_copies;      /* svmhip_solve: copies of the per-iteration exchange records, 1..4
 				     (0 = the launch table, else 4 with 64 or more workgroups and 1
 				     below); results are identical for every value */

@@ -184,6 +184,151 @@ __global__ void __launch_bounds__(GBLOCK) k_gram_f16(View V, const _Float16 *xs,
   }
 }
 
+
+// ------------------------------------------------- prediction Gram tiles
+//
+// Opt-in fast prediction (svmhip_predict_batch with mfma_rows = 1) for
+// models with one coefficient per support vector (two classes, regression,
+// one-class): decision values within ~1e-6 relative instead of FP64's
+// k_function arithmetic.  The test rows and the support vectors are split
+// into dense FP16 hi/lo copies (the layout of k_split_f16, from their FP64
+// values); k_predict_gram_f16 computes 128 test rows x 128 support vectors
+// of dots per workgroup with the three MFMA products of k_gram_f16, the
+// kernel values from the dots and the FP64 squared norms, and the tile's
+// sum of coef_s * K(t, s) per test row (FP64, in a fixed order);
+// k_predict_gram_sum adds the tiles' partial sums in support-vector order.
+
+// Dense split copy of SELL-64 rows with VT values: row r at out + r * rh
+// halves; out must be zero (a sparse row writes only its stored features).
+// sq[r] = sum of x^2 over the row in feature order (FP64).  *bad is set if
+// a value is out of FP16's range.
+template <class VT>
+__global__ void __launch_bounds__(GBLOCK) k_split_rows_f16(PRows R, long long rh, _Float16 *out, double *sq,
+                                                           int *bad) {
+  const int r = blockIdx.x * GBLOCK + threadIdx.x;
+  if (r >= R.l) return;
+  const int len = R.row_len[r];
+  const long long base = R.slice_ptr[r / SLICE] + (r % SLICE);
+  const VT *vals = static_cast<const VT *>(R.vals);
+  _Float16 *row = out + (long long)r * rh;
+  double s2 = 0.0;
+  int b = 0;
+  for (int p = 0; p < len; ++p) {
+    const double x = (double)vals[base + (long long)p * SLICE];
+    const int c = R.dense ? p : R.cols[base + (long long)p * SLICE];
+    s2 = s2 + x * x;
+    b |= !(fabs(x) < 32768.0);
+    const _Float16 h = (_Float16)x;
+    row[(c / 8) * 16 + c % 8] = h;
+    row[(c / 8) * 16 + 8 + c % 8] = (_Float16)(x - (double)h);
+  }
+  sq[r] = s2;
+  if (b) atomicOr(bad, 1);
+}
+
+// Kernel value from a dot product and the FP64 squared norms.
+template <int KT>
+__device__ __forceinline__ double pgram_value(const PredictArgs &A, double xt, double xs, double dot) {
+  if (KT == K_LINEAR) return dot;
+  if (KT == K_POLY) return powi(A.gamma * dot + A.coef0, A.degree);
+  if (KT == K_RBF) return exp(-A.gamma * ((xt + xs) - 2 * dot));
+  return tanh(A.gamma * dot + A.coef0);
+}
+
+// Workgroup b: test-row tile b % ntt, support-vector tile b / ntt.  Wave w
+// takes test rows 64 (w / 2) .. + 63 and support vectors 64 (w % 2) .. + 63
+// of the tile, as k_gram_f16's waves take q and r.  partial[bs * nt + t] =
+// sum over the tile's support vectors s of coef_s * K(t, s).
+template <int KT>
+__global__ void __launch_bounds__(GBLOCK) k_predict_gram_f16(PredictArgs A, const _Float16 *xt, const _Float16 *xs,
+                                                             long long rh, const double *sqt, const double *sqs,
+                                                             double *partial) {
+  __shared__ double red[4][64];
+  const int nt = A.tv.l, ns = A.sv.l;
+  const int ntt = (nt + GT - 1) / GT;
+  const int bt = blockIdx.x % ntt, bs = blockIdx.x / ntt;
+  const int wave = threadIdx.x / 64, lane = threadIdx.x % 64;
+  const int q0 = bt * GT + (wave / 2) * 64, r0 = bs * GT + (wave % 2) * 64;
+  // rows past the ends read the last row (valid memory); their terms are
+  // dropped below
+  const half8 *aq[2], *br[2];
+#pragma unroll
+  for (int b = 0; b < 2; ++b) {
+    const int q = min(q0 + b * 32 + lane % 32, nt - 1), r = min(r0 + b * 32 + lane % 32, ns - 1);
+    aq[b] = reinterpret_cast<const half8 *>(xt + (long long)q * rh) + 2 * (lane / 32);
+    br[b] = reinterpret_cast<const half8 *>(xs + (long long)r * rh) + 2 * (lane / 32);
+  }
+  float16v acc[2][2];
+#pragma unroll
+  for (int i = 0; i < 2; ++i)
+#pragma unroll
+    for (int j = 0; j < 2; ++j)
+#pragma unroll
+      for (int e = 0; e < 16; ++e) acc[i][j][e] = 0.0f;
+  const int steps = (int)(rh / 32);
+  for (int s = 0; s < steps; ++s) {
+    half8 ah[2], al[2], bh[2], bl[2];
+#pragma unroll
+    for (int b = 0; b < 2; ++b) {
+      ah[b] = aq[b][4 * s];
+      al[b] = aq[b][4 * s + 1];
+      bh[b] = br[b][4 * s];
+      bl[b] = br[b][4 * s + 1];
+    }
+#pragma unroll
+    for (int i = 0; i < 2; ++i)
+#pragma unroll
+      for (int j = 0; j < 2; ++j) acc[i][j] = __builtin_amdgcn_mfma_f32_32x32x16_f16(ah[i], bh[j], acc[i][j], 0, 0, 0);
+#pragma unroll
+    for (int i = 0; i < 2; ++i)
+#pragma unroll
+      for (int j = 0; j < 2; ++j) acc[i][j] = __builtin_amdgcn_mfma_f32_32x32x16_f16(ah[i], bl[j], acc[i][j], 0, 0, 0);
+#pragma unroll
+    for (int i = 0; i < 2; ++i)
+#pragma unroll
+      for (int j = 0; j < 2; ++j) acc[i][j] = __builtin_amdgcn_mfma_f32_32x32x16_f16(al[i], bh[j], acc[i][j], 0, 0, 0);
+  }
+  // D[m][n] of block (i, j) in lane element e: m = 8 (e / 4) + 4 (lane / 32)
+  // + e % 4 (test row), n = lane % 32 (support vector)
+  double w[2], ys[2];
+#pragma unroll
+  for (int j = 0; j < 2; ++j) {
+    const int sv = r0 + j * 32 + lane % 32;
+    w[j] = sv < ns ? A.coef[sv] : 0.0;
+    ys[j] = sqs[min(sv, ns - 1)];
+  }
+#pragma unroll
+  for (int i = 0; i < 2; ++i)
+#pragma unroll
+    for (int e = 0; e < 16; ++e) {
+      const int m = i * 32 + 8 * (e / 4) + 4 * (lane / 32) + e % 4;
+      const double xq = sqt[min(q0 + m, nt - 1)];
+      double v = w[0] * pgram_value<KT>(A, xq, ys[0], (double)acc[i][0][e]);
+      v = v + w[1] * pgram_value<KT>(A, xq, ys[1], (double)acc[i][1][e]);
+      // sum over the 32 support vectors of this half-wave
+#pragma unroll
+      for (int k = 0; k < 5; ++k) v = v + __shfl_xor(v, 16 >> k, 32);
+      if (lane % 32 == 0) red[wave][m] = v;
+    }
+  __syncthreads();
+  // waves 2k and 2k + 1 hold the two support-vector halves of test rows
+  // 64k .. 64k + 63
+  if (threadIdx.x < 128) {
+    const int k = threadIdx.x / 64, m = threadIdx.x % 64, t = bt * GT + k * 64 + m;
+    if (t < nt) partial[(long long)bs * nt + t] = red[2 * k][m] + red[2 * k + 1][m];
+  }
+}
+
+// dec[t] = sum over support-vector tiles (in order) of partial - rho.
+__global__ void __launch_bounds__(GBLOCK) k_predict_gram_sum(const double *partial, int nt, int nst, const double *rho,
+                                                             double *dec) {
+  const int t = blockIdx.x * GBLOCK + threadIdx.x;
+  if (t >= nt) return;
+  double a = 0.0;
+  for (int b = 0; b < nst; ++b) a = a + partial[(long long)b * nt + t];
+  dec[t] = a - rho[0];
+}
+
 }  // namespace smo
 
 extern "C" {
@@ -206,6 +351,60 @@ hipError_t smo_launch_gram_f16(const smo::View *V, const _Float16 *xs, hipStream
   hipLaunchKernelGGL(smo::k_gram_f16, dim3((unsigned)grid), dim3(smo::GBLOCK), 0, st, *V, xs, ntiles, nt);
   return hipGetLastError();
 }
+
+// Prediction Gram tiles (see k_predict_gram_f16): the split copies of
+// A->tv and A->sv (each zeroed by the caller, smo_predict_split_halves per
+// row), their squared norms, partial sums of (nt x ceil(ns / 128)) doubles.
+hipError_t smo_launch_predict_split(const smo::PRows *R, _Float16 *out, double *sq, int *bad, int d,
+                                    hipStream_t st) {
+  if (R->l <= 0) return hipSuccess;
+  const long long rh = smo::split_row_halves(d);
+  const unsigned nb = (unsigned)((R->l + smo::GBLOCK - 1) / smo::GBLOCK);
+  if (R->f64)
+    hipLaunchKernelGGL(smo::k_split_rows_f16<double>, dim3(nb), dim3(smo::GBLOCK), 0, st, *R, rh, out, sq, bad);
+  else
+    hipLaunchKernelGGL(smo::k_split_rows_f16<float>, dim3(nb), dim3(smo::GBLOCK), 0, st, *R, rh, out, sq, bad);
+  return hipGetLastError();
+}
+
+hipError_t smo_launch_predict_gram(const smo::PredictArgs *A, const _Float16 *xt, const _Float16 *xs,
+                                   const double *sqt, const double *sqs, double *partial, hipStream_t st) {
+  const int nt = A->tv.l, ns = A->sv.l;
+  if (nt <= 0) return hipSuccess;
+  if (ns <= 0 || A->npair != 1) return hipErrorInvalidValue;
+  const long long rh = smo::split_row_halves(A->ld);
+  const long long ntt = (nt + smo::GT - 1) / smo::GT, nst = (ns + smo::GT - 1) / smo::GT;
+  if (ntt * nst > 0x7fffffffLL) return hipErrorInvalidValue;
+  const dim3 grid((unsigned)(ntt * nst));
+  switch (A->ktype) {
+    case smo::K_LINEAR:
+      hipLaunchKernelGGL(smo::k_predict_gram_f16<smo::K_LINEAR>, grid, dim3(smo::GBLOCK), 0, st, *A, xt, xs, rh, sqt, sqs,
+                         partial);
+      break;
+    case smo::K_POLY:
+      hipLaunchKernelGGL(smo::k_predict_gram_f16<smo::K_POLY>, grid, dim3(smo::GBLOCK), 0, st, *A, xt, xs, rh, sqt, sqs,
+                         partial);
+      break;
+    case smo::K_RBF:
+      hipLaunchKernelGGL(smo::k_predict_gram_f16<smo::K_RBF>, grid, dim3(smo::GBLOCK), 0, st, *A, xt, xs, rh, sqt, sqs,
+                         partial);
+      break;
+    case smo::K_SIGMOID:
+      hipLaunchKernelGGL(smo::k_predict_gram_f16<smo::K_SIGMOID>, grid, dim3(smo::GBLOCK), 0, st, *A, xt, xs, rh, sqt,
+                         sqs, partial);
+      break;
+    default: return hipErrorInvalidValue;
+  }
+  const hipError_t e = hipGetLastError();
+  if (e != hipSuccess) return e;
+  hipLaunchKernelGGL(smo::k_predict_gram_sum, dim3((unsigned)((nt + smo::GBLOCK - 1) / smo::GBLOCK)), dim3(smo::GBLOCK),
+                     0, st, partial, nt, (int)nst, A->rho, A->dec);
+  return hipGetLastError();
+}
+
+long long smo_predict_split_halves(int d) { return smo::split_row_halves(d); }
+
+int smo_predict_gram_tiles(int ns) { return (ns + smo::GT - 1) / smo::GT; }
 
 size_t smo_split_f16_bytes(int l, int d) { return (size_t)l * (size_t)smo::split_row_halves(d) * sizeof(_Float16); }
 

@@ -261,6 +261,18 @@ svmhip_status predict_run(const svm_model *model, const svm_node *const *x, int 
       A.tb = tb;
       A.regression_like = reg ? 1 : 0;
       A.ld = ld;
+      // mfma_rows = 1: models with one coefficient per support vector on
+      // Gram tiles (FP16 hi/lo split, FP32 accumulation), when memory is
+      // allocated (the no-alloc entry point always runs the FP64 kernel)
+      const bool mfma = opt && opt->mfma_rows == 1 && !ar.scratch && (reg || k == 2);
+      _Float16 *sv_split = nullptr;
+      double *sv_sq = nullptr;
+      int *dbad = nullptr;
+      if (mfma) {
+        dbad = (int *)ar.take(sizeof(int));
+        if (!dbad) pfail(SVMHIP_ERR_NOMEM, "range flag does not fit");
+        pck(hipMemset(dbad, 0, sizeof(int)), "clear range flag");
+      }
       // test rows in chunks of whole slices: all of them when memory is
       // allocated, as many as the scratch area has room for otherwise
       const size_t mark = ar.used;
@@ -290,7 +302,31 @@ svmhip_status predict_run(const svm_model *model, const svm_node *const *x, int 
         A.tv = upload_rows(ar, T, s0, s1);
         A.dec = (double *)ar.take(sizeof(double) * (size_t)(r1 - r0) * npair);
         if (!A.dec) pfail(SVMHIP_ERR_NOMEM, "decision values do not fit");
-        pck(smo_launch_predict(&A, r1 - r0, device_cus(dev), 0), "prediction kernel");
+        bool done = false;
+        if (mfma) {  // Gram tiles on the matrix cores, unless a value is out of FP16's range
+          const long long rh = smo_predict_split_halves(ld);
+          auto split = [&](const smo::PRows &R, const char *what, double *&sq) -> _Float16 * {
+            const size_t bytes = sizeof(_Float16) * (size_t)R.l * (size_t)rh;
+            _Float16 *p = (_Float16 *)ar.take(bytes);
+            sq = (double *)ar.take(sizeof(double) * (size_t)std::max(R.l, 1));
+            if (!p || !sq) pfail(SVMHIP_ERR_NOMEM, "%s: FP16 split copy does not fit", what);
+            pck(hipMemset(p, 0, bytes), "clear split copy");
+            pck(smo_launch_predict_split(&R, p, sq, dbad, ld, 0), "split rows");
+            return p;
+          };
+          if (!sv_split) sv_split = split(A.sv, "support vectors", sv_sq);
+          double *tv_sq = nullptr;
+          const _Float16 *tv_split = split(A.tv, "test rows", tv_sq);
+          int bad = 0;
+          pck(hipMemcpy(&bad, dbad, sizeof bad, hipMemcpyDeviceToHost), "range check");
+          if (!bad) {
+            double *partial = (double *)ar.take(sizeof(double) * (size_t)smo_predict_gram_tiles(nsv) * (r1 - r0));
+            if (!partial) pfail(SVMHIP_ERR_NOMEM, "partial sums do not fit");
+            pck(smo_launch_predict_gram(&A, tv_split, sv_split, tv_sq, sv_sq, partial, 0), "prediction Gram tiles");
+            done = true;
+          }
+        }
+        if (!done) pck(smo_launch_predict(&A, r1 - r0, device_cus(dev), 0), "prediction kernel");
         pck(hipMemcpy(dec_values + (long long)r0 * npair, A.dec, sizeof(double) * (size_t)(r1 - r0) * npair,
                       hipMemcpyDeviceToHost),
             "fetch decision values");

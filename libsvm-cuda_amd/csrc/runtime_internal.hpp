@@ -42,6 +42,11 @@ int smo_predict_tb(int ld, size_t lds_bytes, size_t *shmem);
 hipError_t smo_launch_split_f16(const smo::View *V, _Float16 *xs, int *bad, hipStream_t st);
 hipError_t smo_launch_gram_f16(const smo::View *V, const _Float16 *xs, hipStream_t st);
 size_t smo_split_f16_bytes(int l, int d);
+hipError_t smo_launch_predict_split(const smo::PRows *R, _Float16 *out, double *sq, int *bad, int d, hipStream_t st);
+hipError_t smo_launch_predict_gram(const smo::PredictArgs *A, const _Float16 *xt, const _Float16 *xs,
+                                   const double *sqt, const double *sqs, double *partial, hipStream_t st);
+long long smo_predict_split_halves(int d);
+int smo_predict_gram_tiles(int ns);
 hipError_t smo_launch_pack_rowset(const smo::View *src, const int *rows, int n, const long long *dst_slice_ptr,
                                   float *dst_vals, int *dst_cols, int *dst_len, double *dst_xsq, hipStream_t st);
 }

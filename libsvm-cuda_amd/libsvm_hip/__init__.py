@@ -404,10 +404,10 @@ class Model:
             dec[n] = buf
         return lab, dec
 
-    def predict_batch(self, problem: Problem, rows=None, noalloc=False):
+    def predict_batch(self, problem: Problem, rows=None, noalloc=False, mfma=False):
         """Decision values and labels of many rows on the device
         (svmhip_predict_batch, or svmhip_predict_batch_noalloc)."""
-        return predict_batch(self.ptr, problem, rows, noalloc)
+        return predict_batch(self.ptr, problem, rows, noalloc, mfma)
 
     def save(self, path):
         if lib().svm_save_model(path.encode(), self.ptr) != 0:
@@ -421,9 +421,11 @@ class Model:
             pass
 
 
-def predict_batch(model_ptr, problem: Problem, rows=None, noalloc=False):
+def predict_batch(model_ptr, problem: Problem, rows=None, noalloc=False, mfma=False):
     """Decision values and labels of problem's rows (all, or `rows`) against
-    the struct svm_model* model_ptr, on the device."""
+    the struct svm_model* model_ptr, on the device.  mfma: the opt-in Gram-
+    tile path (mfma_rows = 1) for models with one coefficient per support
+    vector."""
     m = model_ptr.contents
     k = m.nr_class
     ndec = 1 if m.param.svm_type in (ONE_CLASS, EPSILON_SVR, NU_SVR) else k * (k - 1) // 2
@@ -434,6 +436,7 @@ def predict_batch(model_ptr, problem: Problem, rows=None, noalloc=False):
     dec = np.zeros((len(rows), ndec))
     lab = np.zeros(len(rows))
     opt = svmhip_options(C.sizeof(svmhip_options), 0, 0, 0, 0, 0.0, 0, 0)
+    opt.mfma_rows = 1 if mfma else 0
     fn = lib().svmhip_predict_batch_noalloc if noalloc else lib().svmhip_predict_batch
     rc = fn(model_ptr, ptrs, len(rows), C.byref(opt), _ptr(dec, C.c_double), _ptr(lab, C.c_double))
     if rc != 0:

@@ -325,6 +325,58 @@ def test_predict_batch_row_counts_across_launch_split(hip, ora, dense):
         print(f"  dense={dense} rows={n}: max |dec - oracle| = {np.max(np.abs(dec_d - dec_r)):.2e}", flush=True)
 
 
+@pytest.mark.parametrize("case", ["rbf-dense", "rbf-sparse", "linear-dense", "poly-sparse", "sigmoid-dense",
+                                  "svr-rbf", "oneclass-rbf", "multiclass-rbf"])
+def test_predict_batch_mfma_close_to_oracle(hip, ora, case):
+    """The opt-in Gram-tile prediction (mfma_rows = 1: FP16 hi/lo split
+    dots on the matrix cores, FP32 accumulation, FP64 kernel values and
+    sums) against the oracle's FP64 predict_values: every decision value
+    within 1e-4 of the decision values' scale (max(1, max |dec|)), the
+    median within 1e-6 of it, and the labels equal wherever |dec| is above
+    that bound.  A model with several coefficient rows (more than two
+    classes) runs the FP64 kernel instead, bit for bit as without the
+    option."""
+    kname, layout = case.split("-")
+    rng = np.random.default_rng(77)
+    n, d = 1200, 96
+
+    def rows(m):
+        X = rng.standard_normal((m, d)) * 0.8
+        if layout == "sparse":
+            X[rng.random((m, d)) < 0.7] = 0.0
+        return X
+
+    X = rows(n)
+    kt = {"rbf": ffi.RBF, "linear": ffi.LINEAR, "poly": ffi.POLY, "sigmoid": ffi.SIGMOID}.get(kname, ffi.RBF)
+    if kname == "svr":
+        y, st = X[:, 0] - 0.4 * X[:, 3] + 0.1 * rng.standard_normal(n), ffi.EPSILON_SVR
+    elif kname == "oneclass":
+        y, st = np.ones(n), ffi.ONE_CLASS
+    elif kname == "multiclass":
+        y, st = np.digitize(X[:, 0] + 0.5 * X[:, 1], [-0.6, 0.0, 0.7]).astype(float), ffi.C_SVC
+    else:
+        y, st = np.where(X[:, 0] + 0.5 * X[:, 1] > 0, 1.0, -1.0), ffi.C_SVC
+    prm = ffi.make_param(svm_type=st, kernel_type=kt, gamma=0.5 / d, coef0=0.2, degree=3, C_=2.0, nu=0.3, p=0.1,
+                         shrinking=0, cuda_flag=0)
+    m = ora.svm_train(ffi.Problem.from_dense(X, y), prm)
+    test = ffi.Problem.from_dense(rows(3000), np.zeros(3000))
+    lab_r, dec_r = m.predict_values(test)
+    lab_m, dec_m = hip.predict_batch(m.ptr, test, mfma=True)
+    scale = max(1.0, float(np.max(np.abs(dec_r))))
+    err = np.abs(dec_m - dec_r)
+    print(f"  {case}: nSV={m.arrays()['l']} max |dec - oracle| = {err.max():.2e}, median {np.median(err):.2e}, "
+          f"scale {scale:.2f}", flush=True)
+    if kname == "multiclass":
+        lab_e, dec_e = hip.predict_batch(m.ptr, test)
+        np.testing.assert_array_equal(dec_m, dec_e)
+        return
+    assert err.max() <= 1e-4 * scale
+    assert np.median(err) <= 1e-6 * scale
+    sure = np.abs(dec_r).ravel() > 1e-4 * scale
+    if st == ffi.C_SVC or st == ffi.ONE_CLASS:
+        np.testing.assert_array_equal(lab_m[sure], lab_r[sure])
+
+
 @pytest.mark.parametrize("kname", ["rbf", "poly", "linear"])
 def test_predict_batch_matches_sklearn_decision_function(hip, kname):
     """The device decision values of a model trained by scikit-learn's
