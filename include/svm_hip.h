@@ -108,6 +108,9 @@ typedef struct {
 	long long device_allocs;  /* device allocations made by the handle since svmhip_create (a
 				     repeated svmhip_setup + svmhip_solve over the same rows adds none
 				     once the first solve has sized the buffers) */
+	int resident_vectors;     /* 1 = the persistent kernel kept each workgroup's owned G, alpha, QD,
+				     labels and statuses in LDS for the solve (they fit); 0 = in
+				     device memory.  Results are identical either way */
 } svmhip_stats;
 
 /* CudaSolver(prob, param, quiet) / CudaSolverNU(...)

@@ -325,6 +325,11 @@ inline double device_clock_hz(int dev) {
   return 1e3 * khz;
 }
 
+// LDS bytes of the persistent kernel's resident vectors for nv variables per
+// row and m rows: G, alpha, QD (FP64), three kernel-row segments (float),
+// labels and statuses (int8).
+inline size_t smo_res_bytes(int nv, size_t m) { return (24 * (size_t)nv + 12 + 2 * (size_t)nv) * m; }
+
 // Dynamic LDS one workgroup may use (bytes).
 inline int device_max_lds(int dev) {
   int v = 0;

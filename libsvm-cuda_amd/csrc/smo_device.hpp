@@ -116,6 +116,10 @@ struct View {
                           // query row) during the launch; 0 = in pm_flags only
   unsigned *drain;        // batched solves: workgroups of the launch that found the queue empty
   int drain_at;           // batched solves: pause once drain reaches this (checked every 32 iterations)
+  int res_off;            // persistent solves: > 0 = the owned variables' G, alpha, QD, labels and
+                          // statuses, and the owned segments of the iteration's kernel rows, live in
+                          // dynamic LDS from this float offset during the launch; 0 = device memory
+  int res_m;              // rows each resident array holds per variable half (>= the owned rows)
 };
 
 // Work queue and kernel-row cache arenas of the batched solver.  Item q of
@@ -211,6 +215,17 @@ __device__ __forceinline__ double wave_mm(double x) {
   x = dmm<MAX>(x, dpp_d<0x143, 0xc>(x));  // row_bcast:31 into rows 2, 3
   return lane63_d(x);
 }
+// Max / min over the first row of 16 lanes (quad_perm xor 1 and xor 2,
+// row_ror 4 and 8), read from lane 0.
+template <bool MAX>
+__device__ __forceinline__ double row_mm(double x) {
+  x = dmm<MAX>(x, dpp_d<0xB1>(x));
+  x = dmm<MAX>(x, dpp_d<0x4E>(x));
+  x = dmm<MAX>(x, dpp_d<0x124>(x));
+  x = dmm<MAX>(x, dpp_d<0x128>(x));
+  return __hiloint2double(__builtin_amdgcn_readlane(__double2hiint(x), 0),
+                          __builtin_amdgcn_readlane(__double2loint(x), 0));
+}
 __device__ __forceinline__ int wave_max_i(int x) {
   x = max(x, dpp_i<0xB1>(x));
   x = max(x, dpp_i<0x4E>(x));
@@ -221,18 +236,27 @@ __device__ __forceinline__ int wave_max_i(int x) {
   return __builtin_amdgcn_readlane(x, 63);
 }
 __device__ __forceinline__ double wave_max(double v) { return wave_mm<true>(v); }
+// Largest i over the lanes of `lanes` (a uniform ballot): one lane read when
+// a single lane holds the extreme value (the usual case), the DPP maximum
+// otherwise.
+__device__ __forceinline__ int lanes_max_i(unsigned long long lanes, bool mine, int i) {
+  if (lanes != 0 && (lanes & (lanes - 1)) == 0) return __builtin_amdgcn_readlane(i, __builtin_ctzll(lanes));
+  return wave_max_i(mine ? i : INT_MIN);
+}
 // (value, index) arg-max / arg-min with ties to the larger index: the extreme
 // value first, then the largest index among the lanes holding it (the same
 // result as any order of pairwise max_idx / min_idx steps).  Every lane
 // receives the result.
 __device__ __forceinline__ void wave_max_idx(double &v, int &i) {
   const double m = wave_mm<true>(v);
-  i = wave_max_i(v == m ? i : INT_MIN);
+  const bool mine = v == m;
+  i = lanes_max_i(__ballot(mine), mine, i);
   v = m;
 }
 __device__ __forceinline__ void wave_min_idx(double &v, int &i) {
   const double m = wave_mm<false>(v);
-  i = wave_max_i(v == m ? i : INT_MIN);
+  const bool mine = v == m;
+  i = lanes_max_i(__ballot(mine), mine, i);
   v = m;
 }
 __device__ __forceinline__ double wave_sum(double v) {
@@ -540,16 +564,27 @@ __device__ __forceinline__ void block_reduce(RedSmem &sm, double (&v)[4], int (&
     sm.i[buf][wave][1] = ix[1];
   }
   __syncthreads();
-  // every thread combines the waves' results (broadcast LDS reads)
-  for (int w = 0; w < nw; ++w) {
-    if (w == wave) continue;
-    int si = 0;
+  // every wave combines the waves' results: lane w < nw reads wave w's (one
+  // LDS round trip), then DPP moves within the first row of 16 lanes and a
+  // lane read; the other lanes hold the identity
+  const bool in = lane < nw;
+  const int src = in ? lane : 0;
+  int si = 0;
 #pragma unroll
-    for (int k = 0; k < N; ++k) {
-      if (kinds[k] == 0) { max_idx(v[k], ix[si], sm.v[buf][w][k], sm.i[buf][w][si]); ++si; }
-      else if (kinds[k] == 1) { min_idx(v[k], ix[si], sm.v[buf][w][k], sm.i[buf][w][si]); ++si; }
-      else v[k] = fmax(v[k], sm.v[buf][w][k]);
+  for (int k = 0; k < N; ++k) {
+    const double x = sm.v[buf][src][k];
+    if (kinds[k] == 2) {
+      v[k] = row_mm<true>(in ? x : -INF);
+      continue;
     }
+    const int xi = sm.i[buf][src][si];
+    const bool mx = kinds[k] == 0;
+    const double xv = in ? x : (mx ? -INF : INF);
+    const double m = mx ? row_mm<true>(xv) : row_mm<false>(xv);
+    const bool mine = in && xv == m;
+    ix[si] = lanes_max_i(__ballot(mine), mine, xi);
+    v[k] = m;
+    ++si;
   }
 }
 
@@ -1332,7 +1367,7 @@ struct Dec {
   double Gmax, Gmax2, Gp, Gn, Gp2, Gn2;
   double Gi, ai, Gip, aip, Gin, ain;
   double QDi, QDip, QDin;
-  int yi, yip, yin, pad2;
+  int yi, yip, yin, yj;
   unsigned long long hits, misses;
   long long t_start;      // device clock at the start of this launch's share of the solve
 };
@@ -1527,6 +1562,8 @@ __device__ __forceinline__ void decide_j(const View &V, Dec &d, const XPart &r) 
   }
   const int yj = r.y0;
   d.j = j;
+  d.yi = yi;
+  d.yj = yj;
   d.slot_j = cache_lookup(V, M, real_of(V, j), d.slot_i, &d.miss_j);
   d.iter += 1;
   double ai2 = ai, aj2 = r.a0;

@@ -304,6 +304,367 @@ __device__ __forceinline__ void spec_rows(const View &V, Dec &d, RedSmem &sm, fl
   __syncthreads();
 }
 
+// ------------------------------------------ resident vectors (persistent kernel)
+//
+// With V.res_off > 0 each workgroup of k_smo keeps its owned variables' G,
+// alpha, QD, labels and statuses in LDS for the whole launch (read at the
+// start; G, alpha and statuses written back at the end), together with the
+// owned segments of the iteration's kernel rows.  A pass then waits on one
+// memory round trip (the cached row segment) instead of two (the vectors,
+// then the winner's scalars for its record), and G is not written to device
+// memory inside the loop.  Variable v of owned row r = real(v) sits at index
+// (v >= l ? m : 0) + r - base.  Within a pass, every entry is read and
+// written by the thread that owns its row in the passes' (wave, lane, slice)
+// mapping; the records read the winners' entries after the pass's
+// reduction barrier.
+struct Res {
+  double *G, *A, *QD;
+  float *seg[3];  // kernel-row segments: C rule [0] row i, [1] row j; nu rule [0] ip, [1] in, [2] j
+  int8_t *y, *st;
+  int m, base;
+};
+
+__device__ __forceinline__ Res res_view(const View &V, int sb) {
+  Res R;
+  const int nv = V.svr ? 2 : 1, m = V.res_m;
+  double *dp = reinterpret_cast<double *>(smo_dyn_lds + V.res_off);
+  R.G = dp;
+  R.A = dp + nv * m;
+  R.QD = dp + 2 * nv * m;
+  float *fp = reinterpret_cast<float *>(dp + 3 * nv * m);
+  R.seg[0] = fp;
+  R.seg[1] = fp + m;
+  R.seg[2] = fp + 2 * m;
+  int8_t *bp = reinterpret_cast<int8_t *>(fp + 3 * m);
+  R.y = bp;
+  R.st = bp + nv * m;
+  R.m = m;
+  R.base = sb * SLICE;
+  return R;
+}
+
+__device__ __forceinline__ int res_index(const View &V, const Res &R, int v) {
+  return v >= V.l ? R.m + (v - V.l) - R.base : v - R.base;
+}
+
+// The owned variables' vectors into LDS (start of a launch).  Ends with a barrier.
+__device__ __forceinline__ void res_load(const View &V, const Res &R, int sb, int se) {
+  const int r0 = sb * SLICE, r1 = min(se * SLICE, V.l), nv = V.svr ? 2 : 1;
+  for (int h = 0; h < nv; ++h)
+    for (int r = r0 + (int)threadIdx.x; r < r1; r += blockDim.x) {
+      const int k = r + h * V.l, x = h * R.m + (r - r0);
+      R.G[x] = V.G[k];
+      R.A[x] = V.alpha[k];
+      R.QD[x] = V.QD[k];
+      R.y[x] = V.y[k];
+      R.st[x] = V.status[k];
+    }
+  __syncthreads();
+}
+
+// G, alpha and statuses back to device memory (end of a launch, after a barrier).
+__device__ __forceinline__ void res_store(const View &V, const Res &R, int sb, int se) {
+  const int r0 = sb * SLICE, r1 = min(se * SLICE, V.l), nv = V.svr ? 2 : 1;
+  for (int h = 0; h < nv; ++h)
+    for (int r = r0 + (int)threadIdx.x; r < r1; r += blockDim.x) {
+      const int k = r + h * V.l, x = h * R.m + (r - r0);
+      V.G[k] = R.G[x];
+      V.alpha[k] = R.A[x];
+      V.status[k] = R.st[x];
+    }
+}
+
+// compute_row_part that also keeps the owned segment of the row in LDS.
+template <bool PIPE>
+__device__ __forceinline__ void compute_row_seg(const View &V, int q, int slot, float *xs, int sb, int se, float *seg,
+                                                int base) {
+  stage_row(V, q, xs);
+  float *dst = V.rows + (long long)slot * V.row_stride;
+  const int wave = threadIdx.x / 64, lane = threadIdx.x % 64, nw = blockDim.x / 64;
+  for (int s = sb + wave; s < se; s += nw) {
+    const int r = s * SLICE + lane;
+    if (r < V.l) {
+      const float kv = kernel_value<PIPE>(V, xs, q, r);
+      dst[r] = kv;
+      seg[r - base] = kv;
+    }
+  }
+  __syncthreads();
+}
+
+// Pass "select j" over the resident vectors (the arithmetic of
+// pass_select_j_nv, libsvm/svm.cpp:866-917 and :1123-1172): the segments of
+// row i (rows ip, in) come from the cache, or are computed on a miss, and
+// are kept for the update pass.
+template <int KIND, int NV, int PS>
+__device__ __forceinline__ Partial rpass_select_j(const View &V, const Snap &c, const Dec &d, float *xs, RedSmem &sm,
+                                                  const Res &R, int sb, int se) {
+  const int qi = KIND == KIND_C ? c.i : c.ip;
+  const int qn = KIND == KIND_C ? -1 : c.in;
+  const int mi = KIND == KIND_C ? c.miss_i : c.miss_ip;
+  const int mn = KIND == KIND_C ? 0 : c.miss_in;
+  const int slot_i = KIND == KIND_C ? c.slot_i : c.slot_ip;
+  if (qi >= 0 && mi) compute_row_seg<true>(V, real_of(V, qi), slot_i, xs, sb, se, R.seg[0], R.base);
+  if (qn >= 0 && mn) compute_row_seg<true>(V, real_of(V, qn), c.slot_in, xs, sb, se, R.seg[1], R.base);
+  const bool li = qi >= 0 && !mi, ln = qn >= 0 && !mn;  // segments read from the cache here
+  const long long ri = qi >= 0 ? (long long)slot_i * V.row_stride : 0;
+  const long long rn = qn >= 0 ? (long long)c.slot_in * V.row_stride : 0;
+  // the nu rule's ip has y = +1 and in has y = -1 by construction
+  const int8_t yi = KIND == KIND_C ? (int8_t)d.yi : (int8_t)1, yn = (int8_t)-1;
+  const double QDi = KIND == KIND_C ? d.QDi : d.QDip, QDn = KIND == KIND_C ? 0.0 : d.QDin;
+  double best = INF;
+  int bj = -1;
+  const int wave = threadIdx.x / 64, lane = threadIdx.x % 64, nw = blockDim.x / 64;
+  for (int s0 = sb + wave; s0 < se; s0 += nw * PS) {
+    float Ki[PS], Kn[PS];
+    bool ok[PS];
+    // the cache loads of PS slices issue back to back (clamped to a valid
+    // owned row, as in pass_select_j_nv)
+#pragma unroll
+    for (int u = 0; u < PS; ++u) {
+      const int r = (s0 + u * nw) * SLICE + lane;
+      ok[u] = s0 + u * nw < se && r < V.l;
+      const int rc = min(min(s0 + u * nw, se - 1) * SLICE + lane, V.l - 1);
+      Ki[u] = li ? V.rows[ri + rc] : (qi >= 0 ? R.seg[0][rc - R.base] : 0.f);
+      Kn[u] = ln ? V.rows[rn + rc] : (qn >= 0 ? R.seg[1][rc - R.base] : 0.f);
+    }
+#pragma unroll
+    for (int u = 0; u < PS; ++u) {
+      if (!ok[u]) continue;
+      const int r = (s0 + u * nw) * SLICE + lane;
+      const int lr = r - R.base;
+      if (li) R.seg[0][lr] = Ki[u];
+      if (ln) R.seg[1][lr] = Kn[u];
+#pragma unroll
+      for (int h = 0; h < NV; ++h) {
+        const int k = r + h * V.l, x = h * R.m + lr;
+        const int8_t y = R.y[x], sk = R.st[x];
+        const double gk = R.G[x], qd = R.QD[x];
+        if (KIND == KIND_C) {
+          double gd;
+          if (y == +1) {
+            if (sk == AT_LOWER) continue;
+            gd = c.Gmax + gk;
+          } else {
+            if (sk == AT_UPPER) continue;
+            gd = c.Gmax - gk;
+          }
+          if (gd > 0) {
+            float Qik = (yi == y) ? Ki[u] : -Ki[u];
+            double t = (2.0 * yi) * (double)Qik;
+            double quad = (y == +1) ? (QDi + qd) - t : (QDi + qd) + t;
+            double od = quad > 0 ? -(gd * gd) / quad : -(gd * gd) / TAU;
+            min_idx(best, bj, od, k);
+          }
+        } else {
+          if (y == +1) {
+            if (sk == AT_LOWER) continue;
+            double gd = c.Gp + gk;
+            if (gd > 0) {
+              float Q = (yi == y) ? Ki[u] : -Ki[u];
+              double quad = (QDi + qd) - (double)(2.0f * Q);
+              double od = quad > 0 ? -(gd * gd) / quad : -(gd * gd) / TAU;
+              min_idx(best, bj, od, k);
+            }
+          } else {
+            if (sk == AT_UPPER) continue;
+            double gd = c.Gn - gk;
+            if (gd > 0) {
+              float Q = (yn == y) ? Kn[u] : -Kn[u];
+              double quad = (QDn + qd) - (double)(2.0f * Q);
+              double od = quad > 0 ? -(gd * gd) / quad : -(gd * gd) / TAU;
+              min_idx(best, bj, od, k);
+            }
+          }
+        }
+      }
+    }
+  }
+  double v[4] = {best, 0, 0, 0};
+  int ix[2] = {bj, -1};
+  const int kinds[4] = {1, 2, 2, 2};
+  block_reduce<1>(sm, v, ix, kinds);
+  Partial pt;
+  pt.v0 = v[0]; pt.v1 = pt.v2 = pt.v3 = 0;
+  pt.i0 = ix[0]; pt.i1 = -1; pt.pad0 = pt.pad1 = 0;
+  return pt;
+}
+
+// Pass "update G, select i" over the resident vectors (the arithmetic of
+// pass_gradient_select_nv, libsvm/svm.cpp:724-736 and :840-863): the owners
+// store the pending pair's alpha and statuses, row j's segment comes from
+// the cache (or is computed on a miss), row i's is the one the j pass kept.
+template <int KIND, int NV, int PS>
+__device__ __forceinline__ Partial rpass_update_select_i(const View &V, const Snap &c, const Dec &d, float *xs,
+                                                         RedSmem &sm, const Res &R, int sb, int se, const Pend &pd) {
+  if (threadIdx.x == 0) {
+    if (owns(V, sb, se, pd.i)) {
+      const int x = res_index(V, R, pd.i);
+      R.A[x] = pd.ai;
+      R.st[x] = (int8_t)pd.si;
+    }
+    if (owns(V, sb, se, pd.j)) {
+      const int x = res_index(V, R, pd.j);
+      R.A[x] = pd.aj;
+      R.st[x] = (int8_t)pd.sj;
+    }
+  }
+  // (the statuses of i and j are taken from pd below, not from those stores)
+  const int sj = KIND == KIND_C ? 1 : 2;
+  const int si = KIND == KIND_C ? 0 : (c.i == c.ip ? 0 : 1);
+  if (c.miss_j) compute_row_seg<true>(V, real_of(V, c.j), c.slot_j, xs, sb, se, R.seg[sj], R.base);
+  const bool lj = !c.miss_j;
+  const long long rj = (long long)c.slot_j * V.row_stride;
+  const int8_t yi = (int8_t)d.yi, yj = (int8_t)d.yj;
+  double v[4] = {-INF, -INF, -INF, -INF};
+  int ix[2] = {-1, -1};
+  const int wave = threadIdx.x / 64, lane = threadIdx.x % 64, nw = blockDim.x / 64;
+  for (int s0 = sb + wave; s0 < se; s0 += nw * PS) {
+    float Kj[PS];
+    bool ok[PS];
+#pragma unroll
+    for (int u = 0; u < PS; ++u) {
+      const int r = (s0 + u * nw) * SLICE + lane;
+      ok[u] = s0 + u * nw < se && r < V.l;
+      const int rc = min(min(s0 + u * nw, se - 1) * SLICE + lane, V.l - 1);
+      Kj[u] = lj ? V.rows[rj + rc] : R.seg[sj][rc - R.base];
+    }
+#pragma unroll
+    for (int u = 0; u < PS; ++u) {
+      if (!ok[u]) continue;
+      const int r = (s0 + u * nw) * SLICE + lane;
+      const int lr = r - R.base;
+      const float Ki = R.seg[si][lr];
+#pragma unroll
+      for (int h = 0; h < NV; ++h) {
+        const int k = r + h * V.l, x = h * R.m + lr;
+        const int8_t yk = R.y[x];
+        double gk = R.G[x];
+        float Qik = (yi == yk) ? Ki : -Ki;
+        float Qjk = (yj == yk) ? Kj[u] : -Kj[u];
+        double t1 = (double)Qik * c.dai;
+        double t2 = (double)Qjk * c.daj;
+        gk = gk + (t1 + t2);
+        R.G[x] = gk;
+        int8_t sk = R.st[x];
+        if (k == pd.i) sk = (int8_t)pd.si;
+        else if (k == pd.j) sk = (int8_t)pd.sj;
+        if (KIND == KIND_C) {
+          if (yk == +1) {
+            if (sk != AT_UPPER) max_idx(v[0], ix[0], -gk, k);
+            if (sk != AT_LOWER) {
+              if (V.spec) max_idx(v[1], ix[1], gk, k);
+              else v[1] = fmax(v[1], gk);
+            }
+          } else {
+            if (sk != AT_LOWER) max_idx(v[0], ix[0], gk, k);
+            if (sk != AT_UPPER) {
+              if (V.spec) max_idx(v[1], ix[1], -gk, k);
+              else v[1] = fmax(v[1], -gk);
+            }
+          }
+        } else {
+          if (yk == +1) {
+            if (sk != AT_UPPER) max_idx(v[0], ix[0], -gk, k);
+            if (sk != AT_LOWER) v[2] = fmax(v[2], gk);
+          } else {
+            if (sk != AT_LOWER) max_idx(v[1], ix[1], gk, k);
+            if (sk != AT_UPPER) v[3] = fmax(v[3], -gk);
+          }
+        }
+      }
+    }
+  }
+  Partial pt;
+  if (KIND == KIND_C) {
+    const int kinds_spec[4] = {0, 0, 2, 2}, kinds[4] = {0, 2, 2, 2};
+    if (V.spec) block_reduce<2>(sm, v, ix, kinds_spec);
+    else block_reduce<2>(sm, v, ix, kinds);
+  } else {
+    const int kinds[4] = {0, 0, 2, 2};
+    block_reduce<4>(sm, v, ix, kinds);
+  }
+  pt.v0 = v[0]; pt.v1 = v[1]; pt.v2 = v[2]; pt.v3 = v[3];
+  pt.i0 = ix[0]; pt.i1 = ix[1]; pt.pad0 = pt.pad1 = 0;
+  return pt;
+}
+
+// make_i_record / make_j_record with the winners' scalars from LDS.
+template <int KIND>
+__device__ __forceinline__ XPart rmake_i_record(const View &V, const Res &R, const Partial &bp) {
+  XPart m = xpart_identity<KIND, PH_I>();
+  m.v0 = bp.v0; m.v1 = bp.v1; m.v2 = bp.v2; m.v3 = bp.v3;
+  m.i0 = bp.i0; m.i1 = bp.i1;
+  if (threadIdx.x == 0) {
+    if (bp.i0 >= 0) {
+      const int x = res_index(V, R, bp.i0);
+      m.g0 = R.G[x]; m.a0 = R.A[x]; m.qd0 = R.QD[x]; m.y0 = R.y[x];
+    }
+    if (KIND == KIND_NU && bp.i1 >= 0) {
+      const int x = res_index(V, R, bp.i1);
+      m.g1 = R.G[x]; m.a1 = R.A[x]; m.qd1 = R.QD[x]; m.y1 = R.y[x];
+    }
+  }
+  return m;
+}
+
+template <int KIND>
+__device__ __forceinline__ XPart rmake_j_record(const View &V, const Res &R, const Dec &d, const Partial &bp) {
+  XPart m = xpart_identity<KIND, PH_J>();
+  m.v0 = bp.v0;
+  m.i0 = bp.i0;
+  if (threadIdx.x == 0 && bp.i0 >= 0) {
+    const int j = bp.i0, x = res_index(V, R, j);
+    const int yj = R.y[x];
+    // Q_ij from the segment of row i (nu rule: of ip for y_j = +1, of in
+    // for y_j = -1, whose labels equal y_j)
+    const int yi = KIND == KIND_C ? d.yi : yj;
+    const float k = R.seg[KIND == KIND_C ? 0 : (yj == +1 ? 0 : 1)][real_of(V, j) - R.base];
+    m.q0 = yi == yj ? k : -k;
+    m.g0 = R.G[x]; m.a0 = R.A[x]; m.qd0 = R.QD[x]; m.y0 = yj;
+  }
+  return m;
+}
+
+// smo_iterations over the resident vectors (same decisions and exchanges).
+template <int KIND, int NV>
+__device__ __forceinline__ void smo_iterations_res(const View &V, Dec &d, RedSmem &sm, XSmem &xsm, float *xs,
+                                                   const Res &R, int sb, int se, int iter_limit) {
+  for (int n = 0; n < iter_limit; ++n) {
+    if (d.stop || d.err) break;
+    if (KIND == KIND_C && V.spec && d.miss_i) spec_rows<KIND>(V, d, sm, xs);
+    Snap c = snap_of(d);
+    Partial bp = rpass_select_j<KIND, NV, PASS_SLICES>(V, c, d, xs, sm, R, sb, se);
+    XPart mine = rmake_j_record<KIND>(V, R, d, bp);
+    unsigned seq = (unsigned)d.epoch + 1u;
+    bool hold = threadIdx.x == 0;
+    XPart r = exchange<KIND, PH_J>(V, sm, xsm, mine, seq, &hold);
+    if (hold) {
+      d.epoch = (int)seq;
+      if (r.seq) d.err = 1;
+      else decide_j<KIND>(V, d, r);
+    }
+    __syncthreads();
+    if (d.stop || d.err) break;
+    c = snap_of(d);
+    Pend pd;
+    pd.i = d.i; pd.j = d.j; pd.ai = d.ai_new; pd.aj = d.aj_new; pd.si = d.st_i; pd.sj = d.st_j;
+    Partial bp2 = rpass_update_select_i<KIND, NV, PASS_SLICES>(V, c, d, xs, sm, R, sb, se, pd);
+    XPart mine2 = rmake_i_record<KIND>(V, R, bp2);
+    mine2.pad = pause_vote(V, d);
+    seq = (unsigned)d.epoch + 1u;
+    bool hold2 = threadIdx.x == 0;
+    XPart r2 = exchange<KIND, PH_I>(V, sm, xsm, mine2, seq, KIND == KIND_C ? &hold2 : nullptr);
+    if (hold2) {
+      d.epoch = (int)seq;
+      d.pend = 0;
+      if (r2.seq) d.err = 1;
+      else decide_i<KIND>(V, d, r2, true);
+    }
+    __syncthreads();
+  }
+}
+
 // The SMO loop across co-resident owners, one iteration per trip:
 //   [select j over owned variables, computing row i on a miss] exchange
 //   [every workgroup: j, row-j slot, alpha step]
@@ -372,8 +733,23 @@ __global__ void __launch_bounds__(BLOCK) k_smo(View V, int iter_limit) {
       if (fbytes[r]) atomicOr(&fbits[r >> 5], 1u << (r & 31));
   }
   __syncthreads();
+  // resident vectors: this workgroup's owned variables in LDS for the launch
+  int sb, se;
+  owned_slices(V, my_owner(V), V.nblocks, sb, se);
+  const bool res = V.res_off > 0;
+  Res R;
+  if (res) {
+    R = res_view(V, sb);
+    res_load(V, R, sb, se);
+  }
   if (d.phase == 0 && !d.stop && !d.err) smo_select_first<KIND, true>(V, d, sm, xsm, xs);
-  smo_iterations<KIND, true, true>(V, d, sm, xsm, xs, iter_limit);
+  if (!res) smo_iterations<KIND, true, true>(V, d, sm, xsm, xs, iter_limit);
+  else if (V.svr) smo_iterations_res<KIND, 2>(V, d, sm, xsm, xs, R, sb, se, iter_limit);
+  else smo_iterations_res<KIND, 1>(V, d, sm, xsm, xs, R, sb, se, iter_limit);
+  if (res) {
+    __syncthreads();
+    res_store(V, R, sb, se);
+  }
   if (V.flag_lds > 0) {
     __syncthreads();
     for (int r = threadIdx.x; r < V.l; r += blockDim.x) fbytes[r] = (int8_t)((fbits[r >> 5] >> (r & 31)) & 1u);

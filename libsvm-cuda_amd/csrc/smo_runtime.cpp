@@ -604,6 +604,25 @@ svmhip_status svmhip_solve(svmhip_solver *h, int max_iter, svmhip_stats *stats) 
         shmem = need;
       }
     }
+    // the owned variables' vectors and row segments resident in LDS past
+    // the query row and the flag bits (SVMHIP_RESIDENT=0 keeps them in
+    // device memory; results are identical either way)
+    {
+      const char *e = getenv("SVMHIP_RESIDENT");
+      const int nv = h->nvar == 2 * h->l ? 2 : 1;
+      const size_t m = (size_t)((h->nslices + h->nblocks - 1) / h->nblocks) * smo::SLICE;
+      const size_t off = std::max<size_t>(4, (shmem + 15) / 16 * 4);
+      const size_t need = off * sizeof(float) + smo_res_bytes(nv, m);
+      int per_cu = 0;
+      if (!(e && e[0] == '0') && h->nvar > 0 && need <= (size_t)device_max_lds(h->dev) &&
+          smo_persistent_capacity((int)h->kind, need, &per_cu) == hipSuccess &&
+          per_cu * device_cus(h->dev) >= h->nblocks) {
+        V.res_off = (int)off;
+        V.res_m = (int)m;
+        shmem = need;
+      }
+      h->stats.resident_vectors = V.res_off > 0 ? 1 : 0;
+    }
     int chunk = h->opt.iters_per_launch > 0 ? h->opt.iters_per_launch : INT_MAX;
     h->ck(hipEventRecord(h->ev[0], h->st), "event");
     h->stats.gram_ms = 0;
