@@ -545,29 +545,40 @@ struct RedSmem {
 // which the caller must not have the next reduction on the same buffer write
 // before a later barrier (the passes use buffer 0 and the exchanges buffer 1,
 // and every pass or exchange has a barrier between two reductions on one
-// buffer).
+// buffer).  nact > 0: only waves [0, nact) hold values (the others hold the
+// identity and skip the wave step).
 template <int N>
 __device__ __forceinline__ void block_reduce(RedSmem &sm, double (&v)[4], int (&ix)[2], const int (&kinds)[4],
-                                             int buf = 0) {
+                                             int buf = 0, int nact = 0) {
   const int wave = threadIdx.x / 64, lane = threadIdx.x % 64, nw = blockDim.x / 64;
-  int slot_i = 0;
+  const int na = nact > 0 && nact < nw ? nact : nw;
+  if (wave < na) {
+    int slot_i = 0;
 #pragma unroll
-  for (int k = 0; k < N; ++k) {
-    if (kinds[k] == 0) { wave_max_idx(v[k], ix[slot_i]); ++slot_i; }
-    else if (kinds[k] == 1) { wave_min_idx(v[k], ix[slot_i]); ++slot_i; }
-    else v[k] = wave_max(v[k]);
-  }
-  if (lane == 0) {
+    for (int k = 0; k < N; ++k) {
+      if (kinds[k] == 0) { wave_max_idx(v[k], ix[slot_i]); ++slot_i; }
+      else if (kinds[k] == 1) { wave_min_idx(v[k], ix[slot_i]); ++slot_i; }
+      else v[k] = wave_max(v[k]);
+    }
+    if (lane == 0) {
 #pragma unroll
-    for (int k = 0; k < N; ++k) sm.v[buf][wave][k] = v[k];
-    sm.i[buf][wave][0] = ix[0];
-    sm.i[buf][wave][1] = ix[1];
+      for (int k = 0; k < N; ++k) sm.v[buf][wave][k] = v[k];
+      sm.i[buf][wave][0] = ix[0];
+      sm.i[buf][wave][1] = ix[1];
+    }
   }
   __syncthreads();
-  // every wave combines the waves' results: lane w < nw reads wave w's (one
+  if (na == 1) {  // one wave held values: its result is the block's
+#pragma unroll
+    for (int k = 0; k < N; ++k) v[k] = sm.v[buf][0][k];
+    ix[0] = sm.i[buf][0][0];
+    ix[1] = sm.i[buf][0][1];
+    return;
+  }
+  // every wave combines the waves' results: lane w < na reads wave w's (one
   // LDS round trip), then DPP moves within the first row of 16 lanes and a
   // lane read; the other lanes hold the identity
-  const bool in = lane < nw;
+  const bool in = lane < na;
   const int src = in ? lane : 0;
   int si = 0;
 #pragma unroll
@@ -1315,6 +1326,29 @@ __device__ __forceinline__ void xpublish(XWord *X, int nb, int t, const XPart &p
       st_agent(xword(X + (size_t)c * xcopy_words(nb), nb, seq & 1u, k, t), ((XWord)seq << 32) | (XWord)w[k]);
 }
 
+// xpublish by a whole wave (every lane calls it; lane 0 holds the record):
+// lane k stores word k % NW of copy k / NW, so all copies of the record go
+// out in one or two store instructions.
+template <int F>
+__device__ __forceinline__ void xpublish_wave(XWord *X, int nb, int t, const XPart &p, unsigned seq, int xrep = 0) {
+  constexpr int NW = xf_words<F>();
+  unsigned w[XW];
+  xpack<F>(p, w);
+  const int lane = threadIdx.x & 63, nc = xcopies(nb, xrep);
+#pragma unroll
+  for (int k0 = 0; k0 < XREP * NW; k0 += 64) {
+    const int k = k0 + lane;
+    const int q = k % NW;
+    unsigned mine = 0;
+#pragma unroll
+    for (int u = 0; u < NW; ++u) {
+      const unsigned s = __builtin_amdgcn_readfirstlane(w[u]);
+      mine = q == u ? s : mine;
+    }
+    if (k < nc * NW) st_agent(xword(X + (size_t)(k / NW) * xcopy_words(nb), nb, seq & 1u, q, t), ((XWord)seq << 32) | mine);
+  }
+}
+
 // The copy of the buffers this workgroup reads.
 __device__ __forceinline__ XWord *xcopy(XWord *X, int nb, int xrep) {
   return X + (size_t)(blockIdx.x % xcopies(nb, xrep)) * xcopy_words(nb);
@@ -1406,10 +1440,8 @@ template <int KIND, int PH>
 __device__ __forceinline__ XPart exchange(const View &V, RedSmem &sm, XSmem &xs, const XPart &mine, unsigned seq,
                                           bool *hold = nullptr) {
   constexpr int F = PH == PH_J ? XF_J : (KIND == KIND_C ? XF_IC : XF_INU);
-  if (threadIdx.x == 0) {
-    xpublish<F>(V.xw, V.nblocks, my_owner(V), mine, seq, V.xrep);
-    xs.err = 0;
-  }
+  if (threadIdx.x < 64) xpublish_wave<F>(V.xw, V.nblocks, my_owner(V), mine, seq, V.xrep);
+  if (threadIdx.x == 0) xs.err = 0;
   XPart q = xpart_identity<KIND, PH>();
   const int t = threadIdx.x;
   if (t < V.nblocks && !xgather<F>(V.xw, V.nblocks, t, seq, q, V.xrep)) xs.err = 1;
@@ -1420,18 +1452,19 @@ __device__ __forceinline__ XPart exchange(const View &V, RedSmem &sm, XSmem &xs,
   if (PH == PH_I && KIND != KIND_C) pause = __syncthreads_or(t < V.nblocks ? q.pad : 0);
   double v[4] = {q.v0, q.v1, q.v2, q.v3};
   int ix[2] = {q.i0, q.i1};
+  const int nact = (V.nblocks + 63) / 64;  // waves holding records
   if (PH == PH_J) {
     const int kinds[4] = {1, 2, 2, 2};
-    block_reduce<1>(sm, v, ix, kinds, 1);
+    block_reduce<1>(sm, v, ix, kinds, 1, nact);
   } else if (KIND == KIND_C) {
     v[2] = (t < V.nblocks && q.pad) ? 1.0 : 0.0;
     const int kinds_spec[4] = {0, 0, 2, 2}, kinds[4] = {0, 2, 2, 2};
-    if (V.spec) block_reduce<3>(sm, v, ix, kinds_spec, 1);
-    else block_reduce<3>(sm, v, ix, kinds, 1);
+    if (V.spec) block_reduce<3>(sm, v, ix, kinds_spec, 1, nact);
+    else block_reduce<3>(sm, v, ix, kinds, 1, nact);
     pause = v[2] > 0.0 ? 1 : 0;
   } else {
     const int kinds[4] = {0, 0, 2, 2};
-    block_reduce<4>(sm, v, ix, kinds, 1);
+    block_reduce<4>(sm, v, ix, kinds, 1, nact);
   }
   if (hold && (PH == PH_J || KIND == KIND_C)) {
     const bool mine_wins = ix[0] >= 0 ? (t < V.nblocks && q.i0 == ix[0]) : (t == 0);

@@ -392,13 +392,34 @@ __device__ __forceinline__ void compute_row_seg(const View &V, int q, int slot, 
   __syncthreads();
 }
 
+// Row segments fetched right after an exchange, before the decisions that
+// follow it (full cache: a row's slot is the row itself): the first trip's
+// PS slices of this wave's part of cached row `row`.  Used by the next pass
+// only if the row turns out to be cached; otherwise discarded (the loads
+// read valid cache memory either way).
+struct Pre {
+  float k[2][PASS_SLICES];
+  int row[2];  // -1: nothing fetched into k[m]
+};
+
+template <int PS>
+__device__ __forceinline__ void prefetch_seg(const View &V, int row, int sb, int se, float (&k)[PS]) {
+  const int wave = threadIdx.x / 64, lane = threadIdx.x % 64, nw = blockDim.x / 64;
+  const long long off = (long long)row * V.row_stride;
+#pragma unroll
+  for (int u = 0; u < PS; ++u) {
+    const int rc = min(min(sb + wave + u * nw, se - 1) * SLICE + lane, V.l - 1);
+    k[u] = V.rows[off + rc];
+  }
+}
+
 // Pass "select j" over the resident vectors (the arithmetic of
 // pass_select_j_nv, libsvm/svm.cpp:866-917 and :1123-1172): the segments of
 // row i (rows ip, in) come from the cache, or are computed on a miss, and
 // are kept for the update pass.
 template <int KIND, int NV, int PS>
 __device__ __forceinline__ Partial rpass_select_j(const View &V, const Snap &c, const Dec &d, float *xs, RedSmem &sm,
-                                                  const Res &R, int sb, int se) {
+                                                  const Res &R, int sb, int se, const Pre &pre) {
   const int qi = KIND == KIND_C ? c.i : c.ip;
   const int qn = KIND == KIND_C ? -1 : c.in;
   const int mi = KIND == KIND_C ? c.miss_i : c.miss_ip;
@@ -409,6 +430,9 @@ __device__ __forceinline__ Partial rpass_select_j(const View &V, const Snap &c, 
   const bool li = qi >= 0 && !mi, ln = qn >= 0 && !mn;  // segments read from the cache here
   const long long ri = qi >= 0 ? (long long)slot_i * V.row_stride : 0;
   const long long rn = qn >= 0 ? (long long)c.slot_in * V.row_stride : 0;
+  // segments fetched after the last exchange (the same rows, full cache)
+  const bool pi = li && V.full && pre.row[0] == real_of(V, qi);
+  const bool pn = ln && V.full && pre.row[1] == real_of(V, qn);
   // the nu rule's ip has y = +1 and in has y = -1 by construction
   const int8_t yi = KIND == KIND_C ? (int8_t)d.yi : (int8_t)1, yn = (int8_t)-1;
   const double QDi = KIND == KIND_C ? d.QDi : d.QDip, QDn = KIND == KIND_C ? 0.0 : d.QDin;
@@ -425,8 +449,9 @@ __device__ __forceinline__ Partial rpass_select_j(const View &V, const Snap &c, 
       const int r = (s0 + u * nw) * SLICE + lane;
       ok[u] = s0 + u * nw < se && r < V.l;
       const int rc = min(min(s0 + u * nw, se - 1) * SLICE + lane, V.l - 1);
-      Ki[u] = li ? V.rows[ri + rc] : (qi >= 0 ? R.seg[0][rc - R.base] : 0.f);
-      Kn[u] = ln ? V.rows[rn + rc] : (qn >= 0 ? R.seg[1][rc - R.base] : 0.f);
+      const bool first = s0 == sb + wave;
+      Ki[u] = li ? (first && pi ? pre.k[0][u] : V.rows[ri + rc]) : (qi >= 0 ? R.seg[0][rc - R.base] : 0.f);
+      Kn[u] = ln ? (first && pn ? pre.k[1][u] : V.rows[rn + rc]) : (qn >= 0 ? R.seg[1][rc - R.base] : 0.f);
     }
 #pragma unroll
     for (int u = 0; u < PS; ++u) {
@@ -496,7 +521,8 @@ __device__ __forceinline__ Partial rpass_select_j(const View &V, const Snap &c, 
 // the cache (or is computed on a miss), row i's is the one the j pass kept.
 template <int KIND, int NV, int PS>
 __device__ __forceinline__ Partial rpass_update_select_i(const View &V, const Snap &c, const Dec &d, float *xs,
-                                                         RedSmem &sm, const Res &R, int sb, int se, const Pend &pd) {
+                                                         RedSmem &sm, const Res &R, int sb, int se, const Pend &pd,
+                                                         const Pre &pre) {
   if (threadIdx.x == 0) {
     if (owns(V, sb, se, pd.i)) {
       const int x = res_index(V, R, pd.i);
@@ -515,6 +541,7 @@ __device__ __forceinline__ Partial rpass_update_select_i(const View &V, const Sn
   if (c.miss_j) compute_row_seg<true>(V, real_of(V, c.j), c.slot_j, xs, sb, se, R.seg[sj], R.base);
   const bool lj = !c.miss_j;
   const long long rj = (long long)c.slot_j * V.row_stride;
+  const bool pj = lj && V.full && pre.row[0] == real_of(V, c.j);  // fetched after the j exchange
   const int8_t yi = (int8_t)d.yi, yj = (int8_t)d.yj;
   double v[4] = {-INF, -INF, -INF, -INF};
   int ix[2] = {-1, -1};
@@ -527,7 +554,7 @@ __device__ __forceinline__ Partial rpass_update_select_i(const View &V, const Sn
       const int r = (s0 + u * nw) * SLICE + lane;
       ok[u] = s0 + u * nw < se && r < V.l;
       const int rc = min(min(s0 + u * nw, se - 1) * SLICE + lane, V.l - 1);
-      Kj[u] = lj ? V.rows[rj + rc] : R.seg[sj][rc - R.base];
+      Kj[u] = lj ? (s0 == sb + wave && pj ? pre.k[0][u] : V.rows[rj + rc]) : R.seg[sj][rc - R.base];
     }
 #pragma unroll
     for (int u = 0; u < PS; ++u) {
@@ -630,15 +657,23 @@ __device__ __forceinline__ XPart rmake_j_record(const View &V, const Res &R, con
 template <int KIND, int NV>
 __device__ __forceinline__ void smo_iterations_res(const View &V, Dec &d, RedSmem &sm, XSmem &xsm, float *xs,
                                                    const Res &R, int sb, int se, int iter_limit) {
+  Pre pi, pj;  // segments fetched for the next j pass (rows i / ip, in) and the next update pass (row j)
+  pi.row[0] = pi.row[1] = pj.row[0] = pj.row[1] = -1;
   for (int n = 0; n < iter_limit; ++n) {
     if (d.stop || d.err) break;
-    if (KIND == KIND_C && V.spec && d.miss_i) spec_rows<KIND>(V, d, sm, xs);
+    if (KIND == KIND_C && V.spec && d.miss_i) {
+      spec_rows<KIND>(V, d, sm, xs);
+      pi.row[0] = -1;  // row i was computed after the fetch
+    }
     Snap c = snap_of(d);
-    Partial bp = rpass_select_j<KIND, NV, PASS_SLICES>(V, c, d, xs, sm, R, sb, se);
+    Partial bp = rpass_select_j<KIND, NV, PASS_SLICES>(V, c, d, xs, sm, R, sb, se, pi);
     XPart mine = rmake_j_record<KIND>(V, R, d, bp);
     unsigned seq = (unsigned)d.epoch + 1u;
     bool hold = threadIdx.x == 0;
     XPart r = exchange<KIND, PH_J>(V, sm, xsm, mine, seq, &hold);
+    // (every thread holds the winning index)
+    pj.row[0] = V.full && r.i0 >= 0 ? real_of(V, r.i0) : -1;
+    if (pj.row[0] >= 0) prefetch_seg<PASS_SLICES>(V, pj.row[0], sb, se, pj.k[0]);
     if (hold) {
       d.epoch = (int)seq;
       if (r.seq) d.err = 1;
@@ -649,12 +684,16 @@ __device__ __forceinline__ void smo_iterations_res(const View &V, Dec &d, RedSme
     c = snap_of(d);
     Pend pd;
     pd.i = d.i; pd.j = d.j; pd.ai = d.ai_new; pd.aj = d.aj_new; pd.si = d.st_i; pd.sj = d.st_j;
-    Partial bp2 = rpass_update_select_i<KIND, NV, PASS_SLICES>(V, c, d, xs, sm, R, sb, se, pd);
+    Partial bp2 = rpass_update_select_i<KIND, NV, PASS_SLICES>(V, c, d, xs, sm, R, sb, se, pd, pj);
     XPart mine2 = rmake_i_record<KIND>(V, R, bp2);
     mine2.pad = pause_vote(V, d);
     seq = (unsigned)d.epoch + 1u;
     bool hold2 = threadIdx.x == 0;
     XPart r2 = exchange<KIND, PH_I>(V, sm, xsm, mine2, seq, KIND == KIND_C ? &hold2 : nullptr);
+    pi.row[0] = V.full && r2.i0 >= 0 ? real_of(V, r2.i0) : -1;
+    pi.row[1] = V.full && KIND == KIND_NU && r2.i1 >= 0 ? real_of(V, r2.i1) : -1;
+    if (pi.row[0] >= 0) prefetch_seg<PASS_SLICES>(V, pi.row[0], sb, se, pi.k[0]);
+    if (pi.row[1] >= 0) prefetch_seg<PASS_SLICES>(V, pi.row[1], sb, se, pi.k[1]);
     if (hold2) {
       d.epoch = (int)seq;
       d.pend = 0;
