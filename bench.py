@@ -577,7 +577,7 @@ def bench_single(args, workload, leg=False):
         # blocking copy of the stopping test, for a bounded number of
         # iterations from the same start
         s.setup_solver(ys, p, a0, st0, Cp, Cn)
-        k_plain = min(args.plain_iters, last["iters"])
+        k_plain = min(args.plain_iters, last["iters"]) if args.plain_iters > 0 else last["iters"]
         t_start = time.perf_counter()
         done = 0
         for _ in range(k_plain):
@@ -595,17 +595,25 @@ def bench_single(args, workload, leg=False):
         sk = s.solve(max_iter=done)
         t_fused = time.perf_counter() - t
         out["config"]["stepwise_path_iters_per_s"] = done / t_plain
-        out["config"]["stepwise_path_sample"] = (f"first {done} iterations of the same solve through "
+        whole = done == last["iters"]
+        out["config"]["stepwise_path_sample"] = (f"{'the whole solve' if whole else f'first {done} iterations'} "
+                                                 f"({done} iterations) through "
                                                  f"svmhip_select_working_set/compute_alpha/update_gradient/"
                                                  f"update_alpha_status, {t_plain:.2f}s; the persistent kernel "
                                                  f"over the same {sk['iters']} iterations: {t_fused:.3f}s")
         out["config"]["speedup_vs_stepwise_path"] = t_plain / t_fused
+        if whole:
+            out["config"]["stepwise_path_train_time_s"] = t_plain
     if workload == "gmm_dense" and not leg:
         out["config"]["error_vs_oracle"] = oracle_errors(last["obj"], last["rho"], alpha_dev, n, d)
     s.close()
     if workload == "gmm_dense" and not leg and not args.mfma:
         out["config"]["precision_variant"] = bench_precision_variant(args, prob, prm, ys, p, a0, st0, Cp, Cn, kind,
                                                                      cache_gb, steps, warmup)
+        t_step = out["config"].get("stepwise_path_train_time_s")
+        if t_step:
+            pv = out["config"]["precision_variant"]
+            pv["speedup_vs_stepwise_path"] = t_step / pv["train_time_s"]
     if not leg:
         # svm_train's own wall clock from the host rows (upload, SELL packing,
         # cache allocation, solve, copy-out, model), the row cache sized by
@@ -822,7 +830,8 @@ def main():
     ap.add_argument("--stage-ms", type=float, default=10.0)
     ap.add_argument("--batch-cache-mb", type=float, default=400.0)
     ap.add_argument("--no-plain", action="store_true")
-    ap.add_argument("--plain-iters", type=int, default=2000)
+    ap.add_argument("--plain-iters", type=int, default=0,
+                    help="iterations of the step-wise comparison (0 = the whole solve)")
     args = ap.parse_args()
     if args.copy_probe_gib > 0:
         copy_probe(args.copy_probe_gib)
