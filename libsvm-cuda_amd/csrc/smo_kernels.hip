@@ -413,10 +413,35 @@ __device__ __forceinline__ void prefetch_seg(const View &V, int row, int sb, int
   }
 }
 
+// The segment of one cached row for this wave's PS slices of a trip (rows
+// clamped to the last owned one, as in pass_select_j_nv), from the fetch
+// made after the last exchange, from the cache (the loads issued back to
+// back), or from the LDS copy of a row computed in this pass.  The branches
+// are uniform over the wave, so no load waits on another.
+template <int PS>
+__device__ __forceinline__ void seg_fetch(const View &V, const Res &R, int s0, int se, int nw, int lane, bool pre,
+                                          const float (&pk)[PS], bool cache, long long off, const float *seg,
+                                          float (&K)[PS]) {
+  if (pre) {
+#pragma unroll
+    for (int u = 0; u < PS; ++u) K[u] = pk[u];
+  } else if (cache) {
+#pragma unroll
+    for (int u = 0; u < PS; ++u) K[u] = V.rows[off + min(min(s0 + u * nw, se - 1) * SLICE + lane, V.l - 1)];
+  } else if (seg) {
+#pragma unroll
+    for (int u = 0; u < PS; ++u) K[u] = seg[min(min(s0 + u * nw, se - 1) * SLICE + lane, V.l - 1) - R.base];
+  } else {
+#pragma unroll
+    for (int u = 0; u < PS; ++u) K[u] = 0.f;
+  }
+}
+
 // Pass "select j" over the resident vectors (the arithmetic of
 // pass_select_j_nv, libsvm/svm.cpp:866-917 and :1123-1172): the segments of
 // row i (rows ip, in) come from the cache, or are computed on a miss, and
-// are kept for the update pass.
+// are kept for the update pass.  The selection is written with selects
+// rather than branches (the same comparisons, in the same order).
 template <int KIND, int NV, int PS>
 __device__ __forceinline__ Partial rpass_select_j(const View &V, const Snap &c, const Dec &d, float *xs, RedSmem &sm,
                                                   const Res &R, int sb, int se, const Pre &pre) {
@@ -431,8 +456,8 @@ __device__ __forceinline__ Partial rpass_select_j(const View &V, const Snap &c, 
   const long long ri = qi >= 0 ? (long long)slot_i * V.row_stride : 0;
   const long long rn = qn >= 0 ? (long long)c.slot_in * V.row_stride : 0;
   // segments fetched after the last exchange (the same rows, full cache)
-  const bool pi = li && V.full && pre.row[0] == real_of(V, qi);
-  const bool pn = ln && V.full && pre.row[1] == real_of(V, qn);
+  bool pi = li && V.full && pre.row[0] == real_of(V, qi);
+  bool pn = ln && V.full && pre.row[1] == real_of(V, qn);
   // the nu rule's ip has y = +1 and in has y = -1 by construction
   const int8_t yi = KIND == KIND_C ? (int8_t)d.yi : (int8_t)1, yn = (int8_t)-1;
   const double QDi = KIND == KIND_C ? d.QDi : d.QDip, QDn = KIND == KIND_C ? 0.0 : d.QDin;
@@ -441,67 +466,40 @@ __device__ __forceinline__ Partial rpass_select_j(const View &V, const Snap &c, 
   const int wave = threadIdx.x / 64, lane = threadIdx.x % 64, nw = blockDim.x / 64;
   for (int s0 = sb + wave; s0 < se; s0 += nw * PS) {
     float Ki[PS], Kn[PS];
-    bool ok[PS];
-    // the cache loads of PS slices issue back to back (clamped to a valid
-    // owned row, as in pass_select_j_nv)
+    seg_fetch<PS>(V, R, s0, se, nw, lane, pi, pre.k[0], li, ri, qi >= 0 ? R.seg[0] : nullptr, Ki);
+    if (KIND == KIND_NU) seg_fetch<PS>(V, R, s0, se, nw, lane, pn, pre.k[1], ln, rn, qn >= 0 ? R.seg[1] : nullptr, Kn);
+    pi = pn = false;
 #pragma unroll
     for (int u = 0; u < PS; ++u) {
       const int r = (s0 + u * nw) * SLICE + lane;
-      ok[u] = s0 + u * nw < se && r < V.l;
-      const int rc = min(min(s0 + u * nw, se - 1) * SLICE + lane, V.l - 1);
-      const bool first = s0 == sb + wave;
-      Ki[u] = li ? (first && pi ? pre.k[0][u] : V.rows[ri + rc]) : (qi >= 0 ? R.seg[0][rc - R.base] : 0.f);
-      Kn[u] = ln ? (first && pn ? pre.k[1][u] : V.rows[rn + rc]) : (qn >= 0 ? R.seg[1][rc - R.base] : 0.f);
-    }
-#pragma unroll
-    for (int u = 0; u < PS; ++u) {
-      if (!ok[u]) continue;
-      const int r = (s0 + u * nw) * SLICE + lane;
+      if (s0 + u * nw >= se || r >= V.l) continue;
       const int lr = r - R.base;
       if (li) R.seg[0][lr] = Ki[u];
-      if (ln) R.seg[1][lr] = Kn[u];
+      if (KIND == KIND_NU && ln) R.seg[1][lr] = Kn[u];
 #pragma unroll
       for (int h = 0; h < NV; ++h) {
         const int k = r + h * V.l, x = h * R.m + lr;
         const int8_t y = R.y[x], sk = R.st[x];
         const double gk = R.G[x], qd = R.QD[x];
+        const bool pos = y == +1;
+        bool cand;
+        double gd, quad;
         if (KIND == KIND_C) {
-          double gd;
-          if (y == +1) {
-            if (sk == AT_LOWER) continue;
-            gd = c.Gmax + gk;
-          } else {
-            if (sk == AT_UPPER) continue;
-            gd = c.Gmax - gk;
-          }
-          if (gd > 0) {
-            float Qik = (yi == y) ? Ki[u] : -Ki[u];
-            double t = (2.0 * yi) * (double)Qik;
-            double quad = (y == +1) ? (QDi + qd) - t : (QDi + qd) + t;
-            double od = quad > 0 ? -(gd * gd) / quad : -(gd * gd) / TAU;
-            min_idx(best, bj, od, k);
-          }
+          cand = pos ? sk != AT_LOWER : sk != AT_UPPER;
+          gd = pos ? c.Gmax + gk : c.Gmax - gk;
+          const float Qik = (yi == y) ? Ki[u] : -Ki[u];
+          const double t = (2.0 * yi) * (double)Qik;
+          quad = pos ? (QDi + qd) - t : (QDi + qd) + t;
         } else {
-          if (y == +1) {
-            if (sk == AT_LOWER) continue;
-            double gd = c.Gp + gk;
-            if (gd > 0) {
-              float Q = (yi == y) ? Ki[u] : -Ki[u];
-              double quad = (QDi + qd) - (double)(2.0f * Q);
-              double od = quad > 0 ? -(gd * gd) / quad : -(gd * gd) / TAU;
-              min_idx(best, bj, od, k);
-            }
-          } else {
-            if (sk == AT_UPPER) continue;
-            double gd = c.Gn - gk;
-            if (gd > 0) {
-              float Q = (yn == y) ? Kn[u] : -Kn[u];
-              double quad = (QDn + qd) - (double)(2.0f * Q);
-              double od = quad > 0 ? -(gd * gd) / quad : -(gd * gd) / TAU;
-              min_idx(best, bj, od, k);
-            }
-          }
+          cand = pos ? sk != AT_LOWER : sk != AT_UPPER;
+          gd = pos ? c.Gp + gk : c.Gn - gk;
+          const float Q = pos ? ((yi == y) ? Ki[u] : -Ki[u]) : ((yn == y) ? Kn[u] : -Kn[u]);
+          quad = ((pos ? QDi : QDn) + qd) - (double)(2.0f * Q);
         }
+        const double od = -(gd * gd) / (quad > 0 ? quad : TAU);
+        const bool take = cand && gd > 0 && (od < best || (od == best && k > bj));
+        best = take ? od : best;
+        bj = take ? k : bj;
       }
     }
   }
@@ -519,6 +517,7 @@ __device__ __forceinline__ Partial rpass_select_j(const View &V, const Snap &c, 
 // pass_gradient_select_nv, libsvm/svm.cpp:724-736 and :840-863): the owners
 // store the pending pair's alpha and statuses, row j's segment comes from
 // the cache (or is computed on a miss), row i's is the one the j pass kept.
+// Selections as selects, as in rpass_select_j.
 template <int KIND, int NV, int PS>
 __device__ __forceinline__ Partial rpass_update_select_i(const View &V, const Snap &c, const Dec &d, float *xs,
                                                          RedSmem &sm, const Res &R, int sb, int se, const Pend &pd,
@@ -541,25 +540,19 @@ __device__ __forceinline__ Partial rpass_update_select_i(const View &V, const Sn
   if (c.miss_j) compute_row_seg<true>(V, real_of(V, c.j), c.slot_j, xs, sb, se, R.seg[sj], R.base);
   const bool lj = !c.miss_j;
   const long long rj = (long long)c.slot_j * V.row_stride;
-  const bool pj = lj && V.full && pre.row[0] == real_of(V, c.j);  // fetched after the j exchange
+  bool pj = lj && V.full && pre.row[0] == real_of(V, c.j);  // fetched after the j exchange
   const int8_t yi = (int8_t)d.yi, yj = (int8_t)d.yj;
   double v[4] = {-INF, -INF, -INF, -INF};
   int ix[2] = {-1, -1};
   const int wave = threadIdx.x / 64, lane = threadIdx.x % 64, nw = blockDim.x / 64;
   for (int s0 = sb + wave; s0 < se; s0 += nw * PS) {
     float Kj[PS];
-    bool ok[PS];
+    seg_fetch<PS>(V, R, s0, se, nw, lane, pj, pre.k[0], lj, rj, R.seg[sj], Kj);
+    pj = false;
 #pragma unroll
     for (int u = 0; u < PS; ++u) {
       const int r = (s0 + u * nw) * SLICE + lane;
-      ok[u] = s0 + u * nw < se && r < V.l;
-      const int rc = min(min(s0 + u * nw, se - 1) * SLICE + lane, V.l - 1);
-      Kj[u] = lj ? (s0 == sb + wave && pj ? pre.k[0][u] : V.rows[rj + rc]) : R.seg[sj][rc - R.base];
-    }
-#pragma unroll
-    for (int u = 0; u < PS; ++u) {
-      if (!ok[u]) continue;
-      const int r = (s0 + u * nw) * SLICE + lane;
+      if (s0 + u * nw >= se || r >= V.l) continue;
       const int lr = r - R.base;
       const float Ki = R.seg[si][lr];
 #pragma unroll
@@ -574,30 +567,34 @@ __device__ __forceinline__ Partial rpass_update_select_i(const View &V, const Sn
         gk = gk + (t1 + t2);
         R.G[x] = gk;
         int8_t sk = R.st[x];
-        if (k == pd.i) sk = (int8_t)pd.si;
-        else if (k == pd.j) sk = (int8_t)pd.sj;
+        sk = k == pd.i ? (int8_t)pd.si : (k == pd.j ? (int8_t)pd.sj : sk);
+        const bool pos = yk == +1;
         if (KIND == KIND_C) {
-          if (yk == +1) {
-            if (sk != AT_UPPER) max_idx(v[0], ix[0], -gk, k);
-            if (sk != AT_LOWER) {
-              if (V.spec) max_idx(v[1], ix[1], gk, k);
-              else v[1] = fmax(v[1], gk);
-            }
+          // I_up: -y G (max_idx); I_low: y G (plain max, or max_idx for speculative rows)
+          const bool up = pos ? sk != AT_UPPER : sk != AT_LOWER;
+          const bool low = pos ? sk != AT_LOWER : sk != AT_UPPER;
+          const double a = pos ? -gk : gk, b = pos ? gk : -gk;
+          const bool t0 = up && (a > v[0] || (a == v[0] && k > ix[0]));
+          v[0] = t0 ? a : v[0];
+          ix[0] = t0 ? k : ix[0];
+          if (V.spec) {
+            const bool t1s = low && (b > v[1] || (b == v[1] && k > ix[1]));
+            v[1] = t1s ? b : v[1];
+            ix[1] = t1s ? k : ix[1];
           } else {
-            if (sk != AT_LOWER) max_idx(v[0], ix[0], gk, k);
-            if (sk != AT_UPPER) {
-              if (V.spec) max_idx(v[1], ix[1], -gk, k);
-              else v[1] = fmax(v[1], -gk);
-            }
+            v[1] = low ? fmax(v[1], b) : v[1];
           }
         } else {
-          if (yk == +1) {
-            if (sk != AT_UPPER) max_idx(v[0], ix[0], -gk, k);
-            if (sk != AT_LOWER) v[2] = fmax(v[2], gk);
-          } else {
-            if (sk != AT_LOWER) max_idx(v[1], ix[1], gk, k);
-            if (sk != AT_UPPER) v[3] = fmax(v[3], -gk);
-          }
+          const bool c0 = pos && sk != AT_UPPER, c2 = pos && sk != AT_LOWER;
+          const bool c1 = !pos && sk != AT_LOWER, c3 = !pos && sk != AT_UPPER;
+          const bool t0 = c0 && (-gk > v[0] || (-gk == v[0] && k > ix[0]));
+          v[0] = t0 ? -gk : v[0];
+          ix[0] = t0 ? k : ix[0];
+          const bool t1 = c1 && (gk > v[1] || (gk == v[1] && k > ix[1]));
+          v[1] = t1 ? gk : v[1];
+          ix[1] = t1 ? k : ix[1];
+          v[2] = c2 ? fmax(v[2], gk) : v[2];
+          v[3] = c3 ? fmax(v[3], -gk) : v[3];
         }
       }
     }
