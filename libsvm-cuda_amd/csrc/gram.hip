@@ -84,6 +84,17 @@ __device__ __forceinline__ void tri_tile(long long t, int nt, int &tq, int &tr) 
   tr = q + (int)(t - (long long)q * (2LL * nt - q + 1) / 2);
 }
 
+// Kernel value from the dot product (FP64 arithmetic, FP32 exp for RBF),
+// for a kernel type fixed at compile time.
+template <int KT>
+__device__ __forceinline__ float gram_value_t(const View &V, double xq, double xr, float acc) {
+  const double dot = (double)acc;
+  if (KT == K_LINEAR) return (float)dot;
+  if (KT == K_POLY) return (float)powi(V.gamma * dot + V.coef0, V.degree);
+  if (KT == K_RBF) return expf((float)(-V.gamma * ((xq + xr) - 2 * dot)));
+  return (float)tanh(V.gamma * dot + V.coef0);
+}
+
 // Kernel value from the dot product (FP64 arithmetic, FP32 exp for RBF).
 __device__ __forceinline__ float gram_value(const View &V, double xq, double xr, float acc) {
   const double dot = (double)acc;
@@ -102,26 +113,66 @@ __device__ __forceinline__ float gram_value(const View &V, double xq, double xr,
 // XCD's L2 serves a band of q rows).  A tile off the diagonal is written to
 // rows q (K(q, r)) and, transposed through LDS, to rows r (K(r, q)); every
 // store is a run of consecutive floats of one cache row.
-__global__ void __launch_bounds__(GBLOCK) k_gram_f16(View V, const _Float16 *xs, long long ntiles, int nt) {
-  __shared__ float tp[4][64][65];  // each wave's 64 x 64 block, transposed on the way out
+//
+// Operands go through LDS in chunks of GKC = 32 features, double buffered:
+// the workgroup's 128 q rows and 128 r rows of a chunk (128 bytes of hi/lo
+// halves per row) are loaded by all 256 threads into registers while the
+// MFMAs of the previous chunk run from LDS, then stored to the other buffer;
+// one barrier per chunk.  Rows are padded to 144 bytes in LDS against bank
+// conflicts of the 16-byte operand reads.
+constexpr int GKC = 32;                   // features per chunk
+constexpr int GROW = GKC * 2 * 2 + 16;    // LDS bytes per row of a chunk: hi/lo halves + pad
+constexpr int GBUF = 2 * GT * GROW;       // one buffer: 128 q rows + 128 r rows
+constexpr int GLDS = 2 * GBUF > 4 * 64 * 65 * 4 ? 2 * GBUF : 4 * 64 * 65 * 4;
+
+template <int KT>
+__global__ void __launch_bounds__(GBLOCK, 2) k_gram_f16(View V, const _Float16 *xs, long long ntiles, int nt) {
+  __shared__ __attribute__((aligned(16))) unsigned char lds[GLDS];
+  __shared__ double sq[2 * GT];  // squared norms of the tile's q rows, then r rows
   const long long per = (ntiles + 7) / 8;
   const long long tile = (blockIdx.x % 8) * per + blockIdx.x / 8;
   if (tile >= ntiles) return;  // uniform over the workgroup
   int tq, tr;
   tri_tile(tile, nt, tq, tr);
   const int wave = threadIdx.x / 64, lane = threadIdx.x % 64;
+  if (KT == K_RBF) {  // read by the epilogue, after the main loop's barriers
+    const int t = (int)threadIdx.x, base = t < GT ? tq * GT : tr * GT - GT;
+    sq[t] = V.xsq[min(base + t, V.l - 1)];
+  }
   const int q0 = tq * GT + (wave / 2) * 64, r0 = tr * GT + (wave % 2) * 64;
   const long long rh = split_row_halves(V.d);
+  const int ngroups = (int)(rh / 16);  // groups of 8 features ([hi x 8][lo x 8], 32 bytes) per row
   const int l = V.l;
-  // rows this lane feeds to the A (q) and B (r) operands of each block;
-  // rows past l read row l - 1 (valid memory) and are masked at the end
-  const half8 *aq[2], *br[2];
+  // chunk loads: thread t moves pieces p = t + 256 j (j < 4) of the q rows
+  // and of the r rows: row p / 8, bytes 16 (p % 8) of the chunk's 128
+  const unsigned char *gx = reinterpret_cast<const unsigned char *>(xs);
+  long long src_q[4], src_r[4];
+  int dst[4], col[4];
 #pragma unroll
-  for (int b = 0; b < 2; ++b) {
-    const int q = min(q0 + b * 32 + lane % 32, l - 1), r = min(r0 + b * 32 + lane % 32, l - 1);
-    aq[b] = reinterpret_cast<const half8 *>(xs + (long long)q * rh) + 2 * (lane / 32);
-    br[b] = reinterpret_cast<const half8 *>(xs + (long long)r * rh) + 2 * (lane / 32);
+  for (int j = 0; j < 4; ++j) {
+    const int pc = (int)threadIdx.x + 256 * j, row = pc / 8;
+    col[j] = pc % 8;
+    src_q[j] = (long long)min(tq * GT + row, l - 1) * rh * 2 + col[j] * 16;
+    src_r[j] = (long long)min(tr * GT + row, l - 1) * rh * 2 + col[j] * 16;
+    dst[j] = row * GROW + col[j] * 16;
   }
+  typedef unsigned int u32x4 __attribute__((ext_vector_type(4)));
+  const int nchunks = (V.d + GKC - 1) / GKC;
+  // one chunk's pieces of this thread into registers (groups past the
+  // row's padded features read as zeros)
+#define GRAM_FETCH(c)                                                                          \
+  _Pragma("unroll") for (int j = 0; j < 4; ++j) {                                              \
+    const bool in = (c) * 4 + col[j] / 2 < ngroups;                                            \
+    const u32x4 z = {0u, 0u, 0u, 0u};                                                          \
+    pq[j] = in ? *reinterpret_cast<const u32x4 *>(gx + src_q[j] + (long long)(c) * 128) : z;   \
+    pr[j] = in ? *reinterpret_cast<const u32x4 *>(gx + src_r[j] + (long long)(c) * 128) : z;   \
+  }
+#define GRAM_STASH(b)                                                                          \
+  _Pragma("unroll") for (int j = 0; j < 4; ++j) {                                              \
+    *reinterpret_cast<u32x4 *>(lds + (b) * GBUF + dst[j]) = pq[j];                             \
+    *reinterpret_cast<u32x4 *>(lds + (b) * GBUF + GT * GROW + dst[j]) = pr[j];                 \
+  }
+  u32x4 pq[4], pr[4];
   float16v acc[2][2];
 #pragma unroll
   for (int i = 0; i < 2; ++i)
@@ -129,58 +180,91 @@ __global__ void __launch_bounds__(GBLOCK) k_gram_f16(View V, const _Float16 *xs,
     for (int j = 0; j < 2; ++j)
 #pragma unroll
       for (int e = 0; e < 16; ++e) acc[i][j][e] = 0.0f;
-  const int steps = (V.d + 15) / 16;
-  for (int s = 0; s < steps; ++s) {
-    // lane holds features 16s + 8 (lane / 32) .. + 7 of its row: group
-    // 2s + lane / 32, i.e. half8 index 4s + 2 (lane / 32) (hi) and + 1 (lo);
-    // the four blocks' products of one term go back to back, so no two
-    // consecutive MFMAs accumulate into the same registers
-    half8 ah[2], al[2], bh[2], bl[2];
-#pragma unroll
-    for (int b = 0; b < 2; ++b) {
-      ah[b] = aq[b][4 * s];
-      al[b] = aq[b][4 * s + 1];
-      bh[b] = br[b][4 * s];
-      bl[b] = br[b][4 * s + 1];
+  GRAM_FETCH(0)
+  GRAM_STASH(0)
+  __syncthreads();
+  for (int c = 0; c < nchunks; ++c) {
+    if (c + 1 < nchunks) {  // in flight during this chunk's MFMAs
+      GRAM_FETCH(c + 1)
     }
+    const unsigned char *A = lds + (c & 1) * GBUF, *B = A + GT * GROW;
 #pragma unroll
-    for (int i = 0; i < 2; ++i)
+    for (int ss = 0; ss < GKC / 16; ++ss) {
+      // lane: row lane % 32 of each 32-row block, features 16 ss + 8 (lane / 32) .. + 7
+      // of the chunk (group 2 ss + lane / 32: hi at +0, lo at +16 bytes);
+      // the four blocks' products of one term go back to back, so no two
+      // consecutive MFMAs accumulate into the same registers
+      const int off = (2 * ss + lane / 32) * 32;
+      half8 ah[2], al[2], bh[2], bl[2];
 #pragma unroll
-      for (int j = 0; j < 2; ++j) acc[i][j] = __builtin_amdgcn_mfma_f32_32x32x16_f16(ah[i], bh[j], acc[i][j], 0, 0, 0);
+      for (int b = 0; b < 2; ++b) {
+        const unsigned char *ra = A + ((wave / 2) * 64 + b * 32 + lane % 32) * GROW + off;
+        const unsigned char *rb = B + ((wave % 2) * 64 + b * 32 + lane % 32) * GROW + off;
+        ah[b] = *reinterpret_cast<const half8 *>(ra);
+        al[b] = *reinterpret_cast<const half8 *>(ra + 16);
+        bh[b] = *reinterpret_cast<const half8 *>(rb);
+        bl[b] = *reinterpret_cast<const half8 *>(rb + 16);
+      }
 #pragma unroll
-    for (int i = 0; i < 2; ++i)
+      for (int i = 0; i < 2; ++i)
 #pragma unroll
-      for (int j = 0; j < 2; ++j) acc[i][j] = __builtin_amdgcn_mfma_f32_32x32x16_f16(ah[i], bl[j], acc[i][j], 0, 0, 0);
+        for (int j = 0; j < 2; ++j)
+          acc[i][j] = __builtin_amdgcn_mfma_f32_32x32x16_f16(ah[i], bh[j], acc[i][j], 0, 0, 0);
 #pragma unroll
-    for (int i = 0; i < 2; ++i)
+      for (int i = 0; i < 2; ++i)
 #pragma unroll
-      for (int j = 0; j < 2; ++j) acc[i][j] = __builtin_amdgcn_mfma_f32_32x32x16_f16(al[i], bh[j], acc[i][j], 0, 0, 0);
+        for (int j = 0; j < 2; ++j)
+          acc[i][j] = __builtin_amdgcn_mfma_f32_32x32x16_f16(ah[i], bl[j], acc[i][j], 0, 0, 0);
+#pragma unroll
+      for (int i = 0; i < 2; ++i)
+#pragma unroll
+        for (int j = 0; j < 2; ++j)
+          acc[i][j] = __builtin_amdgcn_mfma_f32_32x32x16_f16(al[i], bh[j], acc[i][j], 0, 0, 0);
+    }
+    if (c + 1 < nchunks) {
+      GRAM_STASH((c + 1) & 1)
+    }
+    __syncthreads();
   }
+#undef GRAM_FETCH
+#undef GRAM_STASH
   // D[m][n] of block (i, j) in lane element e: m = 8 (e / 4) + 4 (lane / 32)
   // + e % 4, n = lane % 32.  Rows q: K(q, r), lanes on consecutive r; the
-  // values also go to LDS for the transposed store.
+  // values also go to LDS (over the operand buffers, after the last
+  // barrier) for the transposed store.
+  float(*tp)[64][65] = reinterpret_cast<float(*)[64][65]>(lds);
   const bool mirror = tq != tr;
+  // a wave's block lies inside the matrix unless it touches row or column
+  // l (only the last tile row / column): then every store is checked
+  const bool inside = q0 + 64 <= l && r0 + 64 <= l;
+  const long long stride = V.row_stride;
+  const int qa = q0 + 4 * (lane / 32), rb = r0 + lane % 32;  // this lane's first row and column
+  float *dst0 = V.rows + (long long)qa * stride + rb;
+  float *tp0 = &tp[wave][4 * (lane / 32)][lane % 32];
+  const double *sqq = &sq[(wave / 2) * 64 + 4 * (lane / 32)];
 #pragma unroll
   for (int j = 0; j < 2; ++j) {
-    const int n = j * 32 + lane % 32, r = r0 + n;
-    const double xr = V.xsq[min(r, l - 1)];
+    const double xr = KT == K_RBF ? sq[GT + (wave % 2) * 64 + j * 32 + lane % 32] : 0.0;
 #pragma unroll
     for (int i = 0; i < 2; ++i)
 #pragma unroll
       for (int e = 0; e < 16; ++e) {
-        const int m = i * 32 + 8 * (e / 4) + 4 * (lane / 32) + e % 4, q = q0 + m;
-        const float K = gram_value(V, V.xsq[min(q, l - 1)], xr, acc[i][j][e]);
-        if (q < l && r < l) V.rows[(long long)q * V.row_stride + r] = K;
-        tp[wave][m][n] = K;
+        // row m = 4 (lane / 32) + mo of the wave's block, column n = 32 j + lane % 32
+        const int mo = i * 32 + 8 * (e / 4) + e % 4;
+        const float K = gram_value_t<KT>(V, KT == K_RBF ? sqq[mo] : 0.0, xr, acc[i][j][e]);
+        if (inside || (qa + mo < l && rb + 32 * j < l)) dst0[(long long)mo * stride + 32 * j] = K;
+        tp0[mo * 65 + 32 * j] = K;
       }
   }
   if (!mirror) return;  // uniform over the workgroup
   __syncthreads();
   // rows r: K(r, q) = K(q, r), lanes on consecutive q
   const int q = q0 + lane;
+  float *mp = V.rows + (long long)r0 * stride + q;
+#pragma unroll 8
   for (int n = 0; n < 64; ++n) {
-    const int r = r0 + n;
-    if (r < l && q < l) V.rows[(long long)r * V.row_stride + q] = tp[wave][lane][n];
+    if (inside || (r0 + n < l && q < l)) *mp = tp[wave][lane][n];
+    mp += stride;
   }
 }
 
@@ -348,7 +432,22 @@ hipError_t smo_launch_gram_f16(const smo::View *V, const _Float16 *xs, hipStream
   if (ntiles <= 0) return hipSuccess;
   const long long grid = (ntiles + 7) / 8 * 8;  // whole rounds of the XCD order
   if (grid > 0x7fffffffLL) return hipErrorInvalidValue;
-  hipLaunchKernelGGL(smo::k_gram_f16, dim3((unsigned)grid), dim3(smo::GBLOCK), 0, st, *V, xs, ntiles, nt);
+  switch (V->ktype) {
+    case smo::K_LINEAR:
+      hipLaunchKernelGGL(smo::k_gram_f16<smo::K_LINEAR>, dim3((unsigned)grid), dim3(smo::GBLOCK), 0, st, *V, xs, ntiles, nt);
+      break;
+    case smo::K_POLY:
+      hipLaunchKernelGGL(smo::k_gram_f16<smo::K_POLY>, dim3((unsigned)grid), dim3(smo::GBLOCK), 0, st, *V, xs, ntiles, nt);
+      break;
+    case smo::K_RBF:
+      hipLaunchKernelGGL(smo::k_gram_f16<smo::K_RBF>, dim3((unsigned)grid), dim3(smo::GBLOCK), 0, st, *V, xs, ntiles, nt);
+      break;
+    case smo::K_SIGMOID:
+      hipLaunchKernelGGL(smo::k_gram_f16<smo::K_SIGMOID>, dim3((unsigned)grid), dim3(smo::GBLOCK), 0, st, *V, xs, ntiles, nt);
+      break;
+    default:
+      return hipErrorInvalidValue;
+  }
   return hipGetLastError();
 }
 
