@@ -303,11 +303,11 @@ struct HostSell {
 void make_host_sell(svmhip_solver *h, const svm_problem *prob, int ktype, HostSell &out);
 
 // Dynamic LDS of a batched launch whose rows have up to maxd features: room
-// for 16 staged query rows (G initialisation), at least one row, at most
-// 64 KiB.
+// for 8 staged query rows (G initialisation, speculative rows), at least one
+// row, at most 64 KiB.
 inline size_t batch_lds_bytes(int maxd) {
   const size_t one = (size_t)std::max(maxd, 1) * sizeof(float);
-  return std::max(one, std::min<size_t>(16 * one, 64 * 1024));
+  return std::max(one, std::min<size_t>(8 * one, 64 * 1024));
 }
 
 // Words of the LDS row-flag bitset a batched launch adds past

@@ -984,12 +984,16 @@ __device__ __forceinline__ void ginit_pass(const View &V, float *xs, int ld, con
   }
 }
 
+// (8 staged rows in the batched kernel, whose other paths leave fewer
+// registers: 16 there spills 296 bytes per lane across the whole kernel)
+constexpr int BATCH_GINIT_M = 8;
+
 __device__ __forceinline__ void batch_ginit(const View &V, float *xs, int lds_floats, int r0, int r1, int from,
                                             int to) {
-  __shared__ int list[GINIT_M];
+  __shared__ int list[BATCH_GINIT_M];
   __shared__ int s_cnt, s_next;
   const int fit = lds_floats / max(V.d, 1);
-  const int M = fit >= GINIT_M ? GINIT_M : (fit >= 4 ? 4 : 1);
+  const int M = fit >= BATCH_GINIT_M ? BATCH_GINIT_M : (fit >= 4 ? 4 : 1);
   int v = from;
   for (;;) {
     if (threadIdx.x == 0) {
@@ -1004,7 +1008,7 @@ __device__ __forceinline__ void batch_ginit(const View &V, float *xs, int lds_fl
     v = s_next;
     if (cnt == 0) break;
     for (int m = 0; m < cnt; ++m) stage_row(V, real_of(V, list[m]), xs + m * V.d);
-    if (M == GINIT_M) ginit_pass<GINIT_M>(V, xs, V.d, list, cnt, r0, r1);
+    if (M == BATCH_GINIT_M) ginit_pass<BATCH_GINIT_M>(V, xs, V.d, list, cnt, r0, r1);
     else if (M >= 4) ginit_pass<4>(V, xs, V.d, list, cnt, r0, r1);
     else ginit_pass<1>(V, xs, V.d, list, cnt, r0, r1);
     __syncthreads();
