@@ -230,7 +230,10 @@ int svmhip_batch_add_rowset(svmhip_batch *b, const int *rows, int n);
  * state (the iterates are those of an uninterrupted solve); unfinished
  * subproblems are re-planned into teams as the set shrinks.  stats (may be
  * null) sums iterations and cache counters; solve_ms is the device time of
- * all stages, launches the number of stages. */
+ * all stages, launches the number of stages.  The device buffers of a solve
+ * (the kernel-row cache arenas, by default 70% of the free device memory,
+ * and the pooled solver state) stay with the handle for the next solve and
+ * are released by svmhip_batch_destroy. */
 svmhip_status svmhip_batch_solve(svmhip_batch *b, svmhip_subproblem *subs, int nsub, svmhip_stats *stats);
 
 void svmhip_batch_destroy(svmhip_batch *b);

@@ -47,6 +47,7 @@ hipError_t smo_launch_predict_gram(const smo::PredictArgs *A, const _Float16 *xt
                                    const double *sqt, const double *sqs, double *partial, hipStream_t st);
 long long smo_predict_split_halves(int d);
 int smo_predict_gram_tiles(int ns);
+hipError_t smo_launch_dots_f64(const smo::View *V, double *D, long long ld, hipStream_t st);
 hipError_t smo_launch_pack_rowset(const smo::View *src, const int *rows, int n, const long long *dst_slice_ptr,
                                   float *dst_vals, int *dst_cols, int *dst_len, double *dst_xsq, hipStream_t st);
 }

@@ -120,6 +120,10 @@ struct View {
                           // statuses, and the owned segments of the iteration's kernel rows, live in
                           // dynamic LDS from this float offset during the launch; 0 = device memory
   int res_m;              // rows each resident array holds per variable half (>= the owned rows)
+  const double *gdot;     // batched solves: the exact FP64 dot products of every pair of the base
+                          // problem's rows (row-major, gdot_ld per row; null = compute from X)
+  long long gdot_ld;
+  const int *grow;        // batched solves with gdot: the base-problem row of each local row
 };
 
 // Work queue and kernel-row cache arenas of the batched solver.  Item q of
@@ -517,9 +521,26 @@ __device__ __forceinline__ float kernel_value(const View &V, const float *xs, in
   return (float)kernel_f64(V, q, r, row_dot<PIPE>(V, xs, r));
 }
 
+// dot(x_q, x_r) from the shared dot products (batched solves over one base
+// problem): the same FP64 value row_dot sums.
+__device__ __forceinline__ double shared_dot(const View &V, int q, int r) {
+  return V.gdot[(long long)V.grow[q] * V.gdot_ld + V.grow[r]];
+}
+
 // Materialise the owned part of kernel row q into cache slot `slot`.
 template <bool PIPE = false>
 __device__ __forceinline__ void compute_row_part(const View &V, int q, int slot, float *xs, int sb, int se) {
+  if (V.gdot) {  // a gather of one row of the shared dot products
+    float *dst = V.rows + (long long)slot * V.row_stride;
+    const double *dq = V.gdot + (long long)V.grow[q] * V.gdot_ld;
+    const int wave = threadIdx.x / 64, lane = threadIdx.x % 64, nw = blockDim.x / 64;
+    for (int s = sb + wave; s < se; s += nw) {
+      const int r = s * SLICE + lane;
+      if (r < V.l) dst[r] = (float)kernel_f64(V, q, r, dq[V.grow[r]]);
+    }
+    __syncthreads();
+    return;
+  }
   stage_row(V, q, xs);
   float *dst = V.rows + (long long)slot * V.row_stride;
   const int wave = threadIdx.x / 64, lane = threadIdx.x % 64, nw = blockDim.x / 64;

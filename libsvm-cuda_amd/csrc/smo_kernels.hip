@@ -205,6 +205,10 @@ constexpr int SPEC_UP = 3, SPEC_LOW = 4;
 // Owned parts of the rows in list[0..cnt) (full cache: slot = row).
 __device__ __forceinline__ void compute_rows_part(const View &V, const int *list, int cnt, float *xs, int sb,
                                                   int se) {
+  if (V.gdot) {  // gathers of the shared dot products
+    for (int m = 0; m < cnt; ++m) compute_row_part(V, list[m], list[m], xs, sb, se);
+    return;
+  }
   for (int m = 0; m < cnt; ++m) stage_row(V, list[m], xs + m * V.d);
   const int wave = threadIdx.x / 64, lane = threadIdx.x % 64, nw = blockDim.x / 64;
   for (int sl = sb + wave; sl < se; sl += nw) {
@@ -966,7 +970,12 @@ __device__ __forceinline__ void ginit_pass(const View &V, float *xs, int ld, con
   for (int r = r0 + threadIdx.x; r < r1; r += blockDim.x) {
     // row r of a slice is lane r % 64 of wave (r % 512) / 64 in the SELL layout
     double acc[M];
-    row_dot_multi<M>(V, xs, ld, r, acc);
+    if (V.gdot) {
+#pragma unroll
+      for (int m = 0; m < M; ++m) acc[m] = m < cnt ? shared_dot(V, real_of(V, list[m]), r) : 0.0;
+    } else {
+      row_dot_multi<M>(V, xs, ld, r, acc);
+    }
 #pragma unroll
     for (int m = 0; m < M; ++m) {  // static indices keep acc in registers
       if (m >= cnt) continue;
@@ -1007,7 +1016,8 @@ __device__ __forceinline__ void batch_ginit(const View &V, float *xs, int lds_fl
     const int cnt = s_cnt;
     v = s_next;
     if (cnt == 0) break;
-    for (int m = 0; m < cnt; ++m) stage_row(V, real_of(V, list[m]), xs + m * V.d);
+    if (!V.gdot)
+      for (int m = 0; m < cnt; ++m) stage_row(V, real_of(V, list[m]), xs + m * V.d);
     if (M == BATCH_GINIT_M) ginit_pass<BATCH_GINIT_M>(V, xs, V.d, list, cnt, r0, r1);
     else if (M >= 4) ginit_pass<4>(V, xs, V.d, list, cnt, r0, r1);
     else ginit_pass<1>(V, xs, V.d, list, cnt, r0, r1);
@@ -1416,7 +1426,8 @@ __global__ void __launch_bounds__(BLOCK) k_smo_batch(const View *views, const in
     } else {
       V.nblocks = 1;
       V.rank = 0;
-      V.spec = pool.spec && V.kind == KIND_C && V.full && V.ktype != K_PRECOMPUTED &&
+      // (no speculative rows over the shared dot products: a row is a gather)
+      V.spec = pool.spec && V.kind == KIND_C && V.full && V.ktype != K_PRECOMPUTED && !V.gdot &&
                pool.lds_floats >= 2 * V.d ? 1 : 0;
       V.flag_lds = V.full && pool.flag_words > 0 && (V.l + 31) / 32 <= pool.flag_words ? pool.lds_floats : 0;
       if (V.kind == KIND_C) solo_solve<KIND_C>(V, d, sm, xs, pool.lds_floats, t0);
@@ -1442,6 +1453,76 @@ __global__ void __launch_bounds__(BLOCK) k_pack_rowset(View src, const int *rows
   }
   dst_len[r] = len;
   dst_xsq[r] = src.xsq[g];
+}
+
+// ------------------------------------------------------------ shared dots
+//
+// D[q][r] = dot(x_q, x_r) for every pair of rows of a dense problem, each
+// summed exactly as row_dot sums it (features in order, fused
+// multiply-adds of exact float-by-float products), so a kernel value from D
+// is bit-identical to one computed from X.  One 64 x 64 tile of the upper
+// triangle per workgroup (one SELL slice per side: a tile's rows are one
+// slice, whose feature-p values are 64 consecutive floats), written to both
+// halves of D.  Each thread sums a 4 x 4 block; the two slices' features
+// are staged in LDS as doubles, 32 at a time.
+constexpr int DT = 64, DKC = 32;
+
+__device__ __forceinline__ void dots_tile(long long t, int nt, int &tq, int &tr) {
+  const double a = 2.0 * nt + 1.0;
+  int q = (int)((a - sqrt(a * a - 8.0 * (double)t)) / 2.0);
+  q = max(0, min(q, nt - 1));
+  while (q > 0 && (long long)q * (2LL * nt - q + 1) / 2 > t) --q;
+  while (q + 1 < nt && (long long)(q + 1) * (2LL * nt - q) / 2 <= t) ++q;
+  tq = q;
+  tr = q + (int)(t - (long long)q * (2LL * nt - q + 1) / 2);
+}
+
+__global__ void __launch_bounds__(256) k_dots_f64(View V, double *D, long long ld, long long ntiles, int nt) {
+  __shared__ double A[DKC][DT + 1], B[DKC][DT + 1];
+  const long long tile = blockIdx.x;
+  if (tile >= ntiles) return;
+  int tq, tr;
+  dots_tile(tile, nt, tq, tr);
+  const int tx = threadIdx.x % 16, ty = threadIdx.x / 16;
+  const float *va = V.vals + V.slice_ptr[tq], *vb = V.vals + V.slice_ptr[tr];
+  double acc[4][4];
+#pragma unroll
+  for (int i = 0; i < 4; ++i)
+#pragma unroll
+    for (int j = 0; j < 4; ++j) acc[i][j] = 0.0;
+  for (int p0 = 0; p0 < V.d; p0 += DKC) {
+    const int pe = min(DKC, V.d - p0);
+    for (int e = threadIdx.x; e < DT * DKC; e += 256) {
+      const int row = e % DT, p = e / DT;
+      A[p][row] = p < pe ? (double)va[(long long)(p0 + p) * SLICE + row] : 0.0;
+      B[p][row] = p < pe ? (double)vb[(long long)(p0 + p) * SLICE + row] : 0.0;
+    }
+    __syncthreads();
+    for (int p = 0; p < pe; ++p) {  // features in order
+      double a[4], b[4];
+#pragma unroll
+      for (int i = 0; i < 4; ++i) {
+        a[i] = A[p][ty * 4 + i];
+        b[i] = B[p][tx * 4 + i];
+      }
+#pragma unroll
+      for (int i = 0; i < 4; ++i)
+#pragma unroll
+        for (int j = 0; j < 4; ++j) acc[i][j] = fma(a[i], b[j], acc[i][j]);
+    }
+    __syncthreads();
+  }
+  const int q0 = tq * DT + ty * 4, r0 = tr * DT + tx * 4;
+#pragma unroll
+  for (int i = 0; i < 4; ++i)
+#pragma unroll
+    for (int j = 0; j < 4; ++j) {
+      const int q = q0 + i, r = r0 + j;
+      if (q < V.l && r < V.l) {
+        D[(long long)q * ld + r] = acc[i][j];
+        if (tq != tr) D[(long long)r * ld + q] = acc[i][j];
+      }
+    }
 }
 
 // ------------------------------------------------------------ prediction
@@ -1602,6 +1683,17 @@ hipError_t smo_launch_batch(const View *views, const int *items, int nitems, int
 hipError_t smo_batch_capacity(size_t shmem, int *blocks_per_cu) {
   return hipOccupancyMaxActiveBlocksPerMultiprocessor(blocks_per_cu, (const void *)smo::k_smo_batch,
                                                       smo::BLOCK, shmem);
+}
+
+// The exact dot products of every pair of the problem's rows (dense rows
+// only) into D (l rows of ld doubles).
+hipError_t smo_launch_dots_f64(const smo::View *V, double *D, long long ld, hipStream_t st) {
+  if (!V->dense || V->l <= 0) return hipErrorInvalidValue;
+  const int nt = (V->l + smo::DT - 1) / smo::DT;
+  const long long ntiles = (long long)nt * (nt + 1) / 2;
+  if (ntiles > 0x7fffffffLL) return hipErrorInvalidValue;
+  hipLaunchKernelGGL(smo::k_dots_f64, dim3((unsigned)ntiles), dim3(256), 0, st, *V, D, ld, ntiles, nt);
+  return hipGetLastError();
 }
 
 hipError_t smo_launch_pack_rowset(const View *src, const int *rows, int n, const long long *dst_slice_ptr,

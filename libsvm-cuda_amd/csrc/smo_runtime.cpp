@@ -717,10 +717,28 @@ struct svmhip_batch {
     float *vals;
     int *cols, *len;
     double *xsq;
+    int *grow;               // the base problem's row of each row of the set
   };
   std::vector<Rowset> rowsets;
   double param_cache_mb = 100;
   std::string err;
+  // exact dot products of every pair of base rows (dense problems, when
+  // they fit): a kernel row of any subproblem is then a gather of one row of
+  // them instead of a pass over the subproblem's X
+  double *dots = nullptr;
+  long long dots_ld = 0;
+  int dots_state = 0;        // 0 = not decided, 1 = built, -1 = not used
+  float dots_ms = 0;
+  // device buffers of the previous solve, kept for the next one (a batch is
+  // usually solved again at the same sizes, and allocating and freeing the
+  // row-cache arenas, most of the device memory, costs more than a solve);
+  // they are released with the handle
+  struct Kept {
+    void *p = nullptr;
+    size_t cap = 0;
+  };
+  enum { K_G, K_ALPHA, K_P, K_QD, K_Y, K_STATUS, K_CTL, K_VIEWS, K_ROWS, K_FLAGS, K_LRU, K_WORDS, K_X, K_ITEMS, K_N };
+  Kept kept[K_N];
 };
 
 extern "C" {
@@ -793,7 +811,7 @@ int svmhip_batch_add_rowset(svmhip_batch *b, const int *rows, int n) {
     smo::View V = h->view();
     h->ck(smo_launch_pack_rowset(&V, d_rows, n, rs.slice_ptr, rs.vals, rs.cols, rs.len, rs.xsq, h->st), "pack row set");
     h->ck(hipStreamSynchronize(h->st), "pack row set");
-    h->dfree(d_rows);
+    rs.grow = d_rows;
     b->rowsets.push_back(rs);
     GUARD_END
   }();
@@ -805,7 +823,6 @@ svmhip_status svmhip_batch_solve(svmhip_batch *b, svmhip_subproblem *subs, int n
   if (!b || (nsub > 0 && !subs)) return SVMHIP_ERR_ARG;
   svmhip_solver *h = b->g;
   svmhip_stats tot{};
-  std::vector<void *> temps;
   svmhip_status rc = [&]() -> svmhip_status {
     GUARD_BEGIN
     h->ck(hipSetDevice(h->dev), "hipSetDevice");
@@ -825,20 +842,43 @@ svmhip_status svmhip_batch_solve(svmhip_batch *b, svmhip_subproblem *subs, int n
       maxn = std::max(maxn, n);
     }
     if (nsub == 0) return SVMHIP_OK;
+    // the shared dot products, once per handle: dense rows, not disabled by
+    // SVMHIP_SHARED_DOTS=0, and at most a quarter of the free device memory
+    if (b->dots_state == 0) {
+      b->dots_state = -1;
+      const char *e = getenv("SVMHIP_SHARED_DOTS");
+      const long long ld = ((long long)h->l + 7) / 8 * 8;
+      const double bytes = (double)h->l * (double)ld * sizeof(double);
+      size_t fb = 0, tb = 0;
+      h->ck(hipMemGetInfo(&fb, &tb), "hipMemGetInfo");
+      if (!(e && e[0] == '0') && h->dense && h->ktype != PRECOMPUTED && h->l >= 2 && bytes <= 0.25 * (double)fb) {
+        b->dots = h->dalloc<double>((size_t)h->l * (size_t)ld, "shared dot products");
+        b->dots_ld = ld;
+        smo::View V = h->view();
+        h->ck(hipEventRecord(h->ev[4], h->st), "event");
+        h->ck(smo_launch_dots_f64(&V, b->dots, ld, h->st), "shared dot products");
+        h->ck(hipEventRecord(h->ev[5], h->st), "event");
+        h->ck(hipEventSynchronize(h->ev[5]), "shared dot products");
+        (void)hipEventElapsedTime(&b->dots_ms, h->ev[4], h->ev[5]);
+        b->dots_state = 1;
+        if (h->opt.verbose)
+          fprintf(stderr, "[svmhip] batch: shared dot products of %d rows (%.1f GB) in %.2f ms\n", h->l, bytes / 1e9,
+                  b->dots_ms);
+      }
+    }
     // pooled per-subproblem state, resident across the stages of the solve
-    auto talloc = [&](size_t bytes, const char *what) {
-      void *p = h->dalloc<char>(bytes, what);
-      temps.push_back(p);
-      return p;
+    auto talloc = [&](int slot, size_t bytes, const char *what) {
+      svmhip_batch::Kept &k = b->kept[slot];
+      return (void *)h->ensure(*(char **)&k.p, k.cap, bytes, what);
     };
-    double *dG = (double *)talloc(sizeof(double) * nv_total, "batch G");
-    double *dA = (double *)talloc(sizeof(double) * nv_total, "batch alpha");
-    double *dP = (double *)talloc(sizeof(double) * nv_total, "batch p");
-    double *dQ = (double *)talloc(sizeof(double) * nv_total, "batch QD");
-    int8_t *dY = (int8_t *)talloc(nv_total, "batch y");
-    int8_t *dS = (int8_t *)talloc(nv_total, "batch status");
-    smo::Ctl *dC = (smo::Ctl *)talloc(sizeof(smo::Ctl) * nsub, "batch control");
-    smo::View *dV = (smo::View *)talloc(sizeof(smo::View) * nsub, "batch views");
+    double *dG = (double *)talloc(svmhip_batch::K_G, sizeof(double) * nv_total, "batch G");
+    double *dA = (double *)talloc(svmhip_batch::K_ALPHA, sizeof(double) * nv_total, "batch alpha");
+    double *dP = (double *)talloc(svmhip_batch::K_P, sizeof(double) * nv_total, "batch p");
+    double *dQ = (double *)talloc(svmhip_batch::K_QD, sizeof(double) * nv_total, "batch QD");
+    int8_t *dY = (int8_t *)talloc(svmhip_batch::K_Y, nv_total, "batch y");
+    int8_t *dS = (int8_t *)talloc(svmhip_batch::K_STATUS, nv_total, "batch status");
+    smo::Ctl *dC = (smo::Ctl *)talloc(svmhip_batch::K_CTL, sizeof(smo::Ctl) * nsub, "batch control");
+    smo::View *dV = (smo::View *)talloc(svmhip_batch::K_VIEWS, sizeof(smo::View) * nsub, "batch views");
     std::vector<double> hA(nv_total), hP(nv_total);
     std::vector<int8_t> hY(nv_total), hS(nv_total);
     std::vector<smo::View> views(nsub);
@@ -873,6 +913,11 @@ svmhip_status svmhip_batch_solve(svmhip_batch *b, svmhip_subproblem *subs, int n
       V.ctl = dC + k; V.part = nullptr; V.xw = nullptr; V.xs_global = nullptr;
       V.use_lds = 1; V.nblocks = 1; V.rank = 0; V.resume = 0; V.iter0 = 0; V.xrep = 0;
       V.ginit_from = 0; V.ginit_to = sp.n_var;
+      if (b->dots_state == 1) {
+        V.gdot = b->dots;
+        V.gdot_ld = b->dots_ld;
+        V.grow = rs.grow;
+      }
       views[k] = V;
       off += sp.n_var;
     }
@@ -895,12 +940,18 @@ svmhip_status svmhip_batch_solve(svmhip_batch *b, svmhip_subproblem *subs, int n
     long long maxstride = ((long long)maxn + 63) / 64 * 64;
     size_t freeb = 0, totb = 0;
     h->ck(hipMemGetInfo(&freeb, &totb), "hipMemGetInfo");
+    svmhip_batch::Kept &kept_rows = b->kept[svmhip_batch::K_ROWS];
+    freeb += kept_rows.cap;  // the previous solve's arenas count as free
     const double frac = h->opt.hbm_fraction > 0 ? std::min(h->opt.hbm_fraction, 0.95) : 0.7;
     const long long share = (long long)(frac * (double)freeb / sizeof(float)) / K;
     long long arena = h->opt.cache_bytes > 0 ? (long long)(h->opt.cache_bytes / sizeof(float)) : share;
     arena = std::max<long long>(std::min(arena, std::max<long long>(share, 2 * maxstride)), 2 * maxstride);
     // no subproblem needs more than its whole matrix
     arena = std::min<long long>(arena, std::max<long long>((long long)maxn * maxstride, 2 * maxstride));
+    // within 5% of the kept arenas: keep them rather than allocate anew
+    // (free memory moves a little between calls)
+    const long long kept_arena = (long long)(kept_rows.cap / sizeof(float)) / K;
+    if (kept_arena >= 2 * maxstride && kept_arena < arena && kept_arena >= arena - arena / 20) arena = kept_arena;
     if (h->opt.verbose)
       fprintf(stderr, "[svmhip] batch: %d kernel-row arenas of %.1f MB (%.1f GB of %.1f GB free, fraction %.2f)\n", K,
               arena * 4.0 / 1048576.0, arena * 4.0 * K / 1073741824.0, freeb / 1073741824.0, frac);
@@ -908,21 +959,25 @@ svmhip_status svmhip_batch_solve(svmhip_batch *b, svmhip_subproblem *subs, int n
     memset(&pool, 0, sizeof pool);
     pool.K = K;
     pool.arena_floats = arena;
-    pool.rows = (float *)talloc(sizeof(float) * (size_t)arena * K, "batch row-cache arenas");
+    pool.rows = (float *)talloc(svmhip_batch::K_ROWS, sizeof(float) * (size_t)arena * K, "batch row-cache arenas");
     pool.flag_stride = maxn;
-    pool.flags = (int8_t *)talloc((size_t)maxn * K, "batch cache flags");
+    pool.flags = (int8_t *)talloc(svmhip_batch::K_FLAGS, (size_t)maxn * K, "batch cache flags");
     pool.lru_stride = 4LL * maxn + 2;
-    pool.lru = (int *)talloc(sizeof(int) * (size_t)pool.lru_stride * K, "batch LRU bookkeeping");
-    pool.queue = (int *)talloc(sizeof(int), "batch queue");
-    pool.arena_next = (int *)talloc(sizeof(int), "batch arena counter");
-    pool.t0 = (unsigned long long *)talloc(sizeof(unsigned long long), "batch launch start");
-    unsigned *dIdle = (unsigned *)talloc(sizeof(unsigned), "batch idle count");
+    pool.lru = (int *)talloc(svmhip_batch::K_LRU, sizeof(int) * (size_t)pool.lru_stride * K, "batch LRU bookkeeping");
+    // queue, arena counter, launch start and idle count (each cleared before
+    // every launch)
+    char *words = (char *)talloc(svmhip_batch::K_WORDS, 32, "batch counters");
+    pool.queue = (int *)words;
+    pool.arena_next = (int *)(words + 4);
+    pool.t0 = (unsigned long long *)(words + 8);
+    unsigned *dIdle = (unsigned *)(words + 16);
     pool.lds_floats = (int)(lds0 / sizeof(float));
     pool.flag_words = flag_words;
     pool.spec = h->opt.speculative_rows >= 0 ? 1 : 0;
-    smo::XWord *dX = (smo::XWord *)talloc(sizeof(smo::XWord) * smo::xrec_words(G), "team exchange records");
+    smo::XWord *dX =
+        (smo::XWord *)talloc(svmhip_batch::K_X, sizeof(smo::XWord) * smo::xrec_words(G), "team exchange records");
     // items: one per subproblem plus the extra members of teams (at most G)
-    int *dItems = (int *)talloc(sizeof(int) * 2 * ((size_t)nsub + (size_t)G), "batch items");
+    int *dItems = (int *)talloc(svmhip_batch::K_ITEMS, sizeof(int) * 2 * ((size_t)nsub + (size_t)G), "batch items");
 
     // Stages.  Every unfinished subproblem runs until it converges or, after
     // about tau of device time, pauses at an iteration boundary (team members
@@ -1108,7 +1163,6 @@ svmhip_status svmhip_batch_solve(svmhip_batch *b, svmhip_subproblem *subs, int n
     tot.grid_blocks = G;
     GUARD_END
   }();
-  for (void *p : temps) h->dfree(p);
   if (stats) *stats = tot;
   return rc;
 }
