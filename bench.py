@@ -211,21 +211,26 @@ def bench_ovo_cv_grid(args, leg=False):
     b = libsvm_hip.BatchSolver(prob, prm, cache_bytes=args.batch_cache_mb * 2 ** 20)
     t0 = time.perf_counter()
     rids = [b.add_rowset(rows) for (_, _, _, rows, _) in jobs]
-    log(f"[bench] {len(jobs)} row sets packed in {time.perf_counter() - t0:.1f}s (n={prob.l})")
+    pack_s = time.perf_counter() - t0
+    log(f"[bench] {len(jobs)} row sets packed in {pack_s:.1f}s (n={prob.l})")
     subs = []
     for rid, (_, _, _, rows, yy) in zip(rids, jobs):
         for g in gammas:
             n = len(rows)
             subs.append(dict(rowset=rid, kind=libsvm_hip.KIND_C, y=yy, p=-np.ones(n), alpha0=np.zeros(n),
                              Cp=C, Cn=C, gamma=g, eps=1e-3))
+    first = {}  # the handle's first solve also builds the shared dot products
     for w in range(warmup):
+        t = time.perf_counter()
         _, st = b.solve(subs)
+        first = first or dict(wall_s=time.perf_counter() - t, shared_dots_ms=st["gram_ms"])
         log(f"[bench] warmup {w}: {st['iters']} iters, device {st['solve_ms']:.0f} ms")
     walls, stats = [], []
     for k in range(steps):
         t = time.perf_counter()
         res, st = b.solve(subs)
         walls.append(time.perf_counter() - t)
+        first = first or dict(wall_s=walls[-1], shared_dots_ms=st["gram_ms"])
         stats.append(st)
         log(f"[bench] step {k}: {len(subs)} solves, {st['iters']} iters, {walls[-1]:.2f}s wall, "
             f"device {st['solve_ms']:.0f} ms in {st['launches']} launches, rows computed {st['row_misses']}")
@@ -261,6 +266,9 @@ def bench_ovo_cv_grid(args, leg=False):
                    "solves_per_s": len(subs) * steps / wall, "train_time_s": wall / steps,
                    "iters_per_step": last["iters"], "rows_computed_per_step": last["row_misses"],
                    "stages_per_step": last["launches"],
+                   "handle_setup": {"rowsets_packed_s": pack_s, "shared_dot_products": bool(last["shared_dots"]),
+                                    "shared_dots_ms": first.get("shared_dots_ms"),
+                                    "first_solve_wall_s": first.get("wall_s")},
                    "plain_path_sample": (f"fold 0, {len(sample)} solves one at a time: {p_iters} iters "
                                          f"in {p_time:.2f}s") if p_time else None,
                    "plain_path_iters_per_s": (p_iters / p_time) if p_time else None,

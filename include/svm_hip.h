@@ -104,13 +104,19 @@ typedef struct {
 	int cache_slots;          /* rows the cache holds */
 	int grid_blocks;          /* workgroups used by the persistent kernel */
 	double gram_ms;           /* device time of the Gram-tile kernel matrix (mfma_rows = 1; 0 if the
-				     solve computed rows on demand) */
+				     solve computed rows on demand); svmhip_batch_solve: device time of
+				     the shared dot products on the call that built them, else 0 */
 	long long device_allocs;  /* device allocations made by the handle since svmhip_create (a
 				     repeated svmhip_setup + svmhip_solve over the same rows adds none
 				     once the first solve has sized the buffers) */
 	int resident_vectors;     /* 1 = the persistent kernel kept each workgroup's owned G, alpha, QD,
 				     labels and statuses in LDS for the solve (they fit); 0 = in
 				     device memory.  Results are identical either way */
+	int shared_dots;          /* svmhip_batch_solve: 1 = kernel rows were gathers from the handle's
+				     exact dot products of all pairs of base rows (dense problems whose
+				     l x l FP64 matrix takes at most a quarter of the free device
+				     memory; SVMHIP_SHARED_DOTS=0 turns it off).  Results are
+				     identical either way */
 } svmhip_stats;
 
 /* CudaSolver(prob, param, quiet) / CudaSolverNU(...)

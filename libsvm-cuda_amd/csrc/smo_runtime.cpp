@@ -861,6 +861,7 @@ svmhip_status svmhip_batch_solve(svmhip_batch *b, svmhip_subproblem *subs, int n
         h->ck(hipEventSynchronize(h->ev[5]), "shared dot products");
         (void)hipEventElapsedTime(&b->dots_ms, h->ev[4], h->ev[5]);
         b->dots_state = 1;
+        tot.gram_ms = b->dots_ms;
         if (h->opt.verbose)
           fprintf(stderr, "[svmhip] batch: shared dot products of %d rows (%.1f GB) in %.2f ms\n", h->l, bytes / 1e9,
                   b->dots_ms);
@@ -1161,6 +1162,8 @@ svmhip_status svmhip_batch_solve(svmhip_batch *b, svmhip_subproblem *subs, int n
     }
     tot.cache_slots = (int)std::min<long long>(INT_MAX, arena / maxstride);
     tot.grid_blocks = G;
+    tot.shared_dots = b->dots_state == 1 ? 1 : 0;
+    tot.device_allocs = h->n_allocs;
     GUARD_END
   }();
   if (stats) *stats = tot;

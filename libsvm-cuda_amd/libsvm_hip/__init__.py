@@ -42,7 +42,7 @@ class svmhip_stats(C.Structure):
                 ("solve_ms", C.c_double), ("kernel_ms", C.c_double), ("launches", C.c_int),
                 ("setup_ms", C.c_double), ("cache_slots", C.c_int),
                 ("grid_blocks", C.c_int), ("gram_ms", C.c_double), ("device_allocs", C.c_longlong),
-                ("resident_vectors", C.c_int)]
+                ("resident_vectors", C.c_int), ("shared_dots", C.c_int)]
 
     def as_dict(self):
         return {k: getattr(self, k) for k, _ in self._fields_}

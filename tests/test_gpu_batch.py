@@ -427,3 +427,63 @@ def test_batch_hbm_fraction_caps_arenas_same_results(hip):
         np.testing.assert_array_equal(a["alpha_raw"], b["alpha_raw"])
     b_def.close()
     b_small.close()
+
+
+def test_batch_shared_dots_dense_match_oracle(hip, ora, monkeypatch):
+    """Dense problems: kernel rows gathered from the handle's exact dot
+    products of the base rows (shared_dots = 1) give the oracle's iterates
+    (C-SVC, nu-SVC and epsilon-SVR subsets, several gammas), bit-identical to
+    rows computed from the subsets' own rows (SVMHIP_SHARED_DOTS=0).  A
+    second solve on the same handle, with fewer and smaller subproblems,
+    allocates nothing and gives the results of a fresh handle."""
+    X, y = datasets.gmm_dense_matrix(n=3000, d=64, seed=21)
+    prob = ffi.Problem.from_dense(X.astype(np.float64), y, keep_zeros=True)
+    prm = ffi.make_param(gamma=1.0 / 64, C_=1.0, eps=1e-3, shrinking=0)
+    rng = np.random.default_rng(4)
+    specs = []
+    for k in range(9):
+        rows = np.sort(rng.choice(prob.l, size=int(rng.integers(300, 2400)), replace=False))
+        svm_type = [ffi.C_SVC, ffi.NU_SVC, ffi.EPSILON_SVR][k % 3]
+        gamma = [1.0 / 64, 0.05, 0.003][k % 3]
+        specs.append((rows, ffi.make_param(svm_type=svm_type, gamma=gamma, C_=[0.5, 2.0, 1.0][k % 3], nu=0.4,
+                                           p=0.1, eps=1e-3, shrinking=0)))
+
+    def subs_for(b, which):
+        out = []
+        for rows, sp in (specs[i] for i in which):
+            sub = prob.subset(rows)
+            yy, p, a0, Cp, Cn, kind = hip.formulate(sub, sp)
+            out.append((sub, sp, yy, dict(rowset=b.add_rowset(rows), kind=kind, y=yy, p=p, alpha0=a0, Cp=Cp,
+                                          Cn=Cn, gamma=sp.gamma, eps=1e-3)))
+        return out
+
+    b = hip.BatchSolver(prob, prm)
+    cases = subs_for(b, range(9))
+    res, st = b.solve([c[3] for c in cases])
+    assert st["shared_dots"] == 1
+    for (sub, sp, yy, _), r in zip(cases, res):
+        ref = ora.train_one(sub, sp)
+        print(f"  l={sub.l} type={sp.svm_type} iters {r['iters']} vs {ref['iters']}", flush=True)
+        assert r["status"] == 0
+        assert r["iters"] == ref["iters"]
+        coef, rho, obj = hip.finish_solution(sub.y, sp, r, yy)
+        np.testing.assert_allclose(coef, ref["alpha"], rtol=0, atol=1e-9)
+        assert abs(obj - ref["obj"]) <= 1e-9 * abs(ref["obj"])
+    # the same handle again: fewer, smaller subproblems over its row sets
+    # (adding a row set allocates; a solve that fits the kept buffers not)
+    small = sorted(range(9), key=lambda i: len(specs[i][0]))[:4]
+    res2, st2 = b.solve([cases[i][3] for i in small])
+    assert st2["device_allocs"] == st["device_allocs"]
+    monkeypatch.setenv("SVMHIP_SHARED_DOTS", "0")
+    b0 = hip.BatchSolver(prob, prm)
+    plain = subs_for(b0, range(9))
+    res0, st0 = b0.solve([c[3] for c in plain])
+    assert st0["shared_dots"] == 0
+    for a, c in zip(res, res0):
+        assert a["iters"] == c["iters"]
+        np.testing.assert_array_equal(a["alpha_raw"], c["alpha_raw"])
+    for i, r in zip(small, res2):
+        assert r["iters"] == res0[i]["iters"]
+        np.testing.assert_array_equal(r["alpha_raw"], res0[i]["alpha_raw"])
+    b.close()
+    b0.close()
