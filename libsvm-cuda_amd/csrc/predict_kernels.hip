@@ -4,7 +4,7 @@
 
 namespace smo {
 
-constexpr int BLOCK = 512;
+// (BLOCK: the solver's workgroup size, 512 threads)
 
 // Decision values of test rows against a model (svm_predict_values,
 // libsvm/svm.cpp:2569-2643).  A workgroup stages tb test rows densely in LDS

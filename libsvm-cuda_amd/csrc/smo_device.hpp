@@ -30,6 +30,14 @@
 
 namespace smo {
 
+// Threads of every workgroup that runs the solver's device functions (every
+// solver kernel is launched with BLOCK threads), and its waves.  Compile-time
+// constants: the block size builtin is read from the dispatch packet by a
+// memory load, which inside a loop costs a round trip and a wait on every
+// load in flight.
+constexpr int BLOCK = 512;
+constexpr int NWAVES = BLOCK / 64;
+
 constexpr int SLICE = 64;
 constexpr double TAU = 1e-12;           // libsvm/svm_defs.h:106, libsvm/svm.cpp:42
 constexpr double INF = __builtin_huge_val();
@@ -298,7 +306,7 @@ __device__ __forceinline__ void owned_slices(const View &V, int b, int nb, int &
 // Stage row q of X densely (by feature column, or by position for
 // PRECOMPUTED) into xs[0..d).  Whole workgroup; ends with a barrier.
 __device__ __forceinline__ void stage_row(const View &V, int q, float *xs) {
-  const int tid = threadIdx.x, nt = blockDim.x;
+  const int tid = threadIdx.x, nt = BLOCK;
   const int len = V.row_len[q];
   const long long base = V.slice_ptr[q / SLICE] + (q % SLICE);
   if (!V.dense) {
@@ -533,7 +541,7 @@ __device__ __forceinline__ void compute_row_part(const View &V, int q, int slot,
   if (V.gdot) {  // a gather of one row of the shared dot products
     float *dst = V.rows + (long long)slot * V.row_stride;
     const double *dq = V.gdot + (long long)V.grow[q] * V.gdot_ld;
-    const int wave = threadIdx.x / 64, lane = threadIdx.x % 64, nw = blockDim.x / 64;
+    const int wave = threadIdx.x / 64, lane = threadIdx.x % 64, nw = NWAVES;
     for (int s = sb + wave; s < se; s += nw) {
       const int r = s * SLICE + lane;
       if (r < V.l) dst[r] = (float)kernel_f64(V, q, r, dq[V.grow[r]]);
@@ -543,7 +551,7 @@ __device__ __forceinline__ void compute_row_part(const View &V, int q, int slot,
   }
   stage_row(V, q, xs);
   float *dst = V.rows + (long long)slot * V.row_stride;
-  const int wave = threadIdx.x / 64, lane = threadIdx.x % 64, nw = blockDim.x / 64;
+  const int wave = threadIdx.x / 64, lane = threadIdx.x % 64, nw = NWAVES;
   for (int s = sb + wave; s < se; s += nw) {
     int r = s * SLICE + lane;
     if (r < V.l) dst[r] = kernel_value<PIPE>(V, xs, q, r);
@@ -571,7 +579,7 @@ struct RedSmem {
 template <int N>
 __device__ __forceinline__ void block_reduce(RedSmem &sm, double (&v)[4], int (&ix)[2], const int (&kinds)[4],
                                              int buf = 0, int nact = 0) {
-  const int wave = threadIdx.x / 64, lane = threadIdx.x % 64, nw = blockDim.x / 64;
+  const int wave = threadIdx.x / 64, lane = threadIdx.x % 64, nw = NWAVES;
   const int na = nact > 0 && nact < nw ? nact : nw;
   if (wave < na) {
     int slot_i = 0;
@@ -832,7 +840,7 @@ __device__ __forceinline__ Partial pass_gradient_select_nv(const View &V, const 
   if (update && c.miss_j) compute_row_part<PIPE>(V, real_of(V, c.j), c.slot_j, xs, sb, se);
   double v[4] = {-INF, -INF, -INF, -INF};
   int ix[2] = {-1, -1};
-  const int wave = threadIdx.x / 64, lane = threadIdx.x % 64, nw = blockDim.x / 64;
+  const int wave = threadIdx.x / 64, lane = threadIdx.x % 64, nw = NWAVES;
   const int8_t yi = update ? V.y[c.i] : (int8_t)0, yj = update ? V.y[c.j] : (int8_t)0;
   // PS slices per trip, all loads issued before any use, so a wave
   // waits for one memory round trip per PS slices rather than per
@@ -945,7 +953,7 @@ __device__ __forceinline__ Partial pass_select_j_nv(const View &V, const Snap &c
   }
   double best = INF;
   int bj = -1;
-  const int wave = threadIdx.x / 64, lane = threadIdx.x % 64, nw = blockDim.x / 64;
+  const int wave = threadIdx.x / 64, lane = threadIdx.x % 64, nw = NWAVES;
   // rows and scalars of the candidates i (C rule) / i_p, i_n (nu rule)
   const int qi = KIND == KIND_C ? c.i : c.ip;
   const int qn = KIND == KIND_C ? -1 : c.in;
@@ -1103,7 +1111,7 @@ __device__ __forceinline__ void partial_combine(Partial &a, const Partial &b) {
 template <int KIND, int PH>
 __device__ Partial reduce_partials(const Partial *src, int first, int count, int stride, RedSmem &sm) {
   Partial a = partial_identity<KIND, PH>();
-  for (int t = threadIdx.x; t < count; t += blockDim.x) partial_combine<KIND, PH>(a, load_partial(&src[first + t * stride]));
+  for (int t = threadIdx.x; t < count; t += BLOCK) partial_combine<KIND, PH>(a, load_partial(&src[first + t * stride]));
   double v[4] = {a.v0, a.v1, a.v2, a.v3};
   int ix[2] = {a.i0, a.i1};
   if (PH == PH_J) {

@@ -27,10 +27,8 @@
 
 namespace smo {
 
-constexpr int BLOCK = 512;
-
 __global__ void __launch_bounds__(BLOCK) k_self_dot(View V, double *xsq) {
-  int r = blockIdx.x * blockDim.x + threadIdx.x;
+  int r = blockIdx.x * BLOCK + threadIdx.x;
   if (r >= V.l) return;
   const int len = V.row_len[r];
   const long long base = V.slice_ptr[r / SLICE] + (r % SLICE);
@@ -51,7 +49,7 @@ __global__ void __launch_bounds__(BLOCK) k_self_dot(View V, double *xsq) {
 }
 
 __global__ void __launch_bounds__(BLOCK) k_qd(View V) {
-  int v = blockIdx.x * blockDim.x + threadIdx.x;
+  int v = blockIdx.x * BLOCK + threadIdx.x;
   if (v >= V.nvar) return;
   int r = real_of(V, v);
   double q;
@@ -78,7 +76,7 @@ __global__ void __launch_bounds__(BLOCK) k_ginit_row(View V, int v) {
   stage_row(V, q, xs);
   const double av = V.alpha[v];
   const int8_t yv = V.y[v];
-  const int wave = threadIdx.x / 64, lane = threadIdx.x % 64, nw = blockDim.x / 64;
+  const int wave = threadIdx.x / 64, lane = threadIdx.x % 64, nw = NWAVES;
   const int nv = V.svr ? 2 : 1;
   for (int s = sb + wave; s < se; s += nw) {
     int r = s * SLICE + lane;
@@ -210,7 +208,7 @@ __device__ __forceinline__ void compute_rows_part(const View &V, const int *list
     return;
   }
   for (int m = 0; m < cnt; ++m) stage_row(V, list[m], xs + m * V.d);
-  const int wave = threadIdx.x / 64, lane = threadIdx.x % 64, nw = blockDim.x / 64;
+  const int wave = threadIdx.x / 64, lane = threadIdx.x % 64, nw = NWAVES;
   for (int sl = sb + wave; sl < se; sl += nw) {
     const int r = sl * SLICE + lane;
     if (r >= V.l) continue;
@@ -248,7 +246,7 @@ __device__ __forceinline__ void spec_rows(const View &V, Dec &d, RedSmem &sm, fl
   // thread 0 merges the waves' lists
   __shared__ double wv[16][SPEC_UP + SPEC_LOW];
   __shared__ int wi[16][SPEC_UP + SPEC_LOW];
-  const int wave = t / 64, lane = t % 64, nw = blockDim.x / 64;
+  const int wave = t / 64, lane = t % 64, nw = NWAVES;
 #pragma unroll
   for (int k = 0; k < SPEC_UP + SPEC_LOW; ++k) {
     const bool up = k < SPEC_UP;
@@ -355,7 +353,7 @@ __device__ __forceinline__ int res_index(const View &V, const Res &R, int v) {
 __device__ __forceinline__ void res_load(const View &V, const Res &R, int sb, int se) {
   const int r0 = sb * SLICE, r1 = min(se * SLICE, V.l), nv = V.svr ? 2 : 1;
   for (int h = 0; h < nv; ++h)
-    for (int r = r0 + (int)threadIdx.x; r < r1; r += blockDim.x) {
+    for (int r = r0 + (int)threadIdx.x; r < r1; r += BLOCK) {
       const int k = r + h * V.l, x = h * R.m + (r - r0);
       R.G[x] = V.G[k];
       R.A[x] = V.alpha[k];
@@ -370,7 +368,7 @@ __device__ __forceinline__ void res_load(const View &V, const Res &R, int sb, in
 __device__ __forceinline__ void res_store(const View &V, const Res &R, int sb, int se) {
   const int r0 = sb * SLICE, r1 = min(se * SLICE, V.l), nv = V.svr ? 2 : 1;
   for (int h = 0; h < nv; ++h)
-    for (int r = r0 + (int)threadIdx.x; r < r1; r += blockDim.x) {
+    for (int r = r0 + (int)threadIdx.x; r < r1; r += BLOCK) {
       const int k = r + h * V.l, x = h * R.m + (r - r0);
       V.G[k] = R.G[x];
       V.alpha[k] = R.A[x];
@@ -384,7 +382,7 @@ __device__ __forceinline__ void compute_row_seg(const View &V, int q, int slot, 
                                                 int base) {
   stage_row(V, q, xs);
   float *dst = V.rows + (long long)slot * V.row_stride;
-  const int wave = threadIdx.x / 64, lane = threadIdx.x % 64, nw = blockDim.x / 64;
+  const int wave = threadIdx.x / 64, lane = threadIdx.x % 64, nw = NWAVES;
   for (int s = sb + wave; s < se; s += nw) {
     const int r = s * SLICE + lane;
     if (r < V.l) {
@@ -408,7 +406,7 @@ struct Pre {
 
 template <int PS>
 __device__ __forceinline__ void prefetch_seg(const View &V, int row, int sb, int se, float (&k)[PS]) {
-  const int wave = threadIdx.x / 64, lane = threadIdx.x % 64, nw = blockDim.x / 64;
+  const int wave = threadIdx.x / 64, lane = threadIdx.x % 64, nw = NWAVES;
   const long long off = (long long)row * V.row_stride;
 #pragma unroll
   for (int u = 0; u < PS; ++u) {
@@ -467,7 +465,7 @@ __device__ __forceinline__ Partial rpass_select_j(const View &V, const Snap &c, 
   const double QDi = KIND == KIND_C ? d.QDi : d.QDip, QDn = KIND == KIND_C ? 0.0 : d.QDin;
   double best = INF;
   int bj = -1;
-  const int wave = threadIdx.x / 64, lane = threadIdx.x % 64, nw = blockDim.x / 64;
+  const int wave = threadIdx.x / 64, lane = threadIdx.x % 64, nw = NWAVES;
   for (int s0 = sb + wave; s0 < se; s0 += nw * PS) {
     float Ki[PS], Kn[PS];
     seg_fetch<PS>(V, R, s0, se, nw, lane, pi, pre.k[0], li, ri, qi >= 0 ? R.seg[0] : nullptr, Ki);
@@ -548,7 +546,7 @@ __device__ __forceinline__ Partial rpass_update_select_i(const View &V, const Sn
   const int8_t yi = (int8_t)d.yi, yj = (int8_t)d.yj;
   double v[4] = {-INF, -INF, -INF, -INF};
   int ix[2] = {-1, -1};
-  const int wave = threadIdx.x / 64, lane = threadIdx.x % 64, nw = blockDim.x / 64;
+  const int wave = threadIdx.x / 64, lane = threadIdx.x % 64, nw = NWAVES;
   for (int s0 = sb + wave; s0 < se; s0 += nw * PS) {
     float Kj[PS];
     seg_fetch<PS>(V, R, s0, se, nw, lane, pj, pre.k[0], lj, rj, R.seg[sj], Kj);
@@ -767,9 +765,9 @@ __global__ void __launch_bounds__(BLOCK) k_smo(View V, int iter_limit) {
   unsigned *fbits = reinterpret_cast<unsigned *>(smo_dyn_lds + V.flag_lds);  // as private_meta
   int8_t *fbytes = V.pm_flags + pm_index(V) * V.l;
   if (V.flag_lds > 0) {
-    for (int w = threadIdx.x; w < (V.l + 31) / 32; w += blockDim.x) fbits[w] = 0u;
+    for (int w = threadIdx.x; w < (V.l + 31) / 32; w += BLOCK) fbits[w] = 0u;
     __syncthreads();
-    for (int r = threadIdx.x; r < V.l; r += blockDim.x)
+    for (int r = threadIdx.x; r < V.l; r += BLOCK)
       if (fbytes[r]) atomicOr(&fbits[r >> 5], 1u << (r & 31));
   }
   __syncthreads();
@@ -792,7 +790,7 @@ __global__ void __launch_bounds__(BLOCK) k_smo(View V, int iter_limit) {
   }
   if (V.flag_lds > 0) {
     __syncthreads();
-    for (int r = threadIdx.x; r < V.l; r += blockDim.x) fbytes[r] = (int8_t)((fbits[r >> 5] >> (r & 31)) & 1u);
+    for (int r = threadIdx.x; r < V.l; r += BLOCK) fbytes[r] = (int8_t)((fbits[r >> 5] >> (r & 31)) & 1u);
   }
   if (blockIdx.x == 0 && threadIdx.x == 0) {
     Ctl *c = V.ctl;
@@ -810,8 +808,8 @@ __global__ void __launch_bounds__(BLOCK) k_smo(View V, int iter_limit) {
 // Resets every workgroup's private LRU bookkeeping (persistent kernel).
 __global__ void __launch_bounds__(BLOCK) k_lru_private_init(View V) {
   int *base = V.pm_lru + (long long)blockIdx.x * V.pm_stride;
-  for (int r = threadIdx.x; r < V.l; r += blockDim.x) base[r] = -1;
-  for (int s = threadIdx.x; s < V.S; s += blockDim.x) {
+  for (int r = threadIdx.x; r < V.l; r += BLOCK) base[r] = -1;
+  for (int s = threadIdx.x; s < V.S; s += BLOCK) {
     base[V.l + s] = -1;
     base[V.l + V.S + s] = s - 1;
     base[V.l + 2 * V.S + s] = s + 1 < V.S ? s + 1 : -1;
@@ -831,7 +829,7 @@ __global__ void __launch_bounds__(BLOCK) k_finalize(View V) {
   // accumulators: C: s, ub, lb | NU: s1, ub1, lb1, s2, ub2, lb2 ; obj
   double s1 = 0, ub1 = INF, lb1 = -INF, s2 = 0, ub2 = INF, lb2 = -INF, ob = 0;
   int n1 = 0, n2 = 0;
-  for (int v = blockIdx.x * blockDim.x + threadIdx.x; v < V.nvar; v += gridDim.x * blockDim.x) {
+  for (int v = blockIdx.x * BLOCK + threadIdx.x; v < V.nvar; v += gridDim.x * BLOCK) {
     double g = V.G[v];
     int8_t yv = V.y[v], st = V.status[v];
     ob += V.alpha[v] * (g + V.p[v]);
@@ -858,7 +856,7 @@ __global__ void __launch_bounds__(BLOCK) k_finalize(View V) {
     }
   }
   // block reduce
-  const int wave = threadIdx.x / 64, lane = threadIdx.x % 64, nw = blockDim.x / 64;
+  const int wave = threadIdx.x / 64, lane = threadIdx.x % 64, nw = NWAVES;
   s1 = wave_sum(s1); s2 = wave_sum(s2); ob = wave_sum(ob);
   lb1 = wave_max(lb1); lb2 = wave_max(lb2);
   ub1 = -wave_max(-ub1); ub2 = -wave_max(-ub2);
@@ -908,7 +906,7 @@ __global__ void __launch_bounds__(BLOCK) k_kernel_row(View V, int q, float *out)
   int sb, se;
   owned_slices(V, blockIdx.x, gridDim.x, sb, se);
   stage_row(V, q, xs);
-  const int wave = threadIdx.x / 64, lane = threadIdx.x % 64, nw = blockDim.x / 64;
+  const int wave = threadIdx.x / 64, lane = threadIdx.x % 64, nw = NWAVES;
   for (int s = sb + wave; s < se; s += nw) {
     int r = s * SLICE + lane;
     if (r < V.l) out[r] = kernel_value(V, xs, q, r);
@@ -932,11 +930,11 @@ __global__ void __launch_bounds__(BLOCK) k_kernel_row(View V, int q, float *out)
 __device__ __forceinline__ void solo_init_cache(View &V) {
   if (V.keep_cache) return;
   if (V.full) {
-    for (int r = threadIdx.x; r < V.l; r += blockDim.x) V.pm_flags[r] = 0;
+    for (int r = threadIdx.x; r < V.l; r += BLOCK) V.pm_flags[r] = 0;
   } else {
     int *base = V.pm_lru;
-    for (int r = threadIdx.x; r < V.l; r += blockDim.x) base[r] = -1;
-    for (int t = threadIdx.x; t < V.S; t += blockDim.x) {
+    for (int r = threadIdx.x; r < V.l; r += BLOCK) base[r] = -1;
+    for (int t = threadIdx.x; t < V.S; t += BLOCK) {
       base[V.l + t] = -1;
       base[V.l + V.S + t] = t - 1;
       base[V.l + 2 * V.S + t] = t + 1 < V.S ? t + 1 : -1;
@@ -950,7 +948,7 @@ __device__ __forceinline__ void solo_init_cache(View &V) {
 
 // QD_v = K(v, v) in FP64 for every variable (gamma is per subproblem).
 __device__ __forceinline__ void batch_qd(const View &V) {
-  for (int v = threadIdx.x; v < V.nvar; v += blockDim.x) {
+  for (int v = threadIdx.x; v < V.nvar; v += BLOCK) {
     int r = real_of(V, v);
     V.QD[v] = kernel_f64(V, r, r, V.xsq[r]);
   }
@@ -967,7 +965,7 @@ constexpr int GINIT_M = 16;
 template <int M>
 __device__ __forceinline__ void ginit_pass(const View &V, float *xs, int ld, const int *list, int cnt, int r0, int r1) {
   const int nv = V.svr ? 2 : 1;
-  for (int r = r0 + threadIdx.x; r < r1; r += blockDim.x) {
+  for (int r = r0 + threadIdx.x; r < r1; r += BLOCK) {
     // row r of a slice is lane r % 64 of wave (r % 512) / 64 in the SELL layout
     double acc[M];
     if (V.gdot) {
@@ -1063,7 +1061,7 @@ __device__ __forceinline__ void final_terms(const View &V, int r0, int r1, doubl
   double s1 = 0, ub1 = INF, lb1 = -INF, s2 = 0, ub2 = INF, lb2 = -INF, ob = 0;
   int n1 = 0, n2 = 0;
   const int nv = V.svr ? 2 : 1;
-  for (int rr = r0 + threadIdx.x; rr < r1; rr += blockDim.x) {
+  for (int rr = r0 + threadIdx.x; rr < r1; rr += BLOCK) {
     for (int h = 0; h < nv; ++h) {
       const int v = rr + h * V.l;
       double g = V.G[v];
@@ -1092,7 +1090,7 @@ __device__ __forceinline__ void final_terms(const View &V, int r0, int r1, doubl
       }
     }
   }
-  const int wave = threadIdx.x / 64, lane = threadIdx.x % 64, nw = blockDim.x / 64;
+  const int wave = threadIdx.x / 64, lane = threadIdx.x % 64, nw = NWAVES;
   s1 = wave_sum(s1); s2 = wave_sum(s2); ob = wave_sum(ob);
   lb1 = wave_max(lb1); lb2 = wave_max(lb2);
   ub1 = -wave_max(-ub1); ub2 = -wave_max(-ub2);
@@ -1185,7 +1183,7 @@ __device__ __forceinline__ void solo_spec_rows(const View &V, Dec &d, RedSmem &s
   __shared__ int s_cnt;
   if (threadIdx.x == 0) {
     const int cap = min(SPEC_ROWS, lds_floats / max(V.d, 1));
-    const int nw = blockDim.x / 64;
+    const int nw = NWAVES;
     CacheMeta M = private_meta(V, d);
     int cnt = 0;
     list[cnt++] = real_of(V, jside ? d.j : d.i);  // the decision has marked it
@@ -1226,10 +1224,10 @@ __device__ __forceinline__ void solo_solve(View &V, Dec &d, RedSmem &sm, float *
   // the arena's flag bytes
   unsigned *fbits = reinterpret_cast<unsigned *>(smo_dyn_lds + V.flag_lds);  // as private_meta
   if (V.flag_lds > 0) {
-    for (int w = threadIdx.x; w < (V.l + 31) / 32; w += blockDim.x) fbits[w] = 0u;
+    for (int w = threadIdx.x; w < (V.l + 31) / 32; w += BLOCK) fbits[w] = 0u;
     __syncthreads();
     if (V.keep_cache)
-      for (int r = threadIdx.x; r < V.l; r += blockDim.x)
+      for (int r = threadIdx.x; r < V.l; r += BLOCK)
         if (V.pm_flags[r]) atomicOr(&fbits[r >> 5], 1u << (r & 31));
     __syncthreads();
   }
@@ -1276,7 +1274,7 @@ __device__ __forceinline__ void solo_solve(View &V, Dec &d, RedSmem &sm, float *
     __syncthreads();
   }
   if (V.flag_lds > 0)
-    for (int r = threadIdx.x; r < V.l; r += blockDim.x) V.pm_flags[r] = (int8_t)((fbits[r >> 5] >> (r & 31)) & 1u);
+    for (int r = threadIdx.x; r < V.l; r += BLOCK) V.pm_flags[r] = (int8_t)((fbits[r >> 5] >> (r & 31)) & 1u);
   double f[7];
   int n[2];
   final_terms<KIND>(V, 0, V.l, f, n);
@@ -1441,7 +1439,7 @@ __global__ void __launch_bounds__(BLOCK) k_smo_batch(const View *views, const in
 // the row set's own SELL arrays (slice pointers precomputed on the host).
 __global__ void __launch_bounds__(BLOCK) k_pack_rowset(View src, const int *rows, int n, const long long *dst_slice_ptr,
                                                       float *dst_vals, int *dst_cols, int *dst_len, double *dst_xsq) {
-  int r = blockIdx.x * blockDim.x + threadIdx.x;
+  int r = blockIdx.x * BLOCK + threadIdx.x;
   if (r >= n) return;
   const int g = rows[r];
   const int len = src.row_len[g];
