@@ -326,7 +326,7 @@ __device__ __forceinline__ void stage_row(const View &V, int q, float *xs) {
 // dot(x_q, x_r) with x_q staged in xs; row r is this lane's row.  Products of
 // two floats are exact in FP64, so fma == multiply-then-add here and the sum
 // is LIBSVM's sequential merge-order sum.  Loads are issued in batches of
-// ROW_UNROLL per wave (16 KB in flight per wave) so a row pass is bound by
+// ROW_UNROLL per lane (8 KB in flight per wave) so a row pass is bound by
 // HBM bandwidth rather than by one memory round trip per few elements; the
 // rest of a row goes in batches of ROW_TAIL, their loads clamped to the
 // row's stored entries and the entries past the row's length left out of the
@@ -343,7 +343,8 @@ constexpr int ROW_PIPE = 16;
 // Sparse row dot with two batches of ROW_PIPE entries in flight (the PIPE
 // form of row_dot): vp / cp are the row's first value and column in its
 // slice, len its length, width the slice's stored width.
-__device__ __forceinline__ double sparse_dot_pipe(const float *vp, const int *cp, const float *xs, int len, int width) {
+template <class XP>
+__device__ __forceinline__ double sparse_dot_pipe(const float *vp, const int *cp, XP xs, int len, int width) {
   constexpr int B = ROW_PIPE;
   double acc = 0.0;
   float a0[B], a1[B];
@@ -384,70 +385,103 @@ __device__ __forceinline__ double sparse_dot_pipe(const float *vp, const int *cp
   return acc;
 }
 
+// The query row in LDS, addressed as LDS: its reads are then LDS reads
+// (counted apart from device-memory loads) rather than flat loads, which
+// count as both and make every wait for one of them a wait for all the
+// row's loads in flight.
+typedef __attribute__((address_space(3))) const float lds_cfloat;
+
+// Dense dot(x_q, x_r), q staged at xs (LDS or device memory): features in
+// order, one fma each (row_dot's sum).  A batch: ROW_UNROLL loads of row r
+// issued, then the batch's query values read at once (one LDS wait rather
+// than one per pair), then the fmas in feature order.  (Measured on the
+// n=100k dense solve: 64-entry batches, or two 32-entry batches in flight,
+// are slower: 191 and 184 ms against 178.5.)
+template <class XP>
+__device__ __forceinline__ double dense_dot(const float *vp, XP xs, int len) {
+  constexpr int U = ROW_UNROLL;
+  double acc = 0.0;
+  int p = 0;
+  for (; p + U <= len; p += U) {
+    float a[U], q[U];
+#pragma unroll
+    for (int u = 0; u < U; ++u) a[u] = vp[(long long)(p + u) * SLICE];
+#pragma unroll
+    for (int u = 0; u < U; ++u) q[u] = xs[p + u];
+#pragma unroll
+    for (int u = 0; u < U; ++u) acc = fma((double)a[u], (double)q[u], acc);
+  }
+  for (; p < len; p += ROW_TAIL) {  // the rest, in clamped batches
+    float a[ROW_TAIL];
+#pragma unroll
+    for (int u = 0; u < ROW_TAIL; ++u) a[u] = vp[(long long)min(p + u, len - 1) * SLICE];
+#pragma unroll
+    for (int u = 0; u < ROW_TAIL; ++u)
+      if (p + u < len) acc = fma((double)a[u], (double)xs[p + u], acc);
+  }
+  return acc;
+}
+
+// Sparse dot(x_q, x_r): the slice's common width walked (rows are
+// zero-padded to it) so the trip count is uniform across the wave.
+template <class XP>
+__device__ __forceinline__ double sparse_dot(const float *vp, const int *cp, XP xs, int len, int width) {
+  constexpr int U = ROW_UNROLL;
+  double acc = 0.0;
+  int p = 0;
+  for (; p + U <= width; p += U) {
+    float a[U];
+    int c[U];
+#pragma unroll
+    for (int u = 0; u < U; ++u) {
+      a[u] = vp[(long long)(p + u) * SLICE];
+      c[u] = cp[(long long)(p + u) * SLICE];
+    }
+#pragma unroll
+    for (int u = 0; u < U; ++u) {
+      double t = (p + u < len) ? (double)a[u] * (double)xs[c[u]] : 0.0;
+      acc = acc + t;
+    }
+  }
+  for (; p < width; p += ROW_TAIL) {  // the rest, in clamped batches
+    float a[ROW_TAIL];
+    int c[ROW_TAIL];
+#pragma unroll
+    for (int u = 0; u < ROW_TAIL; ++u) {
+      const long long q = min(p + u, width - 1);
+      a[u] = vp[q * SLICE];
+      c[u] = cp[q * SLICE];
+    }
+#pragma unroll
+    for (int u = 0; u < ROW_TAIL; ++u) {
+      double t = (p + u < len) ? (double)a[u] * (double)xs[c[u]] : 0.0;
+      acc = acc + t;
+    }
+  }
+  return acc;
+}
+
 template <bool PIPE = false>
 __device__ __forceinline__ double row_dot(const View &V, const float *xs, int r) {
   const int s = r / SLICE;
   const long long base = V.slice_ptr[s] + (r % SLICE);
   const float *vp = V.vals + base;
-  double acc = 0.0;
-  constexpr int U = ROW_UNROLL;
   if (V.dense) {
-    const int len = V.d;
-    int p = 0;
-    for (; p + U <= len; p += U) {
-      float a[U];
-#pragma unroll
-      for (int u = 0; u < U; ++u) a[u] = vp[(long long)(p + u) * SLICE];
-#pragma unroll
-      for (int u = 0; u < U; ++u) acc = fma((double)a[u], (double)xs[p + u], acc);
-    }
-    for (; p < len; p += ROW_TAIL) {  // the rest, in clamped batches
-      float a[ROW_TAIL];
-#pragma unroll
-      for (int u = 0; u < ROW_TAIL; ++u) a[u] = vp[(long long)min(p + u, len - 1) * SLICE];
-#pragma unroll
-      for (int u = 0; u < ROW_TAIL; ++u)
-        if (p + u < len) acc = fma((double)a[u], (double)xs[p + u], acc);
-    }
-  } else {
-    // walk the slice's common width (rows are zero-padded to it) so the trip
-    // count is uniform across the wave
-    const int len = V.row_len[r];
-    const int width = (int)((V.slice_ptr[s + 1] - V.slice_ptr[s]) / SLICE);
-    const int *cp = V.cols + base;
-    int p = 0;
-    if (PIPE) return sparse_dot_pipe(vp, cp, xs, len, width);
-    for (; p + U <= width; p += U) {
-      float a[U];
-      int c[U];
-#pragma unroll
-      for (int u = 0; u < U; ++u) {
-        a[u] = vp[(long long)(p + u) * SLICE];
-        c[u] = cp[(long long)(p + u) * SLICE];
-      }
-#pragma unroll
-      for (int u = 0; u < U; ++u) {
-        double t = (p + u < len) ? (double)a[u] * (double)xs[c[u]] : 0.0;
-        acc = acc + t;
-      }
-    }
-    for (; p < width; p += ROW_TAIL) {  // the rest, in clamped batches
-      float a[ROW_TAIL];
-      int c[ROW_TAIL];
-#pragma unroll
-      for (int u = 0; u < ROW_TAIL; ++u) {
-        const long long q = min(p + u, width - 1);
-        a[u] = vp[q * SLICE];
-        c[u] = cp[q * SLICE];
-      }
-#pragma unroll
-      for (int u = 0; u < ROW_TAIL; ++u) {
-        double t = (p + u < len) ? (double)a[u] * (double)xs[c[u]] : 0.0;
-        acc = acc + t;
-      }
-    }
+#if defined(__HIP_DEVICE_COMPILE__)  // (the builtin exists in the device pass only)
+    if (__builtin_amdgcn_is_shared((const void *)xs)) return dense_dot(vp, (lds_cfloat *)xs, V.d);
+#endif
+    return dense_dot(vp, xs, V.d);
   }
-  return acc;
+  const int len = V.row_len[r];
+  const int width = (int)((V.slice_ptr[s + 1] - V.slice_ptr[s]) / SLICE);
+  const int *cp = V.cols + base;
+#if defined(__HIP_DEVICE_COMPILE__)
+  if (__builtin_amdgcn_is_shared((const void *)xs)) {
+    lds_cfloat *xl = (lds_cfloat *)xs;
+    return PIPE ? sparse_dot_pipe(vp, cp, xl, len, width) : sparse_dot(vp, cp, xl, len, width);
+  }
+#endif
+  return PIPE ? sparse_dot_pipe(vp, cp, xs, len, width) : sparse_dot(vp, cp, xs, len, width);
 }
 
 // dot(x_q(m), x_r) for the M rows staged at xs + m * ld, each summed in the
