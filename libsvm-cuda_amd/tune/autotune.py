@@ -34,6 +34,11 @@ import libsvm_hip  # noqa: E402
 from libsvm_hip import datasets, ffi  # noqa: E402
 
 GRIDS = [32, 64, 128, 256]
+# dense rows with the whole kernel matrix cached: also grids between the
+# powers of two (how many 64-row slices each workgroup owns changes how
+# evenly the row passes end: n=100k d=256 exact rows, 256 / 240 / 224 / 208 /
+# 196 workgroups: 178.6 / 177.4 / 176.5 / 176.0 / 177.8 ms)
+GRIDS_DENSE_FULL = [32, 64, 96, 128, 160, 192, 208, 224, 240, 256]
 COPIES = [1, 2, 4]
 
 
@@ -98,6 +103,8 @@ def main():
         prm = ffi.make_param(gamma=gamma, C_=C, eps=1e-3, shrinking=0)
         full = 1 if cache >= 4.0 * l * ((l + 63) // 64 * 64) else 0
         grids, copies = (GRIDS, COPIES) if full else ([128, 256], [2, 4])  # LRU solves are long: fewer points
+        if full and kind == "dense":
+            grids = GRIDS_DENSE_FULL
         for mode in (("exact", "gram") if kind == "dense" and full else ("exact",)):
             results = []
             for g in grids:
