@@ -1431,8 +1431,11 @@ __device__ __forceinline__ bool xgather(XWord *X, int nb, int t, unsigned seq, X
   // a record takes about a microsecond to travel: a short sleep before the
   // first poll keeps the readers' polls off the memory pipeline meanwhile
   // (n=100k solve, 256 workgroups: 191 -> 188 ms; Gram-tile loop on 128:
-  // 89 -> 82 ms)
-  __builtin_amdgcn_s_sleep(8);
+  // 89 -> 82 ms).  Polls are what the exchange is most sensitive to: 12
+  // sleep units rather than 8 (Gram-tile loop on 192 workgroups 72.0 ->
+  // 70.9 ms; 14, 16 and 20: 71.9, 72.6, 74.3 ms), and a second poll kept in
+  // flight, which doubles the polls, 72.0 -> 87 ms
+  __builtin_amdgcn_s_sleep(12);
   for (;;) {
     while ((unsigned)(ld_acq(xword(X, nb, seq & 1u, NW - 1, t)) >> 32) != seq) {
       if (++spins > SPIN_LIMIT) return false;
