@@ -399,8 +399,9 @@ __device__ __forceinline__ void compute_row_seg(const View &V, int q, int slot, 
 // PS slices of this wave's part of cached row `row`.  Used by the next pass
 // only if the row turns out to be cached; otherwise discarded (the loads
 // read valid cache memory either way).
+template <int PS>
 struct Pre {
-  float k[2][PASS_SLICES];
+  float k[2][PS];
   int row[2];  // -1: nothing fetched into k[m]
 };
 
@@ -446,7 +447,7 @@ __device__ __forceinline__ void seg_fetch(const View &V, const Res &R, int s0, i
 // rather than branches (the same comparisons, in the same order).
 template <int KIND, int NV, int PS>
 __device__ __forceinline__ Partial rpass_select_j(const View &V, const Snap &c, const Dec &d, float *xs, RedSmem &sm,
-                                                  const Res &R, int sb, int se, const Pre &pre) {
+                                                  const Res &R, int sb, int se, const Pre<PS> &pre) {
   const int qi = KIND == KIND_C ? c.i : c.ip;
   const int qn = KIND == KIND_C ? -1 : c.in;
   const int mi = KIND == KIND_C ? c.miss_i : c.miss_ip;
@@ -523,7 +524,7 @@ __device__ __forceinline__ Partial rpass_select_j(const View &V, const Snap &c, 
 template <int KIND, int NV, int PS>
 __device__ __forceinline__ Partial rpass_update_select_i(const View &V, const Snap &c, const Dec &d, float *xs,
                                                          RedSmem &sm, const Res &R, int sb, int se, const Pend &pd,
-                                                         const Pre &pre) {
+                                                         const Pre<PS> &pre) {
   if (threadIdx.x == 0) {
     if (owns(V, sb, se, pd.i)) {
       const int x = res_index(V, R, pd.i);
@@ -653,10 +654,12 @@ __device__ __forceinline__ XPart rmake_j_record(const View &V, const Res &R, con
 }
 
 // smo_iterations over the resident vectors (same decisions and exchanges).
-template <int KIND, int NV>
+// PS: slices a wave takes per trip of a pass (1 when each wave owns at most
+// one slice: the unrolled form's idle copies cost registers and time).
+template <int KIND, int NV, int PS>
 __device__ __forceinline__ void smo_iterations_res(const View &V, Dec &d, RedSmem &sm, XSmem &xsm, float *xs,
                                                    const Res &R, int sb, int se, int iter_limit) {
-  Pre pi, pj;  // segments fetched for the next j pass (rows i / ip, in) and the next update pass (row j)
+  Pre<PS> pi, pj;  // segments fetched for the next j pass (rows i / ip, in) and the next update pass (row j)
   pi.row[0] = pi.row[1] = pj.row[0] = pj.row[1] = -1;
   for (int n = 0; n < iter_limit; ++n) {
     if (d.stop || d.err) break;
@@ -665,14 +668,14 @@ __device__ __forceinline__ void smo_iterations_res(const View &V, Dec &d, RedSme
       pi.row[0] = -1;  // row i was computed after the fetch
     }
     Snap c = snap_of(d);
-    Partial bp = rpass_select_j<KIND, NV, PASS_SLICES>(V, c, d, xs, sm, R, sb, se, pi);
+    Partial bp = rpass_select_j<KIND, NV, PS>(V, c, d, xs, sm, R, sb, se, pi);
     XPart mine = rmake_j_record<KIND>(V, R, d, bp);
     unsigned seq = (unsigned)d.epoch + 1u;
     bool hold = threadIdx.x == 0;
     XPart r = exchange<KIND, PH_J>(V, sm, xsm, mine, seq, &hold);
     // (every thread holds the winning index)
     pj.row[0] = V.full && r.i0 >= 0 ? real_of(V, r.i0) : -1;
-    if (pj.row[0] >= 0) prefetch_seg<PASS_SLICES>(V, pj.row[0], sb, se, pj.k[0]);
+    if (pj.row[0] >= 0) prefetch_seg<PS>(V, pj.row[0], sb, se, pj.k[0]);
     if (hold) {
       d.epoch = (int)seq;
       if (r.seq) d.err = 1;
@@ -683,7 +686,7 @@ __device__ __forceinline__ void smo_iterations_res(const View &V, Dec &d, RedSme
     c = snap_of(d);
     Pend pd;
     pd.i = d.i; pd.j = d.j; pd.ai = d.ai_new; pd.aj = d.aj_new; pd.si = d.st_i; pd.sj = d.st_j;
-    Partial bp2 = rpass_update_select_i<KIND, NV, PASS_SLICES>(V, c, d, xs, sm, R, sb, se, pd, pj);
+    Partial bp2 = rpass_update_select_i<KIND, NV, PS>(V, c, d, xs, sm, R, sb, se, pd, pj);
     XPart mine2 = rmake_i_record<KIND>(V, R, bp2);
     mine2.pad = pause_vote(V, d);
     seq = (unsigned)d.epoch + 1u;
@@ -691,8 +694,8 @@ __device__ __forceinline__ void smo_iterations_res(const View &V, Dec &d, RedSme
     XPart r2 = exchange<KIND, PH_I>(V, sm, xsm, mine2, seq, KIND == KIND_C ? &hold2 : nullptr);
     pi.row[0] = V.full && r2.i0 >= 0 ? real_of(V, r2.i0) : -1;
     pi.row[1] = V.full && KIND == KIND_NU && r2.i1 >= 0 ? real_of(V, r2.i1) : -1;
-    if (pi.row[0] >= 0) prefetch_seg<PASS_SLICES>(V, pi.row[0], sb, se, pi.k[0]);
-    if (pi.row[1] >= 0) prefetch_seg<PASS_SLICES>(V, pi.row[1], sb, se, pi.k[1]);
+    if (pi.row[0] >= 0) prefetch_seg<PS>(V, pi.row[0], sb, se, pi.k[0]);
+    if (pi.row[1] >= 0) prefetch_seg<PS>(V, pi.row[1], sb, se, pi.k[1]);
     if (hold2) {
       d.epoch = (int)seq;
       d.pend = 0;
@@ -782,8 +785,14 @@ __global__ void __launch_bounds__(BLOCK) k_smo(View V, int iter_limit) {
   }
   if (d.phase == 0 && !d.stop && !d.err) smo_select_first<KIND, true>(V, d, sm, xsm, xs);
   if (!res) smo_iterations<KIND, true, true>(V, d, sm, xsm, xs, iter_limit);
-  else if (V.svr) smo_iterations_res<KIND, 2>(V, d, sm, xsm, xs, R, sb, se, iter_limit);
-  else smo_iterations_res<KIND, 1>(V, d, sm, xsm, xs, R, sb, se, iter_limit);
+  else if (se - sb <= NWAVES) {  // every wave of this workgroup owns at most one slice
+    if (V.svr) smo_iterations_res<KIND, 2, 1>(V, d, sm, xsm, xs, R, sb, se, iter_limit);
+    else smo_iterations_res<KIND, 1, 1>(V, d, sm, xsm, xs, R, sb, se, iter_limit);
+  } else if (V.svr) {
+    smo_iterations_res<KIND, 2, PASS_SLICES>(V, d, sm, xsm, xs, R, sb, se, iter_limit);
+  } else {
+    smo_iterations_res<KIND, 1, PASS_SLICES>(V, d, sm, xsm, xs, R, sb, se, iter_limit);
+  }
   if (res) {
     __syncthreads();
     res_store(V, R, sb, se);
