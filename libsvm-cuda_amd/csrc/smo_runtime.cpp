@@ -734,7 +734,7 @@ struct svmhip_batch {
   // row-cache arenas, most of the device memory, costs more than a solve);
   // they are released with the handle
   struct Kept {
-    void *p = nullptr;
+    char *p = nullptr;
     size_t cap = 0;
   };
   enum { K_G, K_ALPHA, K_P, K_QD, K_Y, K_STATUS, K_CTL, K_VIEWS, K_ROWS, K_FLAGS, K_LRU, K_WORDS, K_X, K_ITEMS, K_N };
@@ -870,7 +870,7 @@ svmhip_status svmhip_batch_solve(svmhip_batch *b, svmhip_subproblem *subs, int n
     // pooled per-subproblem state, resident across the stages of the solve
     auto talloc = [&](int slot, size_t bytes, const char *what) {
       svmhip_batch::Kept &k = b->kept[slot];
-      return (void *)h->ensure(*(char **)&k.p, k.cap, bytes, what);
+      return (void *)h->ensure(k.p, k.cap, bytes, what);
     };
     double *dG = (double *)talloc(svmhip_batch::K_G, sizeof(double) * nv_total, "batch G");
     double *dA = (double *)talloc(svmhip_batch::K_ALPHA, sizeof(double) * nv_total, "batch alpha");

@@ -175,7 +175,10 @@ def test_small_cache_lru_same_result(hip, ora):
 @pytest.mark.parametrize("grid", [1, 3, 7, 64])
 def test_grid_sizes_agree(hip, grid):
     """Any persistent-grid size gives bit-identical iterates (the reductions
-    break ties by index, so the result does not depend on the split)."""
+    break ties by index, so the result does not depend on the split).  The
+    1500 rows are 24 slices: on one workgroup each wave takes several slices
+    per pass (the unrolled loop), on the default grid each owns at most one
+    (the one-slice loop)."""
     prob, d = zipf_problem(1500)
     prm = ffi.make_param(gamma=1.0 / d, C_=1.0, eps=1e-3, shrinking=0)
     ref = hip.solve_one(prob, prm, grid_blocks=0)
