@@ -703,8 +703,11 @@ __device__ __forceinline__ bool flag_test_set(CacheMeta &M, int row) {
   return was;
 }
 
+// FULL: the caller knows the cache holds every row (V.full), so the LRU
+// bookkeeping is left out of its code.
+template <bool FULL = false>
 __device__ __forceinline__ int cache_lookup(const View &V, CacheMeta &M, int row, int exclude, int *miss) {
-  if (V.full) {
+  if (FULL || V.full) {
     *miss = flag_test_set(M, row) ? 0 : 1;
     if (*miss) ++*M.misses; else ++*M.hits;
     return row;
@@ -1623,7 +1626,7 @@ __device__ __forceinline__ Snap snap_of(const Dec &d) {
 }
 
 // Decisions after the i exchange (every workgroup, thread 0).
-template <int KIND>
+template <int KIND, bool FULL = false>
 __device__ __forceinline__ void decide_i(const View &V, Dec &d, const XPart &r, bool count_maxiter) {
   CacheMeta M = private_meta(V, d);
   int stop = STOP_NONE;
@@ -1632,7 +1635,7 @@ __device__ __forceinline__ void decide_i(const View &V, Dec &d, const XPart &r, 
     d.Gmax = r.v0; d.Gmax2 = r.v1; d.i = r.i0; d.Gi = r.g0; d.ai = r.a0; d.QDi = r.qd0; d.yi = r.y0;
     if (!stop && r.v0 + r.v1 < V.eps) stop = STOP_OPTIMAL;   // libsvm/svm.cpp:919
     if (!stop && r.pad) stop = STOP_PAUSE;
-    if (!stop) d.slot_i = cache_lookup(V, M, real_of(V, r.i0), -1, &d.miss_i);
+    if (!stop) d.slot_i = cache_lookup<FULL>(V, M, real_of(V, r.i0), -1, &d.miss_i);
   } else {
     d.Gp = r.v0; d.Gn = r.v1; d.Gp2 = r.v2; d.Gn2 = r.v3;
     d.ip = r.i0; d.in = r.i1;
@@ -1643,8 +1646,8 @@ __device__ __forceinline__ void decide_i(const View &V, Dec &d, const XPart &r, 
     if (!stop) {
       int sp = -1;
       d.miss_ip = d.miss_in = 0;
-      if (d.ip >= 0) { sp = cache_lookup(V, M, real_of(V, d.ip), -1, &d.miss_ip); d.slot_ip = sp; }
-      if (d.in >= 0) d.slot_in = cache_lookup(V, M, real_of(V, d.in), sp, &d.miss_in);
+      if (d.ip >= 0) { sp = cache_lookup<FULL>(V, M, real_of(V, d.ip), -1, &d.miss_ip); d.slot_ip = sp; }
+      if (d.in >= 0) d.slot_in = cache_lookup<FULL>(V, M, real_of(V, d.in), sp, &d.miss_in);
     }
   }
   d.stop = stop;
@@ -1652,7 +1655,7 @@ __device__ __forceinline__ void decide_i(const View &V, Dec &d, const XPart &r, 
 
 // Decisions after the j exchange: the pair, the row-j slot and the alpha
 // step, computed identically in every workgroup (thread 0).
-template <int KIND>
+template <int KIND, bool FULL = false>
 __device__ __forceinline__ void decide_j(const View &V, Dec &d, const XPart &r) {
   CacheMeta M = private_meta(V, d);
   const int j = r.i0;
@@ -1673,7 +1676,7 @@ __device__ __forceinline__ void decide_j(const View &V, Dec &d, const XPart &r) 
   d.j = j;
   d.yi = yi;
   d.yj = yj;
-  d.slot_j = cache_lookup(V, M, real_of(V, j), d.slot_i, &d.miss_j);
+  d.slot_j = cache_lookup<FULL>(V, M, real_of(V, j), d.slot_i, &d.miss_j);
   d.iter += 1;
   double ai2 = ai, aj2 = r.a0;
   alpha_pair(yi, yj, yi > 0 ? V.Cp : V.Cn, yj > 0 ? V.Cp : V.Cn, QDi, r.qd0, Gi, r.g0, r.q0, ai2, aj2);

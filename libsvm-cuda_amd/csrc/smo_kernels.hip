@@ -656,7 +656,7 @@ __device__ __forceinline__ XPart rmake_j_record(const View &V, const Res &R, con
 // smo_iterations over the resident vectors (same decisions and exchanges).
 // PS: slices a wave takes per trip of a pass (1 when each wave owns at most
 // one slice: the unrolled form's idle copies cost registers and time).
-template <int KIND, int NV, int PS>
+template <int KIND, int NV, int PS, bool FULL>
 __device__ __forceinline__ void smo_iterations_res(const View &V, Dec &d, RedSmem &sm, XSmem &xsm, float *xs,
                                                    const Res &R, int sb, int se, int iter_limit) {
   Pre<PS> pi, pj;  // segments fetched for the next j pass (rows i / ip, in) and the next update pass (row j)
@@ -679,7 +679,7 @@ __device__ __forceinline__ void smo_iterations_res(const View &V, Dec &d, RedSme
     if (hold) {
       d.epoch = (int)seq;
       if (r.seq) d.err = 1;
-      else decide_j<KIND>(V, d, r);
+      else decide_j<KIND, FULL>(V, d, r);
     }
     __syncthreads();
     if (d.stop || d.err) break;
@@ -700,7 +700,7 @@ __device__ __forceinline__ void smo_iterations_res(const View &V, Dec &d, RedSme
       d.epoch = (int)seq;
       d.pend = 0;
       if (r2.seq) d.err = 1;
-      else decide_i<KIND>(V, d, r2, true);
+      else decide_i<KIND, FULL>(V, d, r2, true);
     }
     __syncthreads();
   }
@@ -786,12 +786,13 @@ __global__ void __launch_bounds__(BLOCK) k_smo(View V, int iter_limit) {
   if (d.phase == 0 && !d.stop && !d.err) smo_select_first<KIND, true>(V, d, sm, xsm, xs);
   if (!res) smo_iterations<KIND, true, true>(V, d, sm, xsm, xs, iter_limit);
   else if (se - sb <= NWAVES) {  // every wave of this workgroup owns at most one slice
-    if (V.svr) smo_iterations_res<KIND, 2, 1>(V, d, sm, xsm, xs, R, sb, se, iter_limit);
-    else smo_iterations_res<KIND, 1, 1>(V, d, sm, xsm, xs, R, sb, se, iter_limit);
+    if (V.svr) smo_iterations_res<KIND, 2, 1, false>(V, d, sm, xsm, xs, R, sb, se, iter_limit);
+    else if (V.full) smo_iterations_res<KIND, 1, 1, true>(V, d, sm, xsm, xs, R, sb, se, iter_limit);
+    else smo_iterations_res<KIND, 1, 1, false>(V, d, sm, xsm, xs, R, sb, se, iter_limit);
   } else if (V.svr) {
-    smo_iterations_res<KIND, 2, PASS_SLICES>(V, d, sm, xsm, xs, R, sb, se, iter_limit);
+    smo_iterations_res<KIND, 2, PASS_SLICES, false>(V, d, sm, xsm, xs, R, sb, se, iter_limit);
   } else {
-    smo_iterations_res<KIND, 1, PASS_SLICES>(V, d, sm, xsm, xs, R, sb, se, iter_limit);
+    smo_iterations_res<KIND, 1, PASS_SLICES, false>(V, d, sm, xsm, xs, R, sb, se, iter_limit);
   }
   if (res) {
     __syncthreads();
