@@ -117,6 +117,9 @@ struct View {
   int own_cache;          // batched solves: rows/S/full/row_stride and the cache metadata
                           // (nblocks copies, pm_stride apart) are the subproblem's own
   int keep_cache;         // batched solves: the own cache is valid from an earlier stage
+  int qpre;               // persistent solves, full cache, rows computed on demand, dense rows of at
+                          // most BLOCK features: fetch the query row of a likely miss right after
+                          // each exchange
   int spec;               // persistent C-rule solves, full cache: speculative rows on an i miss
   int xrep;               // persistent solves: copies of the exchange records (0 = by the grid size)
   int flag_lds;           // persistent solves, full cache: > 0 = the workgroup's row flags live as

@@ -376,11 +376,41 @@ __device__ __forceinline__ void res_store(const View &V, const Res &R, int sb, i
     }
 }
 
-// compute_row_part that also keeps the owned segment of the row in LDS.
+// Query rows fetched right after an exchange, before the decisions that
+// follow it, for rows the next pass will likely compute (their cache flag
+// was clear): dense rows of at most BLOCK features, one value per thread.
+// The pass stages a row from here when the tag names it, else from memory
+// (the flag test is a hint: the values are X's either way).
+struct QPre {
+  float v[2];
+  int row[2];  // -1: nothing fetched into v[m]
+};
+
+__device__ __forceinline__ void prefetch_query(const View &V, int row, float &v, int &tag) {
+  tag = -1;
+  if (row < 0 || !V.dense || V.d > BLOCK || V.flag_lds <= 0) return;
+  const unsigned *bits = reinterpret_cast<const unsigned *>(smo_dyn_lds + V.flag_lds);
+  // (the decision may be setting the bit meanwhile: a hint, read atomically)
+  const unsigned w = __hip_atomic_load(&bits[row >> 5], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_WORKGROUP);
+  if ((w >> (row & 31)) & 1u) return;  // cached: no pass computes it
+  // dense rows are stored at width d in every slice
+  const long long base = (long long)(row / SLICE) * SLICE * V.d + (row % SLICE);
+  v = V.vals[base + (long long)min((int)threadIdx.x, V.d - 1) * SLICE];
+  tag = row;
+}
+
+// compute_row_part that also keeps the owned segment of the row in LDS; the
+// query row from qv when qtag names it (dense rows: the values stage_row
+// reads).
 template <bool PIPE>
 __device__ __forceinline__ void compute_row_seg(const View &V, int q, int slot, float *xs, int sb, int se, float *seg,
-                                                int base) {
-  stage_row(V, q, xs);
+                                                int base, float qv = 0.f, int qtag = -1) {
+  if (qtag == q) {
+    if ((int)threadIdx.x < V.d) xs[threadIdx.x] = qv;
+    __syncthreads();
+  } else {
+    stage_row(V, q, xs);
+  }
   float *dst = V.rows + (long long)slot * V.row_stride;
   const int wave = threadIdx.x / 64, lane = threadIdx.x % 64, nw = NWAVES;
   for (int s = sb + wave; s < se; s += nw) {
@@ -447,14 +477,19 @@ __device__ __forceinline__ void seg_fetch(const View &V, const Res &R, int s0, i
 // rather than branches (the same comparisons, in the same order).
 template <int KIND, int NV, int PS>
 __device__ __forceinline__ Partial rpass_select_j(const View &V, const Snap &c, const Dec &d, float *xs, RedSmem &sm,
-                                                  const Res &R, int sb, int se, const Pre<PS> &pre) {
+                                                  const Res &R, int sb, int se, const Pre<PS> &pre,
+                                                  const QPre *qp = nullptr) {
   const int qi = KIND == KIND_C ? c.i : c.ip;
   const int qn = KIND == KIND_C ? -1 : c.in;
   const int mi = KIND == KIND_C ? c.miss_i : c.miss_ip;
   const int mn = KIND == KIND_C ? 0 : c.miss_in;
   const int slot_i = KIND == KIND_C ? c.slot_i : c.slot_ip;
-  if (qi >= 0 && mi) compute_row_seg<true>(V, real_of(V, qi), slot_i, xs, sb, se, R.seg[0], R.base);
-  if (qn >= 0 && mn) compute_row_seg<true>(V, real_of(V, qn), c.slot_in, xs, sb, se, R.seg[1], R.base);
+  if (qi >= 0 && mi)
+    compute_row_seg<true>(V, real_of(V, qi), slot_i, xs, sb, se, R.seg[0], R.base, qp ? qp->v[0] : 0.f,
+                          qp ? qp->row[0] : -1);
+  if (qn >= 0 && mn)
+    compute_row_seg<true>(V, real_of(V, qn), c.slot_in, xs, sb, se, R.seg[1], R.base, qp ? qp->v[1] : 0.f,
+                          qp ? qp->row[1] : -1);
   const bool li = qi >= 0 && !mi, ln = qn >= 0 && !mn;  // segments read from the cache here
   const long long ri = qi >= 0 ? (long long)slot_i * V.row_stride : 0;
   const long long rn = qn >= 0 ? (long long)c.slot_in * V.row_stride : 0;
@@ -524,7 +559,7 @@ __device__ __forceinline__ Partial rpass_select_j(const View &V, const Snap &c, 
 template <int KIND, int NV, int PS>
 __device__ __forceinline__ Partial rpass_update_select_i(const View &V, const Snap &c, const Dec &d, float *xs,
                                                          RedSmem &sm, const Res &R, int sb, int se, const Pend &pd,
-                                                         const Pre<PS> &pre) {
+                                                         const Pre<PS> &pre, const QPre *qp = nullptr) {
   if (threadIdx.x == 0) {
     if (owns(V, sb, se, pd.i)) {
       const int x = res_index(V, R, pd.i);
@@ -540,7 +575,9 @@ __device__ __forceinline__ Partial rpass_update_select_i(const View &V, const Sn
   // (the statuses of i and j are taken from pd below, not from those stores)
   const int sj = KIND == KIND_C ? 1 : 2;
   const int si = KIND == KIND_C ? 0 : (c.i == c.ip ? 0 : 1);
-  if (c.miss_j) compute_row_seg<true>(V, real_of(V, c.j), c.slot_j, xs, sb, se, R.seg[sj], R.base);
+  if (c.miss_j)
+    compute_row_seg<true>(V, real_of(V, c.j), c.slot_j, xs, sb, se, R.seg[sj], R.base, qp ? qp->v[0] : 0.f,
+                          qp ? qp->row[0] : -1);
   const bool lj = !c.miss_j;
   const long long rj = (long long)c.slot_j * V.row_stride;
   bool pj = lj && V.full && pre.row[0] == real_of(V, c.j);  // fetched after the j exchange
@@ -656,11 +693,17 @@ __device__ __forceinline__ XPart rmake_j_record(const View &V, const Res &R, con
 // smo_iterations over the resident vectors (same decisions and exchanges).
 // PS: slices a wave takes per trip of a pass (1 when each wave owns at most
 // one slice: the unrolled form's idle copies cost registers and time).
-template <int KIND, int NV, int PS, bool FULL>
+template <int KIND, int NV, int PS, bool FULL, bool QF = false>
 __device__ __forceinline__ void smo_iterations_res(const View &V, Dec &d, RedSmem &sm, XSmem &xsm, float *xs,
                                                    const Res &R, int sb, int se, int iter_limit) {
   Pre<PS> pi, pj;  // segments fetched for the next j pass (rows i / ip, in) and the next update pass (row j)
   pi.row[0] = pi.row[1] = pj.row[0] = pj.row[1] = -1;
+  // QF: query rows fetched for the same passes (V.qpre; the other forms,
+  // and solves that never miss, would pay more for its registers than the
+  // fetch saves)
+  QPre qi, qj;
+  qi.row[0] = qi.row[1] = qj.row[0] = qj.row[1] = -1;
+  qi.v[0] = qi.v[1] = qj.v[0] = qj.v[1] = 0.f;
   for (int n = 0; n < iter_limit; ++n) {
     if (d.stop || d.err) break;
     if (KIND == KIND_C && V.spec && d.miss_i) {
@@ -668,7 +711,7 @@ __device__ __forceinline__ void smo_iterations_res(const View &V, Dec &d, RedSme
       pi.row[0] = -1;  // row i was computed after the fetch
     }
     Snap c = snap_of(d);
-    Partial bp = rpass_select_j<KIND, NV, PS>(V, c, d, xs, sm, R, sb, se, pi);
+    Partial bp = rpass_select_j<KIND, NV, PS>(V, c, d, xs, sm, R, sb, se, pi, QF ? &qi : nullptr);
     XPart mine = rmake_j_record<KIND>(V, R, d, bp);
     unsigned seq = (unsigned)d.epoch + 1u;
     bool hold = threadIdx.x == 0;
@@ -676,6 +719,7 @@ __device__ __forceinline__ void smo_iterations_res(const View &V, Dec &d, RedSme
     // (every thread holds the winning index)
     pj.row[0] = V.full && r.i0 >= 0 ? real_of(V, r.i0) : -1;
     if (pj.row[0] >= 0) prefetch_seg<PS>(V, pj.row[0], sb, se, pj.k[0]);
+    if (QF) prefetch_query(V, pj.row[0], qj.v[0], qj.row[0]);
     if (hold) {
       d.epoch = (int)seq;
       if (r.seq) d.err = 1;
@@ -686,7 +730,7 @@ __device__ __forceinline__ void smo_iterations_res(const View &V, Dec &d, RedSme
     c = snap_of(d);
     Pend pd;
     pd.i = d.i; pd.j = d.j; pd.ai = d.ai_new; pd.aj = d.aj_new; pd.si = d.st_i; pd.sj = d.st_j;
-    Partial bp2 = rpass_update_select_i<KIND, NV, PS>(V, c, d, xs, sm, R, sb, se, pd, pj);
+    Partial bp2 = rpass_update_select_i<KIND, NV, PS>(V, c, d, xs, sm, R, sb, se, pd, pj, QF ? &qj : nullptr);
     XPart mine2 = rmake_i_record<KIND>(V, R, bp2);
     mine2.pad = pause_vote(V, d);
     seq = (unsigned)d.epoch + 1u;
@@ -696,6 +740,10 @@ __device__ __forceinline__ void smo_iterations_res(const View &V, Dec &d, RedSme
     pi.row[1] = V.full && KIND == KIND_NU && r2.i1 >= 0 ? real_of(V, r2.i1) : -1;
     if (pi.row[0] >= 0) prefetch_seg<PS>(V, pi.row[0], sb, se, pi.k[0]);
     if (pi.row[1] >= 0) prefetch_seg<PS>(V, pi.row[1], sb, se, pi.k[1]);
+    if (QF) {
+      prefetch_query(V, pi.row[0], qi.v[0], qi.row[0]);
+      prefetch_query(V, pi.row[1], qi.v[1], qi.row[1]);
+    }
     if (hold2) {
       d.epoch = (int)seq;
       d.pend = 0;
@@ -787,6 +835,7 @@ __global__ void __launch_bounds__(BLOCK) k_smo(View V, int iter_limit) {
   if (!res) smo_iterations<KIND, true, true>(V, d, sm, xsm, xs, iter_limit);
   else if (se - sb <= NWAVES) {  // every wave of this workgroup owns at most one slice
     if (V.svr) smo_iterations_res<KIND, 2, 1, false>(V, d, sm, xsm, xs, R, sb, se, iter_limit);
+    else if (V.full && V.qpre) smo_iterations_res<KIND, 1, 1, true, true>(V, d, sm, xsm, xs, R, sb, se, iter_limit);
     else if (V.full) smo_iterations_res<KIND, 1, 1, true>(V, d, sm, xsm, xs, R, sb, se, iter_limit);
     else smo_iterations_res<KIND, 1, 1, false>(V, d, sm, xsm, xs, R, sb, se, iter_limit);
   } else if (V.svr) {

@@ -604,6 +604,7 @@ svmhip_status svmhip_solve(svmhip_solver *h, int max_iter, svmhip_stats *stats) 
         shmem = need;
       }
     }
+    V.qpre = h->full && !use_gram && h->dense && h->d <= smo::BLOCK && V.flag_lds > 0 ? 1 : 0;
     // the owned variables' vectors and row segments resident in LDS past
     // the query row and the flag bits (SVMHIP_RESIDENT=0 keeps them in
     // device memory; results are identical either way)
