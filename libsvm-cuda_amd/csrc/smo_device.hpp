@@ -467,7 +467,9 @@ __device__ __forceinline__ double sparse_dot(const float *vp, const int *cp, XP 
 template <bool PIPE = false>
 __device__ __forceinline__ double row_dot(const View &V, const float *xs, int r) {
   const int s = r / SLICE;
-  const long long base = V.slice_ptr[s] + (r % SLICE);
+  // dense rows are stored at width d in every slice: their offset needs no
+  // load of the slice pointers (a round trip before the row's loads)
+  const long long base = V.dense ? (long long)s * SLICE * V.d + (r % SLICE) : V.slice_ptr[s] + (r % SLICE);
   const float *vp = V.vals + base;
   if (V.dense) {
 #if defined(__HIP_DEVICE_COMPILE__)  // (the builtin exists in the device pass only)
@@ -546,14 +548,20 @@ __device__ __forceinline__ void row_dot_multi(const View &V, const float *xs, in
 }
 
 // K(q, r) in FP64 given dot(x_q, x_r) (Kernel::kernel_*, libsvm/svm.cpp:243-255).
-__device__ __forceinline__ double kernel_f64(const View &V, int q, int r, double dot) {
+// The same, the two squared norms given (RBF; read before the dot so their
+// loads are not a round trip after it).
+__device__ __forceinline__ double kernel_f64_sq(const View &V, double xq, double xr, double dot) {
   switch (V.ktype) {
     case K_LINEAR: return dot;
     case K_POLY: return powi(V.gamma * dot + V.coef0, V.degree);
-    case K_RBF: return exp(-V.gamma * ((V.xsq[q] + V.xsq[r]) - 2 * dot));
+    case K_RBF: return exp(-V.gamma * ((xq + xr) - 2 * dot));
     case K_SIGMOID: return tanh(V.gamma * dot + V.coef0);
     default: return 0.0;
   }
+}
+
+__device__ __forceinline__ double kernel_f64(const View &V, int q, int r, double dot) {
+  return kernel_f64_sq(V, V.ktype == K_RBF ? V.xsq[q] : 0.0, V.ktype == K_RBF ? V.xsq[r] : 0.0, dot);
 }
 
 // Cached (float-rounded) kernel value K(q, r); q staged in xs.
@@ -563,7 +571,9 @@ __device__ __forceinline__ float kernel_value(const View &V, const float *xs, in
     int s = V.serial[r];
     return (s >= 0 && s < V.d) ? xs[s] : 0.0f;
   }
-  return (float)kernel_f64(V, q, r, row_dot<PIPE>(V, xs, r));
+  const bool rbf = V.ktype == K_RBF;
+  const double xq = rbf ? V.xsq[q] : 0.0, xr = rbf ? V.xsq[r] : 0.0;
+  return (float)kernel_f64_sq(V, xq, xr, row_dot<PIPE>(V, xs, r));
 }
 
 // dot(x_q, x_r) from the shared dot products (batched solves over one base
