@@ -8,7 +8,7 @@
  *
  * The one field the fork adds, svm_parameter.cuda_flag (libsvm/svm.h:49),
  * stays the FIRST member.  In this library cuda_flag = 1 selects the HIP
- * solver on the MI355X; this build carries no CPU solver, so
+ * solver on the MI355X; the library carries no CPU solver, so
  * svm_check_parameter() rejects cuda_flag != 1 with an error string.
  */
 #ifndef LIBSVM_MI355X_SVM_H

@@ -24,7 +24,7 @@ void print_null(const char *) {}
   printf(
       "Usage: svm-train [options] training_set_file [model_file]\n"
       "options:\n"
-      "-C : device solver (always on in this build; accepted for compatibility)\n"
+      "-C : device solver (always on: the library has no CPU solver; accepted for compatibility)\n"
       "-s svm_type : set type of SVM (default 0)\n"
       "\t0 -- C-SVC\t\t(multi-class classification)\n"
       "\t1 -- nu-SVC\t\t(multi-class classification)\n"

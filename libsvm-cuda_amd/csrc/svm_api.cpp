@@ -1007,7 +1007,7 @@ void svm_destroy_param(svm_parameter *param) {
   free(param->weight);
 }
 
-// libsvm/svm.cpp:3097-3216, plus the two restrictions of this build.
+// libsvm/svm.cpp:3097-3216, plus the two restrictions of this library.
 const char *svm_check_parameter(const svm_problem *prob, const svm_parameter *param) {
   int t = param->svm_type;
   if (t != C_SVC && t != NU_SVC && t != ONE_CLASS && t != EPSILON_SVR && t != NU_SVR) return "unknown svm type";

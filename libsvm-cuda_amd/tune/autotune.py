@@ -16,7 +16,7 @@ between the measured ones, split at geometric midpoints).  Results do not
 depend on either parameter, so the table only changes speed.
 
 Run on the device, from the repository root:
-    python libsvm-cuda_amd/tune/autotune.py [--quick] [--reps N] [--out PATH]
+    python libsvm-cuda_amd/tune/autotune.py [--quick] [--reps N] [--out PATH] [--log PATH]
 """
 import argparse
 import json
@@ -93,6 +93,7 @@ def main():
     # is ~1%: the committed table was written with --reps 3)
     ap.add_argument("--reps", type=int, default=3)
     ap.add_argument("--out", default=os.path.join(HERE, "launch_table.txt"))
+    ap.add_argument("--log", default=None, help="also write every timed point to this JSON file")
     args = ap.parse_args()
     libsvm_hip.set_quiet(True)
     best = {}  # (mode, full, kind, d) -> {l: (grid, copies, ms)}
@@ -136,9 +137,9 @@ def main():
             lines.append(f"{mode} {full} {dense} {lo} {hi} {dmin} {dmax} {g} {c}   # measured l={l} d={d}: {ms:.1f} ms")
     with open(args.out, "w") as f:
         f.write("\n".join(lines) + "\n")
-    os.makedirs("gpurun_out", exist_ok=True)
-    with open(os.path.join("gpurun_out", "autotune_log.json"), "w") as f:
-        json.dump(log, f, indent=1)
+    if args.log:  # every timed point, as JSON
+        with open(args.log, "w") as f:
+            json.dump(log, f, indent=1)
     print(f"wrote {args.out}", flush=True)
 
 
