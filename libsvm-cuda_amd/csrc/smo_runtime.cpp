@@ -983,7 +983,7 @@ svmhip_status svmhip_batch_solve(svmhip_batch *b, svmhip_subproblem *subs, int n
 
     // Stages.  Every unfinished subproblem runs until it converges or, after
     // about tau of device time, pauses at an iteration boundary (team members
-    // agree through their exchange); tau starts at 0.4 s and grows 4x each
+    // agree through their exchange); tau starts at 0.2 s and grows 3x each
     // stage.  A stage also ends once three quarters of its workgroups have
     // found the queue empty (the drain rule).  While more subproblems remain
     // than the grid has workgroups, each is solved by one workgroup (largest
@@ -994,8 +994,10 @@ svmhip_status svmhip_batch_solve(svmhip_batch *b, svmhip_subproblem *subs, int n
     // nothing: a resumed subproblem continues the same sequence of working
     // sets, but its kernel-row cache starts empty, so stages are long; the
     // first stage is the one that learns which subproblems run long
-    // (measured on 900 solves of 3k-14k rows, one box: first stages of
-    // 0.2 / 0.4 / 0.8 s took 3.83 / 3.59 / 3.93 s).
+    // (measured on 900 solves of 3k-14k rows, one box, all 900 in 1.3-1.4 s:
+    // first stages of 0.1 / 0.15 / 0.2 / 0.3 / 0.4 / 0.6 s, 4x growth, took
+    // 1.42 / 1.30 / 1.31 / 1.36 / 1.37 / 1.46 s; 0.2 s with 3x growth
+    // 1.30 s, with 6x 1.31 s).
     const int team_max = h->opt.grid_blocks > 0 ? std::min(h->opt.grid_blocks, 64) : 32;
     const int fixed_chunk = h->opt.iters_per_launch;
     std::vector<int> rem(nsub), iters(nsub, 0);
@@ -1004,7 +1006,7 @@ svmhip_status svmhip_batch_solve(svmhip_batch *b, svmhip_subproblem *subs, int n
     for (int k = 0; k < nsub; ++k) rem[k] = k;
     std::vector<smo::Ctl> hC(nsub);
     std::vector<int> T(nsub, 1), items;
-    double tau = 0.4;
+    double tau = 0.2;
     // team weights: with fewer subproblems than workgroups, n x (iterations
     // so far + 1); with more, an estimate of a subproblem's whole work,
     // n^1.5 x (iterations so far + n) (a one-workgroup iteration costs about
@@ -1146,7 +1148,7 @@ svmhip_status svmhip_batch_solve(svmhip_batch *b, svmhip_subproblem *subs, int n
                 stages, rem.size(), nteam, tmax, nitems, ms, next.size());
       }
       rem.swap(next);
-      tau *= 4;
+      tau *= 3;
     }
     h->ck(hipMemcpyAsync(hA.data(), dA, sizeof(double) * nv_total, hipMemcpyDeviceToHost, h->st), "fetch batch alpha");
     h->ck(hipStreamSynchronize(h->st), "fetch batch alpha");
