@@ -252,7 +252,9 @@ __global__ void __launch_bounds__(GBLOCK, 2) k_gram_f16(View V, const _Float16 *
         // row m = 4 (lane / 32) + mo of the wave's block, column n = 32 j + lane % 32
         const int mo = i * 32 + 8 * (e / 4) + e % 4;
         const float K = gram_value_t<KT>(V, KT == K_RBF ? sqq[mo] : 0.0, xr, acc[i][j][e]);
-        if (inside || (qa + mo < l && rb + 32 * j < l)) dst0[(long long)mo * stride + 32 * j] = K;
+        // (non-temporal: the 40 GB of a full matrix are not read back soon;
+        // n=100k: 19.4 -> 17.6 ms)
+        if (inside || (qa + mo < l && rb + 32 * j < l)) __builtin_nontemporal_store(K, &dst0[(long long)mo * stride + 32 * j]);
         tp0[mo * 65 + 32 * j] = K;
       }
   }
@@ -263,7 +265,7 @@ __global__ void __launch_bounds__(GBLOCK, 2) k_gram_f16(View V, const _Float16 *
   float *mp = V.rows + (long long)r0 * stride + q;
 #pragma unroll 8
   for (int n = 0; n < 64; ++n) {
-    if (inside || (r0 + n < l && q < l)) *mp = tp[wave][lane][n];
+    if (inside || (r0 + n < l && q < l)) __builtin_nontemporal_store(tp[wave][lane][n], mp);
     mp += stride;
   }
 }
