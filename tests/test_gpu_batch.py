@@ -487,3 +487,44 @@ def test_batch_shared_dots_dense_match_oracle(hip, ora, monkeypatch):
         np.testing.assert_array_equal(r["alpha_raw"], res0[i]["alpha_raw"])
     b.close()
     b0.close()
+
+
+def test_batch_kept_buffers_grow_for_a_larger_solve(hip):
+    """A handle whose first solve was small and whose second needs more of
+    every buffer (more subproblems, more variables, SVR doubling them)
+    reallocates and gives the results of a fresh handle."""
+    X, y = datasets.gmm_dense_matrix(n=2500, d=32, seed=31)
+    prob = ffi.Problem.from_dense(X.astype(np.float64), y, keep_zeros=True)
+    prm = ffi.make_param(gamma=1.0 / 32, C_=1.0, eps=1e-3, shrinking=0)
+    rng = np.random.default_rng(8)
+
+    def subs(b, sizes, svr_every):
+        out = []
+        for k, n in enumerate(sizes):
+            rows = np.sort(rng.choice(prob.l, size=n, replace=False))
+            sub = prob.subset(rows)
+            sp = ffi.make_param(svm_type=ffi.EPSILON_SVR if k % svr_every == 0 else ffi.C_SVC, gamma=1.0 / 32,
+                                C_=1.0, p=0.1, eps=1e-3, shrinking=0)
+            yy, p, a0, Cp, Cn, kind = hip.formulate(sub, sp)
+            out.append((rows, dict(rowset=b.add_rowset(rows), kind=kind, y=yy, p=p, alpha0=a0, Cp=Cp, Cn=Cn,
+                                   gamma=sp.gamma, eps=1e-3), sp))
+        return out
+
+    b = hip.BatchSolver(prob, prm)
+    small = subs(b, [200, 300], 99)
+    _, st1 = b.solve([s[1] for s in small])
+    big = subs(b, [900, 1500, 2200, 1100, 700], 2)
+    res, st2 = b.solve([s[1] for s in big])
+    assert st2["device_allocs"] > st1["device_allocs"]
+    fresh = hip.BatchSolver(prob, prm)
+    ref = []
+    for rows, d, sp in big:
+        d2 = dict(d)
+        d2["rowset"] = fresh.add_rowset(rows)
+        ref.append(d2)
+    res0, _ = fresh.solve(ref)
+    for a, c in zip(res, res0):
+        assert a["status"] == 0 and a["iters"] == c["iters"]
+        np.testing.assert_array_equal(a["alpha_raw"], c["alpha_raw"])
+    b.close()
+    fresh.close()
