@@ -27,24 +27,6 @@ struct Sell64 {
   std::vector<int> rl;           // stored entries per row
 };
 
-// f(i0, i1) over [0, n) split among host threads when the work (n items of
-// `cost` each) is large enough to pay for them.
-template <class F>
-void par_ranges(int n, double cost, F f) {
-  const unsigned hw = std::max(1u, std::thread::hardware_concurrency());
-  const int nt = (int)std::min<double>({(double)std::min(hw, 16u), (double)n, n * cost / 2e5});
-  if (nt <= 1) {
-    f(0, n);
-    return;
-  }
-  std::vector<std::thread> th;
-  for (int t = 0; t < nt; ++t) {
-    const int i0 = (int)((long long)n * t / nt), i1 = (int)((long long)n * (t + 1) / nt);
-    th.emplace_back([=, &f] { f(i0, i1); });
-  }
-  for (auto &x : th) x.join();
-}
-
 bool pack_sell64(const svm_node *const *x, int n, Sell64 &o, std::string &err) {
   std::vector<int> len(std::max(n, 0));
   for (int r = 0; r < n; ++r)

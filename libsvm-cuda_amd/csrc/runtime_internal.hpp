@@ -11,8 +11,11 @@
 #include <algorithm>
 #include <climits>
 #include <map>
+#include <memory>
 #include <string>
+#include <thread>
 #include <unordered_map>
+#include <utility>
 #include <vector>
 
 #include "../../include/svm_hip.h"
@@ -289,12 +292,51 @@ struct svmhip_solver {
 
 // Rows of a problem packed on the host in the SELL-64 layout
 // (see smo_device.hpp), ready for upload.
+// Allocator whose vectors leave new elements uninitialised (the packers
+// zero and fill them in parallel, by slice ranges).
+template <class T>
+struct uninit_alloc : std::allocator<T> {
+  template <class U>
+  struct rebind {
+    using other = uninit_alloc<U>;
+  };
+  uninit_alloc() = default;
+  template <class U>
+  uninit_alloc(const uninit_alloc<U> &) {}
+  template <class U>
+  void construct(U *p) noexcept {
+    ::new ((void *)p) U;
+  }
+  template <class U, class... A>
+  void construct(U *p, A &&...a) {
+    ::new ((void *)p) U(std::forward<A>(a)...);
+  }
+};
+
+// f(i0, i1) over [0, n) split among host threads (at most 16) when the work
+// (n items of `cost` each) is large enough to pay for them.
+template <class F>
+void par_ranges(int n, double cost, F f) {
+  const unsigned hw = std::max(1u, std::thread::hardware_concurrency());
+  const int nt = (int)std::min<double>({(double)std::min(hw, 16u), (double)n, n * cost / 2e5});
+  if (nt <= 1) {
+    f(0, n);
+    return;
+  }
+  std::vector<std::thread> th;
+  for (int t = 0; t < nt; ++t) {
+    const int i0 = (int)((long long)n * t / nt), i1 = (int)((long long)n * (t + 1) / nt);
+    th.emplace_back([=, &f] { f(i0, i1); });
+  }
+  for (auto &x : th) x.join();
+}
+
 struct HostSell {
   int l = 0, d = 1, nslices = 0, dense = 0;
   long long total = 0;
   std::vector<long long> sp;   // nslices + 1 slice offsets
-  std::vector<float> vals;
-  std::vector<int> cols;       // empty when dense
+  std::vector<float, uninit_alloc<float>> vals;
+  std::vector<int, uninit_alloc<int>> cols;  // empty when dense
   std::vector<int> rl;         // stored entries per row
   std::vector<int> serial;     // PRECOMPUTED only
 };

@@ -8,6 +8,8 @@
 // the persistent cooperative kernel (svmhip_solve).
 #include <dlfcn.h>
 
+#include <mutex>
+
 #include "runtime_internal.hpp"
 
 // ------------------------------------------------------- launch table
@@ -81,23 +83,54 @@ void make_host_sell(svmhip_solver *h, const svm_problem *prob, int ktype, HostSe
   std::vector<int> len(l);
   long long nnz = 0;
   int maxidx = -1, maxlen = 0;
-  for (int r = 0; r < l; ++r) {
-    const svm_node *x = prob->x[r];
-    if (!x) h->fail(SVMHIP_ERR_ARG, "row %d is null", r);
-    int n = 0, prev = INT_MIN;
-    for (; x[n].index != -1; ++n) {
-      if (!pre) {
-        if (x[n].index < 0) h->fail(SVMHIP_ERR_ARG, "row %d: negative feature index %d", r, x[n].index);
-        if (x[n].index <= prev) h->fail(SVMHIP_ERR_ARG, "row %d: feature indices not ascending", r);
-        prev = x[n].index;
-        maxidx = std::max(maxidx, x[n].index);
+  // row lengths and checks, in parallel over rows; the first bad row is the
+  // one reported (kind 1: null, 2: negative index, 3: not ascending, 4:
+  // precomputed row without its serial)
+  int bad_row = INT_MAX, bad_kind = 0, bad_index = 0;
+  std::mutex mu;
+  par_ranges(l, 64.0, [&](int r0, int r1) {
+    long long z = 0;
+    int mx = -1, ml = 0, br = INT_MAX, bk = 0, bi = 0;
+    for (int r = r0; r < r1 && br == INT_MAX; ++r) {
+      const svm_node *x = prob->x[r];
+      if (!x) {
+        br = r, bk = 1;
+        break;
       }
+      int n = 0, prev = INT_MIN;
+      for (; x[n].index != -1; ++n) {
+        if (!pre) {
+          if (x[n].index < 0) {
+            br = r, bk = 2, bi = x[n].index;
+            break;
+          }
+          if (x[n].index <= prev) {
+            br = r, bk = 3;
+            break;
+          }
+          prev = x[n].index;
+          mx = std::max(mx, x[n].index);
+        }
+      }
+      if (br != INT_MAX) break;
+      if (pre && n == 0) {
+        br = r, bk = 4;
+        break;
+      }
+      len[r] = n;
+      z += n;
+      ml = std::max(ml, n);
     }
-    if (pre && n == 0) h->fail(SVMHIP_ERR_ARG, "row %d: precomputed row needs 0:serial", r);
-    len[r] = n;
-    nnz += n;
-    maxlen = std::max(maxlen, n);
-  }
+    std::lock_guard<std::mutex> g(mu);
+    nnz += z;
+    maxidx = std::max(maxidx, mx);
+    maxlen = std::max(maxlen, ml);
+    if (br < bad_row) bad_row = br, bad_kind = bk, bad_index = bi;
+  });
+  if (bad_kind == 1) h->fail(SVMHIP_ERR_ARG, "row %d is null", bad_row);
+  if (bad_kind == 2) h->fail(SVMHIP_ERR_ARG, "row %d: negative feature index %d", bad_row, bad_index);
+  if (bad_kind == 3) h->fail(SVMHIP_ERR_ARG, "row %d: feature indices not ascending", bad_row);
+  if (bad_kind == 4) h->fail(SVMHIP_ERR_ARG, "row %d: precomputed row needs 0:serial", bad_row);
   o.l = l;
   o.d = pre ? std::max(maxlen, 1) : std::max(maxidx + 1, 1);
   o.dense = (!pre && l > 0 && (double)nnz >= 0.5 * (double)l * (double)o.d) ? 1 : 0;
@@ -113,24 +146,34 @@ void make_host_sell(svmhip_solver *h, const svm_problem *prob, int ktype, HostSe
     o.sp[s + 1] = o.sp[s] + (long long)w * smo::SLICE;
   }
   o.total = o.sp[o.nslices];
-  o.vals.assign(std::max<long long>(o.total, 1), 0.0f);
+  o.vals.resize(std::max<long long>(o.total, 1));  // (not initialised: zeroed below)
   o.cols.clear();
-  if (!o.dense) o.cols.assign(std::max<long long>(o.total, 1), 0);
+  if (!o.dense) o.cols.resize(std::max<long long>(o.total, 1));
+  if (o.total == 0) {
+    o.vals[0] = 0.0f;
+    if (!o.dense) o.cols[0] = 0;
+  }
   o.serial.clear();
   if (pre) o.serial.resize(l);
-  for (int r = 0; r < l; ++r) {
-    const svm_node *x = prob->x[r];
-    const long long base = o.sp[r / smo::SLICE] + (r % smo::SLICE);
-    if (pre) o.serial[r] = (int)x[0].value;
-    for (int p = 0; p < len[r]; ++p) {
-      if (o.dense) {
-        o.vals[base + (long long)x[p].index * smo::SLICE] = (float)x[p].value;
-      } else {
-        o.vals[base + (long long)p * smo::SLICE] = (float)x[p].value;
-        o.cols[base + (long long)p * smo::SLICE] = pre ? p : x[p].index;
+  // each thread zeroes and fills whole slices
+  const double per_slice = o.nslices ? (double)o.total / o.nslices : 0.0;
+  par_ranges(o.nslices, per_slice, [&](int s0, int s1) {
+    std::fill(o.vals.data() + o.sp[s0], o.vals.data() + o.sp[s1], 0.0f);
+    if (!o.dense) std::fill(o.cols.data() + o.sp[s0], o.cols.data() + o.sp[s1], 0);
+    for (int r = s0 * smo::SLICE; r < std::min(l, s1 * smo::SLICE); ++r) {
+      const svm_node *x = prob->x[r];
+      const long long base = o.sp[r / smo::SLICE] + (r % smo::SLICE);
+      if (pre) o.serial[r] = (int)x[0].value;
+      for (int p = 0; p < len[r]; ++p) {
+        if (o.dense) {
+          o.vals[base + (long long)x[p].index * smo::SLICE] = (float)x[p].value;
+        } else {
+          o.vals[base + (long long)p * smo::SLICE] = (float)x[p].value;
+          o.cols[base + (long long)p * smo::SLICE] = pre ? p : x[p].index;
+        }
       }
     }
-  }
+  });
 }
 
 static void build_sell(svmhip_solver *h, const svm_problem *prob) {
