@@ -693,6 +693,8 @@ __device__ __forceinline__ XPart rmake_j_record(const View &V, const Res &R, con
 // smo_iterations over the resident vectors (same decisions and exchanges).
 // PS: slices a wave takes per trip of a pass (1 when each wave owns at most
 // one slice: the unrolled form's idle copies cost registers and time).
+// FULL: the row cache holds every row (V.full).  QF: query rows of likely
+// misses fetched after each exchange (V.qpre).
 template <int KIND, int NV, int PS, bool FULL, bool QF = false>
 __device__ __forceinline__ void smo_iterations_res(const View &V, Dec &d, RedSmem &sm, XSmem &xsm, float *xs,
                                                    const Res &R, int sb, int se, int iter_limit) {
@@ -832,6 +834,10 @@ __global__ void __launch_bounds__(BLOCK) k_smo(View V, int iter_limit) {
     res_load(V, R, sb, se);
   }
   if (d.phase == 0 && !d.stop && !d.err) smo_select_first<KIND, true>(V, d, sm, xsm, xs);
+  // one instantiation of the resident loop per case, each carrying only the
+  // code its case runs (fewer live registers in the loop): variables per
+  // row, slices per trip, full cache (no LRU bookkeeping), query rows
+  // fetched after the exchanges.  All give the same iterates.
   if (!res) smo_iterations<KIND, true, true>(V, d, sm, xsm, xs, iter_limit);
   else if (se - sb <= NWAVES) {  // every wave of this workgroup owns at most one slice
     if (V.svr) smo_iterations_res<KIND, 2, 1, false>(V, d, sm, xsm, xs, R, sb, se, iter_limit);
