@@ -81,7 +81,8 @@ typedef struct {
 				     (0 = the launch table, else 4 with 64 or more workgroups and 1
 				     below); results are identical for every value */
 	double hbm_fraction;      /* batched solves: at most this fraction of the free device memory
-				     goes to kernel-row cache arenas (0 = 0.7); svm_train and
+				     goes to kernel-row cache arenas (0 = 0.7, and at most 32 GiB in
+				     all unless cache_bytes or hbm_fraction is given); svm_train and
 				     svm_cross_validation take it from the environment variable
 				     SVMHIP_HBM_FRACTION */
 } svmhip_options;
@@ -198,7 +199,9 @@ const char *svmhip_last_error(const svmhip_solver *h);
  * 1 = one workgroup per subproblem); iters_per_launch = iterations per
  * subproblem per launch (0 = sized by the scheduler); cache_bytes = arena
  * per workgroup (0 = an equal share of hbm_fraction, default 70%, of free
- * device memory); verbose = 1 prints the arena size chosen.
+ * device memory, at most 32 GiB in all when hbm_fraction is not given either:
+ * allocating most of the device memory again after freeing it takes
+ * seconds); verbose = 1 prints the arena size chosen.
  * ------------------------------------------------------------------------- */
 
 typedef struct svmhip_batch svmhip_batch;

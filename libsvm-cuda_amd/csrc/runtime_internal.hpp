@@ -59,6 +59,14 @@ namespace svmhip_detail {
 
 constexpr size_t kLdsQueryLimit = 64 * 1024;  // query row staged in LDS up to this size
 constexpr size_t kFlagLdsLimit = 64 * 1024;  // persistent kernel: query row + row-flag bitset in LDS
+// Batched solves with neither cache_bytes nor hbm_fraction given: all the
+// kernel-row arenas together at most this.  A device allocation of most of
+// the free memory right after freeing as much takes seconds (the freed
+// memory is cleared again: 190 GB in 5.7 s, 100 GB up to 3 s, 38 GB at once
+// on one MI355X), so a handle made per call (svm_train, svm_cross_validation)
+// paid it on every call after the first; rows gathered from the shared dot
+// products make a larger cache worth little.
+constexpr size_t kBatchArenaDefaultBytes = 32ull << 30;
 
 struct Failure {
   svmhip_status code;

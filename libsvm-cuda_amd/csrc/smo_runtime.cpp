@@ -991,6 +991,10 @@ svmhip_status svmhip_batch_solve(svmhip_batch *b, svmhip_subproblem *subs, int n
     const long long share = (long long)(frac * (double)freeb / sizeof(float)) / K;
     long long arena = h->opt.cache_bytes > 0 ? (long long)(h->opt.cache_bytes / sizeof(float)) : share;
     arena = std::max<long long>(std::min(arena, std::max<long long>(share, 2 * maxstride)), 2 * maxstride);
+    // the default budget (kBatchArenaDefaultBytes in all)
+    if (h->opt.cache_bytes <= 0 && h->opt.hbm_fraction <= 0)
+      arena = std::max<long long>(std::min<long long>(arena, (long long)(kBatchArenaDefaultBytes / sizeof(float)) / K),
+                                  2 * maxstride);
     // no subproblem needs more than its whole matrix
     arena = std::min<long long>(arena, std::max<long long>((long long)maxn * maxstride, 2 * maxstride));
     // within 5% of the kept arenas: keep them rather than allocate anew

@@ -528,3 +528,36 @@ def test_batch_kept_buffers_grow_for_a_larger_solve(hip):
         np.testing.assert_array_equal(a["alpha_raw"], c["alpha_raw"])
     b.close()
     fresh.close()
+
+
+def test_batch_default_arena_budget(hip):
+    """With neither cache_bytes nor hbm_fraction given, the kernel-row arenas
+    take at most 32 GiB in all, so a handle per call (svm_train,
+    svm_cross_validation) does not re-allocate most of the device memory each
+    time; the solve matches one with smaller arenas given by cache_bytes."""
+    X, y = datasets.gmm_dense_matrix(n=20000, d=16, seed=41)
+    prob = ffi.Problem.from_dense(X.astype(np.float64), y, keep_zeros=True)
+    prm = ffi.make_param(gamma=1.0 / 16, C_=1.0, eps=1e-3, shrinking=0)
+    rows = np.arange(0, 20000, 2)
+    n = len(rows)
+    stride = (n + 63) // 64 * 64
+    sub = prob.subset(rows)
+    yy, p, a0, Cp, Cn, kind = hip.formulate(sub, prm)
+    out = []
+    for cache_bytes in (0, 64 << 20):
+        b = hip.BatchSolver(prob, prm, cache_bytes=cache_bytes)
+        rid = b.add_rowset(rows)
+        res, st = b.solve([dict(rowset=rid, kind=kind, y=yy, p=p, alpha0=a0, Cp=Cp, Cn=Cn, gamma=prm.gamma,
+                                eps=1e-3)])
+        total = st["cache_slots"] * stride * 4 * st["grid_blocks"]
+        print(f"  cache_bytes={cache_bytes}: {st['cache_slots']} slots x {st['grid_blocks']} arenas = "
+              f"{total / 2 ** 30:.1f} GiB, iters {res[0]['iters']}", flush=True)
+        assert res[0]["status"] == 0
+        out.append((res[0], st))
+        b.close()
+    (r0, st0), (r1, st1) = out
+    assert st0["cache_slots"] * stride * 4 * st0["grid_blocks"] <= 32 * 2 ** 30
+    assert st0["cache_slots"] < n  # the whole matrix in each arena would take 100 GB
+    assert st1["cache_slots"] == (64 << 20) // (stride * 4)
+    assert r0["iters"] == r1["iters"]
+    np.testing.assert_array_equal(r0["alpha_raw"], r1["alpha_raw"])
