@@ -114,19 +114,28 @@ __device__ __forceinline__ float gram_value(const View &V, double xq, double xr,
 // rows q (K(q, r)) and, transposed through LDS, to rows r (K(r, q)); every
 // store is a run of consecutive floats of one cache row.
 //
-// Operands go through LDS in chunks of GKC = 32 features, double buffered:
-// the workgroup's 128 q rows and 128 r rows of a chunk (128 bytes of hi/lo
+// Operands go through LDS in chunks of GKC = 16 features, double buffered:
+// the workgroup's 128 q rows and 128 r rows of a chunk (64 bytes of hi/lo
 // halves per row) are loaded by all 256 threads into registers while the
 // MFMAs of the previous chunk run from LDS, then stored to the other buffer;
-// one barrier per chunk.  Rows are padded to 144 bytes in LDS against bank
-// conflicts of the 16-byte operand reads.
-constexpr int GKC = 32;                   // features per chunk
+// one barrier per chunk.  Rows are padded to 80 bytes in LDS against bank
+// conflicts of the 16-byte operand reads.  The MFMAs accumulate 16 features
+// at a time in feature order whatever the chunk width, so the chunking does
+// not change a value.
+constexpr int GKC = 16;                   // features per chunk
 constexpr int GROW = GKC * 2 * 2 + 16;    // LDS bytes per row of a chunk: hi/lo halves + pad
 constexpr int GBUF = 2 * GT * GROW;       // one buffer: 128 q rows + 128 r rows
-constexpr int GLDS = 2 * GBUF > 4 * 64 * 65 * 4 ? 2 * GBUF : 4 * 64 * 65 * 4;
+constexpr int GPPR = GKC / 4;             // 16-byte pieces per row of a chunk
+constexpr int GNP = GT * GPPR / GBLOCK;   // pieces per thread of the q (and of the r) rows
+constexpr int GTP = 4 * 64 * 33 * 4;      // transpose: each wave's 64 x 32 half-block, padded
+constexpr int GLDS = 2 * GBUF > GTP ? 2 * GBUF : GTP;
+// 42 KB of LDS (operands 40 KB + norms 2 KB) and at most 168 VGPRs: three
+// workgroups per CU, so one's MFMA phase overlaps the others' loads and
+// epilogues (n=100k d=256: 19.6 -> 17.9 ms against 32-feature chunks, a
+// 64 x 64 transpose and two workgroups per CU, 74 KB of LDS)
 
 template <int KT>
-__global__ void __launch_bounds__(GBLOCK, 2) k_gram_f16(View V, const _Float16 *xs, long long ntiles, int nt) {
+__global__ void __launch_bounds__(GBLOCK, 3) k_gram_f16(View V, const _Float16 *xs, long long ntiles, int nt) {
   __shared__ __attribute__((aligned(16))) unsigned char lds[GLDS];
   __shared__ double sq[2 * GT];  // squared norms of the tile's q rows, then r rows
   const long long per = (ntiles + 7) / 8;
@@ -143,15 +152,15 @@ __global__ void __launch_bounds__(GBLOCK, 2) k_gram_f16(View V, const _Float16 *
   const long long rh = split_row_halves(V.d);
   const int ngroups = (int)(rh / 16);  // groups of 8 features ([hi x 8][lo x 8], 32 bytes) per row
   const int l = V.l;
-  // chunk loads: thread t moves pieces p = t + 256 j (j < 4) of the q rows
-  // and of the r rows: row p / 8, bytes 16 (p % 8) of the chunk's 128
+  // chunk loads: thread t moves pieces p = t + 256 j (j < GNP) of the q rows
+  // and of the r rows: row p / GPPR, bytes 16 (p % GPPR) of the chunk's 4 GKC
   const unsigned char *gx = reinterpret_cast<const unsigned char *>(xs);
-  long long src_q[4], src_r[4];
-  int dst[4], col[4];
+  long long src_q[GNP], src_r[GNP];
+  int dst[GNP], col[GNP];
 #pragma unroll
-  for (int j = 0; j < 4; ++j) {
-    const int pc = (int)threadIdx.x + 256 * j, row = pc / 8;
-    col[j] = pc % 8;
+  for (int j = 0; j < GNP; ++j) {
+    const int pc = (int)threadIdx.x + GBLOCK * j, row = pc / GPPR;
+    col[j] = pc % GPPR;
     src_q[j] = (long long)min(tq * GT + row, l - 1) * rh * 2 + col[j] * 16;
     src_r[j] = (long long)min(tr * GT + row, l - 1) * rh * 2 + col[j] * 16;
     dst[j] = row * GROW + col[j] * 16;
@@ -161,18 +170,18 @@ __global__ void __launch_bounds__(GBLOCK, 2) k_gram_f16(View V, const _Float16 *
   // one chunk's pieces of this thread into registers (groups past the
   // row's padded features read as zeros)
 #define GRAM_FETCH(c)                                                                          \
-  _Pragma("unroll") for (int j = 0; j < 4; ++j) {                                              \
-    const bool in = (c) * 4 + col[j] / 2 < ngroups;                                            \
+  _Pragma("unroll") for (int j = 0; j < GNP; ++j) {                                            \
+    const bool in = (c) * (GKC / 8) + col[j] / 2 < ngroups;                                    \
     const u32x4 z = {0u, 0u, 0u, 0u};                                                          \
-    pq[j] = in ? *reinterpret_cast<const u32x4 *>(gx + src_q[j] + (long long)(c) * 128) : z;   \
-    pr[j] = in ? *reinterpret_cast<const u32x4 *>(gx + src_r[j] + (long long)(c) * 128) : z;   \
+    pq[j] = in ? *reinterpret_cast<const u32x4 *>(gx + src_q[j] + (long long)(c) * (GKC * 4)) : z; \
+    pr[j] = in ? *reinterpret_cast<const u32x4 *>(gx + src_r[j] + (long long)(c) * (GKC * 4)) : z; \
   }
 #define GRAM_STASH(b)                                                                          \
-  _Pragma("unroll") for (int j = 0; j < 4; ++j) {                                              \
+  _Pragma("unroll") for (int j = 0; j < GNP; ++j) {                                            \
     *reinterpret_cast<u32x4 *>(lds + (b) * GBUF + dst[j]) = pq[j];                             \
     *reinterpret_cast<u32x4 *>(lds + (b) * GBUF + GT * GROW + dst[j]) = pr[j];                 \
   }
-  u32x4 pq[4], pr[4];
+  u32x4 pq[GNP], pr[GNP];
   float16v acc[2][2];
 #pragma unroll
   for (int i = 0; i < 2; ++i)
@@ -232,7 +241,7 @@ __global__ void __launch_bounds__(GBLOCK, 2) k_gram_f16(View V, const _Float16 *
   // + e % 4, n = lane % 32.  Rows q: K(q, r), lanes on consecutive r; the
   // values also go to LDS (over the operand buffers, after the last
   // barrier) for the transposed store.
-  float(*tp)[64][65] = reinterpret_cast<float(*)[64][65]>(lds);
+  float(*tp)[64][33] = reinterpret_cast<float(*)[64][33]>(lds);
   const bool mirror = tq != tr;
   // a wave's block lies inside the matrix unless it touches row or column
   // l (only the last tile row / column): then every store is checked
@@ -242,8 +251,11 @@ __global__ void __launch_bounds__(GBLOCK, 2) k_gram_f16(View V, const _Float16 *
   float *dst0 = V.rows + (long long)qa * stride + rb;
   float *tp0 = &tp[wave][4 * (lane / 32)][lane % 32];
   const double *sqq = &sq[(wave / 2) * 64 + 4 * (lane / 32)];
+  const int q = q0 + lane;
 #pragma unroll
   for (int j = 0; j < 2; ++j) {
+    // columns 32 j .. 32 j + 31 of the wave's block: rows q, then (through
+    // LDS, 64 x 32 per wave) rows r
     const double xr = KT == K_RBF ? sq[GT + (wave % 2) * 64 + j * 32 + lane % 32] : 0.0;
 #pragma unroll
     for (int i = 0; i < 2; ++i)
@@ -255,18 +267,18 @@ __global__ void __launch_bounds__(GBLOCK, 2) k_gram_f16(View V, const _Float16 *
         // (non-temporal: the 40 GB of a full matrix are not read back soon;
         // n=100k: 19.4 -> 17.6 ms)
         if (inside || (qa + mo < l && rb + 32 * j < l)) __builtin_nontemporal_store(K, &dst0[(long long)mo * stride + 32 * j]);
-        tp0[mo * 65 + 32 * j] = K;
+        tp0[mo * 33] = K;
       }
-  }
-  if (!mirror) return;  // uniform over the workgroup
-  __syncthreads();
-  // rows r: K(r, q) = K(q, r), lanes on consecutive q
-  const int q = q0 + lane;
-  float *mp = V.rows + (long long)r0 * stride + q;
+    if (!mirror) continue;  // uniform over the workgroup
+    __syncthreads();
+    // rows r: K(r, q) = K(q, r), lanes on consecutive q
+    float *mp = V.rows + (long long)(r0 + 32 * j) * stride + q;
 #pragma unroll 8
-  for (int n = 0; n < 64; ++n) {
-    if (inside || (r0 + n < l && q < l)) __builtin_nontemporal_store(tp[wave][lane][n], mp);
-    mp += stride;
+    for (int n = 0; n < 32; ++n) {
+      if (inside || (r0 + 32 * j + n < l && q < l)) __builtin_nontemporal_store(tp[wave][lane][n], mp);
+      mp += stride;
+    }
+    __syncthreads();  // before the next half overwrites tp
   }
 }
 
