@@ -118,24 +118,31 @@ __device__ __forceinline__ float gram_value(const View &V, double xq, double xr,
 // the workgroup's 128 q rows and 128 r rows of a chunk (64 bytes of hi/lo
 // halves per row) are loaded by all 256 threads into registers while the
 // MFMAs of the previous chunk run from LDS, then stored to the other buffer;
-// one barrier per chunk.  Rows are padded to 80 bytes in LDS against bank
-// conflicts of the 16-byte operand reads.  The MFMAs accumulate 16 features
+// one barrier per chunk.  The 16-byte pieces of a row are swizzled in LDS
+// (gswz) against bank conflicts of the operand reads.  The MFMAs accumulate 16 features
 // at a time in feature order whatever the chunk width, so the chunking does
 // not change a value.
 constexpr int GKC = 16;                   // features per chunk
-constexpr int GROW = GKC * 2 * 2 + 16;    // LDS bytes per row of a chunk: hi/lo halves + pad
+constexpr int GROW = GKC * 2 * 2;         // LDS bytes per row of a chunk: hi/lo halves
 constexpr int GBUF = 2 * GT * GROW;       // one buffer: 128 q rows + 128 r rows
 constexpr int GPPR = GKC / 4;             // 16-byte pieces per row of a chunk
 constexpr int GNP = GT * GPPR / GBLOCK;   // pieces per thread of the q (and of the r) rows
 constexpr int GTP = 4 * 64 * 33 * 4;      // transpose: each wave's 64 x 32 half-block, padded
 constexpr int GLDS = 2 * GBUF > GTP ? 2 * GBUF : GTP;
-// 42 KB of LDS (operands 40 KB + norms 2 KB) and at most 168 VGPRs: three
-// workgroups per CU, so one's MFMA phase overlaps the others' loads and
-// epilogues (n=100k d=256: 19.6 -> 17.9 ms against 32-feature chunks, a
-// 64 x 64 transpose and two workgroups per CU, 74 KB of LDS)
+static_assert((GPPR & (GPPR - 1)) == 0, "pieces per row: a power of two");
+// 16-byte piece c of LDS row r sits at position c ^ ((r / 4) mod GPPR): the
+// 16 rows one ds_read_b128 group of lanes reads (same piece, consecutive
+// rows at a 64-byte stride) then cover all 64 banks once
+__device__ __forceinline__ int gswz(int r, int c) { return c ^ ((r >> 2) & (GPPR - 1)); }
+// 35 KB of LDS (operands 32 KB + norms 2 KB; the transpose reuses the
+// operand buffers) and at most 128 VGPRs: four workgroups per CU, so one's
+// MFMA phase overlaps the others' loads and epilogues (n=100k d=256: 16.2
+// ms; 17.6 ms with rows padded to 80 bytes instead of swizzled, three
+// workgroups per CU; 19.6 ms with 32-feature chunks, a 64 x 64 transpose
+// and two workgroups per CU)
 
 template <int KT>
-__global__ void __launch_bounds__(GBLOCK, 3) k_gram_f16(View V, const _Float16 *xs, long long ntiles, int nt) {
+__global__ void __launch_bounds__(GBLOCK, 4) k_gram_f16(View V, const _Float16 *xs, long long ntiles, int nt) {
   __shared__ __attribute__((aligned(16))) unsigned char lds[GLDS];
   __shared__ double sq[2 * GT];  // squared norms of the tile's q rows, then r rows
   const long long per = (ntiles + 7) / 8;
@@ -163,7 +170,7 @@ __global__ void __launch_bounds__(GBLOCK, 3) k_gram_f16(View V, const _Float16 *
     col[j] = pc % GPPR;
     src_q[j] = (long long)min(tq * GT + row, l - 1) * rh * 2 + col[j] * 16;
     src_r[j] = (long long)min(tr * GT + row, l - 1) * rh * 2 + col[j] * 16;
-    dst[j] = row * GROW + col[j] * 16;
+    dst[j] = row * GROW + gswz(row, col[j]) * 16;
   }
   typedef unsigned int u32x4 __attribute__((ext_vector_type(4)));
   const int nchunks = (V.d + GKC - 1) / GKC;
@@ -203,16 +210,15 @@ __global__ void __launch_bounds__(GBLOCK, 3) k_gram_f16(View V, const _Float16 *
       // of the chunk (group 2 ss + lane / 32: hi at +0, lo at +16 bytes);
       // the four blocks' products of one term go back to back, so no two
       // consecutive MFMAs accumulate into the same registers
-      const int off = (2 * ss + lane / 32) * 32;
+      const int pg = 2 * (2 * ss + lane / 32);  // the group's hi piece; lo is pg + 1
       half8 ah[2], al[2], bh[2], bl[2];
 #pragma unroll
       for (int b = 0; b < 2; ++b) {
-        const unsigned char *ra = A + ((wave / 2) * 64 + b * 32 + lane % 32) * GROW + off;
-        const unsigned char *rb = B + ((wave % 2) * 64 + b * 32 + lane % 32) * GROW + off;
-        ah[b] = *reinterpret_cast<const half8 *>(ra);
-        al[b] = *reinterpret_cast<const half8 *>(ra + 16);
-        bh[b] = *reinterpret_cast<const half8 *>(rb);
-        bl[b] = *reinterpret_cast<const half8 *>(rb + 16);
+        const int ia = (wave / 2) * 64 + b * 32 + lane % 32, ib = (wave % 2) * 64 + b * 32 + lane % 32;
+        ah[b] = *reinterpret_cast<const half8 *>(A + ia * GROW + gswz(ia, pg) * 16);
+        al[b] = *reinterpret_cast<const half8 *>(A + ia * GROW + gswz(ia, pg + 1) * 16);
+        bh[b] = *reinterpret_cast<const half8 *>(B + ib * GROW + gswz(ib, pg) * 16);
+        bl[b] = *reinterpret_cast<const half8 *>(B + ib * GROW + gswz(ib, pg + 1) * 16);
       }
 #pragma unroll
       for (int i = 0; i < 2; ++i)
