@@ -136,10 +136,11 @@ static_assert((GPPR & (GPPR - 1)) == 0, "pieces per row: a power of two");
 __device__ __forceinline__ int gswz(int r, int c) { return c ^ ((r >> 2) & (GPPR - 1)); }
 // 35 KB of LDS (operands 32 KB + norms 2 KB; the transpose reuses the
 // operand buffers) and at most 128 VGPRs: four workgroups per CU, so one's
-// MFMA phase overlaps the others' loads and epilogues (n=100k d=256: 16.2
-// ms; 17.6 ms with rows padded to 80 bytes instead of swizzled, three
-// workgroups per CU; 19.6 ms with 32-feature chunks, a 64 x 64 transpose
-// and two workgroups per CU)
+// MFMA phase overlaps the others' loads and epilogues.  n=100k d=256: 15.5
+// ms; 16.5 ms with 4-byte stores of each value straight from the MFMA
+// registers (4x the store instructions); 17.6 ms with rows padded to 80
+// bytes instead of swizzled, three workgroups per CU; 19.6 ms with
+// 32-feature chunks, a 64 x 64 transpose and two workgroups per CU.
 
 template <int KT>
 __global__ void __launch_bounds__(GBLOCK, 4) k_gram_f16(View V, const _Float16 *xs, long long ntiles, int nt) {
@@ -258,10 +259,13 @@ __global__ void __launch_bounds__(GBLOCK, 4) k_gram_f16(View V, const _Float16 *
   float *tp0 = &tp[wave][4 * (lane / 32)][lane % 32];
   const double *sqq = &sq[(wave / 2) * 64 + 4 * (lane / 32)];
   const int q = q0 + lane;
+  typedef float f4 __attribute__((ext_vector_type(4)));
 #pragma unroll
   for (int j = 0; j < 2; ++j) {
-    // columns 32 j .. 32 j + 31 of the wave's block: rows q, then (through
-    // LDS, 64 x 32 per wave) rows r
+    // columns 32 j .. 32 j + 31 of the wave's block, through LDS (64 x 32
+    // per wave): a block inside the matrix goes out in 16-byte stores, 8
+    // rows q (K(q, r)) and 4 rows r (K(r, q) = K(q, r)) per instruction;
+    // one on the matrix's edge in checked 4-byte stores
     const double xr = KT == K_RBF ? sq[GT + (wave % 2) * 64 + j * 32 + lane % 32] : 0.0;
 #pragma unroll
     for (int i = 0; i < 2; ++i)
@@ -270,19 +274,40 @@ __global__ void __launch_bounds__(GBLOCK, 4) k_gram_f16(View V, const _Float16 *
         // row m = 4 (lane / 32) + mo of the wave's block, column n = 32 j + lane % 32
         const int mo = i * 32 + 8 * (e / 4) + e % 4;
         const float K = gram_value_t<KT>(V, KT == K_RBF ? sqq[mo] : 0.0, xr, acc[i][j][e]);
-        // (non-temporal: the 40 GB of a full matrix are not read back soon;
-        // n=100k: 19.4 -> 17.6 ms)
-        if (inside || (qa + mo < l && rb + 32 * j < l)) __builtin_nontemporal_store(K, &dst0[(long long)mo * stride + 32 * j]);
+        // (non-temporal: the 40 GB of a full matrix are not read back soon)
+        if (!inside && qa + mo < l && rb + 32 * j < l) __builtin_nontemporal_store(K, &dst0[(long long)mo * stride + 32 * j]);
         tp0[mo * 33] = K;
       }
-    if (!mirror) continue;  // uniform over the workgroup
     __syncthreads();
-    // rows r: K(r, q) = K(q, r), lanes on consecutive q
-    float *mp = V.rows + (long long)(r0 + 32 * j) * stride + q;
+    if (inside) {
+      // rows q: row 8 it + lane / 8 of the block, columns 4 (lane % 8) .. + 3
+      const int c4 = 4 * (lane % 8);
+      float *qrow = V.rows + (long long)(q0 + lane / 8) * stride + r0 + 32 * j + c4;
+#pragma unroll
+      for (int it = 0; it < 8; ++it) {
+        const int m = 8 * it + lane / 8;
+        const f4 v = {tp[wave][m][c4], tp[wave][m][c4 + 1], tp[wave][m][c4 + 2], tp[wave][m][c4 + 3]};
+        __builtin_nontemporal_store(v, reinterpret_cast<f4 *>(qrow + (long long)(8 * it) * stride));
+      }
+      if (mirror) {
+        // rows r: row 4 it + lane / 16 of the half, columns q0 + 4 (lane % 16) .. + 3
+        const int m4 = 4 * (lane % 16);
+        float *rrow = V.rows + (long long)(r0 + 32 * j + lane / 16) * stride + q0 + m4;
+#pragma unroll
+        for (int it = 0; it < 8; ++it) {
+          const int n = 4 * it + lane / 16;
+          const f4 v = {tp[wave][m4][n], tp[wave][m4 + 1][n], tp[wave][m4 + 2][n], tp[wave][m4 + 3][n]};
+          __builtin_nontemporal_store(v, reinterpret_cast<f4 *>(rrow + (long long)(4 * it) * stride));
+        }
+      }
+    } else if (mirror) {
+      // rows r: K(r, q) = K(q, r), lanes on consecutive q
+      float *mp = V.rows + (long long)(r0 + 32 * j) * stride + q;
 #pragma unroll 8
-    for (int n = 0; n < 32; ++n) {
-      if (inside || (r0 + 32 * j + n < l && q < l)) __builtin_nontemporal_store(tp[wave][lane][n], mp);
-      mp += stride;
+      for (int n = 0; n < 32; ++n) {
+        if (r0 + 32 * j + n < l && q < l) __builtin_nontemporal_store(tp[wave][lane][n], mp);
+        mp += stride;
+      }
     }
     __syncthreads();  // before the next half overwrites tp
   }
